@@ -1,0 +1,149 @@
+"""Benchmark: Kerr RKF45 1080p rays/s (FP32-bulk and plain FP64), one MI355X.
+
+A step is one full frame of the C1 workload: Kerr a = 0.95 (Boyer-Lindquist),
+adaptive RKF45 at rtol = atol = 1e-8, 1920x1080 with 2x2 regular SSAA
+(8,294,400 primary rays), a textured sphere at r = 8, then the SSAA combine
+and FXAA + RGB8; inputs (textures, frame table) are resident in HBM and the
+images are written to device buffers.  `value` is primary rays per second of
+the FP32-bulk form; the plain FP64 form is timed in the same run.
+
+roofline: the dominant kernel is the persistent integrator, bound by vector
+ALU throughput (no dense contraction: no MFMA).  Its algorithmic FLOP count
+comes from the kernel's own attempt counters times the per-attempt model of
+capi.cpp (FP32 stage flops and FP64 flops kept apart); `achieved` divides by
+the kernel's duration measured with HIP events on its own stream.  frac =
+(F32/P32 + F64/P64) / t against MI355X's published vector peaks
+(157.3 TFLOP/s FP32, 78.6 TFLOP/s FP64).
+
+cpu_baseline: the CPU FP64 oracle (oracle/, a restatement of the reference
+solver) on a stratified row subset of the same frame, all host cores, OpenMP.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+P32, P64 = 157.3e12, 78.6e12
+METRIC = "Kerr RKF45 1080p rays/s + adjoint rays/s, FP32-bulk vs FP64 at ≤1e-6 rad"
+
+
+def run_form(ctx, frame, precision, steps, warmup, out8, outf, torch):
+    from parrelray_amd import _abi as A
+    from parrelray_amd import configs as CF
+    ctx.set_tracer(CF.tracer_kerr(precision=precision))
+    for _ in range(warmup):
+        ctx.render_device([frame], rgb8=[out8], rgbf=[outf])
+    torch.cuda.synchronize()
+    per = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.render_device([frame], rgb8=[out8], rgbf=[outf])
+        per.append(ctx.stats())
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    ms = (t1 - t0) * 1e3 / steps
+    integ = sum(s["integrate_ms"] for s in per) / steps
+    st = per[-1]
+    f32, f64 = st["flops_fp32"], st["flops_fp64"]
+    t_int = integ / 1e3
+    return dict(ms_per_step=ms, integrate_ms=integ, ssaa_ms=sum(s["ssaa_ms"] for s in per) / steps,
+                fxaa_ms=sum(s["fxaa_ms"] for s in per) / steps, rays=st["rays"], attempts=st["attempts"],
+                accepted=st["accepted"], attempts_fp64=st["attempts_fp64"], flops_fp32=f32, flops_fp64=f64,
+                rays_per_s=st["rays"] / (ms / 1e3), attempts_per_s=st["attempts"] / t_int,
+                achieved_tflops=(f32 + f64) / t_int / 1e12,
+                roofline_frac=(f32 / P32 + f64 / P64) / t_int,
+                ssaa_gbs=st["ssaa_bytes"] / (sum(s["ssaa_ms"] for s in per) / steps / 1e3) / 1e9,
+                fxaa_gbs=st["fxaa_bytes"] / max(1e-9, sum(s["fxaa_ms"] for s in per) / steps / 1e3) / 1e9)
+
+
+def cpu_baseline(frame, row_stride=16):
+    """Oracle (FP64, Christoffel form) on every row_stride-th image row, all cores."""
+    import numpy as np
+
+    import oracle as O
+    from parrelray_amd import _abi as A
+    from parrelray_amd import configs as CF
+    S = O.Scene(A.PRR_KERR_BL, 1.0, 0.95, CF.tracer_kerr())
+    S.set_texture(0, CF.sky_texture())
+    S.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
+    per_row = frame.width * frame.samples
+    rows = np.arange(0, frame.height, row_stride)
+    ids = (rows[:, None] * per_row + np.arange(per_row)[None, :]).ravel()
+    cores = len(os.sched_getaffinity(0))
+    r = S.render_rays(frame, ids, nthreads=cores)
+    return dict(value=ids.size / r["seconds"], unit="rays/s", cores=cores, kind="port",
+                sample=f"C1 rows j % {row_stride} == 0 ({ids.size} rays of {frame.nrays}), FP64 RKF45 tol 1e-8, "
+                       f"{r['seconds']:.1f} s")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: the integrator has no multi-GPU path in this build "
+              f"(--gpus {args.gpus} requested).", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    from parrelray_amd import _abi as A
+    from parrelray_amd import api
+    from parrelray_amd import configs as CF
+
+    frame = CF.frame_c1()
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.95)
+    ctx.set_texture(0, CF.sky_texture())
+    ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
+    out8 = torch.empty((frame.height, frame.width, 3), dtype=torch.uint8, device="cuda")
+    outf = torch.empty((frame.height, frame.width, 3), dtype=torch.float32, device="cuda")
+    fp64 = run_form(ctx, frame, A.PRR_FP64, args.steps, args.warmup, out8, outf, torch)
+    print(json.dumps({"progress": "fp64 done", "ms_per_step": fp64["ms_per_step"]}), file=sys.stderr, flush=True)
+    fp32 = run_form(ctx, frame, A.PRR_FP32_BULK, args.steps, args.warmup, out8, outf, torch)
+    print(json.dumps({"progress": "fp32_bulk done", "ms_per_step": fp32["ms_per_step"]}), file=sys.stderr,
+          flush=True)
+    cpu = None if args.no_cpu else cpu_baseline(frame)
+    t_int = fp32["integrate_ms"] / 1e3
+    f32, f64 = fp32["flops_fp32"], fp32["flops_fp64"]
+    t_ideal = f32 / P32 + f64 / P64
+    line = {
+        "metric": METRIC,
+        "value": fp32["rays_per_s"],
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": fp32["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32-bulk/fp64",
+        "data": "synthetic (seeded checker sky 512x256, value-noise sphere texture 1024x512)",
+        "config": {"workload": "C1: Kerr a=0.95 BL, RKF45 rtol=atol=1e-8, 1920x1080, 2x2 SSAA + FXAA, "
+                               "camera r=20M i=80deg, sphere R=1M at r=8M", "rays_per_frame": frame.nrays},
+        "roofline": {"bound": "valu", "kernel": "k_integrate<Kerr,RKF45,FP32-bulk>",
+                     "achieved": (f32 + f64) / t_int / 1e12, "peak": (f32 + f64) / t_ideal / 1e12,
+                     "unit": "TFLOP/s", "frac": t_ideal / t_int, "traffic": None,
+                     "flops_fp32_per_launch": f32, "flops_fp64_per_launch": f64},
+        "cpu_baseline": cpu,
+        "fp64_plain": {"rays_per_s": fp64["rays_per_s"], "ms_per_step": fp64["ms_per_step"],
+                       "integrate_ms": fp64["integrate_ms"], "roofline_frac": fp64["roofline_frac"],
+                       "achieved_tflops": fp64["achieved_tflops"]},
+        "fp32_bulk": {k: fp32[k] for k in ("integrate_ms", "ssaa_ms", "fxaa_ms", "attempts", "accepted",
+                                            "attempts_fp64", "attempts_per_s", "achieved_tflops",
+                                            "roofline_frac", "ssaa_gbs", "fxaa_gbs")},
+        "speedup_fp32_bulk_vs_fp64": fp64["ms_per_step"] / fp32["ms_per_step"],
+        "adjoint_rays_per_s": None,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

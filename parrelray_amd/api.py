@@ -1,0 +1,281 @@
+"""Python host API over the C ABI (include/prr.h).
+
+Mirrors the reference's host objects: the `.params` store (Parameters,
+main/Parameters.{h,cpp}), the spacetime / tracer / camera set-up of
+createScene (prr.cpp:159-198) and one render call per batch of frames
+(renderframe, prr.cpp:237-341).  All computation runs in libprr.so's HIP
+kernels; numpy arrays carry host inputs and outputs, torch tensors (when
+given) stay on the GPU.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _abi as A
+from ._lib import check, lib
+
+
+def _p(a):
+    return None if a is None else C.c_void_p(a.ctypes.data)
+
+
+# ---------------------------------------------------------------- textures
+def checker_texture(w=512, h=256, cw=16, ch=8, seed=1):
+    out = np.empty((h, w, 3), np.uint8)
+    check(lib().prr_texture_checker(_p(out), w, h, cw, ch, seed))
+    return out
+
+
+def noise_texture(w=1024, h=512, octaves=4, seed=2):
+    out = np.empty((h, w, 3), np.uint8)
+    check(lib().prr_texture_value_noise(_p(out), w, h, octaves, seed))
+    return out
+
+
+# ---------------------------------------------------------------- .params
+class Params:
+    """A parsed .params store (readParams, main/io.h:21-59)."""
+
+    def __init__(self, path=None, sim=None):
+        h = C.c_void_p()
+        if sim is not None:
+            check(lib().prr_params_load_sim(C.byref(h), sim.encode()))
+        else:
+            check(lib().prr_params_read(C.byref(h), path.encode()))
+        self.h = h
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            lib().prr_params_free(self.h)
+            self.h = None
+
+    def read_into(self, path):
+        check(lib().prr_params_read_into(self.h, path.encode()))
+
+    def header_exists(self, header):
+        return bool(lib().prr_params_header_exists(self.h, header.encode()))
+
+    def par_exists(self, header, key):
+        return bool(lib().prr_params_par_exists(self.h, header.encode(), key.encode()))
+
+    def par(self, header, key):
+        n = lib().prr_params_par(self.h, header.encode(), key.encode(), None, 0)
+        buf = C.create_string_buffer(n + 1)
+        lib().prr_params_par(self.h, header.encode(), key.encode(), buf, n + 1)
+        return buf.value.decode()
+
+    def pari(self, header, key):
+        return lib().prr_params_pari(self.h, header.encode(), key.encode())
+
+    def parf(self, header, key, default=0.0):
+        return lib().prr_params_parf(self.h, header.encode(), key.encode(), default)
+
+    def frame_count(self):
+        return lib().prr_params_frame_count(self.h)
+
+    def spacetime(self):
+        m = C.c_int32()
+        M, a, q = C.c_double(), C.c_double(), C.c_double()
+        pv = (C.c_double * 16)()
+        check(lib().prr_params_spacetime(self.h, C.byref(m), C.byref(M), C.byref(a), C.byref(q), pv))
+        return dict(metric=m.value, M=M.value, a=a.value, q=q.value, p=list(pv))
+
+    def tracer(self):
+        t = A.Tracer()
+        check(lib().prr_params_tracer(self.h, C.byref(t)))
+        return t
+
+    def frame(self, i):
+        f = A.Frame()
+        check(lib().prr_params_frame(self.h, i, C.byref(f)))
+        return f
+
+
+# ---------------------------------------------------------------- context
+class Context:
+    """One prr_ctx: device buffers, spacetime, tracer, textures and launch shape."""
+
+    def __init__(self, device=0):
+        h = C.c_void_p()
+        check(lib().prr_create(C.byref(h), device))
+        self.h = h
+        self.metric = A.PRR_SCHW_KS_CART
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().prr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def _check(self, st):
+        check(st, self.h)
+
+    def set_spacetime(self, metric, M=1.0, a=0.0, q=0.0, p=None):
+        pv = (C.c_double * 16)(*(p or [0.0] * 16))
+        self._check(lib().prr_set_spacetime(self.h, metric, M, a, q, pv))
+        self.metric = metric
+
+    def set_tracer(self, tracer):
+        self._check(lib().prr_set_tracer(self.h, C.byref(tracer)))
+
+    def set_texture(self, slot, rgb8, filt=A.PRR_FILTER_SQAVG):
+        if rgb8 is None:
+            self._check(lib().prr_set_texture(self.h, slot, None, 0, 0, filt))
+            return
+        rgb8 = np.ascontiguousarray(rgb8, dtype=np.uint8)
+        h, w, _ = rgb8.shape
+        self._check(lib().prr_set_texture(self.h, slot, _p(rgb8), w, h, filt))
+
+    def set_spheres(self, spheres):
+        arr = (A.Sphere * max(1, len(spheres)))()
+        for i, (c, r, tex) in enumerate(spheres):
+            arr[i].c[:] = list(c)
+            arr[i].radius = r
+            arr[i].tex = tex
+        self._check(lib().prr_set_spheres(self.h, arr, len(spheres)))
+
+    def set_launch(self, block=256, blocks_per_cu=0, refill=8, checkpoint=64):
+        l = A.Launch(block, blocks_per_cu, refill, checkpoint)
+        self._check(lib().prr_set_launch(self.h, C.byref(l)))
+
+    def get_launch(self):
+        l = A.Launch()
+        self._check(lib().prr_get_launch(self.h, C.byref(l)))
+        return dict(block=l.block, blocks_per_cu=l.blocks_per_cu, refill=l.refill, checkpoint=l.checkpoint)
+
+    def stats(self):
+        s = A.Stats()
+        self._check(lib().prr_get_stats(self.h, C.byref(s)))
+        return s.as_dict()
+
+    def render(self, frames, rays=False, rgbf=True):
+        """Render frames into host arrays.
+
+        Returns a list of dicts per frame with 'rgb8' (HxWx3 uint8), 'rgbf'
+        (HxWx3 float32) and, with rays=True, per-ray 'cls', 'endpoint',
+        'steps', 'attempts' and 'color'.
+        """
+        if isinstance(frames, A.Frame):
+            frames = [frames]
+        n = len(frames)
+        farr = (A.Frame * n)(*frames)
+        outs, imgs = [], (A.Image * n)()
+        for i, f in enumerate(frames):
+            o = dict(rgb8=np.empty((f.height, f.width, 3), np.uint8))
+            if rgbf:
+                o["rgbf"] = np.empty((f.height, f.width, 3), np.float32)
+            imgs[i].rgb8 = o["rgb8"].ctypes.data
+            imgs[i].rgbf = o["rgbf"].ctypes.data if rgbf else None
+            imgs[i].on_device = 0
+            outs.append(o)
+        ro = None
+        if rays:
+            total = sum(f.nrays for f in frames)
+            rr = dict(cls=np.empty(total, np.int8), endpoint=np.empty((total, 3)), steps=np.empty(total, np.int32),
+                      attempts=np.empty(total, np.int32), color=np.empty((total, 3), np.float32))
+            ro = A.RayOut(rr["cls"].ctypes.data, rr["endpoint"].ctypes.data, rr["steps"].ctypes.data,
+                          rr["attempts"].ctypes.data, rr["color"].ctypes.data, 0, 0)
+        self._check(lib().prr_render(self.h, farr, n, imgs, C.byref(ro) if ro is not None else None, None))
+        if rays:
+            off = 0
+            for o, f in zip(outs, frames):
+                k = f.nrays
+                for key, arr in rr.items():
+                    o[key] = arr[off:off + k]
+                off += k
+        return outs
+
+    def render_device(self, frames, rgb8=None, rgbf=None, stream=None):
+        """Render into device buffers (lists of torch tensors, one per frame, or None)."""
+        if isinstance(frames, A.Frame):
+            frames = [frames]
+        n = len(frames)
+        farr = (A.Frame * n)(*frames)
+        imgs = (A.Image * n)()
+        for i in range(n):
+            imgs[i].rgb8 = rgb8[i].data_ptr() if rgb8 is not None else None
+            imgs[i].rgbf = rgbf[i].data_ptr() if rgbf is not None else None
+            imgs[i].on_device = 1
+        self._check(lib().prr_render(self.h, farr, n, imgs, None, C.c_void_p(stream) if stream else None))
+
+    def trace(self, x0u0):
+        """Integrate explicit initial states (n x 8): returns xend, cls, steps, attempts."""
+        x = np.ascontiguousarray(x0u0, dtype=np.float64).reshape(-1, 8)
+        n = x.shape[0]
+        xend = np.empty((n, 8))
+        cls = np.empty(n, np.int32)
+        steps = np.empty(n, np.int32)
+        att = np.empty(n, np.int32)
+        self._check(lib().prr_trace(self.h, _p(x), n, _p(xend), _p(cls), _p(steps), _p(att)))
+        return dict(xend=xend, cls=cls, steps=steps, attempts=att)
+
+
+class Graph:
+    """A captured multi-frame render (prr_graph_create / prr_graph_launch)."""
+
+    def __init__(self, ctx, frames, rgb8=None, rgbf=None):
+        n = len(frames)
+        self.ctx = ctx
+        self._frames = (A.Frame * n)(*frames)
+        imgs = (A.Image * n)()
+        for i in range(n):
+            imgs[i].rgb8 = rgb8[i].data_ptr() if rgb8 is not None else None
+            imgs[i].rgbf = rgbf[i].data_ptr() if rgbf is not None else None
+            imgs[i].on_device = 1
+        h = C.c_void_p()
+        ctx._check(lib().prr_graph_create(ctx.h, self._frames, n, imgs, C.byref(h)))
+        self.h = h
+
+    def launch(self, stream=None):
+        self.ctx._check(lib().prr_graph_launch(self.h, C.c_void_p(stream) if stream else None))
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().prr_graph_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+
+# ---------------------------------------------------------------- host evaluation of the device formulas
+def state_size(metric):
+    return lib().prr_debug_state_size(metric)
+
+
+def rhs(metric, y, M=1.0, a=0.0, E=0.0, L=0.0, fp32=False):
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    f = np.empty(state_size(metric))
+    check(lib().prr_debug_rhs(metric, M, a, E, L, _p(y), int(fp32), _p(f)))
+    return f
+
+
+def state_from_xu(metric, x, u, M=1.0, a=0.0):
+    x, u = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(u, np.float64)
+    y = np.empty(state_size(metric))
+    EL = np.empty(2)
+    check(lib().prr_debug_state_from_xu(metric, M, a, _p(x), _p(u), _p(y), _p(EL)))
+    return y, EL
+
+
+def xu_from_state(metric, y, M=1.0, a=0.0, E=0.0, L=0.0):
+    y = np.ascontiguousarray(y, np.float64)
+    x, u = np.empty(4), np.empty(4)
+    check(lib().prr_debug_xu_from_state(metric, M, a, E, L, _p(y), _p(x), _p(u)))
+    return x, u
+
+
+def init_ray(metric, pos, d, M=1.0, a=0.0):
+    pos, d = np.ascontiguousarray(pos, np.float64), np.ascontiguousarray(d, np.float64)
+    y = np.empty(state_size(metric))
+    EL = np.empty(2)
+    check(lib().prr_debug_init_ray(metric, M, a, _p(pos), _p(d), _p(y), _p(EL)))
+    return y, EL
+
+
+def camera_ray(frame, local):
+    d = np.empty(3)
+    check(lib().prr_debug_camera_ray(C.byref(frame), local, _p(d)))
+    return d

@@ -1,0 +1,742 @@
+// capi.cpp -- the C ABI of include/prr.h: context, device buffers, frame
+// set-up and the per-call launch sequence (counter reset -> persistent
+// integrator -> SSAA -> FXAA + RGB8), all on one HIP stream.
+//
+// Replaces the reference's host orchestration (prr.cpp:237-404): global
+// Tracer/Spacetime/Camera objects, managed ray buffers, seven synchronous
+// launches per frame and exit(99) on any error.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+#include "kernels.hpp"
+#include "prr_host.hpp"
+
+using namespace prr;
+
+namespace {
+
+thread_local std::string g_error;
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+};
+
+} // namespace
+
+namespace prr {
+void set_global_error(const std::string &msg) { g_error = msg; }
+} // namespace prr
+
+struct prr_ctx {
+    int device = 0;
+    int cus = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    int32_t metric = PRR_SCHW_KS_CART;
+    double M = 1.0, a = 0.0, q = 0.0, p[16] = {};
+    prr_tracer tracer{};
+    prr_launch launch{256, 0, 8, 64};
+    TexDev tex[PRR_MAX_TEXTURES] = {};
+    std::vector<prr_sphere> spheres;
+    // device buffers, grown on demand
+    DevBuf frames, counters, color, post, img_scratch, stage_rgbf, stage_rgb8, cls, endpoint, steps, attempts,
+        state8, x0u0;
+    hipEvent_t ev[5] = {};
+    prr_stats stats{};
+};
+
+struct prr_graph {
+    prr_ctx *ctx = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int64_t total_rays = 0, pixels = 0;
+    double ssaa_bytes = 0, fxaa_bytes = 0;
+};
+
+namespace {
+
+prr_status fail(prr_ctx *ctx, prr_status st, const char *fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof buf, fmt, ap);
+    va_end(ap);
+    if (ctx) ctx->err = buf;
+    g_error = buf;
+    return st;
+}
+
+#define HIPTRY(ctx, call)                                                                                       \
+    do {                                                                                                        \
+        hipError_t e_ = (call);                                                                                 \
+        if (e_ != hipSuccess) return fail((ctx), PRR_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
+    } while (0)
+
+prr_status ensure(prr_ctx *ctx, DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return PRR_OK;
+    if (b.p) {
+        hipError_t e = hipFree(b.p);
+        if (e != hipSuccess) return fail(ctx, PRR_EHIP, "hipFree failed: %s", hipGetErrorString(e));
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    const size_t alloc = std::max<size_t>(bytes, 256);
+    hipError_t e = hipMalloc(&b.p, alloc);
+    if (e != hipSuccess) return fail(ctx, PRR_ENOMEM, "hipMalloc(%zu) failed: %s", alloc, hipGetErrorString(e));
+    b.bytes = alloc;
+    return PRR_OK;
+}
+
+void release(DevBuf &b) {
+    if (b.p) (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+}
+
+double horizon_radius(int metric, double M, double a) {
+    if (metric == PRR_FLAT) return 0.0;
+    if (metric == PRR_KERR_BL) return M + std::sqrt(std::max(M * M - a * a, 0.0));
+    return 2.0 * M;
+}
+
+prr_status validate_tracer(prr_ctx *ctx, const prr_tracer &t) {
+    if (t.solver != PRR_EULER && t.solver != PRR_RKF45)
+        return fail(ctx, PRR_EINVAL, "unknown solver %d", t.solver);
+    if (t.precision != PRR_FP64 && t.precision != PRR_FP32_BULK)
+        return fail(ctx, PRR_EINVAL, "unknown precision %d", t.precision);
+    if (!(t.h > 0.0) || !std::isfinite(t.h)) return fail(ctx, PRR_EINVAL, "step h must be positive");
+    if (t.solver == PRR_RKF45) {
+        if (!(t.hmin > 0.0) || !(t.hmax >= t.hmin) || !std::isfinite(t.hmax))
+            return fail(ctx, PRR_EINVAL, "RKF45 needs 0 < hmin <= hmax < inf");
+        if (!(t.atol >= 0.0) || !(t.rtol >= 0.0) || !(t.atol + t.rtol > 0.0))
+            return fail(ctx, PRR_EINVAL, "RKF45 needs atol, rtol >= 0, not both 0");
+        // every ray ends within max_attempts loop trips: the persistent kernel's exit bound
+        if (t.max_attempts < 0 || t.max_attempts > 100000000LL)
+            return fail(ctx, PRR_EINVAL, "max_attempts must be in [0, 1e8]");
+    } else {
+        // Euler: steps are bounded by max_steps and lambda_max / h
+        const double by_lambda = t.lambda_max / t.h;
+        if (std::min<double>(double(t.max_steps), by_lambda) > 1e8)
+            return fail(ctx, PRR_EINVAL, "Euler needs min(max_steps, lambda_max/h) <= 1e8");
+    }
+    if (t.max_steps < 0) return fail(ctx, PRR_EINVAL, "max_steps must be >= 0");
+    if (!(t.r_escape > 0.0)) return fail(ctx, PRR_EINVAL, "r_escape must be positive");
+    return PRR_OK;
+}
+
+TracerDev tracer_dev(const prr_tracer &t) {
+    TracerDev d{};
+    d.solver = t.solver;
+    d.precision = t.precision;
+    d.h = t.h;
+    d.hmin = t.hmin;
+    d.hmax = t.hmax;
+    d.atol = t.atol;
+    d.rtol = t.rtol;
+    d.r_escape = t.r_escape;
+    d.lambda_max = t.lambda_max;
+    d.r_fp64 = t.r_fp64;
+    d.max_steps = t.max_steps;
+    d.max_attempts = t.max_attempts;
+    return d;
+}
+
+prr_status validate_frame(prr_ctx *ctx, const prr_frame &f, int idx) {
+    if (f.width <= 0 || f.height <= 0 || f.width > 65536 || f.height > 65536)
+        return fail(ctx, PRR_EINVAL, "frame %d: bad image size %dx%d", idx, f.width, f.height);
+    if (f.ssaa == PRR_SSAA_NONE) {
+        if (f.samples != 1)
+            return fail(ctx, PRR_EINVAL, "frame %d: SSAA none needs samples = 1 (got %d)", idx, f.samples);
+    } else if (f.ssaa == PRR_SSAA_REGULAR) {
+        const int n = int(std::lround(std::sqrt(double(f.samples))));
+        if (f.samples < 1 || n * n != f.samples || n > 16)
+            return fail(ctx, PRR_EINVAL, "frame %d: regular SSAA needs samples = n*n, n <= 16 (got %d)", idx,
+                        f.samples);
+    } else {
+        return fail(ctx, PRR_EINVAL, "frame %d: unsupported SSAA distribution %d", idx, f.ssaa);
+    }
+    if (f.projection != PRR_PROJ_PINHOLE && f.projection != PRR_PROJ_SPHERICAL)
+        return fail(ctx, PRR_EINVAL, "frame %d: unknown projection %d", idx, f.projection);
+    if (!(f.vfov > 0.0 && f.vfov < M_PI)) return fail(ctx, PRR_EINVAL, "frame %d: vfov must be in (0, pi)", idx);
+    if (f.n_spheres < 0 || f.n_spheres > PRR_MAX_SPHERES)
+        return fail(ctx, PRR_EINVAL, "frame %d: n_spheres must be in [0, %d]", idx, PRR_MAX_SPHERES);
+    for (int k = 0; k < f.n_spheres; ++k)
+        if (!(f.spheres[k].radius > 0.0)) return fail(ctx, PRR_EINVAL, "frame %d: sphere %d radius", idx, k);
+    if (!(f.mass >= 0.0)) return fail(ctx, PRR_EINVAL, "frame %d: mass must be >= 0", idx);
+    if (ctx->metric == PRR_KERR_BL && !(std::fabs(f.spin) <= f.mass))
+        return fail(ctx, PRR_EINVAL, "frame %d: Kerr needs |a| <= M", idx);
+    const double r = std::sqrt(f.pos[0] * f.pos[0] + f.pos[1] * f.pos[1] + f.pos[2] * f.pos[2]);
+    if ((ctx->metric == PRR_SCHW_SPH || ctx->metric == PRR_KERR_BL) &&
+        !(f.pos[0] * f.pos[0] + f.pos[1] * f.pos[1] > 0.0))
+        return fail(ctx, PRR_EINVAL, "frame %d: camera on the polar axis of a spherical chart", idx);
+    if (!(r < ctx->tracer.r_escape)) return fail(ctx, PRR_EINVAL, "frame %d: camera outside r_escape", idx);
+    return PRR_OK;
+}
+
+FrameDev frame_dev(const prr_ctx *ctx, const prr_frame &f, int64_t begin) {
+    FrameDev d{};
+    for (int i = 0; i < 3; ++i) d.pos[i] = f.pos[i];
+    double F[3], U[3], R[3];
+    camera_basis(f.alpha, f.beta, F, U, R);
+    for (int i = 0; i < 3; ++i) {
+        d.F[i] = F[i];
+        d.U[i] = U[i];
+        d.R[i] = R[i];
+    }
+    d.alpha = f.alpha;
+    d.beta = f.beta;
+    d.vfov = f.vfov;
+    d.tan_half = std::tan(f.vfov / 2.0);
+    d.aspect = double(f.width) / double(f.height);
+    d.W = f.width;
+    d.H = f.height;
+    d.s = f.samples;
+    d.n = (f.ssaa == PRR_SSAA_NONE || f.samples == 1) ? 0 : int(std::lround(std::sqrt(double(f.samples))));
+    d.projection = f.projection;
+    d.n_spheres = f.n_spheres;
+    d.M = f.mass;
+    d.a = ctx->metric == PRR_KERR_BL ? f.spin : 0.0;
+    const double rh = horizon_radius(ctx->metric, f.mass, d.a);
+    d.r_hor = ctx->metric == PRR_FLAT ? -1.0
+                                      : rh * (1.0 + ctx->tracer.horizon_eps_rel) + ctx->tracer.horizon_eps_abs;
+    d.ray_begin = begin;
+    for (int k = 0; k < f.n_spheres; ++k) {
+        for (int c = 0; c < 3; ++c) d.sph[k][c] = f.spheres[k].c[c];
+        d.sph[k][3] = f.spheres[k].radius;
+        d.sph_tex[k] = f.spheres[k].tex;
+    }
+    return d;
+}
+
+int blocks_per_cu(const prr_ctx *ctx) {
+    if (ctx->launch.blocks_per_cu > 0) return ctx->launch.blocks_per_cu;
+    return std::max(1, 2048 / std::max(64, ctx->launch.block) / 4); // 2 waves per SIMD
+}
+
+// Algorithmic flops of one attempt.  Counting: FMA = 2; add, sub, mul, abs,
+// min, max, compare = 1; div, sqrt and each transcendental (sin and cos
+// separately) = 1.  F_rhs is counted from the formulas in geodesic.hpp.
+// Per state component an RKF45 attempt adds: 6 scalings k = h f, 15 stage
+// FMAs (30), the 4th-order update (4 FMAs + 1 add = 9), the error estimate
+// (5 nonzero b5 - b4 terms = 9) and the scaled error norm (8); plus 6 for
+// the step controller.  The stage part (6 F_rhs + 36 per component) runs in
+// FP32 in the FP32-bulk form, the rest always in FP64.  Euler: F_rhs + 2 per
+// component.
+struct FlopModel {
+    double rhs;
+    int n;
+};
+FlopModel flop_model(int metric) {
+    switch (metric) {
+    case PRR_FLAT: return {0.0, 8};
+    case PRR_SCHW_KS_CART: return {49.0, 8};
+    case PRR_SCHW_SPH: return {44.0, 8};
+    default: return {73.0, 5};
+    }
+}
+
+void fill_flops(prr_ctx *ctx, const CountersDev &c) {
+    const FlopModel fm = flop_model(ctx->metric);
+    const bool rkf = ctx->tracer.solver == PRR_RKF45;
+    const double f_stage = rkf ? 6.0 * fm.rhs + 36.0 * fm.n : fm.rhs;
+    const double f_rest = rkf ? 26.0 * fm.n + 6.0 : 2.0 * fm.n;
+    const double a64 = double(c.attempts_fp64), a32 = double(c.attempts) - double(c.attempts_fp64);
+    ctx->stats.flops_fp64 = a64 * (f_stage + f_rest) + a32 * f_rest;
+    ctx->stats.flops_fp32 = a32 * f_stage;
+}
+
+} // namespace
+
+// ======================================================================== API
+extern "C" {
+
+int prr_abi_version(void) { return PRR_ABI_VERSION; }
+const char *prr_global_error(void) { return g_error.c_str(); }
+const char *prr_last_error(const prr_ctx *ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
+
+prr_status prr_create(prr_ctx **out, int hip_device) {
+    if (!out) return fail(nullptr, PRR_EINVAL, "prr_create: out is NULL");
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return fail(nullptr, PRR_ENODEV, "no HIP device");
+    if (hip_device < 0 || hip_device >= n) return fail(nullptr, PRR_EINVAL, "device %d out of range", hip_device);
+    hipError_t e = hipSetDevice(hip_device);
+    if (e != hipSuccess) return fail(nullptr, PRR_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
+    hipDeviceProp_t prop;
+    e = hipGetDeviceProperties(&prop, hip_device);
+    if (e != hipSuccess) return fail(nullptr, PRR_EHIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));
+    prr_ctx *ctx = new prr_ctx();
+    ctx->device = hip_device;
+    ctx->cus = prop.multiProcessorCount;
+    ctx->tracer = prr_tracer{PRR_RKF45, PRR_FP64, 0.01, 1e-4, 0.1, 1e-3, 0.0, 70.0, 0.0, 1e-3, 1000.0,
+                             1000000000LL, 1000000LL, 3.0};
+    e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        delete ctx;
+        return fail(nullptr, PRR_EHIP, "hipStreamCreate: %s", hipGetErrorString(e));
+    }
+    for (auto &ev : ctx->ev) {
+        e = hipEventCreate(&ev);
+        if (e != hipSuccess) {
+            prr_destroy(ctx);
+            return fail(nullptr, PRR_EHIP, "hipEventCreate: %s", hipGetErrorString(e));
+        }
+    }
+    *out = ctx;
+    return PRR_OK;
+}
+
+void prr_destroy(prr_ctx *ctx) {
+    if (!ctx) return;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    for (auto &t : ctx->tex)
+        if (t.texels) (void)hipFree(const_cast<float4 *>(t.texels));
+    DevBuf *bufs[] = {&ctx->frames, &ctx->counters, &ctx->color, &ctx->post, &ctx->img_scratch,
+                      &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
+                      &ctx->attempts, &ctx->state8, &ctx->x0u0};
+    for (DevBuf *b : bufs) release(*b);
+    for (auto &ev : ctx->ev)
+        if (ev) (void)hipEventDestroy(ev);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+}
+
+prr_status prr_set_spacetime(prr_ctx *ctx, int32_t metric, double M, double a, double q, const double p[16]) {
+    if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
+    if (metric < PRR_FLAT || metric > PRR_KERR_BL) return fail(ctx, PRR_EINVAL, "unknown metric %d", metric);
+    if (!(M >= 0.0)) return fail(ctx, PRR_EINVAL, "mass must be >= 0");
+    if (metric == PRR_KERR_BL && !(std::fabs(a) <= M)) return fail(ctx, PRR_EINVAL, "Kerr needs |a| <= M");
+    ctx->metric = metric;
+    ctx->M = M;
+    ctx->a = a;
+    ctx->q = q;
+    for (int i = 0; i < 16; ++i) ctx->p[i] = p ? p[i] : 0.0;
+    return PRR_OK;
+}
+
+prr_status prr_set_tracer(prr_ctx *ctx, const prr_tracer *t) {
+    if (!ctx || !t) return fail(ctx, PRR_EINVAL, "NULL argument");
+    prr_status st = validate_tracer(ctx, *t);
+    if (st != PRR_OK) return st;
+    ctx->tracer = *t;
+    return PRR_OK;
+}
+
+prr_status prr_set_launch(prr_ctx *ctx, const prr_launch *l) {
+    if (!ctx || !l) return fail(ctx, PRR_EINVAL, "NULL argument");
+    if (l->block < 64 || l->block > 256 || l->block % 64)
+        return fail(ctx, PRR_EINVAL, "block must be 64, 128, 192 or 256");
+    if (l->blocks_per_cu < 0 || l->blocks_per_cu > 32) return fail(ctx, PRR_EINVAL, "blocks_per_cu in [0, 32]");
+    if (l->refill < 1 || l->refill > 64) return fail(ctx, PRR_EINVAL, "refill in [1, 64]");
+    if (l->checkpoint < 1) return fail(ctx, PRR_EINVAL, "checkpoint must be >= 1");
+    ctx->launch = *l;
+    return PRR_OK;
+}
+
+prr_status prr_get_launch(const prr_ctx *ctx, prr_launch *l) {
+    if (!ctx || !l) return fail(nullptr, PRR_EINVAL, "NULL argument");
+    *l = ctx->launch;
+    return PRR_OK;
+}
+
+prr_status prr_set_texture(prr_ctx *ctx, int slot, const uint8_t *rgb8, int w, int h, int32_t filter) {
+    if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
+    if (slot < 0 || slot >= PRR_MAX_TEXTURES) return fail(ctx, PRR_EINVAL, "texture slot %d out of range", slot);
+    if (filter != PRR_FILTER_SQAVG && filter != PRR_FILTER_BILINEAR)
+        return fail(ctx, PRR_EINVAL, "unknown filter %d", filter);
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    TexDev &t = ctx->tex[slot];
+    if (t.texels) {
+        HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+        HIPTRY(ctx, hipFree(const_cast<float4 *>(t.texels)));
+        t = TexDev{};
+    }
+    if (!rgb8 || w <= 0 || h <= 0) return PRR_OK; // slot cleared
+    if (w < 2 || h < 2) return fail(ctx, PRR_EINVAL, "texture must be at least 2x2");
+    std::vector<float4> host(size_t(w) * h);
+    for (size_t i = 0; i < host.size(); ++i)
+        host[i] = make_float4(float(rgb8[3 * i]) / 255.0f, float(rgb8[3 * i + 1]) / 255.0f,
+                              float(rgb8[3 * i + 2]) / 255.0f, 1.0f);
+    float4 *d = nullptr;
+    hipError_t e = hipMalloc(&d, host.size() * sizeof(float4));
+    if (e != hipSuccess) return fail(ctx, PRR_ENOMEM, "texture hipMalloc: %s", hipGetErrorString(e));
+    e = hipMemcpy(d, host.data(), host.size() * sizeof(float4), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        (void)hipFree(d);
+        return fail(ctx, PRR_EHIP, "texture upload: %s", hipGetErrorString(e));
+    }
+    t.texels = d;
+    t.w = w;
+    t.h = h;
+    t.filter = filter;
+    return PRR_OK;
+}
+
+prr_status prr_set_spheres(prr_ctx *ctx, const prr_sphere *s, int n) {
+    if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
+    if (n < 0 || n > PRR_MAX_SPHERES || (n > 0 && !s)) return fail(ctx, PRR_EINVAL, "bad sphere list");
+    for (int k = 0; k < n; ++k)
+        if (!(s[k].radius > 0.0)) return fail(ctx, PRR_EINVAL, "sphere %d radius must be positive", k);
+    ctx->spheres.assign(s, s + n);
+    return PRR_OK;
+}
+
+prr_status prr_get_stats(const prr_ctx *ctx, prr_stats *out) {
+    if (!ctx || !out) return fail(nullptr, PRR_EINVAL, "NULL argument");
+    *out = ctx->stats;
+    return PRR_OK;
+}
+
+} // extern "C"
+
+namespace {
+
+struct RenderPlan {
+    std::vector<FrameDev> fd;
+    std::vector<PostFrameDev> pf;
+    int64_t total = 0, pixels = 0, max_pix = 0;
+    int max_w = 0, max_h = 0;
+    bool any_fxaa = false;
+    double ssaa_bytes = 0, fxaa_bytes = 0;
+};
+
+// Builds the device frame table and post-processing descriptors and makes
+// sure every buffer exists.  Images given on the device are written in
+// place; host images go through staging buffers.
+prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_image *images, RenderPlan &P,
+                       bool need_host_staging) {
+    P.fd.resize(n);
+    for (int i = 0; i < n; ++i) {
+        prr_status st = validate_frame(ctx, frames[i], i);
+        if (st != PRR_OK) return st;
+        P.fd[i] = frame_dev(ctx, frames[i], P.total);
+        const int64_t np = int64_t(frames[i].width) * frames[i].height;
+        P.total += np * frames[i].samples;
+        P.pixels += np;
+        P.max_pix = std::max(P.max_pix, np);
+        P.max_w = std::max(P.max_w, frames[i].width);
+        P.max_h = std::max(P.max_h, frames[i].height);
+        P.any_fxaa = P.any_fxaa || frames[i].fxaa;
+        P.ssaa_bytes += double(np) * frames[i].samples * 12.0 + double(np) * (frames[i].fxaa ? 12.0 : 15.0);
+        if (frames[i].fxaa) P.fxaa_bytes += double(np) * (12.0 + 15.0);
+    }
+    prr_status st;
+    if ((st = ensure(ctx, ctx->frames, sizeof(FrameDev) * n)) != PRR_OK) return st;
+    if ((st = ensure(ctx, ctx->counters, sizeof(CountersDev))) != PRR_OK) return st;
+    if ((st = ensure(ctx, ctx->color, sizeof(float) * 3 * size_t(P.total))) != PRR_OK) return st;
+    if ((st = ensure(ctx, ctx->post, sizeof(PostFrameDev) * n)) != PRR_OK) return st;
+    if (P.any_fxaa && (st = ensure(ctx, ctx->img_scratch, sizeof(float) * 3 * size_t(P.pixels))) != PRR_OK)
+        return st;
+    if (need_host_staging) {
+        if ((st = ensure(ctx, ctx->stage_rgbf, sizeof(float) * 3 * size_t(P.pixels))) != PRR_OK) return st;
+        if ((st = ensure(ctx, ctx->stage_rgb8, 3 * size_t(P.pixels))) != PRR_OK) return st;
+    }
+    P.pf.resize(n);
+    int64_t pix_off = 0;
+    for (int i = 0; i < n; ++i) {
+        PostFrameDev &q = P.pf[i];
+        const int64_t np = int64_t(frames[i].width) * frames[i].height;
+        q.ray_color = static_cast<float *>(ctx->color.p) + 3 * P.fd[i].ray_begin;
+        q.img = P.any_fxaa ? static_cast<float *>(ctx->img_scratch.p) + 3 * pix_off : nullptr;
+        q.W = frames[i].width;
+        q.H = frames[i].height;
+        q.s = frames[i].samples;
+        q.fxaa = frames[i].fxaa;
+        const prr_image *im = images ? &images[i] : nullptr;
+        if (im && im->on_device) {
+            q.final_rgbf = im->rgbf;
+            q.rgb8 = im->rgb8;
+        } else if (im) {
+            q.final_rgbf = im->rgbf ? static_cast<float *>(ctx->stage_rgbf.p) + 3 * pix_off : nullptr;
+            q.rgb8 = im->rgb8 ? static_cast<uint8_t *>(ctx->stage_rgb8.p) + 3 * pix_off : nullptr;
+        } else {
+            q.final_rgbf = nullptr;
+            q.rgb8 = nullptr;
+        }
+        pix_off += np;
+    }
+    return PRR_OK;
+}
+
+JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
+    JobDev job{};
+    job.frames = static_cast<const FrameDev *>(ctx->frames.p);
+    job.nframes = int(P.fd.size());
+    job.refill = ctx->launch.refill;
+    job.total = P.total;
+    job.explicit_x0u0 = nullptr;
+    job.tr = tracer_dev(ctx->tracer);
+    for (int i = 0; i < PRR_MAX_TEXTURES; ++i) job.tex[i] = ctx->tex[i];
+    job.out.color = static_cast<float *>(ctx->color.p);
+    job.ctr = static_cast<CountersDev *>(ctx->counters.p);
+    return job;
+}
+
+// Enqueue reset -> integrate -> SSAA -> FXAA with timing events.
+prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events) {
+    if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
+    HIPTRY(ctx, launch_reset(job.ctr, st));
+    if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
+    const int grid = ctx->cus * blocks_per_cu(ctx);
+    HIPTRY(ctx, launch_integrate(ctx->metric, job, grid, ctx->launch.block, st));
+    if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
+    HIPTRY(ctx, launch_ssaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_pix, st));
+    if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[3], st));
+    if (P.any_fxaa)
+        HIPTRY(ctx, launch_fxaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_w, P.max_h,
+                                st));
+    if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[4], st));
+    return PRR_OK;
+}
+
+prr_status read_stats(prr_ctx *ctx, const RenderPlan &P, hipStream_t st, bool events) {
+    CountersDev c{};
+    HIPTRY(ctx, hipMemcpyAsync(&c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipStreamSynchronize(st));
+    prr_stats &s = ctx->stats;
+    s = prr_stats{};
+    s.rays = int64_t(c.rays);
+    s.attempts = int64_t(c.attempts);
+    s.accepted = int64_t(c.accepted);
+    s.attempts_fp64 = int64_t(c.attempts_fp64);
+    s.pixels = P.pixels;
+    if (events) {
+        float ms = 0.f;
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
+        s.integrate_ms = ms;
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        s.ssaa_ms = ms;
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
+        s.fxaa_ms = ms;
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]));
+        s.total_ms = ms;
+    }
+    s.ssaa_bytes = P.ssaa_bytes;
+    s.fxaa_bytes = P.fxaa_bytes;
+    fill_flops(ctx, c);
+    if (c.rays != uint64_t(P.total))
+        return fail(ctx, PRR_EHIP, "integrator processed %llu rays, expected %lld", c.rays, (long long)P.total);
+    return PRR_OK;
+}
+
+} // namespace
+
+extern "C" {
+
+prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *images, prr_ray_out *rays,
+                      void *stream) {
+    if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
+    if (!frames || n <= 0) return fail(ctx, PRR_EINVAL, "no frames");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    bool host_images = false;
+    if (images)
+        for (int i = 0; i < n; ++i) host_images = host_images || !images[i].on_device;
+    RenderPlan P;
+    prr_status s = plan_render(ctx, frames, n, images, P, host_images);
+    if (s != PRR_OK) return s;
+    JobDev job = make_job(ctx, P);
+    // optional per-ray diagnostics
+    if (rays) {
+        const size_t T = size_t(P.total);
+        auto bind = [&](void *user, DevBuf &buf, size_t bytes, auto *&dst) -> prr_status {
+            using Ptr = std::remove_reference_t<decltype(dst)>;
+            if (!user) {
+                dst = nullptr;
+                return PRR_OK;
+            }
+            if (rays->on_device) {
+                dst = static_cast<Ptr>(user);
+                return PRR_OK;
+            }
+            prr_status e = ensure(ctx, buf, bytes);
+            dst = static_cast<Ptr>(buf.p);
+            return e;
+        };
+        if ((s = bind(rays->cls, ctx->cls, T, job.out.cls)) != PRR_OK) return s;
+        if ((s = bind(rays->endpoint, ctx->endpoint, 24 * T, job.out.endpoint)) != PRR_OK) return s;
+        if ((s = bind(rays->steps, ctx->steps, 4 * T, job.out.steps)) != PRR_OK) return s;
+        if ((s = bind(rays->attempts, ctx->attempts, 4 * T, job.out.attempts)) != PRR_OK) return s;
+    }
+    HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev) * n, hipMemcpyHostToDevice, st));
+    if ((s = enqueue_render(ctx, P, job, st, true)) != PRR_OK) return s;
+    if ((s = read_stats(ctx, P, st, true)) != PRR_OK) return s;
+    // copy results to host buffers
+    if (images && host_images) {
+        int64_t off = 0;
+        for (int i = 0; i < n; ++i) {
+            const int64_t np = int64_t(frames[i].width) * frames[i].height;
+            if (!images[i].on_device) {
+                if (images[i].rgbf)
+                    HIPTRY(ctx, hipMemcpyAsync(images[i].rgbf, static_cast<float *>(ctx->stage_rgbf.p) + 3 * off,
+                                               12 * np, hipMemcpyDeviceToHost, st));
+                if (images[i].rgb8)
+                    HIPTRY(ctx, hipMemcpyAsync(images[i].rgb8, static_cast<uint8_t *>(ctx->stage_rgb8.p) + 3 * off,
+                                               3 * np, hipMemcpyDeviceToHost, st));
+            }
+            off += np;
+        }
+    }
+    if (rays) {
+        const size_t T = size_t(P.total);
+        if (rays->color) {
+            HIPTRY(ctx, hipMemcpyAsync(rays->color, ctx->color.p, 12 * T,
+                                       rays->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+        }
+        if (!rays->on_device) {
+            if (rays->cls) HIPTRY(ctx, hipMemcpyAsync(rays->cls, ctx->cls.p, T, hipMemcpyDeviceToHost, st));
+            if (rays->endpoint)
+                HIPTRY(ctx, hipMemcpyAsync(rays->endpoint, ctx->endpoint.p, 24 * T, hipMemcpyDeviceToHost, st));
+            if (rays->steps) HIPTRY(ctx, hipMemcpyAsync(rays->steps, ctx->steps.p, 4 * T, hipMemcpyDeviceToHost, st));
+            if (rays->attempts)
+                HIPTRY(ctx, hipMemcpyAsync(rays->attempts, ctx->attempts.p, 4 * T, hipMemcpyDeviceToHost, st));
+        }
+    }
+    HIPTRY(ctx, hipStreamSynchronize(st));
+    return PRR_OK;
+}
+
+prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int32_t *cls, int32_t *steps,
+                     int32_t *attempts) {
+    if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
+    if (!x0u0 || n <= 0) return fail(ctx, PRR_EINVAL, "no initial states");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    prr_frame f{};
+    f.width = f.height = 1;
+    f.samples = 1;
+    f.mass = ctx->M;
+    f.spin = ctx->a;
+    f.n_spheres = int(ctx->spheres.size());
+    for (int k = 0; k < f.n_spheres; ++k) f.spheres[k] = ctx->spheres[k];
+    RenderPlan P;
+    P.fd.push_back(frame_dev(ctx, f, 0));
+    P.total = n;
+    prr_status s;
+    if ((s = ensure(ctx, ctx->frames, sizeof(FrameDev))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->counters, sizeof(CountersDev))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->x0u0, 64 * size_t(n))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->state8, 64 * size_t(n))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->steps, 4 * size_t(n))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->attempts, 4 * size_t(n))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->cls, size_t(n))) != PRR_OK) return s;
+    JobDev job = make_job(ctx, P);
+    job.explicit_x0u0 = static_cast<const double *>(ctx->x0u0.p);
+    job.out = OutDev{};
+    job.out.state8 = static_cast<double *>(ctx->state8.p);
+    job.out.steps = static_cast<int32_t *>(ctx->steps.p);
+    job.out.attempts = static_cast<int32_t *>(ctx->attempts.p);
+    job.out.cls = static_cast<int8_t *>(ctx->cls.p);
+    HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipMemcpyAsync(ctx->x0u0.p, x0u0, 64 * size_t(n), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
+    HIPTRY(ctx, launch_reset(job.ctr, st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
+    const int grid = std::min<int64_t>(ctx->cus * blocks_per_cu(ctx), (int64_t(n) + 63) / 64 + 1);
+    HIPTRY(ctx, launch_integrate(ctx->metric, job, grid, ctx->launch.block, st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[3], st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[4], st));
+    std::vector<int8_t> c8(n);
+    if (xend) HIPTRY(ctx, hipMemcpyAsync(xend, ctx->state8.p, 64 * size_t(n), hipMemcpyDeviceToHost, st));
+    if (steps) HIPTRY(ctx, hipMemcpyAsync(steps, ctx->steps.p, 4 * size_t(n), hipMemcpyDeviceToHost, st));
+    if (attempts) HIPTRY(ctx, hipMemcpyAsync(attempts, ctx->attempts.p, 4 * size_t(n), hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipMemcpyAsync(c8.data(), ctx->cls.p, size_t(n), hipMemcpyDeviceToHost, st));
+    if ((s = read_stats(ctx, P, st, true)) != PRR_OK) return s;
+    if (cls)
+        for (int i = 0; i < n; ++i) cls[i] = c8[i];
+    return PRR_OK;
+}
+
+// ---------------------------------------------------------------- graphs
+prr_status prr_graph_create(prr_ctx *ctx, const prr_frame *frames, int n, const prr_image *images, prr_graph **out) {
+    if (!ctx || !out) return fail(ctx, PRR_EINVAL, "NULL argument");
+    *out = nullptr;
+    if (!frames || n <= 0) return fail(ctx, PRR_EINVAL, "no frames");
+    if (images)
+        for (int i = 0; i < n; ++i)
+            if (!images[i].on_device) return fail(ctx, PRR_EINVAL, "graph images must be device buffers");
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    RenderPlan P;
+    prr_status s = plan_render(ctx, frames, n, images, P, false);
+    if (s != PRR_OK) return s;
+    JobDev job = make_job(ctx, P);
+    HIPTRY(ctx, hipMemcpy(ctx->frames.p, P.fd.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice));
+    HIPTRY(ctx, hipMemcpy(ctx->post.p, P.pf.data(), sizeof(PostFrameDev) * n, hipMemcpyHostToDevice));
+    prr_graph *g = new prr_graph();
+    g->ctx = ctx;
+    g->total_rays = P.total;
+    g->pixels = P.pixels;
+    g->ssaa_bytes = P.ssaa_bytes;
+    g->fxaa_bytes = P.fxaa_bytes;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) {
+        delete g;
+        return fail(ctx, PRR_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    s = enqueue_render(ctx, P, job, ctx->stream, false);
+    hipGraph_t graph = nullptr;
+    e = hipStreamEndCapture(ctx->stream, &graph);
+    if (s != PRR_OK || e != hipSuccess) {
+        if (graph) (void)hipGraphDestroy(graph);
+        delete g;
+        return s != PRR_OK ? s : fail(ctx, PRR_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
+    }
+    g->graph = graph;
+    e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        prr_graph_destroy(g);
+        return fail(ctx, PRR_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    }
+    *out = g;
+    return PRR_OK;
+}
+
+prr_status prr_graph_launch(prr_graph *g, void *stream) {
+    if (!g) return fail(nullptr, PRR_EINVAL, "NULL graph");
+    prr_ctx *ctx = g->ctx;
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
+    HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
+    HIPTRY(ctx, hipGraphLaunch(g->exec, st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[4], st));
+    CountersDev c{};
+    HIPTRY(ctx, hipMemcpyAsync(&c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipStreamSynchronize(st));
+    prr_stats &s = ctx->stats;
+    s = prr_stats{};
+    float ms = 0.f;
+    HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[4]));
+    s.total_ms = ms;
+    s.rays = int64_t(c.rays);
+    s.attempts = int64_t(c.attempts);
+    s.accepted = int64_t(c.accepted);
+    s.attempts_fp64 = int64_t(c.attempts_fp64);
+    s.pixels = g->pixels;
+    s.ssaa_bytes = g->ssaa_bytes;
+    s.fxaa_bytes = g->fxaa_bytes;
+    fill_flops(ctx, c);
+    if (c.rays != uint64_t(g->total_rays))
+        return fail(ctx, PRR_EHIP, "graph processed %llu rays, expected %lld", c.rays, (long long)g->total_rays);
+    return PRR_OK;
+}
+
+void prr_graph_destroy(prr_graph *g) {
+    if (!g) return;
+    if (g->exec) (void)hipGraphExecDestroy(g->exec);
+    if (g->graph) (void)hipGraphDestroy(g->graph);
+    delete g;
+}
+
+} // extern "C"

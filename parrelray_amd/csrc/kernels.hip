@@ -1,0 +1,520 @@
+// kernels.hip -- gfx950 kernels: the persistent geodesic integrator and the
+// SSAA / FXAA / RGB8 passes.
+//
+// Integrator (replaces initRays + trace_Euler/trace_RKF + rayToRGB,
+// kernels/solver.cuh:26-220, metrics/schwarzschild.cuh:151-183, launched by
+// prr.cpp:279-323 as one thread per ray in 64-row bands):
+//   * one persistent grid sized to the CUs; every lane holds one ray's state
+//     in VGPRs and runs one Euler step / RKF45 attempt per loop trip;
+//   * when at least `refill` lanes of a wave are idle the wave claims that
+//     many ray ids from a device queue with a single atomic (ballot + mbcnt
+//     prefix sum) and initialises them in place, so lanes never wait for the
+//     slowest ray of their wave;
+//   * rays are generated from their id (no ray buffer), coloured on
+//     termination, and only the colour (plus optional diagnostics) is written.
+#include <hip/hip_runtime.h>
+
+#include "geodesic.hpp"
+#include "kernels.hpp"
+
+namespace prr {
+
+// Fehlberg 4(5) tableau (kernels/solver.cuh:125-131).
+__device__ constexpr double kA[6][5] = {
+    {0, 0, 0, 0, 0},
+    {1.0 / 4.0, 0, 0, 0, 0},
+    {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0},
+    {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0},
+    {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0},
+    {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0}};
+__device__ constexpr double kB4[6] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -1.0 / 5.0, 0.0};
+// b5 - b4: the embedded error estimate.
+__device__ constexpr double kE[6] = {16.0 / 135.0 - 25.0 / 216.0,       0.0,
+                                     6656.0 / 12825.0 - 1408.0 / 2565.0, 28561.0 / 56430.0 - 2197.0 / 4104.0,
+                                     -9.0 / 50.0 + 1.0 / 5.0,            2.0 / 55.0};
+
+// Six RK stages in precision T from the FP64 state y; returns the 4th-order
+// increment d4 and the error vector e in FP64.
+template <class Mt, class T>
+__device__ inline void rkf_stages(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
+    constexpr int N = Mt::N;
+    const Consts<T> c{T(cd.M), T(cd.a), T(cd.E), T(cd.L)};
+    const T hT = T(h);
+    T yT[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) yT[i] = T(y[i]);
+    T k[6][N];
+    Mt::template rhs<T>(yT, c, k[0]);
+#pragma unroll
+    for (int i = 0; i < N; ++i) k[0][i] *= hT;
+#pragma unroll
+    for (int s = 1; s < 6; ++s) {
+        T ys[N];
+#pragma unroll
+        for (int i = 0; i < N; ++i) {
+            T acc = yT[i];
+#pragma unroll
+            for (int j = 0; j < s; ++j) acc += T(kA[s][j]) * k[j][i];
+            ys[i] = acc;
+        }
+        Mt::template rhs<T>(ys, c, k[s]);
+#pragma unroll
+        for (int i = 0; i < N; ++i) k[s][i] *= hT;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        double a4 = 0.0, ae = 0.0;
+#pragma unroll
+        for (int s = 0; s < 6; ++s) {
+            if (kB4[s] != 0.0) a4 += kB4[s] * double(k[s][i]); // zero terms folded at compile time
+            if (kE[s] != 0.0) ae += kE[s] * double(k[s][i]);
+        }
+        d4[i] = a4;
+        e[i] = ae;
+    }
+}
+
+template <class Mt, class T>
+__device__ inline void euler_rhs(const double *y, const Consts<double> &cd, double *f) {
+    constexpr int N = Mt::N;
+    const Consts<T> c{T(cd.M), T(cd.a), T(cd.E), T(cd.L)};
+    T yT[N], fT[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) yT[i] = T(y[i]);
+    Mt::template rhs<T>(yT, c, fT);
+#pragma unroll
+    for (int i = 0; i < N; ++i) f[i] = double(fT[i]);
+}
+
+__device__ inline int find_frame(const JobDev &job, int64_t id) {
+    int lo = 0, hi = job.nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (job.frames[mid].ray_begin <= id) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Lane state of one ray.
+template <class Mt> struct Lane {
+    double y[Mt::N];
+    Consts<double> cs;
+    double h, lam;
+    int32_t steps, attempts, att64;
+    int32_t frame;
+    int64_t rid;
+    D3 P0; // Cartesian position at the start of the step (sphere chord)
+};
+
+template <class Mt>
+__device__ inline void lane_init(const JobDev &job, Lane<Mt> &L, int64_t id) {
+    L.rid = id;
+    L.frame = job.explicit_x0u0 ? 0 : find_frame(job, id);
+    const FrameDev &fr = job.frames[L.frame];
+    if (job.explicit_x0u0) {
+        const double *s = job.explicit_x0u0 + 8 * id;
+        state_from_xu<Mt>(s, s + 4, fr.M, fr.a, L.y, L.cs);
+    } else {
+        const D3 d = camera_ray(fr, id - fr.ray_begin);
+        init_state<Mt>(d3(fr.pos[0], fr.pos[1], fr.pos[2]), d, fr.M, fr.a, L.y, L.cs);
+    }
+    L.h = job.tr.h;
+    L.lam = 0.0;
+    L.steps = L.attempts = L.att64 = 0;
+    if (fr.n_spheres > 0) L.P0 = cart_of<Mt>(L.y);
+}
+
+template <class Mt>
+__device__ inline bool finite_state(const double *y) {
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < Mt::N; ++i) ok = ok && isfinite(y[i]);
+    return ok;
+}
+
+// Writes the outputs of a finished ray (rayToRGB, metrics/schwarzschild.cuh:151-183).
+template <class Mt>
+__device__ void lane_finish(const JobDev &job, const Lane<Mt> &L, int cls, D3 endpoint, float3 color) {
+    const int64_t id = L.rid;
+    if (cls == PRR_ESCAPE) {
+        const FrameDev &fr = job.frames[L.frame];
+        (void)fr;
+        const D3 n = escape_dir<Mt>(L.y, L.cs);
+        endpoint = n;
+        double th, ph;
+        if (Mt::kind == PRR_SCHW_SPH) {
+            const double two_pi = 6.28318530717958647692, pi = 3.14159265358979323846;
+            th = L.y[1];
+            ph = L.y[2];
+            if (ph == two_pi) ph = 0.0; // main/base.h:86-89
+            if (th == pi) th = 0.0;
+        } else {
+            th = acos(fmin(fmax(n.z, -1.0), 1.0));
+            ph = atan2(n.y, n.x);
+            if (ph < 0.0) ph += 6.28318530717958647692;
+        }
+        color = sky_lookup(job.tex[0], th, ph);
+    } else if (cls != PRR_SPHERE) {
+        endpoint = cart_of<Mt>(L.y);
+        color = make_float3(0.f, 0.f, 0.f);
+    }
+    const OutDev &o = job.out;
+    if (o.color) {
+        o.color[3 * id] = color.x;
+        o.color[3 * id + 1] = color.y;
+        o.color[3 * id + 2] = color.z;
+    }
+    if (o.cls) o.cls[id] = int8_t(cls);
+    if (o.endpoint) {
+        o.endpoint[3 * id] = endpoint.x;
+        o.endpoint[3 * id + 1] = endpoint.y;
+        o.endpoint[3 * id + 2] = endpoint.z;
+    }
+    if (o.steps) o.steps[id] = L.steps;
+    if (o.attempts) o.attempts[id] = L.attempts;
+    if (o.state8) {
+        double x[4], u[4];
+        xu_from_state<Mt>(L.y, L.cs, x, u);
+        for (int c = 0; c < 4; ++c) {
+            o.state8[8 * id + c] = x[c];
+            o.state8[8 * id + 4 + c] = u[c];
+        }
+    }
+}
+
+// Per-step termination tests after an accepted step from yo (User_UPDATE /
+// User_ENDLOOP, metrics/schwarzschild.cuh:55-79): sphere chord, spherical
+// wraps, horizon, escape.  Returns the class or -1 to continue.
+template <class Mt>
+__device__ inline int after_step(const JobDev &job, Lane<Mt> &L, D3 &endpoint, float3 &color) {
+    const FrameDev &fr = job.frames[L.frame];
+    if (fr.n_spheres > 0) {
+        const D3 P1 = cart_of<Mt>(L.y);
+        double t;
+        const int k = chord_hit(fr, L.P0, P1, t);
+        if (k >= 0) {
+            const D3 H = L.P0 + t * (P1 - L.P0);
+            const D3 n = (1.0 / fr.sph[k][3]) * (H - d3(fr.sph[k][0], fr.sph[k][1], fr.sph[k][2]));
+            endpoint = n;
+            const int slot = fr.sph_tex[k];
+            color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, n);
+            return PRR_SPHERE;
+        }
+        L.P0 = P1;
+    }
+    if (Mt::kind == PRR_SCHW_SPH) {
+        sph_post_step(L.y);
+        if (fr.n_spheres > 0) L.P0 = cart_of<Mt>(L.y);
+    }
+    const double r = radius_of<Mt>(L.y);
+    if (r <= fr.r_hor) return PRR_HORIZON;
+    if (r >= job.tr.r_escape) return PRR_ESCAPE;
+    return -1;
+}
+
+// One Euler step or one RKF45 attempt.  Returns the class when the ray ends.
+template <class Mt, int SOLVER, bool MIXED>
+__device__ inline int lane_advance(const JobDev &job, Lane<Mt> &L, D3 &endpoint, float3 &color) {
+    constexpr int N = Mt::N;
+    const TracerDev &tr = job.tr;
+    if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max) return PRR_STEP_CAP;
+    const bool fp64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
+    if (SOLVER == PRR_EULER) {
+        double f[N];
+        if (fp64) euler_rhs<Mt, double>(L.y, L.cs, f);
+        else euler_rhs<Mt, float>(L.y, L.cs, f);
+        bool ok = true;
+#pragma unroll
+        for (int i = 0; i < N; ++i) ok = ok && isfinite(f[i]);
+        if (!ok) return PRR_NAN;
+        const double h = L.h;
+#pragma unroll
+        for (int i = 0; i < N; ++i) L.y[i] += h * f[i];
+        L.steps++;
+        L.attempts++;
+        L.att64 += fp64 ? 1 : 0;
+        L.lam += h;
+        if (!finite_state<Mt>(L.y)) return PRR_NAN;
+        return after_step<Mt>(job, L, endpoint, color);
+    }
+    if (L.attempts >= tr.max_attempts) return PRR_STEP_CAP;
+    L.attempts++;
+    L.att64 += fp64 ? 1 : 0;
+    double d4[N], e[N];
+    if (fp64) rkf_stages<Mt, double>(L.y, L.h, L.cs, d4, e);
+    else rkf_stages<Mt, float>(L.y, L.h, L.cs, d4, e);
+    double errn = 0.0;
+    double y4[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) {
+        y4[i] = L.y[i] + d4[i];
+        const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(y4[i]));
+        errn = fmax(errn, fabs(e[i]) / sc);
+    }
+    // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
+    double hn;
+    if (MIXED) hn = 0.9 * L.h * double(powf(float(errn), -0.2f));
+    else hn = 0.9 * L.h * pow(errn, -0.2);
+    hn = fmin(fmax(hn, tr.hmin), tr.hmax);
+    if (errn > 1.0 && L.h != tr.hmin) { // reject (:185-188)
+        L.h = hn;
+        return -1;
+    }
+#pragma unroll
+    for (int i = 0; i < N; ++i) L.y[i] = y4[i];
+    if (!finite_state<Mt>(L.y)) return PRR_NAN;
+    L.steps++;
+    L.lam += L.h;
+    L.h = hn;
+    return after_step<Mt>(job, L, endpoint, color);
+}
+
+__device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
+    const int lo = __shfl(int(v & 0xffffffffull), src);
+    const int hi = __shfl(int(v >> 32), src);
+    return (unsigned long long)(unsigned(lo)) | ((unsigned long long)(unsigned(hi)) << 32);
+}
+
+template <class Mt, int SOLVER, bool MIXED>
+__global__ void __launch_bounds__(256) k_integrate(JobDev job) {
+    const int lane = int(threadIdx.x & 63);
+    Lane<Mt> L;
+    bool active = false, exhausted = false;
+    unsigned long long acc_att = 0, acc_ok = 0, acc_64 = 0, acc_rays = 0;
+    while (true) {
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= job.refill) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&job.ctr->queue, (unsigned long long)nidle);
+            base = shfl_u64(base, leader);
+            if (base + (unsigned long long)nidle >= (unsigned long long)job.total) exhausted = true;
+            if (!active) {
+                const unsigned rank =
+                    __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
+                const int64_t id = int64_t(base) + rank;
+                if (id < job.total) {
+                    lane_init<Mt>(job, L, id);
+                    active = true;
+                    if (!finite_state<Mt>(L.y)) {
+                        lane_finish<Mt>(job, L, PRR_NAN, d3(0, 0, 0), make_float3(0.f, 0.f, 0.f));
+                        acc_rays++;
+                        active = false;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            D3 endpoint = d3(0, 0, 0);
+            float3 color = make_float3(0.f, 0.f, 0.f);
+            const int cls = lane_advance<Mt, SOLVER, MIXED>(job, L, endpoint, color);
+            if (cls >= 0) {
+                lane_finish<Mt>(job, L, cls, endpoint, color);
+                acc_att += (unsigned long long)L.attempts;
+                acc_ok += (unsigned long long)L.steps;
+                acc_64 += (unsigned long long)L.att64;
+                acc_rays++;
+                active = false;
+            }
+        }
+    }
+    // one atomic per wave for the counters
+    for (int off = 32; off > 0; off >>= 1) {
+        acc_att += shfl_u64(acc_att, lane ^ off);
+        acc_ok += shfl_u64(acc_ok, lane ^ off);
+        acc_64 += shfl_u64(acc_64, lane ^ off);
+        acc_rays += shfl_u64(acc_rays, lane ^ off);
+    }
+    if (lane == 0) {
+        atomicAdd(&job.ctr->attempts, acc_att);
+        atomicAdd(&job.ctr->accepted, acc_ok);
+        atomicAdd(&job.ctr->attempts_fp64, acc_64);
+        atomicAdd(&job.ctr->rays, acc_rays);
+    }
+}
+
+// Counter reset by a kernel (no memset node, so the sequence can be captured
+// in a graph).
+__global__ void k_reset_counters(CountersDev *c) {
+    if (threadIdx.x == 0) {
+        c->queue = 0;
+        c->attempts = 0;
+        c->accepted = 0;
+        c->attempts_fp64 = 0;
+        c->rays = 0;
+    }
+}
+
+// ----------------------------------------------------------------- AA passes
+__device__ inline float clamp01(float c) { return fminf(fmaxf(c, 0.0f), 1.0f); }
+
+__device__ inline uint8_t to_rgb8(float c) {
+    float v = roundf(255.0f * c); // cudamaths/image.cu:31
+    v = fminf(fmaxf(v, 0.0f), 255.0f);
+    return uint8_t(v);
+}
+
+// combineSamples (kernels/ssaa.cuh:109-131): one thread per pixel, frames
+// along grid.y.  Writes RGB8 directly when no FXAA follows.
+__global__ void __launch_bounds__(256) k_ssaa(const PostFrameDev *pfs) {
+#pragma clang fp contract(off)
+    const PostFrameDev pf = pfs[blockIdx.y];
+    const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const int64_t npix = int64_t(pf.W) * pf.H;
+    if (p >= npix) return;
+    const int s = pf.s;
+    float out[3];
+    if (s == 1) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) out[ch] = clamp01(pf.ray_color[3 * p + ch]);
+    } else {
+        float acc[3] = {0.f, 0.f, 0.f};
+        const float *src = pf.ray_color + 3 * p * s;
+        for (int k = 0; k < s; ++k) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) {
+                const float c = src[3 * k + ch];
+                acc[ch] += clamp01(c * c);
+            }
+        }
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) out[ch] = sqrtf(acc[ch] / float(s));
+    }
+    float *dst = pf.fxaa ? pf.img : pf.final_rgbf;
+    if (dst) {
+        dst[3 * p] = out[0];
+        dst[3 * p + 1] = out[1];
+        dst[3 * p + 2] = out[2];
+    }
+    if (!pf.fxaa && pf.rgb8) {
+        pf.rgb8[3 * p] = to_rgb8(out[0]);
+        pf.rgb8[3 * p + 1] = to_rgb8(out[1]);
+        pf.rgb8[3 * p + 2] = to_rgb8(out[2]);
+    }
+}
+
+// luma (kernels/ssaa.cuh:141-143): double literals, float result.
+__device__ inline float luma(float r, float g, float b) {
+#pragma clang fp contract(off)
+    const double k = double(1.0f / 255.0f);
+    return float(double(r) * 0.299 * k + double(g) * 0.587 * k + double(b) * 0.114 * k);
+}
+
+__device__ inline float3 load3(const float *im, int W, int x, int y) {
+    const float *p = im + 3 * (int64_t(y) * W + x);
+    return make_float3(p[0], p[1], p[2]);
+}
+
+// Clamp-to-edge bilinear tap at pixel-space (X, Y).
+__device__ inline float3 fxaa_tap(const float *im, int W, int H, float X, float Y) {
+#pragma clang fp contract(off)
+    const float fx0 = floorf(X), fy0 = floorf(Y);
+    const float fx = X - fx0, fy = Y - fy0;
+    const int x0 = int(fx0), y0 = int(fy0);
+    const int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
+    const int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
+    const float3 c00 = load3(im, W, xa, ya), c10 = load3(im, W, xb, ya), c01 = load3(im, W, xa, yb),
+                 c11 = load3(im, W, xb, yb);
+    const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy), w01 = (1.0f - fx) * fy, w11 = fx * fy;
+    return make_float3(c00.x * w00 + c10.x * w10 + c01.x * w01 + c11.x * w11,
+                       c00.y * w00 + c10.y * w10 + c01.y * w01 + c11.y * w11,
+                       c00.z * w00 + c10.z * w10 + c01.z * w01 + c11.z * w11);
+}
+
+// fxaa (kernels/ssaa.cuh:150-182) on every pixel with clamp-to-edge taps,
+// fused with the RGB8 quantisation.  16x16 tiles; neighbour taps hit L1/L2.
+__global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
+#pragma clang fp contract(off)
+    const PostFrameDev pf = pfs[blockIdx.z];
+    const int x = int(blockIdx.x) * 16 + int(threadIdx.x & 15);
+    const int y = int(blockIdx.y) * 16 + int(threadIdx.x >> 4);
+    const int W = pf.W, H = pf.H;
+    if (!pf.fxaa || x >= W || y >= H) return;
+    const float *im = pf.img;
+    auto L = [&](int xx, int yy) {
+        xx = min(max(xx, 0), W - 1);
+        yy = min(max(yy, 0), H - 1);
+        const float3 c = load3(im, W, xx, yy);
+        return luma(c.x, c.y, c.z);
+    };
+    const float lNW = L(x - 1, y - 1), lNE = L(x + 1, y - 1), lSW = L(x - 1, y + 1), lSE = L(x + 1, y + 1);
+    const float lM = L(x, y);
+    const float lMin = fminf(lM, fminf(fminf(lNW, lNE), fminf(lSW, lSE)));
+    const float lMax = fmaxf(lM, fmaxf(fmaxf(lNW, lNE), fmaxf(lSW, lSE)));
+    float dx = -((lNW + lNE) - (lSW + lSE));
+    float dy = ((lNW + lSW) - (lNE + lSE));
+    const float reduce = fmaxf((lNW + lNE + lSW + lSE) * (0.25f * (1.0f / 8.0f)), 1.0f / 128.0f);
+    const float rcp = 1.0f / (fminf(fabsf(dx), fabsf(dy)) + reduce);
+    dx = fminf(fmaxf(dx * rcp, -8.0f), 8.0f);
+    dy = fminf(fmaxf(dy * rcp, -8.0f), 8.0f);
+    const float k1 = 1.0f / 3.0f - 0.5f, k2 = 2.0f / 3.0f - 0.5f;
+    const float fxp = float(x), fyp = float(y);
+    const float3 t1 = fxaa_tap(im, W, H, fxp + dx * k1, fyp + dy * k1);
+    const float3 t2 = fxaa_tap(im, W, H, fxp + dx * k2, fyp + dy * k2);
+    const float3 t3 = fxaa_tap(im, W, H, fxp + dx * -0.5f, fyp + dy * -0.5f);
+    const float3 t4 = fxaa_tap(im, W, H, fxp + dx * 0.5f, fyp + dy * 0.5f);
+    const float3 A = make_float3(t1.x * 0.5f + t2.x * 0.5f, t1.y * 0.5f + t2.y * 0.5f, t1.z * 0.5f + t2.z * 0.5f);
+    const float3 B = make_float3(0.5f * A.x + (t3.x * 0.25f + t4.x * 0.25f), 0.5f * A.y + (t3.y * 0.25f + t4.y * 0.25f),
+                                 0.5f * A.z + (t3.z * 0.25f + t4.z * 0.25f));
+    const float lB = luma(B.x, B.y, B.z);
+    const float3 o = (lB < lMin || lB > lMax) ? A : B;
+    const int64_t p = int64_t(y) * W + x;
+    if (pf.final_rgbf) {
+        pf.final_rgbf[3 * p] = o.x;
+        pf.final_rgbf[3 * p + 1] = o.y;
+        pf.final_rgbf[3 * p + 2] = o.z;
+    }
+    if (pf.rgb8) {
+        pf.rgb8[3 * p] = to_rgb8(o.x);
+        pf.rgb8[3 * p + 1] = to_rgb8(o.y);
+        pf.rgb8[3 * p + 2] = to_rgb8(o.z);
+    }
+}
+
+// ----------------------------------------------------------------- dispatch
+template <class Mt>
+static hipError_t launch_metric(int solver, int precision, const JobDev &job, dim3 grid, dim3 block,
+                                hipStream_t st) {
+    const bool mixed = precision == PRR_FP32_BULK;
+    if (solver == PRR_EULER) {
+        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true>), grid, block, 0, st, job);
+        else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false>), grid, block, 0, st, job);
+    } else {
+        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true>), grid, block, 0, st, job);
+        else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false>), grid, block, 0, st, job);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st) {
+    const dim3 g(grid), b(block);
+    switch (metric) {
+    case PRR_FLAT: return launch_metric<Flat>(job.tr.solver, job.tr.precision, job, g, b, st);
+    case PRR_SCHW_KS_CART: return launch_metric<SchwKS>(job.tr.solver, job.tr.precision, job, g, b, st);
+    case PRR_SCHW_SPH: return launch_metric<SchwSph>(job.tr.solver, job.tr.precision, job, g, b, st);
+    case PRR_KERR_BL: return launch_metric<Kerr>(job.tr.solver, job.tr.precision, job, g, b, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_reset(CountersDev *c, hipStream_t st) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c);
+    return hipGetLastError();
+}
+
+hipError_t launch_ssaa(const PostFrameDev *pfs, int nframes, int64_t max_pixels, hipStream_t st) {
+    const dim3 g(unsigned((max_pixels + 255) / 256), unsigned(nframes)), b(256);
+    hipLaunchKernelGGL(k_ssaa, g, b, 0, st, pfs);
+    return hipGetLastError();
+}
+
+hipError_t launch_fxaa(const PostFrameDev *pfs, int nframes, int max_w, int max_h, hipStream_t st) {
+    const dim3 g(unsigned((max_w + 15) / 16), unsigned((max_h + 15) / 16), unsigned(nframes)), b(256);
+    hipLaunchKernelGGL(k_fxaa, g, b, 0, st, pfs);
+    return hipGetLastError();
+}
+
+} // namespace prr

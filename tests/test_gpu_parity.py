@@ -1,0 +1,172 @@
+"""GPU parity: the HIP integrator and AA passes against the CPU oracle.
+
+All tests call libprr.so through the C ABI (parrelray_amd.api) on cuda:0 and
+compare with oracle/ on the same seeded inputs.
+"""
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+from parrelray_amd import _abi as A
+from parrelray_amd import api
+from parrelray_amd import configs as CF
+
+from parity import angle, compare_rays, gpu_context, oracle_scene, rgb8_agreement
+
+pytestmark = pytest.mark.gpu
+RNG = np.random.default_rng(11)
+
+
+def random_states(metric, n, a=0.0):
+    out = []
+    for _ in range(n):
+        pos = RNG.normal(size=3)
+        pos = pos / np.linalg.norm(pos) * RNG.uniform(8.0, 30.0)
+        d = -pos / np.linalg.norm(pos) + 0.6 * RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        x, u = O.init_ray(metric, 1.0, a, pos, d)
+        out.append(np.concatenate([x, u]))
+    return np.array(out)
+
+
+def trace_tracer(solver):
+    if solver == A.PRR_EULER:
+        return A.make_tracer(solver=solver, h=0.01, r_escape=60.0, horizon_eps_rel=1e-3, horizon_eps_abs=0.0,
+                             lambda_max=1e9, max_steps=3000, max_attempts=3000)
+    return A.make_tracer(solver=solver, h=0.05, hmin=1e-9, hmax=5.0, atol=1e-10, rtol=1e-10, r_escape=60.0,
+                         horizon_eps_rel=1e-3, horizon_eps_abs=0.0, lambda_max=1e9, max_steps=20000,
+                         max_attempts=200000)
+
+
+@pytest.mark.parametrize("metric", [A.PRR_FLAT, A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH])
+@pytest.mark.parametrize("solver", [A.PRR_EULER, A.PRR_RKF45])
+def test_trace_second_order_metrics_match_oracle(metric, solver):
+    """Same algorithm, same state: classes agree and final states agree tightly."""
+    T = trace_tracer(solver)
+    x0 = random_states(metric, 512)
+    ctx = api.Context(0)
+    ctx.set_spacetime(metric, 1.0, 0.0)
+    ctx.set_tracer(T)
+    g = ctx.trace(x0)
+    S = O.Scene(metric, 1.0, 0.0, T)
+    o = S.trace(x0)
+    agree = g["cls"] == o["cls"]
+    assert agree.mean() >= 0.99, np.nonzero(~agree)
+    err = np.abs(g["xend"][agree] - o["xend"][agree]).max(axis=1) / (np.abs(o["xend"][agree]).max(axis=1) + 1)
+    assert np.median(err) < 1e-12
+    assert (err < 1e-6).mean() >= 0.99, np.sort(err)[-10:]
+    steps_same = g["steps"][agree] == o["steps"][agree]
+    assert steps_same.mean() >= 0.97
+
+
+@pytest.mark.parametrize("a", [0.0, 0.6, 0.95])
+def test_trace_kerr_conserves_constants_and_matches_oracle(a):
+    T = trace_tracer(A.PRR_RKF45)
+    x0 = random_states(A.PRR_KERR_BL, 256, a)
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_KERR_BL, 1.0, a)
+    ctx.set_tracer(T)
+    g = ctx.trace(x0)
+    S = O.Scene(A.PRR_KERR_BL, 1.0, a, A.make_tracer(**{**{k: getattr(T, k) for k, _ in A.Tracer._fields_},
+                                                         "atol": 1e-12, "rtol": 1e-12}))
+    o = S.trace(x0)
+    agree = g["cls"] == o["cls"]
+    assert agree.mean() >= 0.98
+    esc = agree & (o["cls"] == A.PRR_ESCAPE)
+    # Carter constant drift of the GPU's final state
+    for i in np.nonzero(esc)[0][:50]:
+        c0 = O.kerr_constants(1.0, a, x0[i, :4], x0[i, 4:])
+        c1 = O.kerr_constants(1.0, a, g["xend"][i, :4], g["xend"][i, 4:])
+        assert abs(c1[0] - c0[0]) < 1e-12 * abs(c0[0])  # E is exact by construction
+        assert abs(c1[1] - c0[1]) < 1e-12 * (abs(c0[1]) + 1)
+        assert abs(c1[2] - c0[2]) < 1e-6 * (abs(c0[2]) + 1)
+
+
+def test_c0_parity_full_frame():
+    T = CF.tracer_c0()
+    f = CF.frame_c0()
+    ctx = gpu_context(A.PRR_SCHW_SPH, T)
+    out = ctx.render(f, rays=True)[0]
+    S = oracle_scene(A.PRR_SCHW_SPH, T)
+    orc = S.render_rays(f)
+    cmp = compare_rays(S, f, out, orc)
+    assert cmp["unexcused"] == 0, cmp["unexcused_ids"][:20]
+    # endpoints of rays stopped by the step cap: final positions
+    same = cmp["same"]
+    d = np.linalg.norm(out["endpoint"][same] - orc["endpoint"][same], axis=1)
+    r = np.linalg.norm(orc["endpoint"][same], axis=1)
+    assert np.quantile(d / r, 0.999) < 1e-6
+    _, img, rgb8 = S.render_frame(f)
+    frac, _ = rgb8_agreement(out["rgb8"], rgb8)
+    assert frac >= 0.999
+
+
+@pytest.mark.parametrize("precision", [A.PRR_FP64, A.PRR_FP32_BULK])
+def test_c1_reduced_parity(precision):
+    """C1 at 192x108, 2x2 SSAA + FXAA, against the FP64 tol-1e-12 oracle."""
+    f = CF.frame_c1(192, 108)
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=precision))
+    out = ctx.render(f, rays=True)[0]
+    S = oracle_scene(A.PRR_KERR_BL, CF.tracer_kerr(tol=1e-12))
+    col, img, rgb8 = S.render_frame(f)
+    orc = S.render_rays(f)
+    cmp = compare_rays(S, f, out, orc)
+    assert cmp["unexcused"] <= 2, cmp["unexcused_ids"][:20]
+    ang = cmp["angles"]
+    assert np.median(ang) < 1e-6
+    assert np.quantile(ang, 0.999) < 1e-4
+    frac, _ = rgb8_agreement(out["rgb8"], rgb8)
+    assert frac >= 0.999
+
+
+def test_aa_passes_bit_exact():
+    """SSAA combine, FXAA and RGB8 on the GPU's own ray colours equal the oracle's passes bit for bit."""
+    f = CF.frame_c1(160, 90)
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr())
+    out = ctx.render(f, rays=True)[0]
+    ss = O.ssaa(out["color"], f.width, f.height, f.samples)
+    fx = O.fxaa(ss)
+    assert np.array_equal(out["rgbf"], fx)
+    assert np.array_equal(out["rgb8"], O.rgb8(fx))
+    f.fxaa = 0
+    out = ctx.render(f, rays=True)[0]
+    ss = O.ssaa(out["color"], f.width, f.height, f.samples)
+    assert np.array_equal(out["rgbf"], ss)
+    assert np.array_equal(out["rgb8"], O.rgb8(ss))
+
+
+def test_every_ray_processed_once_and_batch_equals_single():
+    frames = CF.frames_c2(nframes=10, scale=0.08)
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    batch = ctx.render(frames, rays=True)
+    st = ctx.stats()
+    assert st["rays"] == sum(f.nrays for f in frames)
+    for f, b in zip(frames, batch):
+        single = ctx.render(f, rays=True)[0]
+        assert np.array_equal(single["cls"], b["cls"])
+        assert np.array_equal(single["rgb8"], b["rgb8"])
+        assert np.array_equal(single["rgbf"], b["rgbf"])
+
+
+@pytest.mark.parametrize("refill,block", [(1, 64), (32, 256), (64, 128)])
+def test_launch_shape_does_not_change_results(refill, block):
+    f = CF.frame_c1(96, 54)
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    ref = ctx.render(f, rays=True)[0]
+    ctx.set_launch(block=block, blocks_per_cu=1, refill=refill)
+    out = ctx.render(f, rays=True)[0]
+    assert np.array_equal(ref["rgbf"], out["rgbf"])
+    assert np.array_equal(ref["steps"], out["steps"])
+
+
+def test_fp32_bulk_close_to_fp64():
+    f = CF.frame_c1(192, 108, ssaa=1, fxaa=0)
+    c64 = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP64)).render(f, rays=True)[0]
+    c32 = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK)).render(f, rays=True)[0]
+    same = c64["cls"] == c32["cls"]
+    assert same.mean() > 0.995
+    m = same & ((c64["cls"] == A.PRR_ESCAPE) | (c64["cls"] == A.PRR_SPHERE))
+    ang = angle(c64["endpoint"][m], c32["endpoint"][m])
+    assert np.median(ang) < 1e-6

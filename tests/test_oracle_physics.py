@@ -1,0 +1,337 @@
+"""Pins the CPU oracle's physics against closed forms and invariants.
+
+Each check is chosen so that a plausible mistake (a dropped Christoffel term,
+a sign, a transposed index, the wrong RKF coupling) fails it; none of them
+re-types the oracle's own formula.
+"""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+from scipy import integrate
+
+import oracle as O
+from parrelray_amd import _abi as A
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+RNG = np.random.default_rng(20240531)
+
+
+def fd_christoffel(kind, M, a, x, eps=1e-5):
+    """Christoffel symbols from central differences of the oracle's metric."""
+    g = O.metric(kind, M, a, x)
+    gi = np.linalg.inv(g)
+    dg = np.zeros((4, 4, 4))
+    for k in range(4):
+        xp, xm = np.array(x, float), np.array(x, float)
+        step = eps * max(1.0, abs(x[k]))
+        xp[k] += step
+        xm[k] -= step
+        dg[k] = (O.metric(kind, M, a, xp) - O.metric(kind, M, a, xm)) / (2 * step)
+    G = np.zeros((4, 4, 4))
+    for al in range(4):
+        for mu in range(4):
+            for nu in range(4):
+                G[al, mu, nu] = 0.5 * sum(gi[al, be] * (dg[mu, be, nu] + dg[nu, be, mu] - dg[be, mu, nu])
+                                          for be in range(4))
+    return G
+
+
+def random_points(kind, n):
+    pts = []
+    for _ in range(n):
+        if kind in (A.PRR_SCHW_SPH, A.PRR_KERR_BL):
+            x = [RNG.uniform(3.0, 40.0), RNG.uniform(0.3, math.pi - 0.3), RNG.uniform(0, 2 * math.pi), 0.0]
+        else:
+            v = RNG.normal(size=3)
+            x = list(v / np.linalg.norm(v) * RNG.uniform(3.0, 40.0)) + [0.0]
+        u = list(RNG.normal(size=4))
+        pts.append((np.array(x), np.array(u)))
+    return pts
+
+
+@pytest.mark.parametrize("kind,a", [(A.PRR_SCHW_KS_CART, 0.0), (A.PRR_SCHW_SPH, 0.0), (A.PRR_KERR_BL, 0.0),
+                                    (A.PRR_KERR_BL, 0.6), (A.PRR_KERR_BL, 0.998)])
+def test_geod_equals_minus_christoffel_of_metric(kind, a):
+    for x, u in random_points(kind, 20):
+        G = fd_christoffel(kind, 1.0, a, x)
+        expect = -np.einsum("amn,m,n->a", G, u, u)
+        got = O.geod(kind, 1.0, a, x, u)
+        scale = np.abs(np.einsum("amn,m,n->amn", np.abs(G), np.abs(u), np.abs(u))).sum(axis=(1, 2)) + 1e-12
+        # FD noise is absolute in G; compare against each component's own scale
+        # plus a small fraction of the largest one.
+        assert np.all(np.abs(got - expect) <= 2e-7 * (scale + 1e-3 * scale.max())), (x, u, got, expect)
+
+
+def test_flat_geod_is_zero():
+    for x, u in random_points(A.PRR_FLAT, 5):
+        assert np.all(O.geod(A.PRR_FLAT, 1.0, 0.0, x, u) == 0.0)
+
+
+def test_kerr_metric_derivatives_match_finite_differences():
+    for a in (0.3, 0.95):
+        for x, _ in random_points(A.PRR_KERR_BL, 10):
+            dg = O.dmetric_kerr(1.0, a, x)
+            for k in (0, 1):
+                e = 1e-6
+                xp, xm = x.copy(), x.copy()
+                xp[k] += e
+                xm[k] -= e
+                fd = (O.metric(A.PRR_KERR_BL, 1.0, a, xp) - O.metric(A.PRR_KERR_BL, 1.0, a, xm)) / (2 * e)
+                assert np.allclose(dg[k], fd, rtol=1e-6, atol=1e-8)
+            assert np.all(dg[2] == 0) and np.all(dg[3] == 0)
+
+
+def test_kerr_reduces_to_schwarzschild_at_zero_spin():
+    for x, u in random_points(A.PRR_SCHW_SPH, 20):
+        assert np.allclose(O.geod(A.PRR_KERR_BL, 1.0, 0.0, x, u), O.geod(A.PRR_SCHW_SPH, 1.0, 0.0, x, u),
+                           rtol=1e-12, atol=1e-14)
+
+
+def test_kerr_metric_known_values():
+    # Horizon r+ = M + sqrt(M^2 - a^2) is where Delta = 0: g_rr diverges.
+    a = 0.95
+    rp = 1 + math.sqrt(1 - a * a)
+    g = O.metric(A.PRR_KERR_BL, 1.0, a, [rp * (1 + 1e-9), 1.0, 0.0, 0.0])
+    assert g[0, 0] > 1e7
+    # ergosphere boundary r = M + sqrt(M^2 - a^2 cos^2 th): g_tt = 0
+    th = 0.7
+    re = 1 + math.sqrt(1 - a * a * math.cos(th) ** 2)
+    assert abs(O.metric(A.PRR_KERR_BL, 1.0, a, [re, th, 0.0, 0.0])[3, 3]) < 1e-13
+
+
+@pytest.mark.parametrize("kind", [A.PRR_FLAT, A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH, A.PRR_KERR_BL])
+def test_initial_state_is_null(kind):
+    for _ in range(20):
+        pos = RNG.normal(size=3)
+        pos = pos / np.linalg.norm(pos) * RNG.uniform(5, 50)
+        d = RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        a = 0.9 if kind == A.PRR_KERR_BL else 0.0
+        x, u = O.init_ray(kind, 1.0, a, pos, d)
+        g = O.metric(kind, 1.0, a, x)
+        norm = u @ g @ u
+        assert abs(norm) < 1e-12 * (np.abs(g).max() * (u @ u)), norm
+        assert u[3] > 0  # future directed
+
+
+def test_radial_ingoing_ks_has_unit_time_component():
+    # In Kerr-Schild coordinates an ingoing radial null ray has dt/dlambda = dr/dlambda magnitude.
+    pos = np.array([7.0, 0.0, 0.0])
+    x, u = O.init_ray(A.PRR_SCHW_KS_CART, 1.0, 0.0, pos, [-1.0, 0.0, 0.0])
+    assert abs(u[3] - 1.0) < 1e-14
+
+
+def tracer(**kw):
+    base = dict(solver=A.PRR_RKF45, h=0.01, hmin=1e-9, hmax=5.0, atol=1e-12, rtol=1e-12, r_escape=200.0,
+                horizon_eps_rel=1e-3, horizon_eps_abs=0.0, lambda_max=1e9, max_steps=10**7, max_attempts=10**8)
+    base.update(kw)
+    return A.make_tracer(**base)
+
+
+def kerr_state_from_constants(M, a, r, th, E, L, ur, uth):
+    """u^mu of a ray at (r, th) with p_t = -E, p_phi = L and given u^r, u^th."""
+    g = O.metric(A.PRR_KERR_BL, M, a, [r, th, 0.0, 0.0])
+    gi = np.linalg.inv(g)
+    ut = gi[3, 3] * (-E) + gi[3, 2] * L
+    uph = gi[2, 3] * (-E) + gi[2, 2] * L
+    return np.array([r, th, 0.0, 0.0]), np.array([ur, uth, uph, ut])
+
+
+def test_kerr_conserves_E_L_Q():
+    a = 0.9
+    S = O.Scene(A.PRR_KERR_BL, 1.0, a, tracer(max_steps=3000, r_escape=60.0))
+    for _ in range(6):
+        pos = RNG.normal(size=3)
+        pos = pos / np.linalg.norm(pos) * 20.0
+        d = -pos / 20.0 + 0.25 * RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        x, u = O.init_ray(A.PRR_KERR_BL, 1.0, a, pos, d)
+        c0 = O.kerr_constants(1.0, a, x, u)
+        res = S.trace(np.concatenate([x, u])[None])
+        xe, ue = res["xend"][0, :4], res["xend"][0, 4:]
+        c1 = O.kerr_constants(1.0, a, xe, ue)
+        assert res["cls"][0] in (A.PRR_ESCAPE, A.PRR_HORIZON, A.PRR_STEP_CAP)
+        if res["cls"][0] == A.PRR_HORIZON:
+            continue  # BL constants lose precision at the horizon
+        scale = np.array([abs(c0[0]), abs(c0[1]) + 1.0, abs(c0[2]) + 1.0])
+        assert np.all(np.abs(c1 - c0) <= 1e-8 * scale), (c0, c1)
+
+
+def test_kerr_equatorial_photon_orbits():
+    # Circular equatorial photon orbits r = 2M{1 + cos[2/3 arccos(-+a/M)]} (Bardeen, Press, Teukolsky 1972).
+    a = 0.7
+    for sign in (+1, -1):  # prograde, retrograde
+        r = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(-sign * a)))
+        xi = -(r ** 3 - 3 * r ** 2 + a * a * r + a * a) / (a * (r - 1))  # L/E
+        x, u = kerr_state_from_constants(1.0, a, r, math.pi / 2, 1.0, xi, 0.0, 0.0)
+        g = O.metric(A.PRR_KERR_BL, 1.0, a, x)
+        assert abs(u @ g @ u) < 1e-12
+        du = O.geod(A.PRR_KERR_BL, 1.0, a, x, u)
+        assert abs(du[0]) < 1e-12 * (abs(u[3]) ** 2)  # no radial acceleration
+        S = O.Scene(A.PRR_KERR_BL, 1.0, a, tracer(max_steps=400, hmax=0.05))
+        res = S.trace(np.concatenate([x, u])[None])
+        assert abs(res["xend"][0, 0] - r) < 1e-5 * r
+        # a slightly smaller radius falls in, a slightly larger one escapes (unstable orbit)
+    r_isco_ph = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(-a)))
+    assert r_isco_ph < 3.0
+
+
+def test_schwarzschild_photon_sphere_is_circular():
+    x = np.array([3.0, math.pi / 2, 0.0, 0.0])
+    L = 3.0 * math.sqrt(3.0)  # b_c = 3 sqrt(3) M
+    u = np.array([0.0, 0.0, L / 9.0, 1.0 / (1 - 2 / 3.0)])
+    g = O.metric(A.PRR_SCHW_SPH, 1.0, 0.0, x)
+    assert abs(u @ g @ u) < 1e-12
+    assert abs(O.geod(A.PRR_SCHW_SPH, 1.0, 0.0, x, u)[0]) < 1e-14
+
+
+def tail(b, R, M=1.0):
+    val, _ = integrate.quad(lambda w: b / math.sqrt(1 - b * b * w * w * (1 - 2 * M * w)), 0.0, 1.0 / R,
+                            epsabs=1e-15, epsrel=1e-13)
+    return val
+
+
+def test_darwin_deflection_schwarzschild_coordinates():
+    rows = json.load(open(os.path.join(GOLDEN, "darwin_deflection.json")))["rows"]
+    R = 400.0
+    # Global RKF45 error grows like tol^(4/5) times the orbit's sensitivity to b,
+    # which diverges as r0 -> 3M; tol 1e-13 leaves < 1e-7 rad at r0 = 3.2M.
+    S = O.Scene(A.PRR_SCHW_SPH, 1.0, 0.0, tracer(r_escape=R, hmax=1.0, atol=1e-13, rtol=1e-13))
+    for row in rows:
+        b = row["b"]
+        ur = -math.sqrt(1 - (1 - 2 / R) * b * b / R ** 2)
+        x = np.array([R, math.pi / 2, 0.0, 0.0])
+        u = np.array([ur, 0.0, b / R ** 2, 1 / (1 - 2 / R)])
+        res = S.trace(np.concatenate([x, u])[None])
+        assert res["cls"][0] == A.PRR_ESCAPE
+        r_end, phi_end = res["xend"][0, 0], res["xend"][0, 2]
+        expect = row["alpha"] + math.pi - tail(b, R) - tail(b, r_end)
+        n = round((expect - phi_end) / (2 * math.pi))
+        bound = 1e-7 if row["r0"] <= 4.0 else 5e-9
+        assert abs(phi_end + 2 * math.pi * n - expect) < bound, (b, phi_end, expect)
+
+
+def test_capture_threshold_is_3sqrt3():
+    R = 300.0
+    S = O.Scene(A.PRR_SCHW_SPH, 1.0, 0.0, tracer(r_escape=R + 1, hmax=20.0, atol=1e-11, rtol=1e-11))
+    bc = 3 * math.sqrt(3)
+    states = []
+    for b in (bc * (1 - 1e-4), bc * (1 + 1e-4)):
+        ur = -math.sqrt(1 - (1 - 2 / R) * b * b / R ** 2)
+        states.append([R, math.pi / 2, 0.0, 0.0, ur, 0.0, b / R ** 2, 1 / (1 - 2 / R)])
+    res = S.trace(np.array(states))
+    assert list(res["cls"]) == [A.PRR_HORIZON, A.PRR_ESCAPE]
+
+
+def test_flat_rays_are_straight():
+    S = O.Scene(A.PRR_FLAT, 0.0, 0.0, tracer(r_escape=100.0))
+    f = A.make_frame([30.0, 0.0, 5.0], math.pi, 0.1, math.radians(60), 16, 12, mass=0.0,
+                     projection=A.PRR_PROJ_SPHERICAL)
+    res = S.render_rays(f)
+    for i in range(f.nrays):
+        uv = O.sample_uv(f, i)
+        d = O.camera_dir(f, *uv)
+        assert res["cls"][i] == A.PRR_ESCAPE
+        assert np.allclose(res["endpoint"][i], d, atol=1e-12)
+
+
+def test_ks_and_schwarzschild_coordinates_trace_the_same_orbit():
+    # Same event and spatial coordinate velocity -> same null geodesic; the spatial
+    # coordinates of Kerr-Schild and Schwarzschild charts coincide.
+    h, N = 0.01, 1500
+    T = tracer(hmin=h, hmax=h, h=h, max_steps=N, r_escape=1e9)
+    Sk = O.Scene(A.PRR_SCHW_KS_CART, 1.0, 0.0, T)
+    Ss = O.Scene(A.PRR_SCHW_SPH, 1.0, 0.0, T)
+    for _ in range(5):
+        pos = np.array([12.0, 3.0, 2.0]) + RNG.normal(size=3)
+        d = -pos / np.linalg.norm(pos) + 0.3 * RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        xk, uk = O.init_ray(A.PRR_SCHW_KS_CART, 1.0, 0.0, pos, d)
+        xs, us = O.init_ray(A.PRR_SCHW_SPH, 1.0, 0.0, pos, d)
+        rk = Sk.trace(np.concatenate([xk, uk])[None])
+        rs = Ss.trace(np.concatenate([xs, us])[None])
+        if rk["cls"][0] != A.PRR_STEP_CAP or rs["cls"][0] != A.PRR_STEP_CAP:
+            continue
+        pk = rk["xend"][0, :3]
+        r, th, ph = rs["xend"][0, :3]
+        ps = r * np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+        assert np.allclose(pk, ps, atol=1e-8), (pk, ps)
+
+
+def _fixed_step_end(solver, h, N, x, u):
+    T = tracer(solver=solver, h=h, hmin=h, hmax=h, max_steps=N, r_escape=1e9)
+    S = O.Scene(A.PRR_SCHW_KS_CART, 1.0, 0.0, T)
+    return S.trace(np.concatenate([x, u])[None])["xend"][0]
+
+
+@pytest.mark.parametrize("solver,order", [(A.PRR_RKF45, 4), (A.PRR_EULER, 1)])
+def test_convergence_order(solver, order):
+    x, u = O.init_ray(A.PRR_SCHW_KS_CART, 1.0, 0.0, [10.0, 0.0, 1.0], [-0.6, 0.8, 0.0])
+    lam = 4.0
+    h = 0.2 if solver == A.PRR_RKF45 else 0.01
+    y1 = _fixed_step_end(solver, h, int(round(lam / h)), x, u)
+    y2 = _fixed_step_end(solver, h / 2, int(round(lam / h * 2)), x, u)
+    y4 = _fixed_step_end(solver, h / 4, int(round(lam / h * 4)), x, u)
+    e1 = np.abs(y1 - y2).max()
+    e2 = np.abs(y2 - y4).max()
+    ratio = e1 / e2
+    assert 0.75 * 2 ** order < ratio < 1.3 * 2 ** order, ratio
+
+
+def test_fehlberg_tableau_conditions():
+    Am, b4, b5 = O.fehlberg()
+    c = Am.sum(axis=1)
+    assert np.allclose(c, [0, 1 / 4, 3 / 8, 12 / 13, 1, 1 / 2])
+    for b, p in ((b4, 4), (b5, 5)):
+        assert abs(b.sum() - 1) < 1e-15
+        for q in range(1, p):  # quadrature conditions sum b c^q = 1/(q+1)
+            assert abs((b * c ** q).sum() - 1 / (q + 1)) < 1e-14
+
+
+def test_ray_sphere_chord_brute_force():
+    spheres = [([8.0, 1.0, 0.5], 1.0), ([8.5, 1.2, 0.3], 0.7)]
+    for _ in range(300):
+        P0 = RNG.uniform(-2, 12, size=3)
+        P1 = P0 + RNG.normal(size=3) * 3
+        k, t = O.chord_hit(spheres, P0, P1)
+        ts = np.linspace(0, 1, 20001)
+        pts = P0[None] + ts[:, None] * (P1 - P0)[None]
+        best = None
+        for j, (c, r) in enumerate(spheres):
+            inside = np.linalg.norm(pts - np.array(c)[None], axis=1) <= r
+            if inside.any():
+                tj = ts[np.argmax(inside)]
+                if best is None or tj < best[1]:
+                    best = (j, tj)
+        if best is None:
+            assert k == -1
+        else:
+            assert k == best[0] and abs(t - best[1]) <= 1e-4
+
+
+def test_camera_centre_ray_is_forward_and_corners_match_fov():
+    f = A.make_frame([30.0, 0.0, 0.0], math.pi, math.radians(5), math.radians(60), 101, 51)
+    Bm = O.camera_basis(f.alpha, f.beta)
+    assert np.allclose(O.camera_dir(f, 0.5, 0.5), Bm[0], atol=1e-15)
+    # top-centre ray makes vfov/2 with forward
+    d = O.camera_dir(f, 0.5, 0.0)
+    assert abs(math.acos(d @ Bm[0]) - math.radians(30)) < 1e-13
+    assert d @ Bm[1] > 0  # row 0 is the top of the image
+    # right edge: horizontal half-angle atan(tan(vfov/2) W/H)
+    d = O.camera_dir(f, 1.0, 0.5)
+    assert abs(math.atan2(d @ Bm[2], d @ Bm[0]) - math.atan(math.tan(math.radians(30)) * 101 / 51)) < 1e-13
+    assert np.allclose(np.cross(Bm[0], Bm[1]), Bm[2])
+
+
+def test_sample_positions_regular_grid():
+    f = A.make_frame([30, 0, 0], math.pi, 0, 1.0, 4, 3, ssaa=A.PRR_SSAA_REGULAR, samples=9)
+    # pixel (1, 2) sample q*3 + p at ((i + (p+1)/4)/W, (j + (q+1)/4)/H)  (kernels/ssaa.cuh:41-46)
+    for q in range(3):
+        for p in range(3):
+            uv = O.sample_uv(f, (2 * 4 + 1) * 9 + q * 3 + p)
+            assert np.allclose(uv, [(1 + (p + 1) / 4) / 4, (2 + (q + 1) / 4) / 3], atol=0, rtol=1e-15)
+    f1 = A.make_frame([30, 0, 0], math.pi, 0, 1.0, 4, 3)
+    assert np.allclose(O.sample_uv(f1, 5), [(1 + 0.5) / 4, (1 + 0.5) / 3])
