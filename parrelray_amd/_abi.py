@@ -16,8 +16,8 @@ STATUS_NAMES = {0: "PRR_OK", 1: "PRR_EINVAL", 2: "PRR_ENOMEM", 3: "PRR_EHIP", 4:
 
 # prr_metric
 PRR_FLAT, PRR_SCHW_KS_CART, PRR_SCHW_SPH, PRR_KERR_BL = 0, 1, 2, 3
-METRIC_NAMES = {"flat": PRR_FLAT, "schwarzschild": PRR_SCHW_KS_CART, "schwarzschild_cart": PRR_SCHW_KS_CART,
-                "schwarzschild_sph": PRR_SCHW_SPH, "kerr": PRR_KERR_BL}
+METRIC_NAMES = {"flat": PRR_FLAT, "schwarzschild": PRR_SCHW_KS_CART, "schwarzschild-cart": PRR_SCHW_KS_CART,
+                "schwarzschild-sph": PRR_SCHW_SPH, "kerr": PRR_KERR_BL}
 # prr_solver (values of the reference's SOLVER enum, main/base.h:26-34)
 PRR_EULER, PRR_RKF45 = 0, 2
 # prr_precision

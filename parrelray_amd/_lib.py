@@ -77,6 +77,11 @@ def lib():
     _sig(L, "prr_texture_value_noise", st, [_vp, C.c_int, C.c_int, C.c_int, C.c_uint64])
     _sig(L, "prr_debug_state_size", C.c_int, [_i32])
     _sig(L, "prr_debug_rhs", st, [_i32, _d, _d, _d, _d, _vp, _i32, _vp])
+    _sig(L, "prr_debug_rhs_vjp", st, [_i32, _d, _d, _d, _d, _vp, _vp, _vp, _vp])
+    _sig(L, "prr_debug_orbit_init", st, [_d, _d, _d, _d, _d, _d, _d, _d, _d, _vp, _vp])
+    _sig(L, "prr_debug_kerr_step", st, [_d, _d, _d, _d, _vp, _d, _vp])
+    _sig(L, "prr_debug_kerr_step_vjp", st, [_d, _d, _d, _d, _vp, _d, _vp])
+    _sig(L, "prr_debug_escape_color", st, [_d, _d, _d, _d, _vp, _vp, C.c_int, C.c_int, _vp, _vp])
     _sig(L, "prr_debug_state_from_xu", st, [_i32, _d, _d, _vp, _vp, _vp, _vp])
     _sig(L, "prr_debug_xu_from_state", st, [_i32, _d, _d, _d, _d, _vp, _vp, _vp])
     _sig(L, "prr_debug_init_ray", st, [_i32, _d, _d, _vp, _vp, _vp, _vp])

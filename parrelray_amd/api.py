@@ -252,6 +252,43 @@ def rhs(metric, y, M=1.0, a=0.0, E=0.0, L=0.0, fp32=False):
     return f
 
 
+def rhs_vjp(metric, y, fb, M=1.0, a=0.0, E=0.0, L=0.0):
+    """(dL/dy, dL/d(E, L, a)) for dL/df = fb (Kerr)."""
+    y, fb = np.ascontiguousarray(y, np.float64), np.ascontiguousarray(fb, np.float64)
+    yb, cb = np.zeros(5), np.zeros(3)
+    check(lib().prr_debug_rhs_vjp(metric, M, a, E, L, _p(y), _p(fb), _p(yb), _p(cb)))
+    return yb, cb
+
+
+def orbit_init(a, r_cam, incl, cam_phi, vfov, aspect, u, v, M=1.0):
+    """Kerr ray set-up z = (r, mu, phi, p_r, p_mu, E, L) and dz/d(a, r_cam, incl)."""
+    z, dz = np.empty(7), np.empty((7, 3))
+    check(lib().prr_debug_orbit_init(M, a, r_cam, incl, cam_phi, vfov, aspect, u, v, _p(z), _p(dz)))
+    return z, dz
+
+
+def kerr_step(y, h, a, E, L, M=1.0):
+    y = np.ascontiguousarray(y, np.float64)
+    out = np.empty(5)
+    check(lib().prr_debug_kerr_step(M, a, E, L, _p(y), h, _p(out)))
+    return out
+
+
+def kerr_step_vjp(y, h, lam, a, E, L, M=1.0):
+    y = np.ascontiguousarray(y, np.float64)
+    lam = np.array(lam, np.float64)
+    check(lib().prr_debug_kerr_step_vjp(M, a, E, L, _p(y), h, _p(lam)))
+    return lam
+
+
+def escape_color(y, a, E, L, rgb8, M=1.0):
+    y = np.ascontiguousarray(y, np.float64)
+    rgb8 = np.ascontiguousarray(rgb8, np.uint8)
+    c, dc = np.empty(3), np.empty((3, 8))
+    check(lib().prr_debug_escape_color(M, a, E, L, _p(y), _p(rgb8), rgb8.shape[1], rgb8.shape[0], _p(c), _p(dc)))
+    return c, dc
+
+
 def state_from_xu(metric, x, u, M=1.0, a=0.0):
     x, u = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(u, np.float64)
     y = np.empty(state_size(metric))

@@ -3,7 +3,10 @@
 // __host__ __device__ templates are available to host code.
 #include <hip/hip_runtime.h>
 
+#include <vector>
+
 #include "../../include/prr_debug.h"
+#include "adjoint.hpp"
 #include "geodesic.hpp"
 #include "prr_host.hpp"
 
@@ -42,6 +45,70 @@ prr_status prr_debug_rhs(int32_t metric, double M, double a, double E, double L,
         }
         return PRR_OK;
     });
+}
+
+prr_status prr_debug_rhs_vjp(int32_t metric, double M, double a, double E, double L, const double *y,
+                             const double *fb, double *yb, double *cb) {
+    if (metric != PRR_KERR_BL) {
+        set_global_error("rhs_vjp: Kerr only");
+        return PRR_EINVAL;
+    }
+    if (!y || !fb || !yb || !cb) return PRR_EINVAL;
+    Kerr::rhs_vjp(y, Consts<double>{M, a, E, L}, fb, yb, cb);
+    return PRR_OK;
+}
+
+prr_status prr_debug_orbit_init(double M, double a, double r_cam, double incl, double cam_phi, double vfov,
+                                double aspect, double u, double v, double z[7], double dz[21]) {
+    if (!z) return PRR_EINVAL;
+    using D = Dual<3>;
+    const D da = D::var(a, 0), dr = D::var(r_cam, 1), di = D::var(incl, 2);
+    V3<D> pos, d;
+    orbit_camera_ray<D>(dr, di, cam_phi, std::tan(vfov / 2.0), aspect, u, v, pos, d);
+    D zz[7];
+    kerr_init_t<D>(pos, d, M, da, zz);
+    for (int i = 0; i < 7; ++i) {
+        z[i] = zz[i].v;
+        if (dz)
+            for (int j = 0; j < 3; ++j) dz[3 * i + j] = zz[i].d[j];
+    }
+    return PRR_OK;
+}
+
+prr_status prr_debug_kerr_step(double M, double a, double E, double L, const double y[5], double h, double out[5]) {
+    if (!y || !out) return PRR_EINVAL;
+    kerr_step(y, h, Consts<double>{M, a, E, L}, out);
+    return PRR_OK;
+}
+
+prr_status prr_debug_kerr_step_vjp(double M, double a, double E, double L, const double y[5], double h,
+                                   double lam[8]) {
+    if (!y || !lam) return PRR_EINVAL;
+    kerr_step_vjp(y, h, Consts<double>{M, a, E, L}, lam);
+    return PRR_OK;
+}
+
+prr_status prr_debug_escape_color(double M, double a, double E, double L, const double y[5], const uint8_t *rgb8,
+                                  int w, int h, double c[3], double dc[24]) {
+    if (!y || !rgb8 || !c || w < 2 || h < 2) return PRR_EINVAL;
+    std::vector<float4> tx(size_t(w) * h);
+    for (size_t i = 0; i < tx.size(); ++i)
+        tx[i] = make_float4(float(rgb8[3 * i]) / 255.0f, float(rgb8[3 * i + 1]) / 255.0f,
+                            float(rgb8[3 * i + 2]) / 255.0f, 1.0f);
+    TexDev t{tx.data(), w, h, PRR_FILTER_BILINEAR, 0};
+    using D = Dual<8>;
+    D yy[5];
+    for (int i = 0; i < 5; ++i) yy[i] = D::var(y[i], i);
+    const D dE = D::var(E, 5), dL = D::var(L, 6), da = D::var(a, 7);
+    const V3<D> n = kerr_escape_dir_t<D>(yy, dE, dL, da, M);
+    D col[3];
+    sky_color_t<D>(t, n, col);
+    for (int k = 0; k < 3; ++k) {
+        c[k] = col[k].v;
+        if (dc)
+            for (int j = 0; j < 8; ++j) dc[8 * k + j] = col[k].d[j];
+    }
+    return PRR_OK;
 }
 
 prr_status prr_debug_state_from_xu(int32_t metric, double M, double a, const double x[4], const double u[4],

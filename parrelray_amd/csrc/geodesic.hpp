@@ -11,10 +11,10 @@
 // State vectors:
 //   Flat, SchwKS, SchwSph: y = (x^1, x^2, x^3, t, u^1, u^2, u^3, u^t), the
 //     reference's Ray r/u pair (main/base.h:172-176); f(y) = (u, getGeod(x, u)).
-//   Kerr (Boyer-Lindquist): y = (r, theta, phi, p_r, p_theta) with the
+//   Kerr (Boyer-Lindquist): y = (r, mu = cos theta, phi, p_r, p_mu) with the
 //     conserved E = -p_t and L = p_phi held per ray; f(y) = Hamilton's
-//     equations of H = g^{mu nu} p_mu p_nu / 2.  t is not integrated (it
-//     enters nothing observable).
+//     equations of H = g^{ab} p_a p_b / 2.  t is not integrated (it enters
+//     nothing observable).
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -128,41 +128,171 @@ struct SchwSph {
 };
 
 // Kerr, Boyer-Lindquist (metric of metrics/kerr.cuh:77-100; the reference's
-// getGeod is a stub).  With Sigma = r^2 + a^2 cos^2, Delta = r^2 - 2Mr + a^2,
-// W = (r^2 + a^2) E - a L and N = 2 Sigma H
-//   = Delta p_r^2 + p_th^2 - W^2/Delta + (L - a E sin^2)^2 / sin^2:
-//   r'  = Delta p_r / Sigma,  th' = p_th / Sigma,
-//   ph' = (a W / Delta + L / sin^2 - a E) / Sigma,
-//   p_r'  = -(dN/dr  - 2 r N / Sigma) / (2 Sigma),
-//   p_th' = -(dN/dth + 2 a^2 sin cos N / Sigma) / (2 Sigma).
+// getGeod is a stub), integrated as Hamilton's equations in the canonical
+// pair (mu = cos th, p_mu = -p_th / sin th), so that the right-hand side is
+// rational: no trigonometry in the hot loop.  With s2 = 1 - mu^2,
+// Sigma = r^2 + a^2 mu^2, Delta = r^2 - 2Mr + a^2, W = (r^2 + a^2) E - a L,
+// Lm = L - a E s2 and N = 2 Sigma H
+//   = Delta p_r^2 + s2 p_mu^2 - W^2/Delta + Lm^2 / s2:
+//   r'  = Delta p_r / Sigma,  mu' = s2 p_mu / Sigma,
+//   ph' = (a W / Delta + Lm / s2) / Sigma,
+//   p_r'  = -(dN/dr - 2 r N / Sigma) / (2 Sigma),
+//     dN/dr = Delta' (p_r^2 + (W/Delta)^2) - 4 r E W / Delta,
+//   p_mu' = -mu (L^2 / s2^2 - a^2 E^2 - p_mu^2 - a^2 N / Sigma) / Sigma.
+// The chart is singular only where a ray with L = 0 crosses the axis
+// exactly (s2 = 0); such a ray ends as NAN.
 struct Kerr {
     static constexpr int N = 5;
     static constexpr int kind = PRR_KERR_BL;
     static constexpr bool spherical = true;
     static constexpr bool hamiltonian = true;
     template <class T> PRR_HD static void rhs(const T *y, const Consts<T> &c, T *f) {
-        const T r = y[0], pr = y[3], pth = y[4];
+        const T r = y[0], mu = y[1], pr = y[3], pm = y[4];
         const T a = c.a, M = c.M, E = c.E, L = c.L;
-        T s, co;
-        sincos_t(y[1], &s, &co);
-        const T s2 = s * s, r2 = r * r, a2 = a * a;
-        const T Sig = r2 + a2 * co * co;
+        const T r2 = r * r, a2 = a * a;
+        const T s2 = T(1) - mu * mu;
+        const T Sig = r2 + a2 * mu * mu;
         const T Del = r2 - T(2) * M * r + a2;
         const T iS = T(1) / Sig, iD = T(1) / Del, is2 = T(1) / s2;
         const T W = (r2 + a2) * E - a * L;
         const T Lm = L - a * E * s2;
         const T WiD = W * iD;
-        const T Nn = Del * pr * pr + pth * pth - W * WiD + Lm * Lm * is2;
-        const T dDel = T(2) * (r - M);
-        const T dNr = dDel * (pr * pr + WiD * WiD) - T(4) * r * E * WiD;
-        const T sc = s * co;
-        const T dNth = T(2) * sc * (a2 * E * E - L * L * is2 * is2);
+        const T Lmis2 = Lm * is2;
+        const T Nn = Del * pr * pr + s2 * pm * pm - W * WiD + Lm * Lmis2;
         const T NiS = Nn * iS;
+        const T dNr = T(2) * (r - M) * (pr * pr + WiD * WiD) - T(4) * r * E * WiD;
+        const T Lis2 = L * is2;
         f[0] = Del * pr * iS;
-        f[1] = pth * iS;
-        f[2] = (a * WiD + L * is2 - a * E) * iS;
+        f[1] = s2 * pm * iS;
+        f[2] = (a * WiD + Lmis2) * iS;
         f[3] = T(-0.5) * iS * (dNr - T(2) * r * NiS);
-        f[4] = T(-0.5) * iS * (dNth + T(2) * a2 * sc * NiS);
+        f[4] = -mu * iS * (Lis2 * Lis2 - a2 * E * E - pm * pm - a2 * NiS);
+    }
+
+    // Reverse mode of rhs: given fb = dL/df, accumulate dL/dy into yb[5]
+    // (yb[2] stays untouched: phi does not enter f) and dL/d(E, L, a) into
+    // cb[0..2].  Written by hand from the forward expression graph above,
+    // in reverse order.
+    PRR_HD static void rhs_vjp(const double *y, const Consts<double> &c, const double *fb, double *yb, double *cb) {
+        const double r = y[0], mu = y[1], pr = y[3], pm = y[4];
+        const double a = c.a, M = c.M, E = c.E, L = c.L;
+        // forward values
+        const double r2 = r * r, a2 = a * a;
+        const double s2 = 1.0 - mu * mu;
+        const double Sig = r2 + a2 * mu * mu;
+        const double Del = r2 - 2.0 * M * r + a2;
+        const double iS = 1.0 / Sig, iD = 1.0 / Del, is2 = 1.0 / s2;
+        const double W = (r2 + a2) * E - a * L;
+        const double Lm = L - a * E * s2;
+        const double WiD = W * iD;
+        const double Lmis2 = Lm * is2;
+        const double Nn = Del * pr * pr + s2 * pm * pm - W * WiD + Lm * Lmis2;
+        const double NiS = Nn * iS;
+        const double Aq = pr * pr + WiD * WiD;
+        const double dNr = 2.0 * (r - M) * Aq - 4.0 * r * E * WiD;
+        const double Lis2 = L * is2;
+        const double g2 = a * WiD + Lmis2;
+        const double g3 = dNr - 2.0 * r * NiS;
+        const double g4 = Lis2 * Lis2 - a2 * E * E - pm * pm - a2 * NiS;
+        // adjoints
+        double rb = 0, mub = 0, prb = 0, pmb = 0, Eb = 0, Lb = 0, ab = 0;
+        double iSb = 0, Delb = 0, s2b = 0, WiDb = 0, Lmis2b = 0, NiSb = 0, dNrb = 0, Lis2b = 0, a2b = 0;
+        // f0 = Del pr iS
+        Delb += fb[0] * pr * iS;
+        prb += fb[0] * Del * iS;
+        iSb += fb[0] * Del * pr;
+        // f1 = s2 pm iS
+        s2b += fb[1] * pm * iS;
+        pmb += fb[1] * s2 * iS;
+        iSb += fb[1] * s2 * pm;
+        // f2 = g2 iS, g2 = a WiD + Lmis2
+        iSb += fb[2] * g2;
+        {
+            const double g2b = fb[2] * iS;
+            ab += g2b * WiD;
+            WiDb += g2b * a;
+            Lmis2b += g2b;
+        }
+        // f3 = -0.5 iS g3, g3 = dNr - 2 r NiS
+        iSb += -0.5 * fb[3] * g3;
+        {
+            const double g3b = -0.5 * fb[3] * iS;
+            dNrb += g3b;
+            rb += g3b * (-2.0 * NiS);
+            NiSb += g3b * (-2.0 * r);
+        }
+        // f4 = -mu iS g4, g4 = Lis2^2 - a2 E^2 - pm^2 - a2 NiS
+        mub += -fb[4] * iS * g4;
+        iSb += -fb[4] * mu * g4;
+        {
+            const double g4b = -fb[4] * mu * iS;
+            Lis2b += g4b * 2.0 * Lis2;
+            a2b += g4b * (-E * E - NiS);
+            Eb += g4b * (-2.0 * a2 * E);
+            pmb += g4b * (-2.0 * pm);
+            NiSb += g4b * (-a2);
+        }
+        // Lis2 = L is2
+        Lb += Lis2b * is2;
+        double is2b = Lis2b * L;
+        // dNr = 2 (r - M) Aq - 4 r E WiD, Aq = pr^2 + WiD^2
+        rb += dNrb * (2.0 * Aq - 4.0 * E * WiD);
+        prb += dNrb * 4.0 * (r - M) * pr;
+        WiDb += dNrb * (4.0 * (r - M) * WiD - 4.0 * r * E);
+        Eb += dNrb * (-4.0 * r * WiD);
+        // NiS = Nn iS
+        const double Nnb = NiSb * iS;
+        iSb += NiSb * Nn;
+        // Nn = Del pr^2 + s2 pm^2 - W WiD + Lm Lmis2
+        Delb += Nnb * pr * pr;
+        prb += Nnb * 2.0 * Del * pr;
+        s2b += Nnb * pm * pm;
+        pmb += Nnb * 2.0 * s2 * pm;
+        double Wb = -Nnb * WiD;
+        WiDb += -Nnb * W;
+        double Lmb = Nnb * Lmis2;
+        Lmis2b += Nnb * Lm;
+        // Lmis2 = Lm is2
+        Lmb += Lmis2b * is2;
+        is2b += Lmis2b * Lm;
+        // WiD = W iD
+        Wb += WiDb * iD;
+        const double iDb = WiDb * W;
+        // Lm = L - a E s2
+        Lb += Lmb;
+        ab += Lmb * (-E * s2);
+        Eb += Lmb * (-a * s2);
+        s2b += Lmb * (-a * E);
+        // W = (r2 + a2) E - a L
+        double r2b = Wb * E;
+        a2b += Wb * E;
+        Eb += Wb * (r2 + a2);
+        ab += Wb * (-L);
+        Lb += Wb * (-a);
+        // reciprocals
+        const double Sigb = -iSb * iS * iS;
+        Delb += -iDb * iD * iD;
+        s2b += -is2b * is2 * is2;
+        // Del = r2 - 2 M r + a2
+        r2b += Delb;
+        rb += Delb * (-2.0 * M);
+        a2b += Delb;
+        // Sig = r2 + a2 mu^2
+        r2b += Sigb;
+        a2b += Sigb * mu * mu;
+        mub += Sigb * 2.0 * a2 * mu;
+        // s2 = 1 - mu^2
+        mub += s2b * (-2.0 * mu);
+        // squares
+        ab += a2b * 2.0 * a;
+        rb += r2b * 2.0 * r;
+        yb[0] += rb;
+        yb[1] += mub;
+        yb[3] += prb;
+        yb[4] += pmb;
+        cb[0] += Eb;
+        cb[1] += Lb;
+        cb[2] += ab;
     }
 };
 
@@ -177,7 +307,12 @@ template <class Mt> PRR_HD double radius_of(const double *y) {
 template <class Mt> PRR_HD D3 cart_of(const double *y) {
     if (!Mt::spherical) return d3(y[0], y[1], y[2]);
     double st, ct, sp, cp;
-    sincos(y[1], &st, &ct);
+    if (Mt::hamiltonian) { // y[1] = cos th
+        ct = y[1];
+        st = sqrt(fmax(1.0 - ct * ct, 0.0));
+    } else {
+        sincos(y[1], &st, &ct);
+    }
     sincos(y[2], &sp, &cp);
     return d3(y[0] * st * cp, y[0] * st * sp, y[0] * ct);
 }
@@ -267,11 +402,13 @@ template <class Mt> PRR_HD void state_from_xu(const double x[4], const double u[
     const KerrG g = kerr_g(M, a, x[0], x[1]);
     cs.E = -(g.tt * u[3] + g.tph * u[2]);
     cs.L = g.phph * u[2] + g.tph * u[3];
+    double st, ct;
+    sincos(x[1], &st, &ct);
     y[0] = x[0];
-    y[1] = x[1];
+    y[1] = ct;
     y[2] = x[2];
     y[3] = g.rr * u[0];
-    y[4] = g.thth * u[1];
+    y[4] = -g.thth * u[1] / st; // p_mu = -p_th / sin th
 }
 
 // Integrator state -> (x, u); for Kerr, t is returned as 0.
@@ -285,14 +422,15 @@ template <class Mt> PRR_HD void xu_from_state(const double *y, const Consts<doub
     }
     double f[5];
     Kerr::rhs<double>(y, cs, f);
-    const KerrG g = kerr_g(cs.M, cs.a, y[0], y[1]);
+    const double th = acos(fmin(fmax(y[1], -1.0), 1.0));
+    const KerrG g = kerr_g(cs.M, cs.a, y[0], th);
     const double det = g.tt * g.phph - g.tph * g.tph;
     x[0] = y[0];
-    x[1] = y[1];
+    x[1] = th;
     x[2] = y[2];
     x[3] = 0.0;
     u[0] = f[0];
-    u[1] = f[1];
+    u[1] = -sqrt(fmax(1.0 - y[1] * y[1], 0.0)) * y[4] / g.Sig; // th' = -sin th p_mu / Sigma
     u[2] = f[2];
     u[3] = -(g.phph * cs.E + g.tph * cs.L) / det;
 }
@@ -376,12 +514,14 @@ template <class Mt> PRR_HD D3 escape_dir(const double *y, const Consts<double> &
     if (Mt::hamiltonian) {
         double f[5];
         Kerr::rhs<double>(y, cs, f);
-        double st, ct, sp, cp;
-        sincos(y[1], &st, &ct);
+        double sp, cp;
+        const double ct = y[1], st = sqrt(fmax(1.0 - ct * ct, 0.0));
         sincos(y[2], &sp, &cp);
         const double r = y[0];
+        const double Sig = r * r + cs.a * cs.a * ct * ct;
+        const double thdot = -st * y[4] / Sig;
         const D3 er = d3(st * cp, st * sp, ct), et = d3(ct * cp, ct * sp, -st), ep = d3(-sp, cp, 0.0);
-        return normalize3(f[0] * er + (r * f[1]) * et + (r * st * f[2]) * ep);
+        return normalize3(f[0] * er + (r * thdot) * et + (r * st * f[2]) * ep);
     }
     return normalize3(d3(y[4], y[5], y[6]));
 }

@@ -224,8 +224,8 @@ double parf(const prr_params *p, const std::string &h, const std::string &k, dou
 
 int metric_from_name(const std::string &s) {
     if (s == "flat") return PRR_FLAT;
-    if (s == "schwarzschild" || s == "schwarzschild_cart" || s.empty()) return PRR_SCHW_KS_CART;
-    if (s == "schwarzschild_sph") return PRR_SCHW_SPH;
+    if (s == "schwarzschild" || s == "schwarzschild-cart" || s.empty()) return PRR_SCHW_KS_CART;
+    if (s == "schwarzschild-sph") return PRR_SCHW_SPH;
     if (s == "kerr") return PRR_KERR_BL;
     return -1;
 }
@@ -304,7 +304,8 @@ int prr_params_frame_count(const prr_params *p) {
 
 // createScene's spacetime part (prr.cpp:163-181).  The metric, fixed at
 // compile time in the reference (main/metric.h:12), is read from the new
-// [prr] block's `metric` key (flat | schwarzschild | schwarzschild_sph | kerr).
+// [prr] block's `metric` key (flat | schwarzschild | schwarzschild-cart | schwarzschild-sph | kerr;
+// no underscores: an underscore anywhere makes a .params line a header).
 prr_status prr_params_spacetime(const prr_params *p, int32_t *metric, double *M, double *a, double *q,
                                 double pvals[16]) {
     if (!p) return perr(PRR_EINVAL, "NULL params");
@@ -336,7 +337,7 @@ prr_status prr_params_tracer(const prr_params *p, prr_tracer *t) {
     t->solver = solver == "rkf" ? PRR_RKF45 : PRR_EULER;
     const std::string prec = par(p, "prr", "precision");
     if (prec.empty() || prec == "fp64") t->precision = PRR_FP64;
-    else if (prec == "fp32_bulk") t->precision = PRR_FP32_BULK;
+    else if (prec == "fp32-bulk") t->precision = PRR_FP32_BULK;
     else return perr(PRR_EINVAL, "unknown precision '" + prec + "'");
     t->hmin = parf(p, "tracer", "hmin", 0.0);
     t->hmax = parf(p, "tracer", "hmax", 0.0);
@@ -344,14 +345,15 @@ prr_status prr_params_tracer(const prr_params *p, prr_tracer *t) {
     t->atol = parf(p, "tracer", "eps", 0.0);
     t->rtol = parf(p, "prr", "rtol", 0.0);
     t->r_escape = parf(p, "tracer", "rout1", 0.0);
+    // [prr] keys use '-' because any '_' turns a .params line into a header.
     // r <= r_s + 0.001 (metrics/schwarzschild.cuh:70) unless overridden
-    t->horizon_eps_rel = parf(p, "prr", "horizon_eps_rel", 0.0);
-    t->horizon_eps_abs = parf(p, "prr", "horizon_eps_abs", 1e-3);
+    t->horizon_eps_rel = parf(p, "prr", "horizon-eps-rel", 0.0);
+    t->horizon_eps_abs = parf(p, "prr", "horizon-eps-abs", 1e-3);
     // lambda < 5000 (Euler, kernels/solver.cuh:84), < 1000 (RKF, :158); 1e6 attempts (:181)
-    t->lambda_max = parf(p, "prr", "lambda_max", t->solver == PRR_EULER ? 5000.0 : 1000.0);
-    t->max_steps = int64_t(parf(p, "prr", "max_steps", 1e9));
-    t->max_attempts = int64_t(parf(p, "prr", "max_attempts", 1e6));
-    t->r_fp64 = parf(p, "prr", "r_fp64", 3.0);
+    t->lambda_max = parf(p, "prr", "lambda-max", t->solver == PRR_EULER ? 5000.0 : 1000.0);
+    t->max_steps = int64_t(parf(p, "prr", "max-steps", 1e9));
+    t->max_attempts = int64_t(parf(p, "prr", "max-attempts", 1e6));
+    t->r_fp64 = parf(p, "prr", "r-fp64", 3.0);
     return PRR_OK;
 }
 

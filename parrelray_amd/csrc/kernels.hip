@@ -103,12 +103,16 @@ template <class Mt> struct Lane {
     double h, lam;
     int32_t steps, attempts, att64;
     int32_t frame;
+    int32_t hit; // sphere index of a SPHERE termination
     int64_t rid;
-    D3 P0; // Cartesian position at the start of the step (sphere chord)
+    D3 P0; // Cartesian position at the start of the step (sphere chord); the hit normal after a hit
 };
 
+// lane_init and lane_finish run once per ray; they are kept out of line so
+// that their transcendental constants and temporaries do not occupy
+// registers across the integration loop.
 template <class Mt>
-__device__ inline void lane_init(const JobDev &job, Lane<Mt> &L, int64_t id) {
+__device__ __noinline__ void lane_init(const JobDev &job, Lane<Mt> &L, int64_t id) {
     L.rid = id;
     L.frame = job.explicit_x0u0 ? 0 : find_frame(job, id);
     const FrameDev &fr = job.frames[L.frame];
@@ -135,11 +139,15 @@ __device__ inline bool finite_state(const double *y) {
 
 // Writes the outputs of a finished ray (rayToRGB, metrics/schwarzschild.cuh:151-183).
 template <class Mt>
-__device__ void lane_finish(const JobDev &job, const Lane<Mt> &L, int cls, D3 endpoint, float3 color) {
+__device__ __noinline__ void lane_finish(const JobDev &job, const Lane<Mt> &L, int cls) {
     const int64_t id = L.rid;
-    if (cls == PRR_ESCAPE) {
-        const FrameDev &fr = job.frames[L.frame];
-        (void)fr;
+    D3 endpoint;
+    float3 color;
+    if (cls == PRR_SPHERE) {
+        endpoint = L.P0;
+        const int slot = job.frames[L.frame].sph_tex[L.hit];
+        color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, endpoint);
+    } else if (cls == PRR_ESCAPE) {
         const D3 n = escape_dir<Mt>(L.y, L.cs);
         endpoint = n;
         double th, ph;
@@ -155,7 +163,7 @@ __device__ void lane_finish(const JobDev &job, const Lane<Mt> &L, int cls, D3 en
             if (ph < 0.0) ph += 6.28318530717958647692;
         }
         color = sky_lookup(job.tex[0], th, ph);
-    } else if (cls != PRR_SPHERE) {
+    } else {
         endpoint = cart_of<Mt>(L.y);
         color = make_float3(0.f, 0.f, 0.f);
     }
@@ -187,7 +195,7 @@ __device__ void lane_finish(const JobDev &job, const Lane<Mt> &L, int cls, D3 en
 // User_ENDLOOP, metrics/schwarzschild.cuh:55-79): sphere chord, spherical
 // wraps, horizon, escape.  Returns the class or -1 to continue.
 template <class Mt>
-__device__ inline int after_step(const JobDev &job, Lane<Mt> &L, D3 &endpoint, float3 &color) {
+__device__ inline int after_step(const JobDev &job, Lane<Mt> &L) {
     const FrameDev &fr = job.frames[L.frame];
     if (fr.n_spheres > 0) {
         const D3 P1 = cart_of<Mt>(L.y);
@@ -195,10 +203,8 @@ __device__ inline int after_step(const JobDev &job, Lane<Mt> &L, D3 &endpoint, f
         const int k = chord_hit(fr, L.P0, P1, t);
         if (k >= 0) {
             const D3 H = L.P0 + t * (P1 - L.P0);
-            const D3 n = (1.0 / fr.sph[k][3]) * (H - d3(fr.sph[k][0], fr.sph[k][1], fr.sph[k][2]));
-            endpoint = n;
-            const int slot = fr.sph_tex[k];
-            color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, n);
+            L.P0 = (1.0 / fr.sph[k][3]) * (H - d3(fr.sph[k][0], fr.sph[k][1], fr.sph[k][2]));
+            L.hit = k;
             return PRR_SPHERE;
         }
         L.P0 = P1;
@@ -215,7 +221,7 @@ __device__ inline int after_step(const JobDev &job, Lane<Mt> &L, D3 &endpoint, f
 
 // One Euler step or one RKF45 attempt.  Returns the class when the ray ends.
 template <class Mt, int SOLVER, bool MIXED>
-__device__ inline int lane_advance(const JobDev &job, Lane<Mt> &L, D3 &endpoint, float3 &color) {
+__device__ inline int lane_advance(const JobDev &job, Lane<Mt> &L) {
     constexpr int N = Mt::N;
     const TracerDev &tr = job.tr;
     if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max) return PRR_STEP_CAP;
@@ -236,7 +242,7 @@ __device__ inline int lane_advance(const JobDev &job, Lane<Mt> &L, D3 &endpoint,
         L.att64 += fp64 ? 1 : 0;
         L.lam += h;
         if (!finite_state<Mt>(L.y)) return PRR_NAN;
-        return after_step<Mt>(job, L, endpoint, color);
+        return after_step<Mt>(job, L);
     }
     if (L.attempts >= tr.max_attempts) return PRR_STEP_CAP;
     L.attempts++;
@@ -267,7 +273,7 @@ __device__ inline int lane_advance(const JobDev &job, Lane<Mt> &L, D3 &endpoint,
     L.steps++;
     L.lam += L.h;
     L.h = hn;
-    return after_step<Mt>(job, L, endpoint, color);
+    return after_step<Mt>(job, L);
 }
 
 __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
@@ -299,7 +305,7 @@ __global__ void __launch_bounds__(256) k_integrate(JobDev job) {
                     lane_init<Mt>(job, L, id);
                     active = true;
                     if (!finite_state<Mt>(L.y)) {
-                        lane_finish<Mt>(job, L, PRR_NAN, d3(0, 0, 0), make_float3(0.f, 0.f, 0.f));
+                        lane_finish<Mt>(job, L, PRR_NAN);
                         acc_rays++;
                         active = false;
                     }
@@ -308,11 +314,9 @@ __global__ void __launch_bounds__(256) k_integrate(JobDev job) {
         }
         if (__ballot(active) == 0ull) break;
         if (active) {
-            D3 endpoint = d3(0, 0, 0);
-            float3 color = make_float3(0.f, 0.f, 0.f);
-            const int cls = lane_advance<Mt, SOLVER, MIXED>(job, L, endpoint, color);
+            const int cls = lane_advance<Mt, SOLVER, MIXED>(job, L);
             if (cls >= 0) {
-                lane_finish<Mt>(job, L, cls, endpoint, color);
+                lane_finish<Mt>(job, L, cls);
                 acc_att += (unsigned long long)L.attempts;
                 acc_ok += (unsigned long long)L.steps;
                 acc_64 += (unsigned long long)L.att64;

@@ -2,7 +2,7 @@
 against the oracle.  No GPU: these run in the CPU suite.
 
 The Kerr check is a cross-formulation test: the kernels integrate Hamilton's
-equations for (r, th, phi, p_r, p_th) with conserved E and L; the oracle uses
+equations for (r, cos th, phi, p_r, p_mu) with conserved E and L; the oracle uses
 the second-order geodesic equation with Christoffel symbols.  They must agree
 on dx/dlambda and on dp/dlambda = d(g u)/dlambda for arbitrary (not only
 null) u, which pins every term of both.
@@ -60,16 +60,23 @@ def test_kerr_hamiltonian_rhs_matches_christoffel_form(a):
         u = RNG.normal(size=4)
         y, EL = api.state_from_xu(A.PRR_KERR_BL, x, u, M, a)
         f = api.rhs(A.PRR_KERR_BL, y, M, a, EL[0], EL[1])
-        # positions: dx/dlambda = u
-        assert np.allclose(f[:3], u[:3], rtol=1e-11, atol=1e-13 * np.abs(u).max())
+        s, c = math.sin(x[1]), math.cos(x[1])
+        # positions: y = (r, mu = cos th, phi): y' = (u^r, -sin th u^th, u^phi)
+        assert np.allclose(f[:3], [u[0], -s * u[1], u[2]], rtol=1e-11, atol=1e-13 * np.abs(u).max())
         # momenta: dp_i/dlambda = d_k g_{i nu} u^k u^nu + g_{i nu} du^nu for i = r, th
         g = O.metric(A.PRR_KERR_BL, M, a, x)
         dg = O.dmetric_kerr(M, a, x)
         du = O.geod(A.PRR_KERR_BL, M, a, x, u)
-        for i, fi in ((0, f[3]), (1, f[4])):
-            expect = sum(dg[k][i] @ u * u[k] for k in range(4)) + g[i] @ du
-            scale = sum(np.abs(dg[k][i]) @ np.abs(u) * abs(u[k]) for k in range(4)) + np.abs(g[i]) @ np.abs(du)
-            assert abs(fi - expect) <= 1e-11 * scale, (a, x, u, i, fi, expect)
+        pdot, pscale = [], []
+        for i in (0, 1):
+            pdot.append(sum(dg[k][i] @ u * u[k] for k in range(4)) + g[i] @ du)
+            pscale.append(sum(np.abs(dg[k][i]) @ np.abs(u) * abs(u[k]) for k in range(4)) + np.abs(g[i]) @ np.abs(du))
+        assert abs(f[3] - pdot[0]) <= 1e-11 * pscale[0], (a, x, u, f[3], pdot[0])
+        # p_mu = -p_th / sin th  ->  p_mu' = -p_th'/sin th + p_th cos th th' / sin^2 th
+        pth = g[1, 1] * u[1]
+        expect = -pdot[1] / s + pth * c * u[1] / (s * s)
+        scale = pscale[1] / s + abs(pth * c * u[1]) / (s * s)
+        assert abs(f[4] - expect) <= 1e-11 * scale, (a, x, u, f[4], expect)
 
 
 def test_kerr_state_round_trip_and_constants():
@@ -122,3 +129,34 @@ def test_fp32_rhs_close_to_fp64(metric):
             y = np.concatenate([x, u])
             f64, f32 = api.rhs(metric, y), api.rhs(metric, y, fp32=True)
         assert np.allclose(f32, f64, rtol=2e-4, atol=2e-5 * np.abs(f64).max())
+
+
+@pytest.mark.parametrize("a", [0.0, 0.7, 0.95])
+def test_kerr_rhs_vjp_matches_central_differences(a):
+    """The adjoint kernel's hand-written reverse mode of the Kerr RHS."""
+    for _ in range(40):
+        x = np.array([RNG.uniform(2.0, 50.0), RNG.uniform(0.3, math.pi - 0.3), RNG.uniform(0, 6), 0.0])
+        u = RNG.normal(size=4)
+        y, EL = api.state_from_xu(A.PRR_KERR_BL, x, u, 1.0, a)
+        fb = RNG.normal(size=5)
+        yb, cb = api.rhs_vjp(A.PRR_KERR_BL, y, fb, 1.0, a, EL[0], EL[1])
+
+        def F(yy, E, L, aa):
+            return fb @ api.rhs(A.PRR_KERR_BL, yy, 1.0, aa, E, L)
+
+        # central differences: truncation ~h^2, rounding ~1e-16 |F| / h
+        scaleF = np.abs(fb) @ np.abs(api.rhs(A.PRR_KERR_BL, y, 1.0, a, EL[0], EL[1]))
+        for i in range(5):
+            h = 1e-6 * max(1.0, abs(y[i]))
+            yp, ym = y.copy(), y.copy()
+            yp[i] += h
+            ym[i] -= h
+            fd = (F(yp, *EL, a) - F(ym, *EL, a)) / (2 * h)
+            tol = 1e-6 * (abs(fd) + 1e-3 * np.abs(yb).max()) + 1e-14 * scaleF / h
+            assert abs(yb[i] - fd) <= tol, (i, yb[i], fd)
+        for j, (dE, dL, da) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
+            h = 1e-6
+            fd = (F(y, EL[0] + h * dE, EL[1] + h * dL, a + h * da) -
+                  F(y, EL[0] - h * dE, EL[1] - h * dL, a - h * da)) / (2 * h)
+            tol = 1e-6 * (abs(fd) + 1e-3 * np.abs(cb).max()) + 1e-14 * scaleF / h
+            assert abs(cb[j] - fd) <= tol, (j, cb[j], fd)
