@@ -60,6 +60,57 @@ def run_form(ctx, frame, precision, steps, warmup, out8, outf, torch):
                 fxaa_gbs=st["fxaa_bytes"] / max(1e-9, sum(s["fxaa_ms"] for s in per) / steps / 1e3) / 1e9)
 
 
+def run_adjoint(api, A, CF, reps):
+    """C3: 512x512 Kerr a=0.7, FP32-bulk RKF45 tol 1e-8; gradient of the L2 loss against the target
+    render w.r.t. (a, r_cam, incl, sphere phi).  Rays/s counts primary rays through the whole
+    prr_adjoint call (forward render, checkpointed forward, replay and reverse sweep)."""
+    import numpy as np
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.7)
+    ctx.set_tracer(CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    ctx.set_texture(0, CF.sky_texture(), A.PRR_FILTER_BILINEAR)
+    ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
+    tmpl = CF.frame_c3()
+    o0, ot = CF.c3_orbit(), CF.c3_orbit(target=True)
+    T = ctx.render(api.frame_from_orbit(tmpl, ot))[0]["rgbf"]
+    I = ctx.render(api.frame_from_orbit(tmpl, o0))[0]["rgbf"]
+    dLdI = (2.0 * (I - T) / I.size).astype(np.float32)
+    g = ctx.adjoint(tmpl, o0, dLdI)  # warm-up
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g = ctx.adjoint(tmpl, o0, dLdI)
+    sec = (time.perf_counter() - t0) / reps
+    st = ctx.stats()
+    return dict(rays=tmpl.nrays, seconds=sec, rays_per_s=tmpl.nrays / sec, adjoint_ms=st["adjoint_ms"],
+                total_ms=st["total_ms"], steps_swept=st["adjoint_steps"],
+                steps_per_s=st["adjoint_steps"] / (st["adjoint_ms"] / 1e3),
+                grad={"a": g[0], "r_cam": g[1], "incl": g[2], "sphere_phi": g[3]},
+                loss=float(((I - T) ** 2).mean()))
+
+
+def run_batch(api, A, CF, reps, torch):
+    """C2: 64 frames (389,520,384 rays) in one persistent launch, replayed from a hipGraph."""
+    frames = CF.frames_c2()
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.0)
+    ctx.set_tracer(CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    ctx.set_texture(0, CF.sky_texture())
+    ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
+    rgb8 = [torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
+    g = api.Graph(ctx, frames, rgb8=rgb8)
+    g.launch()  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        g.launch()
+    torch.cuda.synchronize()
+    sec = (time.perf_counter() - t0) / reps
+    st = ctx.stats()
+    g.close()
+    return dict(frames=len(frames), rays=st["rays"], seconds=sec, rays_per_s=st["rays"] / sec,
+                attempts=st["attempts"], attempts_fp64=st["attempts_fp64"], graph_ms=st["total_ms"])
+
+
 def cpu_baseline(frame, row_stride=16):
     """Oracle (FP64, Christoffel form) on every row_stride-th image row, all cores."""
     import numpy as np
@@ -86,6 +137,8 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the C3 adjoint leg")
+    ap.add_argument("--no-batch", action="store_true", help="skip the C2 64-frame batch leg")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the integrator has no multi-GPU path in this build "
@@ -109,6 +162,10 @@ def main():
     fp32 = run_form(ctx, frame, A.PRR_FP32_BULK, args.steps, args.warmup, out8, outf, torch)
     print(json.dumps({"progress": "fp32_bulk done", "ms_per_step": fp32["ms_per_step"]}), file=sys.stderr,
           flush=True)
+    adj = None if args.no_adjoint else run_adjoint(api, A, CF, max(1, args.steps // 2))
+    print(json.dumps({"progress": "adjoint done", "adjoint": adj}), file=sys.stderr, flush=True)
+    batch = None if args.no_batch else run_batch(api, A, CF, 2, torch)
+    print(json.dumps({"progress": "batch done", "batch": batch}), file=sys.stderr, flush=True)
     cpu = None if args.no_cpu else cpu_baseline(frame)
     t_int = fp32["integrate_ms"] / 1e3
     f32, f64 = fp32["flops_fp32"], fp32["flops_fp64"]
@@ -140,7 +197,9 @@ def main():
                                             "attempts_fp64", "attempts_per_s", "achieved_tflops",
                                             "roofline_frac", "ssaa_gbs", "fxaa_gbs")},
         "speedup_fp32_bulk_vs_fp64": fp64["ms_per_step"] / fp32["ms_per_step"],
-        "adjoint_rays_per_s": None,
+        "adjoint_rays_per_s": adj["rays_per_s"] if adj else None,
+        "adjoint": adj,
+        "batch_c2": batch,
     }
     print(json.dumps(line))
 

@@ -174,6 +174,8 @@ typedef struct prr_stats {
     double flops_fp32;      /* algorithmic FP32 flops of the integration */
     double ssaa_bytes;      /* algorithmic HBM bytes of the SSAA pass */
     double fxaa_bytes;      /* algorithmic HBM bytes of the FXAA pass */
+    double adjoint_ms;      /* prr_adjoint: checkpointed forward + backward passes */
+    int64_t adjoint_steps;  /* prr_adjoint: accepted steps swept backwards */
 } prr_stats;
 
 /* Differentiable parameters of an orbiting-camera frame (adjoint). The

@@ -81,7 +81,8 @@ class Stats(C.Structure):
                 ("attempts_fp64", C.c_int64), ("pixels", C.c_int64),
                 ("integrate_ms", C.c_double), ("ssaa_ms", C.c_double), ("fxaa_ms", C.c_double),
                 ("total_ms", C.c_double), ("flops_fp64", C.c_double), ("flops_fp32", C.c_double),
-                ("ssaa_bytes", C.c_double), ("fxaa_bytes", C.c_double)]
+                ("ssaa_bytes", C.c_double), ("fxaa_bytes", C.c_double), ("adjoint_ms", C.c_double),
+                ("adjoint_steps", C.c_int64)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -10,7 +10,8 @@ import os
 from . import _abi as A
 
 LIB_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib")
-LIB_PATH = os.path.join(LIB_DIR, "libprr.so")
+# PRR_LIB selects another build of the same library (e.g. a different register budget)
+LIB_PATH = os.environ.get("PRR_LIB") or os.path.join(LIB_DIR, "libprr.so")
 
 _lib = None
 
@@ -41,6 +42,13 @@ def lib():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise RuntimeError(f"HIP library not built: {LIB_PATH} is missing (run __graft_entry__.build())")
+    # One HIP runtime per process: torch ships its own libamdhip64 (same soname).
+    # Loading torch first makes libprr.so bind to that runtime instead of
+    # bringing in a second copy, which would hide the GPU from torch.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(LIB_PATH)
     st = C.c_int
     _sig(L, "prr_abi_version", C.c_int, [])
@@ -86,9 +94,8 @@ def lib():
     _sig(L, "prr_debug_xu_from_state", st, [_i32, _d, _d, _d, _d, _vp, _vp, _vp])
     _sig(L, "prr_debug_init_ray", st, [_i32, _d, _d, _vp, _vp, _vp, _vp])
     _sig(L, "prr_debug_camera_ray", st, [C.POINTER(A.Frame), _i64, _vp])
-    if hasattr(L, "prr_adjoint"):
-        _sig(L, "prr_frame_from_orbit", st, [C.POINTER(A.Frame), C.POINTER(A.Orbit), C.POINTER(A.Frame)])
-        _sig(L, "prr_adjoint", st, [_vp, C.POINTER(A.Frame), C.POINTER(A.Orbit), _vp, _i32, _dp, _vp])
+    _sig(L, "prr_frame_from_orbit", st, [C.POINTER(A.Frame), C.POINTER(A.Orbit), C.POINTER(A.Frame)])
+    _sig(L, "prr_adjoint", st, [_vp, C.POINTER(A.Frame), C.POINTER(A.Orbit), _vp, _i32, _dp, _vp])
     if L.prr_abi_version() != A.PRR_ABI_VERSION:
         raise RuntimeError("libprr.so ABI version mismatch")
     _lib = L

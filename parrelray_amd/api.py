@@ -200,6 +200,24 @@ class Context:
             imgs[i].on_device = 1
         self._check(lib().prr_render(self.h, farr, n, imgs, None, C.c_void_p(stream) if stream else None))
 
+    def adjoint(self, tmpl, orbit, dL_dimg, img_out=None):
+        """Gradient of L w.r.t. (spin, r_cam, incl, sphere_phi) given dL/dI of the frame's image.
+
+        dL_dimg: HxWx3 float32 numpy array or CUDA torch tensor.  img_out (same
+        kind, optional) receives the forward image.  Returns a numpy array of 4.
+        """
+        grad = (C.c_double * 4)()
+        on_dev = not isinstance(dL_dimg, np.ndarray)
+        if on_dev:
+            ptr = C.c_void_p(dL_dimg.data_ptr())
+            out = C.c_void_p(img_out.data_ptr()) if img_out is not None else None
+        else:
+            dL_dimg = np.ascontiguousarray(dL_dimg, np.float32)
+            ptr = C.c_void_p(dL_dimg.ctypes.data)
+            out = C.c_void_p(img_out.ctypes.data) if img_out is not None else None
+        self._check(lib().prr_adjoint(self.h, C.byref(tmpl), C.byref(orbit), ptr, int(on_dev), grad, out))
+        return np.array(list(grad))
+
     def trace(self, x0u0):
         """Integrate explicit initial states (n x 8): returns xend, cls, steps, attempts."""
         x = np.ascontiguousarray(x0u0, dtype=np.float64).reshape(-1, 8)
@@ -238,6 +256,12 @@ class Graph:
 
     def __del__(self):
         self.close()
+
+
+def frame_from_orbit(tmpl, orbit):
+    out = A.Frame()
+    check(lib().prr_frame_from_orbit(C.byref(tmpl), C.byref(orbit), C.byref(out)))
+    return out
 
 
 # ---------------------------------------------------------------- host evaluation of the device formulas

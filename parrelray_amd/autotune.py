@@ -1,0 +1,110 @@
+"""Autotuner for the persistent integrator's launch shape.
+
+Searches workgroup size, resident workgroups per CU and the wave refill
+threshold on a short calibration workload (a scaled-down copy of the batch
+to be rendered), timing the integrator kernel with the library's own HIP
+events, and the adjoint's checkpoint interval on a small adjoint problem.
+Results are cached per (device, metric, solver, precision) in a JSON file.
+
+Every launch shape yields bit-identical images (the per-ray work does not
+depend on which lanes share a wave), so tuning only changes speed; the
+tests check that invariance.
+"""
+import json
+import os
+import time
+
+from . import _abi as A
+
+CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "autotune.json")
+
+BLOCKS = (64, 128, 256)
+BLOCKS_PER_CU = (1, 2, 3, 4, 6, 8)
+REFILLS = (1, 4, 8, 16, 32)
+CHECKPOINTS = (16, 32, 64, 128)
+
+
+def _key(device_name, metric, solver, precision):
+    return f"{device_name}|m{metric}|s{solver}|p{precision}"
+
+
+def _load():
+    try:
+        with open(CACHE) as fh:
+            return json.load(fh)
+    except (OSError, ValueError):
+        return {}
+
+
+def _save(d):
+    os.makedirs(os.path.dirname(CACHE), exist_ok=True)
+    tmp = CACHE + ".tmp"
+    with open(tmp, "w") as fh:
+        json.dump(d, fh, indent=1, sort_keys=True)
+    os.replace(tmp, CACHE)
+
+
+def _time_integrate(ctx, frames, reps=2):
+    best = float("inf")
+    for _ in range(reps):
+        ctx.render(frames, rgbf=False)
+        best = min(best, ctx.stats()["integrate_ms"])
+    return best
+
+
+def tune_launch(ctx, frames, metric, tracer, device_name="gpu0", budget_s=60.0, use_cache=True):
+    """Pick (block, blocks_per_cu, refill) for rendering `frames`; sets it on ctx and returns it."""
+    cache = _load()
+    key = _key(device_name, metric, tracer.solver, tracer.precision)
+    if use_cache and key in cache:
+        best = cache[key]
+        ctx.set_launch(best["block"], best["blocks_per_cu"], best["refill"], best.get("checkpoint", 64))
+        return best
+    start = time.perf_counter()
+    results = []
+    # coarse: occupancy-like dimension first at the default refill, then refill at the best shape
+    for block in BLOCKS:
+        for bpc in BLOCKS_PER_CU:
+            waves = block // 64 * bpc
+            if waves > 32:
+                continue
+            ctx.set_launch(block, bpc, 8, 64)
+            results.append((_time_integrate(ctx, frames), block, bpc, 8))
+            if time.perf_counter() - start > budget_s / 2:
+                break
+    results.sort()
+    _, block, bpc, _ = results[0]
+    for refill in REFILLS:
+        ctx.set_launch(block, bpc, refill, 64)
+        results.append((_time_integrate(ctx, frames), block, bpc, refill))
+        if time.perf_counter() - start > budget_s:
+            break
+    results.sort()
+    ms, block, bpc, refill = results[0]
+    best = {"block": block, "blocks_per_cu": bpc, "refill": refill, "checkpoint": 64, "integrate_ms": ms,
+            "trials": len(results)}
+    ctx.set_launch(block, bpc, refill, 64)
+    cache[key] = best
+    _save(cache)
+    return best
+
+
+def tune_checkpoint(ctx, tmpl, orbit, dLdI, device_name="gpu0", use_cache=True):
+    """Pick the adjoint's checkpoint interval (accepted steps between checkpoints)."""
+    cache = _load()
+    key = _key(device_name, A.PRR_KERR_BL, A.PRR_RKF45, -1) + "|adjoint"
+    cur = ctx.get_launch()
+    if use_cache and key in cache:
+        k = cache[key]["checkpoint"]
+    else:
+        timings = []
+        for k in CHECKPOINTS:
+            ctx.set_launch(cur["block"], cur["blocks_per_cu"], cur["refill"], k)
+            ctx.adjoint(tmpl, orbit, dLdI)
+            timings.append((ctx.stats()["adjoint_ms"], k))
+        timings.sort()
+        k = timings[0][1]
+        cache[key] = {"checkpoint": k, "adjoint_ms": timings[0][0], "trials": len(timings)}
+        _save(cache)
+    ctx.set_launch(cur["block"], cur["blocks_per_cu"], cur["refill"], k)
+    return k

@@ -50,6 +50,8 @@ struct prr_ctx {
     // device buffers, grown on demand
     DevBuf frames, counters, color, post, img_scratch, stage_rgbf, stage_rgb8, cls, endpoint, steps, attempts,
         state8, x0u0;
+    // adjoint
+    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_seed, adj_seg, adj_grad_ray, adj_grad;
     hipEvent_t ev[5] = {};
     prr_stats stats{};
 };
@@ -303,7 +305,9 @@ void prr_destroy(prr_ctx *ctx) {
         if (t.texels) (void)hipFree(const_cast<float4 *>(t.texels));
     DevBuf *bufs[] = {&ctx->frames, &ctx->counters, &ctx->color, &ctx->post, &ctx->img_scratch,
                       &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
-                      &ctx->attempts, &ctx->state8, &ctx->x0u0};
+                      &ctx->attempts, &ctx->state8, &ctx->x0u0, &ctx->adj_img, &ctx->adj_dldi,
+                      &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_seed, &ctx->adj_seg, &ctx->adj_grad_ray,
+                      &ctx->adj_grad};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -737,6 +741,130 @@ void prr_graph_destroy(prr_graph *g) {
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
     delete g;
+}
+
+// ---------------------------------------------------------------- adjoint
+prr_status prr_frame_from_orbit(const prr_frame *tmpl, const prr_orbit *o, prr_frame *out) {
+    if (!tmpl || !o || !out) return fail(nullptr, PRR_EINVAL, "NULL argument");
+    prr_frame f = *tmpl;
+    const double si = std::sin(o->incl), ci = std::cos(o->incl);
+    f.pos[0] = o->r_cam * si * std::cos(o->cam_phi);
+    f.pos[1] = o->r_cam * si * std::sin(o->cam_phi);
+    f.pos[2] = o->r_cam * ci;
+    f.alpha = o->cam_phi + M_PI;
+    f.beta = M_PI / 2.0 - o->incl;
+    f.spin = o->spin;
+    if (o->has_sphere) {
+        if (f.n_spheres < 1) f.n_spheres = 1;
+        f.spheres[0].c[0] = o->sphere_r * std::cos(o->sphere_phi);
+        f.spheres[0].c[1] = o->sphere_r * std::sin(o->sphere_phi);
+        f.spheres[0].c[2] = 0.0;
+        f.spheres[0].radius = o->sphere_radius;
+        f.spheres[0].tex = o->sphere_tex;
+    }
+    *out = f;
+    return PRR_OK;
+}
+
+prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orbit, const float *dL_dimg,
+                       int32_t on_device, double grad[4], float *img_out) {
+    if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
+    if (!tmpl || !orbit || !dL_dimg || !grad) return fail(ctx, PRR_EINVAL, "NULL argument");
+    if (ctx->metric != PRR_KERR_BL) return fail(ctx, PRR_EINVAL, "the adjoint supports the Kerr metric only");
+    if (ctx->tracer.solver != PRR_RKF45) return fail(ctx, PRR_EINVAL, "the adjoint needs the RKF45 solver");
+    if (tmpl->fxaa) return fail(ctx, PRR_EINVAL, "the adjoint differentiates the SSAA image: set fxaa = 0");
+    if (orbit->has_sphere && tmpl->n_spheres > 1)
+        return fail(ctx, PRR_EINVAL, "the adjoint's orbit sphere must be the frame's only sphere");
+    const int K = ctx->launch.checkpoint;
+    HIPTRY(ctx, hipSetDevice(ctx->device));
+    hipStream_t st = ctx->stream;
+    prr_frame f;
+    prr_status s = prr_frame_from_orbit(tmpl, orbit, &f);
+    if (s != PRR_OK) return s;
+    const int64_t npix = int64_t(f.width) * f.height;
+    if ((s = ensure(ctx, ctx->adj_img, 12 * size_t(npix))) != PRR_OK) return s;
+    prr_image im{nullptr, static_cast<float *>(ctx->adj_img.p), 1, 0};
+    RenderPlan P;
+    if ((s = plan_render(ctx, &f, 1, &im, P, false)) != PRR_OK) return s;
+    const int64_t T = P.total;
+    if ((s = ensure(ctx, ctx->steps, 4 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_off, 8 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_seed, 96 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_grad_ray, 32 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_grad, 32)) != PRR_OK) return s;
+    JobDev job = make_job(ctx, P);
+    job.out.steps = static_cast<int32_t *>(ctx->steps.p);
+    HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
+    // pass 1: the ordinary render (step counts, ray colours, SSAA image)
+    if ((s = enqueue_render(ctx, P, job, st, false)) != PRR_OK) return s;
+    std::vector<int32_t> steps(T);
+    HIPTRY(ctx, hipMemcpyAsync(steps.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
+    if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises
+    prr_stats fwd = ctx->stats;
+    // checkpoint offsets: ray i keeps checkpoints at steps 0, K, 2K, ... <= N_i
+    std::vector<int64_t> off(T);
+    int64_t nck = 0, swept = 0;
+    for (int64_t i = 0; i < T; ++i) {
+        off[i] = nck;
+        nck += steps[i] / K + 1;
+        swept += steps[i];
+    }
+    if ((s = ensure(ctx, ctx->adj_ckpt, 64 * size_t(nck))) != PRR_OK) return s;
+    const int grid = ctx->cus * blocks_per_cu(ctx);
+    const int64_t lanes = int64_t(grid) * ctx->launch.block;
+    if ((s = ensure(ctx, ctx->adj_seg, 48 * size_t(K) * size_t(lanes))) != PRR_OK) return s;
+    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_off.p, off.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
+    const float *dldi = dL_dimg;
+    if (!on_device) {
+        if ((s = ensure(ctx, ctx->adj_dldi, 12 * size_t(npix))) != PRR_OK) return s;
+        HIPTRY(ctx, hipMemcpyAsync(ctx->adj_dldi.p, dL_dimg, 12 * size_t(npix), hipMemcpyHostToDevice, st));
+        dldi = static_cast<const float *>(ctx->adj_dldi.p);
+    }
+    AdjDev ad{};
+    ad.K = K;
+    ad.steps = static_cast<const int32_t *>(ctx->steps.p);
+    ad.ckpt_off = static_cast<const int64_t *>(ctx->adj_off.p);
+    ad.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
+    ad.dLdI = dldi;
+    ad.img = static_cast<const float *>(ctx->adj_img.p);
+    ad.ray_color = static_cast<const float *>(ctx->color.p);
+    ad.seed = static_cast<double *>(ctx->adj_seed.p);
+    ad.seg = static_cast<double *>(ctx->adj_seg.p);
+    ad.seg_lanes = lanes;
+    ad.grad_ray = static_cast<double *>(ctx->adj_grad_ray.p);
+    ad.cam_phi = orbit->cam_phi;
+    ad.sphere_r = orbit->sphere_r;
+    ad.replay_mismatch = &static_cast<CountersDev *>(ctx->counters.p)->mismatch;
+    JobDev job2 = job;
+    job2.out = OutDev{};
+    HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
+    // pass 2: checkpointed forward, seeds
+    HIPTRY(ctx, launch_reset(job.ctr, st));
+    HIPTRY(ctx, launch_adjoint(2, ctx->tracer.precision, job2, ad, grid, ctx->launch.block, st));
+    // pass 3: replay + reverse sweep, one ray per lane
+    HIPTRY(ctx, launch_reset(job.ctr, st));
+    HIPTRY(ctx, launch_adjoint(3, ctx->tracer.precision, job2, ad, grid, ctx->launch.block, st));
+    HIPTRY(ctx, launch_grad_reduce(ad.grad_ray, T, static_cast<double *>(ctx->adj_grad.p), st));
+    HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
+    CountersDev c{};
+    HIPTRY(ctx, hipMemcpyAsync(&c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipMemcpyAsync(grad, ctx->adj_grad.p, 32, hipMemcpyDeviceToHost, st));
+    if (img_out)
+        HIPTRY(ctx, hipMemcpyAsync(img_out, ctx->adj_img.p, 12 * size_t(npix),
+                                   on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipStreamSynchronize(st));
+    float ms_all = 0.f, ms_adj = 0.f;
+    HIPTRY(ctx, hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[2]));
+    HIPTRY(ctx, hipEventElapsedTime(&ms_adj, ctx->ev[1], ctx->ev[2]));
+    ctx->stats = fwd;
+    ctx->stats.total_ms = ms_all;
+    ctx->stats.adjoint_ms = ms_adj;
+    ctx->stats.adjoint_steps = swept;
+    if (c.mismatch != 0)
+        return fail(ctx, PRR_EHIP, "adjoint replay did not reproduce %llu checkpoint segments", c.mismatch);
+    return PRR_OK;
 }
 
 } // extern "C"

@@ -331,11 +331,10 @@ PRR_HD void cart_to_sph(D3 p, D3 d, double xs[3], double us[3]) {
 
 // createRays_* (kernels/ssaa.cuh:36-48, 92-99) and rayDir_* (main/Camera.h:30-42)
 // for ray `local` of frame fr.
-PRR_HD D3 camera_ray(const FrameDev &fr, int64_t local) {
+PRR_HD void ray_uv(const FrameDev &fr, int64_t local, double &u, double &v) {
     const int64_t pix = local / fr.s;
     const int k = int(local - pix * fr.s);
     const int j = int(pix / fr.W), i = int(pix - int64_t(j) * fr.W);
-    double u, v;
     if (fr.n == 0) {
         u = (i + 0.5) / fr.W;
         v = (j + 0.5) / fr.H;
@@ -344,6 +343,11 @@ PRR_HD D3 camera_ray(const FrameDev &fr, int64_t local) {
         u = (i + double(p + 1) / (fr.n + 1)) / fr.W;
         v = (j + double(q + 1) / (fr.n + 1)) / fr.H;
     }
+}
+
+PRR_HD D3 camera_ray(const FrameDev &fr, int64_t local) {
+    double u, v;
+    ray_uv(fr, local, u, v);
     if (fr.projection == PRR_PROJ_SPHERICAL) {
         const double nu = (0.5 - v) * fr.vfov, mu = (0.5 - u) * fr.aspect * fr.vfov;
         double sb, cb, sa, ca;

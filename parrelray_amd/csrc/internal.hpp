@@ -65,7 +65,8 @@ struct CountersDev {
     unsigned long long accepted;
     unsigned long long attempts_fp64;
     unsigned long long rays;
-    unsigned long long pad[3];
+    unsigned long long mismatch; // adjoint: segments whose replay did not reproduce the checkpoint
+    unsigned long long pad[2];
 };
 
 struct JobDev {
@@ -78,6 +79,25 @@ struct JobDev {
     TexDev tex[PRR_MAX_TEXTURES];
     OutDev out;
     CountersDev *ctr;
+};
+
+// Kerr adjoint (prr_adjoint): checkpoints, seeds and per-ray gradients.
+struct AdjDev {
+    int32_t K;               // accepted steps between checkpoints
+    int32_t pad;
+    const int32_t *steps;    // accepted steps of every ray (from the first forward pass)
+    const int64_t *ckpt_off; // index of each ray's first checkpoint
+    double *ckpt;            // 8 doubles per checkpoint: y[5], h, lambda, attempts
+    const float *dLdI;       // dL/dI, W*H*3
+    const float *img;        // forward linear image, W*H*3
+    const float *ray_color;  // forward ray colours, 3 per ray
+    double *seed;            // 12 doubles per ray: dL/dz_N[8], dL/dz_{N-1}(r, mu, phi)[3], dL/dphi_s
+    double *seg;             // per-lane segment buffers: K * 6 doubles per lane, lane-minor
+    int64_t seg_lanes;       // lanes of the backward grid
+    double *grad_ray;        // 4 doubles per ray: dL/d(a, r_cam, incl, phi_s)
+    double cam_phi;          // fixed orbit parameters
+    double sphere_r;
+    unsigned long long *replay_mismatch; // checkpoints the replay did not reproduce bit for bit
 };
 
 // Post-processing of one frame (SSAA combine, FXAA, RGB8).

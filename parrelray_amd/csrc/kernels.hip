@@ -14,8 +14,17 @@
 //     termination, and only the colour (plus optional diagnostics) is written.
 #include <hip/hip_runtime.h>
 
+#include "adjoint.hpp"
 #include "geodesic.hpp"
 #include "kernels.hpp"
+
+// Register budget of the integrator: waves per SIMD the compiler must fit
+// (build-time knob, -DPRR_WAVES_PER_EU=n; unset = the compiler's choice).
+#ifdef PRR_WAVES_PER_EU
+#define PRR_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PRR_WAVES_PER_EU, 8)))
+#else
+#define PRR_OCCUPANCY
+#endif
 
 namespace prr {
 
@@ -283,7 +292,7 @@ __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
 }
 
 template <class Mt, int SOLVER, bool MIXED>
-__global__ void __launch_bounds__(256) k_integrate(JobDev job) {
+__global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
     const int lane = int(threadIdx.x & 63);
     Lane<Mt> L;
     bool active = false, exhausted = false;
@@ -349,6 +358,7 @@ __global__ void k_reset_counters(CountersDev *c) {
         c->accepted = 0;
         c->attempts_fp64 = 0;
         c->rays = 0;
+        c->mismatch = 0;
     }
 }
 
@@ -476,6 +486,296 @@ __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
         pf.rgb8[3 * p + 1] = to_rgb8(o.y);
         pf.rgb8[3 * p + 2] = to_rgb8(o.z);
     }
+}
+
+// ----------------------------------------------------------------- adjoint (Kerr)
+// Pass 1 is k_integrate itself (step counts and colours).  Pass 2 re-runs
+// the same integration, stores a checkpoint every K accepted steps and the
+// terminal seed dL/dz_N.  Pass 3 walks each ray's segments backwards:
+// replay K steps from a checkpoint (same lane_advance, so the same accepted
+// steps bit for bit), then the reverse sweep kerr_step_vjp over them.
+
+// Weights dL/dc of one ray's colour from dL/dI of its pixel through the SSAA
+// combine (kernels/ssaa.cuh:116-130): s = 1, I = clamp(c); s > 1,
+// I = sqrt(mean(clamp(c^2))).
+__device__ inline void color_weights(const JobDev &job, const AdjDev &ad, const FrameDev &fr, int64_t local,
+                                     const float3 c, double w[3]) {
+    const int64_t pix = local / fr.s;
+    const float cc[3] = {c.x, c.y, c.z};
+    for (int ch = 0; ch < 3; ++ch) {
+        const double g = double(ad.dLdI[3 * pix + ch]);
+        const double cv = double(cc[ch]);
+        if (fr.s == 1) {
+            w[ch] = (cv > 0.0 && cv < 1.0) ? g : 0.0;
+        } else {
+            const double I = double(ad.img[3 * pix + ch]);
+            w[ch] = (I > 0.0 && cv * cv < 1.0) ? g * cv / (fr.s * I) : 0.0;
+        }
+    }
+}
+
+// Terminal seed of a finished ray: dL/dz_N (8), dL/dz_{N-1} (r, mu, phi) and
+// dL/dphi_s, by forward-mode duals through the colour of its class.
+__device__ __noinline__ void adj_seed(const JobDev &job, const AdjDev &ad, const Lane<Kerr> &L, int cls,
+                                      const double yp[3]) {
+    double *s = ad.seed + 12 * L.rid;
+    for (int i = 0; i < 12; ++i) s[i] = 0.0;
+    if (cls != PRR_ESCAPE && cls != PRR_SPHERE) return;
+    const FrameDev &fr = job.frames[L.frame];
+    const float *rc = ad.ray_color + 3 * L.rid;
+    double w[3];
+    color_weights(job, ad, fr, L.rid - fr.ray_begin, make_float3(rc[0], rc[1], rc[2]), w);
+    if (w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0) return;
+    using D = Dual<8>;
+    D col[3];
+    if (cls == PRR_ESCAPE) {
+        D y[5];
+        for (int i = 0; i < 5; ++i) y[i] = D::var(L.y[i], i);
+        const V3<D> n = kerr_escape_dir_t<D>(y, D::var(L.cs.E, 5), D::var(L.cs.L, 6), D::var(L.cs.a, 7), L.cs.M);
+        sky_color_t<D>(job.tex[0], n, col);
+        for (int i = 0; i < 8; ++i) s[i] = w[0] * col[0].d[i] + w[1] * col[1].d[i] + w[2] * col[2].d[i];
+        return;
+    }
+    // SPHERE: directions 0-2 = (r, mu, phi) at step N-1, 3-5 = at step N, 6 = phi_s
+    D y0[5], y1[5];
+    for (int i = 0; i < 5; ++i) {
+        y0[i] = D(0.0);
+        y1[i] = D(L.y[i]);
+    }
+    for (int i = 0; i < 3; ++i) {
+        y0[i] = D::var(yp[i], i);
+        y1[i] = D::var(L.y[i], 3 + i);
+    }
+    const int k = L.hit;
+    V3<D> c = v3t<D>(D(fr.sph[k][0]), D(fr.sph[k][1]), D(fr.sph[k][2]));
+    if (k == 0) { // the orbit's sphere: centre (R_orbit cos phi_s, R_orbit sin phi_s, 0)
+        const double phs = atan2(fr.sph[0][1], fr.sph[0][0]);
+        D sp, cp;
+        sincos(D::var(phs, 6), &sp, &cp);
+        c = v3t<D>(cp * ad.sphere_r, sp * ad.sphere_r, D(0.0));
+    }
+    const int slot = fr.sph_tex[k];
+    sphere_hit_color_t<D>((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, kerr_cart_t<D>(y0),
+                          kerr_cart_t<D>(y1), c, fr.sph[k][3], col);
+    for (int i = 0; i < 3; ++i) {
+        s[8 + i] = w[0] * col[0].d[i] + w[1] * col[1].d[i] + w[2] * col[2].d[i];
+        s[i] = w[0] * col[0].d[3 + i] + w[1] * col[1].d[3 + i] + w[2] * col[2].d[3 + i];
+    }
+    s[11] = w[0] * col[0].d[6] + w[1] * col[1].d[6] + w[2] * col[2].d[6];
+}
+
+__device__ inline void ckpt_store(double *c, const Lane<Kerr> &L) {
+    for (int i = 0; i < 5; ++i) c[i] = L.y[i];
+    c[5] = L.h;
+    c[6] = L.lam;
+    c[7] = double(L.attempts);
+}
+
+__device__ inline void ckpt_load(const double *c, Lane<Kerr> &L, int32_t steps) {
+    for (int i = 0; i < 5; ++i) L.y[i] = c[i];
+    L.h = c[5];
+    L.lam = c[6];
+    L.attempts = int32_t(c[7]);
+    L.steps = steps;
+    L.P0 = cart_of<Kerr>(L.y);
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_adj_forward(JobDev job, AdjDev ad) {
+    const int lane = int(threadIdx.x & 63);
+    Lane<Kerr> L;
+    double yp[3] = {0.0, 0.0, 0.0};
+    bool active = false, exhausted = false;
+    while (true) {
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= job.refill) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&job.ctr->queue, (unsigned long long)nidle);
+            base = shfl_u64(base, leader);
+            if (base + (unsigned long long)nidle >= (unsigned long long)job.total) exhausted = true;
+            if (!active) {
+                const unsigned rank =
+                    __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
+                const int64_t id = int64_t(base) + rank;
+                if (id < job.total) {
+                    lane_init<Kerr>(job, L, id);
+                    active = true;
+                    ckpt_store(ad.ckpt + 8 * ad.ckpt_off[id], L);
+                    if (!finite_state<Kerr>(L.y)) {
+                        adj_seed(job, ad, L, PRR_NAN, yp);
+                        active = false;
+                    }
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (active) {
+            const double y0 = L.y[0], y1 = L.y[1], y2 = L.y[2];
+            const int32_t before = L.steps;
+            const int cls = lane_advance<Kerr, PRR_RKF45, MIXED>(job, L);
+            if (L.steps != before) {
+                yp[0] = y0;
+                yp[1] = y1;
+                yp[2] = y2;
+                if (L.steps % ad.K == 0 && cls < 0) ckpt_store(ad.ckpt + 8 * (ad.ckpt_off[L.rid] + L.steps / ad.K), L);
+            }
+            if (cls >= 0) {
+                adj_seed(job, ad, L, cls, yp);
+                active = false;
+            }
+        }
+    }
+}
+
+// Ray set-up Jacobian dz_0/d(a, r_cam, incl) contracted with lambda_0.
+__device__ __noinline__ void adj_init_grad(const JobDev &job, const AdjDev &ad, const Lane<Kerr> &L,
+                                           const double lam[8], double g[4]) {
+    const FrameDev &fr = job.frames[L.frame];
+    double u, v;
+    ray_uv(fr, L.rid - fr.ray_begin, u, v);
+    const double r_cam = sqrt(fr.pos[0] * fr.pos[0] + fr.pos[1] * fr.pos[1] + fr.pos[2] * fr.pos[2]);
+    const double incl = acos(fr.pos[2] / r_cam);
+    using D = Dual<3>;
+    V3<D> pos, d;
+    orbit_camera_ray<D>(D::var(r_cam, 1), D::var(incl, 2), ad.cam_phi, fr.tan_half, fr.aspect, u, v, pos, d);
+    D z[7];
+    kerr_init_t<D>(pos, d, fr.M, D::var(fr.a, 0), z);
+    for (int k = 0; k < 3; ++k) {
+        double acc = k == 0 ? lam[7] : 0.0; // a is itself part of z
+        for (int i = 0; i < 7; ++i) acc += lam[i] * z[i].d[k];
+        g[k] = acc;
+    }
+}
+
+template <bool MIXED>
+__global__ void __launch_bounds__(256) k_adj_backward(JobDev job, AdjDev ad) {
+    const int lane = int(threadIdx.x & 63);
+    const int64_t gl = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; // this lane's segment buffer
+    Lane<Kerr> L;
+    bool active = false, exhausted = false;
+    while (true) {
+        const unsigned long long idle = __ballot(!active);
+        const int nidle = __popcll(idle);
+        if (!exhausted && nidle >= 1) {
+            const int leader = __ffsll((long long)idle) - 1;
+            unsigned long long base = 0;
+            if (lane == leader) base = atomicAdd(&job.ctr->queue, (unsigned long long)nidle);
+            base = shfl_u64(base, leader);
+            if (base + (unsigned long long)nidle >= (unsigned long long)job.total) exhausted = true;
+            if (!active) {
+                const unsigned rank =
+                    __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
+                const int64_t id = int64_t(base) + rank;
+                if (id < job.total) {
+                    L.rid = id;
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active) == 0ull) break;
+        if (!active) continue;
+        // one whole ray per lane: replay + reverse every segment
+        const int64_t id = L.rid;
+        const double *sd = ad.seed + 12 * id;
+        double g[4] = {0.0, 0.0, 0.0, sd[11]};
+        bool any = false;
+        for (int i = 0; i < 11; ++i) any = any || sd[i] != 0.0;
+        const int32_t N = ad.steps[id];
+        if (any && N > 0) {
+            lane_init<Kerr>(job, L, id); // constants E, L and the frame
+            double lam[8];
+            for (int i = 0; i < 8; ++i) lam[i] = sd[i];
+            const int64_t c0 = ad.ckpt_off[id];
+            bool ok = true;
+            for (int32_t j = (N - 1) / ad.K; j >= 0; --j) {
+                ckpt_load(ad.ckpt + 8 * (c0 + j), L, j * ad.K);
+                const int32_t end = min((j + 1) * ad.K, N);
+                // replay: record (y_n, h_n) of every accepted step of the segment
+                while (L.steps < end) {
+                    double ys[5];
+                    for (int i = 0; i < 5; ++i) ys[i] = L.y[i];
+                    const double hs = L.h;
+                    const int32_t before = L.steps;
+                    const int cls = lane_advance<Kerr, PRR_RKF45, MIXED>(job, L);
+                    if (L.steps != before) {
+                        double *slot = ad.seg + (int64_t(before - j * ad.K) * 6) * ad.seg_lanes + gl;
+                        for (int i = 0; i < 5; ++i) slot[i * ad.seg_lanes] = ys[i];
+                        slot[5 * ad.seg_lanes] = hs;
+                    }
+                    // a ray only ends at step N; ending earlier means the replay diverged
+                    if (cls >= 0 && L.steps < end) {
+                        atomicAdd(ad.replay_mismatch, 1ull);
+                        break;
+                    }
+                }
+                if (L.steps < end) { // diverged: this ray contributes nothing
+                    ok = false;
+                    g[3] = 0.0;
+                    break;
+                }
+                if (end < N && end % ad.K == 0) { // the replay must land on the next checkpoint
+                    const double *nx = ad.ckpt + 8 * (c0 + j + 1);
+                    bool same = L.steps == end;
+                    for (int i = 0; i < 5; ++i) same = same && L.y[i] == nx[i];
+                    if (!same) atomicAdd(ad.replay_mismatch, 1ull);
+                }
+                // reverse sweep
+                for (int32_t n = end - 1; n >= j * ad.K; --n) {
+                    const double *slot = ad.seg + (int64_t(n - j * ad.K) * 6) * ad.seg_lanes + gl;
+                    double yn[5];
+                    for (int i = 0; i < 5; ++i) yn[i] = slot[i * ad.seg_lanes];
+                    kerr_step_vjp(yn, slot[5 * ad.seg_lanes], L.cs, lam);
+                    if (n == N - 1) {
+                        lam[0] += sd[8];
+                        lam[1] += sd[9];
+                        lam[2] += sd[10];
+                    }
+                }
+            }
+            if (ok) adj_init_grad(job, ad, L, lam, g);
+        }
+        double *out = ad.grad_ray + 4 * id;
+        for (int k = 0; k < 4; ++k) out[k] = g[k];
+        active = false;
+    }
+}
+
+// Deterministic sum of the per-ray gradients: one workgroup, fixed order.
+__global__ void __launch_bounds__(1024) k_grad_reduce(const double *grad_ray, int64_t n, double *out) {
+    __shared__ double part[4][1024];
+    const int t = int(threadIdx.x);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = t; i < n; i += 1024)
+        for (int k = 0; k < 4; ++k) acc[k] += grad_ray[4 * i + k];
+    for (int k = 0; k < 4; ++k) part[k][t] = acc[k];
+    __syncthreads();
+    for (int s = 512; s > 0; s >>= 1) {
+        if (t < s)
+            for (int k = 0; k < 4; ++k) part[k][t] += part[k][t + s];
+        __syncthreads();
+    }
+    if (t == 0)
+        for (int k = 0; k < 4; ++k) out[k] = part[k][0];
+}
+
+hipError_t launch_adjoint(int pass, int precision, const JobDev &job, const AdjDev &ad, int grid, int block,
+                          hipStream_t st) {
+    const bool mixed = precision == PRR_FP32_BULK;
+    if (pass == 2) {
+        if (mixed) hipLaunchKernelGGL((k_adj_forward<true>), dim3(grid), dim3(block), 0, st, job, ad);
+        else hipLaunchKernelGGL((k_adj_forward<false>), dim3(grid), dim3(block), 0, st, job, ad);
+    } else {
+        if (mixed) hipLaunchKernelGGL((k_adj_backward<true>), dim3(grid), dim3(block), 0, st, job, ad);
+        else hipLaunchKernelGGL((k_adj_backward<false>), dim3(grid), dim3(block), 0, st, job, ad);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_grad_reduce(const double *grad_ray, int64_t n, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(1024), 0, st, grad_ray, n, out);
+    return hipGetLastError();
 }
 
 // ----------------------------------------------------------------- dispatch

@@ -1,0 +1,75 @@
+"""Batched multi-frame rendering in one persistent launch, captured in a
+hipGraph, and the launch-shape autotuner."""
+import numpy as np
+import pytest
+
+from parrelray_amd import _abi as A
+from parrelray_amd import api
+from parrelray_amd import configs as CF
+
+pytestmark = pytest.mark.gpu
+
+
+def context():
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.0)
+    ctx.set_tracer(CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    ctx.set_texture(0, CF.sky_texture())
+    ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
+    return ctx
+
+
+def test_graph_replays_match_eager_render():
+    torch = pytest.importorskip("torch")
+    frames = CF.frames_c2(nframes=12, scale=0.06)
+    ctx = context()
+    eager = ctx.render(frames)
+    rgb8 = [torch.zeros((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
+    rgbf = [torch.zeros((f.height, f.width, 3), dtype=torch.float32, device="cuda") for f in frames]
+    g = api.Graph(ctx, frames, rgb8=rgb8, rgbf=rgbf)
+    for rep in range(3):
+        for t in rgbf:  # eager work on the same buffers between replays
+            t.fill_(-1.0)
+        torch.cuda.synchronize()
+        g.launch()
+        st = ctx.stats()
+        assert st["rays"] == sum(f.nrays for f in frames)
+        for e, b8, bf in zip(eager, rgb8, rgbf):
+            assert np.array_equal(b8.cpu().numpy(), e["rgb8"]), rep
+            assert np.array_equal(bf.cpu().numpy(), e["rgbf"]), rep
+    g.close()
+
+
+def test_autotuner_picks_a_shape_and_results_do_not_change(tmp_path, monkeypatch):
+    from parrelray_amd import autotune
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tune.json"))
+    frames = CF.frames_c2(nframes=5, scale=0.05)
+    ctx = context()
+    ref = ctx.render(frames)
+    best = autotune.tune_launch(ctx, frames, A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK),
+                                budget_s=20.0, use_cache=False)
+    assert best["block"] in autotune.BLOCKS and 1 <= best["refill"] <= 64
+    out = ctx.render(frames)
+    for a, b in zip(ref, out):
+        assert np.array_equal(a["rgbf"], b["rgbf"])
+    again = autotune.tune_launch(ctx, frames, A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    assert again["block"] == best["block"] and again["refill"] == best["refill"]
+
+
+def test_invalid_inputs_are_rejected_without_touching_the_gpu_state():
+    from parrelray_amd._lib import PrrError
+    ctx = context()
+    f = CF.frame_c1(32, 18)
+    bad = A.Frame.from_buffer_copy(f)
+    bad.samples = 5  # not a square for regular SSAA
+    with pytest.raises(PrrError):
+        ctx.render(bad)
+    bad = A.Frame.from_buffer_copy(f)
+    bad.spin = 1.5  # |a| > M
+    with pytest.raises(PrrError):
+        ctx.render(bad)
+    with pytest.raises(PrrError):
+        ctx.set_tracer(A.make_tracer(hmin=0.0))
+    # the context still works
+    out = ctx.render(f)[0]
+    assert out["rgb8"].shape == (18, 32, 3)
