@@ -44,6 +44,15 @@
 //       pixel, clamp-to-edge taps.
 //   D19 the lambda counter adds the *next* step size (kernels/solver.cuh:198,215):
 //       it adds the step actually taken.
+//   D21 RKF45 escape endpoint = the state at the r = r_escape crossing inside
+//       the last step (escape_locate), not the first state beyond it.
+//   D22 RKF45 sphere hits are located on the integrated path
+//       (sphere_locate: for steps whose chord passes within R + |chord|/2 of
+//       the centre, the path's end or its minimum distance, found by
+//       successive parabolic interpolation over sub-steps, brackets the
+//       entry, which Illinois regula falsi refines), not on the step chord;
+//       the hit normal is (P - c) / |P - c|.  Euler keeps the reference's
+//       chord and first-step semantics.
 //   D20 vel_cartTOspher's phi rate has the wrong sign
 //       (cudamaths/transform.cuh:29) and the spherical sky lookup mirrors
 //       phi back (main/base.h:84); both are corrected, which leaves images of
@@ -552,6 +561,257 @@ inline const Tableau &fehlberg() {
     return T;
 }
 
+// One RKF45 attempt of size h from y = (x, u): the 4th-order state y4 and
+// the scaled error norm (kernels/solver.cuh:142-184 with D2, D3).
+inline double rkf_attempt(const Scene &S, double M, double a, const double x[4], const double u[4], double h,
+                          double y4[8]) {
+    const Tableau &TB = fehlberg();
+    const prr_tracer &T = S.tr;
+    double y[8], k[6][8];
+    for (int i = 0; i < 4; ++i) {
+        y[i] = x[i];
+        y[4 + i] = u[i];
+    }
+    for (int st = 0; st < 6; ++st) {
+        double ys[8];
+        for (int i = 0; i < 8; ++i) {
+            double acc = y[i];
+            for (int j = 0; j < st; ++j) acc += TB.A[st][j] * k[j][i];
+            ys[i] = acc;
+        }
+        double du[4];
+        geod(S.metric, M, a, ys, ys + 4, du);
+        for (int i = 0; i < 4; ++i) {
+            k[st][i] = h * ys[4 + i]; // x' = u (D2)
+            k[st][4 + i] = h * du[i];
+        }
+    }
+    double errn = 0.0;
+    for (int i = 0; i < 8; ++i) {
+        double s4 = y[i], s5 = y[i];
+        for (int st = 0; st < 6; ++st) {
+            s4 += TB.b4[st] * k[st][i];
+            s5 += TB.b5[st] * k[st][i];
+        }
+        y4[i] = s4;
+        const double sc = T.atol + T.rtol * std::max(std::fabs(y[i]), std::fabs(s4));
+        errn = std::max(errn, std::fabs(s5 - s4) / sc);
+    }
+    return errn;
+}
+
+// Sub-step of size h (4th-order solution) for the event location below.
+inline void rk_substep(const Scene &S, double M, double a, const double x[4], const double u[4], double h,
+                       double xs[4], double us[4]) {
+    double y4[8];
+    rkf_attempt(S, M, a, x, u, h, y4);
+    for (int i = 0; i < 4; ++i) {
+        xs[i] = y4[i];
+        us[i] = y4[4 + i];
+    }
+}
+
+// D21: the escape endpoint is the state where r crosses r_escape inside the
+// last step (Newton on the step fraction theta over fresh sub-steps of
+// theta h from the step's start), not the first state beyond r_escape,
+// which depends on the step sequence.
+inline void escape_g(int kind, const double x[4], const double u[4], double r_esc, double &g, double &dg) {
+    if (spherical_coords(kind)) {
+        g = x[0] - r_esc;
+        dg = u[0];
+    } else {
+        const double r = std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
+        g = r - r_esc;
+        dg = (x[0] * u[0] + x[1] * u[1] + x[2] * u[2]) / r;
+    }
+}
+
+inline void escape_locate(const Scene &S, double M, double a, const double xo[4], const double uo[4],
+                          const double xn[4], const double un[4], double h, double xs[4], double us[4]) {
+    const double r_esc = S.tr.r_escape;
+    double go, gn, dg;
+    escape_g(S.metric, xo, uo, r_esc, go, dg);
+    escape_g(S.metric, xn, un, r_esc, gn, dg);
+    if (!(go < 0.0)) {
+        std::memcpy(xs, xo, 4 * sizeof(double));
+        std::memcpy(us, uo, 4 * sizeof(double));
+        return;
+    }
+    double th = std::min(std::max(go / (go - gn), 0.0), 1.0);
+    for (int it = 0; it < 8; ++it) {
+        rk_substep(S, M, a, xo, uo, th * h, xs, us);
+        double g;
+        escape_g(S.metric, xs, us, r_esc, g, dg);
+        if (std::fabs(g) <= 1e-13 * r_esc || !(dg * h != 0.0)) return;
+        th = std::min(std::max(th - g / (dg * h), 0.0), 1.0);
+    }
+    rk_substep(S, M, a, xo, uo, th * h, xs, us);
+}
+
+// D22: a sphere hit is the first point of the integrated path inside the
+// sphere, not the chord's: for a step whose chord passes within R + |chord|/2
+// of the centre, the path is sampled at the chord's closest point; if that
+// point is inside, the entry in [0, t_c] is found by Illinois regula falsi on
+// |P(theta) - c|^2 - R^2 over sub-steps.  Returns the sphere index or -1.
+inline int sphere_locate(const Scene &S, double M, double a, const std::vector<prr_sphere> &sph,
+                         const double xo[4], const double uo[4], const double xn[4], const double un[4], double h,
+                         double xs[4], double us[4]) {
+    int best = -1;
+    double theta = 2.0;
+    const V3 P0 = cart_of(S.metric, xo), P1 = cart_of(S.metric, xn), dP = P1 - P0;
+    const double A = dot(dP, dP);
+    for (size_t k = 0; k < sph.size(); ++k) {
+        const V3 c = v3(sph[k].c[0], sph[k].c[1], sph[k].c[2]);
+        const double R2 = sph[k].radius * sph[k].radius;
+        const V3 m = P0 - c;
+        const double G0 = dot(m, m) - R2;
+        if (G0 <= 0.0) {
+            if (0.0 < theta) {
+                theta = 0.0;
+                best = int(k);
+                std::memcpy(xs, xo, 4 * sizeof(double));
+                std::memcpy(us, uo, 4 * sizeof(double));
+            }
+            continue;
+        }
+        double tc = A > 0.0 ? -dot(m, dP) / A : 1.0;
+        tc = std::min(std::max(tc, 0.0), 1.0);
+        if (norm(m + tc * dP) > sph[k].radius + 0.5 * std::sqrt(A)) continue; // screen
+        auto G = [&](const double *xx) {
+            const V3 q = cart_of(S.metric, xx) - c;
+            return dot(q, q) - R2;
+        };
+        // the path's entry: a bracket [0, hi] with G(0) > 0 >= G(hi)
+        double xt[4], ut[4], hi, Ghi;
+        const double G1 = G(xn);
+        if (G1 <= 0.0) {
+            hi = 1.0;
+            Ghi = G1;
+            std::memcpy(xt, xn, sizeof xt);
+            std::memcpy(ut, un, sizeof ut);
+        } else {
+            // interior dip: sample at the chord's closest point, then refine the
+            // path's minimum by successive parabolic interpolation
+            double t1 = A > 0.0 ? -dot(m, dP) / A : 0.5;
+            t1 = std::min(std::max(t1, 0.02), 0.98);
+            double ta = 0.0, Ga = G0, tb = 1.0, Gb = G1;
+            rk_substep(S, M, a, xo, uo, t1 * h, xt, ut);
+            double G1s = G(xt);
+            for (int it = 0; it < 4 && G1s > 0.0; ++it) {
+                const double p = (t1 - ta) * (G1s - Gb), q = (t1 - tb) * (G1s - Ga);
+                const double den = 2.0 * (p - q);
+                if (!(den != 0.0)) break;
+                const double tv = t1 - ((t1 - ta) * p - (t1 - tb) * q) / den;
+                if (!(tv > ta && tv < tb) || std::fabs(tv - t1) < 1e-9) break;
+                double xv[4], uv[4];
+                rk_substep(S, M, a, xo, uo, tv * h, xv, uv);
+                const double Gv = G(xv);
+                if (Gv <= 0.0) {
+                    t1 = tv;
+                    G1s = Gv;
+                    std::memcpy(xt, xv, sizeof xt);
+                    std::memcpy(ut, uv, sizeof ut);
+                    break;
+                }
+                if (Gv < G1s) {
+                    if (tv < t1) {
+                        tb = t1;
+                        Gb = G1s;
+                    } else {
+                        ta = t1;
+                        Ga = G1s;
+                    }
+                    t1 = tv;
+                    G1s = Gv;
+                } else if (tv < t1) {
+                    ta = tv;
+                    Ga = Gv;
+                } else {
+                    tb = tv;
+                    Gb = Gv;
+                }
+            }
+            if (G1s > 0.0) continue; // the path stays outside: a miss
+            hi = t1;
+            Ghi = G1s;
+        }
+        if (hi >= theta) continue;
+        double lo = 0.0, Glo = G0;
+        int side = 0;
+        double xb[4], ub[4];
+        std::memcpy(xb, xt, sizeof xb);
+        std::memcpy(ub, ut, sizeof ub);
+        const double gtol = 1e-14 * (R2 + dot(c, c));
+        for (int it = 0; it < 60 && std::fabs(Ghi) > gtol && hi - lo > 1e-15; ++it) {
+            double t = (lo * Ghi - hi * Glo) / (Ghi - Glo);
+            if (!(t > lo && t < hi)) t = 0.5 * (lo + hi);
+            rk_substep(S, M, a, xo, uo, t * h, xt, ut);
+            const double Gt = G(xt);
+            if (Gt > 0.0) {
+                lo = t;
+                Glo = Gt;
+                if (side == -1) Ghi *= 0.5;
+                side = -1;
+            } else {
+                hi = t;
+                Ghi = Gt;
+                std::memcpy(xb, xt, sizeof xb);
+                std::memcpy(ub, ut, sizeof ub);
+                if (side == 1) Glo *= 0.5;
+                side = 1;
+            }
+        }
+        if (hi < theta) {
+            theta = hi;
+            best = int(k);
+            std::memcpy(xs, xb, 4 * sizeof(double));
+            std::memcpy(us, ub, 4 * sizeof(double));
+        }
+    }
+    return best;
+}
+
+// Termination tests after an accepted RKF45 step (xo, uo) -> (x, u) of size
+// h: path-located sphere hit (D22), spherical wraps, horizon, located escape
+// (D21).  Returns true when the ray stops.
+inline bool rkf_events(const Scene &S, double M, double a, const std::vector<prr_sphere> &sph, const double xo[4],
+                       const double uo[4], double h, double x[4], double u[4], RayResult &R) {
+    if (!sph.empty()) {
+        double xs[4], us[4];
+        const int k = sphere_locate(S, M, a, sph, xo, uo, x, u, h, xs, us);
+        if (k >= 0) {
+            std::memcpy(x, xs, sizeof xs);
+            std::memcpy(u, us, sizeof us);
+            const V3 c = v3(sph[k].c[0], sph[k].c[1], sph[k].c[2]);
+            const V3 n = normalize(cart_of(S.metric, x) - c);
+            R.cls = PRR_SPHERE;
+            R.endpoint = n;
+            const int slot = sph[k].tex;
+            R.color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &S.tex[slot] : nullptr, n);
+            return true;
+        }
+    }
+    const double r = radius_of(S.metric, x);
+    if (r >= S.tr.r_escape) {
+        double xs[4], us[4];
+        escape_locate(S, M, a, xo, uo, x, u, h, xs, us);
+        std::memcpy(x, xs, sizeof xs);
+        std::memcpy(u, us, sizeof us);
+        if (S.metric == PRR_SCHW_SPH) sph_post_step(x, u);
+        R.cls = PRR_ESCAPE;
+        return true;
+    }
+    if (S.metric == PRR_SCHW_SPH) sph_post_step(x, u);
+    if (S.metric != PRR_FLAT) {
+        const double rh = horizon_radius(S.metric, M, a);
+        if (r <= rh * (1.0 + S.tr.horizon_eps_rel) + S.tr.horizon_eps_abs) {
+            R.cls = PRR_HORIZON;
+            return true;
+        }
+    }
+    return false;
+}
+
 // trace_Euler (kernels/solver.cuh:68-116) and trace_RKF (:142-220) on the
 // 8-component state y = (x, u), with the resolutions listed at the top.
 inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr_sphere> &sph, const double x0[4],
@@ -594,41 +854,12 @@ inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr
         return finish(PRR_STEP_CAP);
     }
     // RKF45
-    const Tableau &TB = fehlberg();
     double h = T.h;
     while (R.steps < T.max_steps && R.lambda < T.lambda_max) {
         if (R.attempts >= T.max_attempts) return finish(PRR_STEP_CAP);
         R.attempts++;
-        double y[8], k[6][8];
-        for (int i = 0; i < 4; ++i) {
-            y[i] = x[i];
-            y[4 + i] = u[i];
-        }
-        for (int st = 0; st < 6; ++st) {
-            double ys[8];
-            for (int i = 0; i < 8; ++i) {
-                double acc = y[i];
-                for (int j = 0; j < st; ++j) acc += TB.A[st][j] * k[j][i];
-                ys[i] = acc;
-            }
-            double du[4];
-            geod(S.metric, M, a, ys, ys + 4, du);
-            for (int i = 0; i < 4; ++i) {
-                k[st][i] = h * ys[4 + i]; // x' = u (D2)
-                k[st][4 + i] = h * du[i];
-            }
-        }
-        double y4[8], errn = 0.0;
-        for (int i = 0; i < 8; ++i) {
-            double s4 = y[i], s5 = y[i];
-            for (int st = 0; st < 6; ++st) {
-                s4 += TB.b4[st] * k[st][i];
-                s5 += TB.b5[st] * k[st][i];
-            }
-            y4[i] = s4;
-            const double sc = T.atol + T.rtol * std::max(std::fabs(y[i]), std::fabs(s4));
-            errn = std::max(errn, std::fabs(s5 - s4) / sc);
-        }
+        double y4[8];
+        const double errn = rkf_attempt(S, M, a, x, u, h, y4);
         // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
         double hn = 0.9 * h * std::pow(errn, -0.2);
         hn = std::fmin(std::fmax(hn, T.hmin), T.hmax);
@@ -636,8 +867,9 @@ inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr
             h = hn;
             continue;
         }
-        double xo[4];
+        double xo[4], uo[4];
         std::memcpy(xo, x, sizeof x);
+        std::memcpy(uo, u, sizeof u);
         for (int i = 0; i < 4; ++i) {
             x[i] = y4[i];
             u[i] = y4[4 + i];
@@ -645,8 +877,9 @@ inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr
         if (!finite8(x, u)) return finish(PRR_NAN);
         R.steps++;
         R.lambda += h;
+        const double hs = h;
         h = hn;
-        if (after_step(S, M, a, sph, xo, x, u, R)) {
+        if (rkf_events(S, M, a, sph, xo, uo, hs, x, u, R)) {
             std::memcpy(R.x, x, sizeof x);
             std::memcpy(R.u, u, sizeof u);
             return R;

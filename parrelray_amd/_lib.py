@@ -84,16 +84,22 @@ def lib():
     _sig(L, "prr_texture_checker", st, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64])
     _sig(L, "prr_texture_value_noise", st, [_vp, C.c_int, C.c_int, C.c_int, C.c_uint64])
     _sig(L, "prr_debug_state_size", C.c_int, [_i32])
-    _sig(L, "prr_debug_rhs", st, [_i32, _d, _d, _d, _d, _vp, _i32, _vp])
-    _sig(L, "prr_debug_rhs_vjp", st, [_i32, _d, _d, _d, _d, _vp, _vp, _vp, _vp])
+    _sig(L, "prr_debug_rhs", st, [_i32, _d, _d, _d, _d, _d, _vp, _i32, _vp])
+    _sig(L, "prr_debug_kerr_delta_rhs", st, [_d, _d, _d, _d, _d, _vp, _vp, _vp])
+    _sig(L, "prr_debug_kerr_stages", st, [_d, _d, _d, _d, _d, _vp, _d, _i32, _vp, _vp])
+    _sig(L, "prr_debug_rhs_vjp", st, [_d, _d, _d, _d, _d, _vp, _vp, _vp, _vp])
     _sig(L, "prr_debug_orbit_init", st, [_d, _d, _d, _d, _d, _d, _d, _d, _d, _vp, _vp])
-    _sig(L, "prr_debug_kerr_step", st, [_d, _d, _d, _d, _vp, _d, _vp])
-    _sig(L, "prr_debug_kerr_step_vjp", st, [_d, _d, _d, _d, _vp, _d, _vp])
-    _sig(L, "prr_debug_escape_color", st, [_d, _d, _d, _d, _vp, _vp, C.c_int, C.c_int, _vp, _vp])
+    _sig(L, "prr_debug_kerr_step", st, [_d, _d, _d, _d, _d, _vp, _d, _vp])
+    _sig(L, "prr_debug_kerr_step_vjp", st, [_d, _d, _d, _d, _d, _vp, _d, _vp])
+    _sig(L, "prr_debug_escape_color", st, [_d, _d, _d, _d, _d, _vp, _vp, C.c_int, C.c_int, _vp, _vp])
     _sig(L, "prr_debug_state_from_xu", st, [_i32, _d, _d, _vp, _vp, _vp, _vp])
-    _sig(L, "prr_debug_xu_from_state", st, [_i32, _d, _d, _d, _d, _vp, _vp, _vp])
+    _sig(L, "prr_debug_xu_from_state", st, [_i32, _d, _d, _d, _d, _d, _vp, _vp, _vp])
     _sig(L, "prr_debug_init_ray", st, [_i32, _d, _d, _vp, _vp, _vp, _vp])
     _sig(L, "prr_debug_camera_ray", st, [C.POINTER(A.Frame), _i64, _vp])
+    _sig(L, "prr_debug_trace_rays", st, [_i32, C.POINTER(A.Tracer), C.POINTER(A.Frame), _vp, _vp, C.c_int, _vp, _vp,
+                                         _vp, _vp, _vp])
+    _sig(L, "prr_debug_trace_log", st, [_i32, C.POINTER(A.Tracer), C.POINTER(A.Frame), _i64, C.c_int, _vp,
+                                        C.POINTER(C.c_int)])
     _sig(L, "prr_frame_from_orbit", st, [C.POINTER(A.Frame), C.POINTER(A.Orbit), C.POINTER(A.Frame)])
     _sig(L, "prr_adjoint", st, [_vp, C.POINTER(A.Frame), C.POINTER(A.Orbit), _vp, _i32, _dp, _vp])
     if L.prr_abi_version() != A.PRR_ABI_VERSION:

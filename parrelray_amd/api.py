@@ -269,71 +269,128 @@ def state_size(metric):
     return lib().prr_debug_state_size(metric)
 
 
-def rhs(metric, y, M=1.0, a=0.0, E=0.0, L=0.0, fp32=False):
-    y = np.ascontiguousarray(y, dtype=np.float64)
-    f = np.empty(state_size(metric))
-    check(lib().prr_debug_rhs(metric, M, a, E, L, _p(y), int(fp32), _p(f)))
+def _f64(a, n=None):
+    a = np.ascontiguousarray(a, np.float64)
+    if n is not None and a.size != n:
+        raise ValueError(f"expected {n} values, got {a.size}")
+    return a
+
+
+def rhs(metric, y, M=1.0, a=0.0, E=0.0, L=0.0, Q=0.0, fp32=False):
+    """Right-hand side f(y) of the integrator state (host evaluation of the device formula)."""
+    y = _f64(y, state_size(metric))
+    f = np.empty_like(y)
+    check(lib().prr_debug_rhs(metric, M, a, E, L, Q, _p(y), int(fp32), _p(f)))
     return f
 
 
-def rhs_vjp(metric, y, fb, M=1.0, a=0.0, E=0.0, L=0.0):
-    """(dL/dy, dL/d(E, L, a)) for dL/df = fb (Kerr)."""
-    y, fb = np.ascontiguousarray(y, np.float64), np.ascontiguousarray(fb, np.float64)
-    yb, cb = np.zeros(5), np.zeros(3)
-    check(lib().prr_debug_rhs_vjp(metric, M, a, E, L, _p(y), _p(fb), _p(yb), _p(cb)))
+def kerr_delta_rhs(y0, d, a, E, L, Q, M=1.0):
+    """FP32-bulk stage function g = f(y0 + d) - f(y0) (Kerr)."""
+    y0, d = _f64(y0, 6), _f64(d, 6)
+    g = np.empty(6)
+    check(lib().prr_debug_kerr_delta_rhs(M, a, E, L, Q, _p(y0), _p(d), _p(g)))
+    return g
+
+
+def kerr_stages(y, h, a, E, L, Q, fp32=False, M=1.0):
+    """One Kerr RKF45 attempt: (d4, e), FP64 or FP32-bulk stages."""
+    y = _f64(y, 6)
+    d4, e = np.empty(6), np.empty(6)
+    check(lib().prr_debug_kerr_stages(M, a, E, L, Q, _p(y), h, int(fp32), _p(d4), _p(e)))
+    return d4, e
+
+
+def rhs_vjp(y, fb, a, E, L, Q, M=1.0):
+    """Kerr: (fb . df/dy [6], fb . df/d(E, L, Q, a) [4])."""
+    y, fb = _f64(y, 6), _f64(fb, 6)
+    yb, cb = np.zeros(6), np.zeros(4)
+    check(lib().prr_debug_rhs_vjp(M, a, E, L, Q, _p(y), _p(fb), _p(yb), _p(cb)))
     return yb, cb
 
 
 def orbit_init(a, r_cam, incl, cam_phi, vfov, aspect, u, v, M=1.0):
-    """Kerr ray set-up z = (r, mu, phi, p_r, p_mu, E, L) and dz/d(a, r_cam, incl)."""
-    z, dz = np.empty(7), np.empty((7, 3))
+    """z = (u, du/ds, mu, dmu/ds, phi, E, L, Q) of an orbiting camera's ray and dz/d(a, r_cam, incl)."""
+    z, dz = np.empty(8), np.empty(24)
     check(lib().prr_debug_orbit_init(M, a, r_cam, incl, cam_phi, vfov, aspect, u, v, _p(z), _p(dz)))
-    return z, dz
+    return z, dz.reshape(8, 3)
 
 
-def kerr_step(y, h, a, E, L, M=1.0):
-    y = np.ascontiguousarray(y, np.float64)
-    out = np.empty(5)
-    check(lib().prr_debug_kerr_step(M, a, E, L, _p(y), h, _p(out)))
+def kerr_step(y, h, a, E, L, Q, M=1.0):
+    y = _f64(y, 6)
+    out = np.empty(6)
+    check(lib().prr_debug_kerr_step(M, a, E, L, Q, _p(y), h, _p(out)))
     return out
 
 
-def kerr_step_vjp(y, h, lam, a, E, L, M=1.0):
-    y = np.ascontiguousarray(y, np.float64)
-    lam = np.array(lam, np.float64)
-    check(lib().prr_debug_kerr_step_vjp(M, a, E, L, _p(y), h, _p(lam)))
+def kerr_step_vjp(y, h, lam, a, E, L, Q, M=1.0):
+    """lam[9] = adjoint of (y'[0..4], E, L, Q, a) -> adjoint of (y[0..4], E, L, Q, a)."""
+    y = _f64(y, 6)
+    lam = _f64(lam, 9).copy()
+    check(lib().prr_debug_kerr_step_vjp(M, a, E, L, Q, _p(y), h, _p(lam)))
     return lam
 
 
-def escape_color(y, a, E, L, rgb8, M=1.0):
-    y = np.ascontiguousarray(y, np.float64)
+def escape_color(y, a, E, L, Q, rgb8, M=1.0):
+    """Bilinear sky colour of an escaping Kerr state y[0..4] and d(colour)/d(y[0..4], E, L, Q, a)."""
+    y = _f64(y[:5], 5)
     rgb8 = np.ascontiguousarray(rgb8, np.uint8)
-    c, dc = np.empty(3), np.empty((3, 8))
-    check(lib().prr_debug_escape_color(M, a, E, L, _p(y), _p(rgb8), rgb8.shape[1], rgb8.shape[0], _p(c), _p(dc)))
-    return c, dc
+    c, dc = np.empty(3), np.empty(27)
+    check(lib().prr_debug_escape_color(M, a, E, L, Q, _p(y), _p(rgb8), rgb8.shape[1], rgb8.shape[0], _p(c), _p(dc)))
+    return c, dc.reshape(3, 9)
 
 
 def state_from_xu(metric, x, u, M=1.0, a=0.0):
-    x, u = np.ascontiguousarray(x, np.float64), np.ascontiguousarray(u, np.float64)
+    """(x, u) -> (integrator state, (E, L, Q))."""
+    x, u = _f64(x, 4), _f64(u, 4)
     y = np.empty(state_size(metric))
-    EL = np.empty(2)
-    check(lib().prr_debug_state_from_xu(metric, M, a, _p(x), _p(u), _p(y), _p(EL)))
-    return y, EL
+    ELQ = np.empty(3)
+    check(lib().prr_debug_state_from_xu(metric, M, a, _p(x), _p(u), _p(y), _p(ELQ)))
+    return y, ELQ
 
 
-def xu_from_state(metric, y, M=1.0, a=0.0, E=0.0, L=0.0):
-    y = np.ascontiguousarray(y, np.float64)
+def xu_from_state(metric, y, M=1.0, a=0.0, E=0.0, L=0.0, Q=0.0):
+    y = _f64(y, state_size(metric))
     x, u = np.empty(4), np.empty(4)
-    check(lib().prr_debug_xu_from_state(metric, M, a, E, L, _p(y), _p(x), _p(u)))
+    check(lib().prr_debug_xu_from_state(metric, M, a, E, L, Q, _p(y), _p(x), _p(u)))
     return x, u
 
 
 def init_ray(metric, pos, d, M=1.0, a=0.0):
-    pos, d = np.ascontiguousarray(pos, np.float64), np.ascontiguousarray(d, np.float64)
+    """initRays for a camera position and direction: (state, (E, L, Q))."""
+    pos, d = _f64(pos, 3), _f64(d, 3)
     y = np.empty(state_size(metric))
-    EL = np.empty(2)
-    check(lib().prr_debug_init_ray(metric, M, a, _p(pos), _p(d), _p(y), _p(EL)))
-    return y, EL
+    ELQ = np.empty(3)
+    check(lib().prr_debug_init_ray(metric, M, a, _p(pos), _p(d), _p(y), _p(ELQ)))
+    return y, ELQ
+
+
+def trace_rays_host(metric, tracer, frame, ids=None, x0u0=None):
+    """The device integrator's per-ray code run on the host (untextured)."""
+    if x0u0 is not None:
+        x0u0 = _f64(x0u0)
+        n = x0u0.size // 8
+        idp = None
+    else:
+        ids = np.ascontiguousarray(ids, np.int64)
+        n = ids.size
+        idp = _p(ids)
+    cls = np.empty(n, np.int32)
+    steps = np.empty(n, np.int32)
+    att = np.empty(n, np.int32)
+    ep = np.empty((n, 3))
+    st8 = np.empty((n, 8))
+    check(lib().prr_debug_trace_rays(metric, C.byref(tracer), C.byref(frame),
+                                     _p(x0u0) if x0u0 is not None else None, idp, n, _p(cls), _p(ep), _p(steps),
+                                     _p(att), _p(st8)))
+    return dict(cls=cls, endpoint=ep, steps=steps, attempts=att, xend=st8)
+
+
+def trace_log(metric, tracer, frame, ray, max_n=100000):
+    """Per accepted step of one ray on the host: (steps, attempts, next h, FP64 attempts, state[8]) rows."""
+    log = np.empty((max_n, 12))
+    n = C.c_int(0)
+    check(lib().prr_debug_trace_log(metric, C.byref(tracer), C.byref(frame), int(ray), max_n, _p(log), C.byref(n)))
+    return log[:n.value]
 
 
 def camera_ray(frame, local):

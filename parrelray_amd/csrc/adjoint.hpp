@@ -2,19 +2,24 @@
 // respect to spin a, camera radius r_cam, inclination i and the sphere's
 // orbital angle phi_s), shared by the adjoint kernels and host evaluation.
 //
-// Per ray the render is the map  p -> z_0 = init(p) -> z_N = Phi_{N-1} o ...
-// o Phi_0 (z_0) -> colour c(z_N [, z_{N-1}], phi_s), with z = (r, mu, phi,
-// p_r, p_mu, E, L, a) (the constants ride along with zero derivative) and
-// Phi_n one accepted RKF45 step with its step size frozen.  The adjoint
-//   lambda_N = dL/dc . dc/dz_N,  lambda_n = lambda_{n+1} . dPhi_n/dz_n,
-//   dL/dp   = lambda_0 . dz_0/dp  (+ the direct dc/dphi_s term)
+// Per ray the render is the map  p -> (y_0, c) = init(p) -> y_{N-1} =
+// Phi_{N-2} o ... o Phi_0 (y_0) -> y* = Phi(y_{N-1}; tau) -> colour C(y*, c,
+// phi_s), with y the Mino-time state (u, u', mu, mu', phi), c = (E, L, Q, a)
+// the per-ray constants, Phi_n one accepted RKF45 step with its size frozen,
+// and tau the located sub-step of the terminal event G(y*, phi_s) = 0
+// (escape: G = 1/r_esc - u; sphere: G = |P(y*) - c_s|^2 - R^2).  The adjoint
+//   lambda* = w . dC/dy*,  kappa = (lambda* . f*) / (grad G . f*),
+//   lambda_{N-1} = (lambda* - kappa grad G) . dPhi(.; tau)/dy,
+//   lambda_n = lambda_{n+1} . dPhi_n/dy_n,
+//   dL/dp = lambda_0 . dy_0/dp + lambda_c . dc/dp (+ dC/dphi_s - kappa dG/dphi_s)
+// (f* = f(y*) = dPhi/dtau; the kappa term is the implicit derivative of tau)
 // uses a hand-written reverse sweep through the six RKF stages
 // (kerr_step_vjp, built on Kerr::rhs_vjp) and forward-mode duals for the
-// once-per-ray Jacobians dz_0/dp and dc/dz_N.
+// once-per-ray Jacobians dy_0/dp and dC/dy*.
 #pragma once
 
 #include "dual.hpp"
-#include "geodesic.hpp"
+#include "lane.hpp"
 
 namespace prr {
 
@@ -54,8 +59,8 @@ PRR_HD void orbit_camera_ray(const T &r_cam, const T &incl, double cam_phi, doub
     d = vnormalize(vadd(F, vadd(vscale(cr, R), vscale(cu, U))));
 }
 
-// init_state<Kerr> (geodesic.hpp) in scalar type T: z = (r, mu, phi, p_r, p_mu, E, L).
-template <class T> PRR_HD void kerr_init_t(const V3<T> &p, const V3<T> &d, double M, const T &a, T z[7]) {
+// init_state<Kerr> in scalar type T: z = (u, u', mu, mu', phi, E, L, Q).
+template <class T> PRR_HD void kerr_init_t(const V3<T> &p, const V3<T> &d, double M, const T &a, T z[8]) {
     const T x2y2 = p.x * p.x + p.y * p.y, r2 = x2y2 + p.z * p.z, r = sqrt(r2);
     const T mu = p.z / r;
     const T ph = atan2(p.y, p.x);
@@ -75,28 +80,30 @@ template <class T> PRR_HD void kerr_init_t(const V3<T> &p, const V3<T> &d, doubl
     const T rt = sqrt(B * B - T(4.0) * g_tt * C);
     const T u1 = (T(0.0) - B + rt) / (T(2.0) * g_tt), u2 = (T(0.0) - B - rt) / (T(2.0) * g_tt);
     const T ut = value(u1) >= value(u2) ? u1 : u2;
-    z[0] = r;
-    z[1] = mu;
-    z[2] = ph;
-    z[3] = g_rr * ur;
-    z[4] = T(0.0) - g_thth * uth / sqrt(s2);
-    z[5] = T(0.0) - (g_tt * ut + g_tph * uph);
-    z[6] = g_phph * uph + g_tph * ut;
+    const T E = T(0.0) - (g_tt * ut + g_tph * uph);
+    const T L = g_phph * uph + g_tph * ut;
+    const T pth = g_thth * uth;
+    z[0] = T(1.0) / r;
+    z[1] = T(0.0) - Sig * ur / r2;
+    z[2] = mu;
+    z[3] = T(0.0) - sqrt(s2) * pth;
+    z[4] = ph;
+    z[5] = E;
+    z[6] = L;
+    z[7] = pth * pth + mu * mu * (L * L / s2 - a2 * E * E);
 }
 
-// escape_dir<Kerr> in scalar type T from z = (y[5], E, L) and spin a.
-template <class T> PRR_HD V3<T> kerr_escape_dir_t(const T y[5], const T &E, const T &L, const T &a, double M) {
-    T f[5];
-    Kerr::rhs<T>(y, Consts<T>{T(M), a, E, L}, f);
-    const T ct = y[1], st = sqrt(T(1.0) - ct * ct);
+// escape_dir<Kerr> in scalar type T from the state y[5] and (E, L, Q, a).
+template <class T>
+PRR_HD V3<T> kerr_escape_dir_t(const T y[5], const T &E, const T &L, const T &Q, const T &a, double M) {
+    T yy[6] = {y[0], y[1], y[2], y[3], y[4], T(0.0)}, f[6];
+    Kerr::rhs<T>(yy, Kerr::coef_of<T>(M, a, E, L, Q), f);
+    const T u = y[0], ct = y[2], st = sqrt(T(1.0) - ct * ct);
     T sp, cp;
-    sincos(y[2], &sp, &cp);
-    const T r = y[0];
-    const T Sig = r * r + a * a * ct * ct;
-    const T thdot = T(0.0) - st * y[4] / Sig;
+    sincos(y[4], &sp, &cp);
     const V3<T> er = v3t<T>(st * cp, st * sp, ct), et = v3t<T>(ct * cp, ct * sp, T(0.0) - st);
     const V3<T> ep = v3t<T>(T(0.0) - sp, cp, T(0.0));
-    return vnormalize(vadd(vscale(f[0], er), vadd(vscale(r * thdot, et), vscale(r * st * f[2], ep))));
+    return vnormalize(vadd(vscale(T(0.0) - y[1], er), vadd(vscale(T(0.0) - u * y[3] / st, et), vscale(u * st * f[4], ep))));
 }
 
 // Bilinear lookup (same cell selection as `bilinear`) in scalar type T:
@@ -143,19 +150,10 @@ template <class T> PRR_HD void sky_color_t(const TexDev &t, const V3<T> &n, T ou
     bilinear_t<T>(t, ph / (2.0 * pi), th / pi, out);
 }
 
-// Sphere colour at the hit of the chord P0 -> P1 with sphere (c, R)
-// (chord_hit + sphere_color) in scalar type T; the entry point (C > 0).
-template <class T>
-PRR_HD void sphere_hit_color_t(const TexDev *t, const V3<T> &P0, const V3<T> &P1, const V3<T> &c, double R,
-                               T out[3]) {
+// Sphere::getColor (scene/sphere.h:59-64) at the outward unit normal n, in
+// scalar type T (bilinear texture filter).
+template <class T> PRR_HD void sphere_color_t(const TexDev *t, const V3<T> &n, T out[3]) {
     const double pi = 3.14159265358979323846;
-    const V3<T> d = vsub(P1, P0), m = vsub(P0, c);
-    const T A = vdot(d, d), B = vdot(m, d), C = vdot(m, m) - R * R;
-    T s;
-    if (value(C) <= 0.0) s = T(0.0);
-    else s = (T(0.0) - B - sqrt(B * B - A * C)) / A;
-    const V3<T> H = vadd(P0, vscale(s, d));
-    const V3<T> n = vscale(1.0 / R, vsub(H, c));
     const T th = acos(fmin(fmax(n.z, -1.0), 1.0));
     const T ph = atan2(n.y, n.x) + pi;
     T col[3];
@@ -166,89 +164,77 @@ PRR_HD void sphere_hit_color_t(const TexDev *t, const V3<T> &P0, const V3<T> &P1
     for (int k = 0; k < 3; ++k) out[k] = col[k] * shade;
 }
 
+// cart_of<Kerr> in scalar type T.
 template <class T> PRR_HD V3<T> kerr_cart_t(const T y[5]) {
-    const T ct = y[1], st = sqrt(T(1.0) - ct * ct);
+    const T r = T(1.0) / y[0], ct = y[2], st = sqrt(T(1.0) - ct * ct);
     T sp, cp;
-    sincos(y[2], &sp, &cp);
-    return v3t<T>(y[0] * st * cp, y[0] * st * sp, y[0] * ct);
+    sincos(y[4], &sp, &cp);
+    return v3t<T>(r * st * cp, r * st * sp, r * ct);
 }
 
-// Fehlberg tableau in host/device constant form.
-struct FehlbergT {
-    static PRR_HD double A(int s, int j) {
-        const double a[6][5] = {{0, 0, 0, 0, 0},
-                                {1.0 / 4.0, 0, 0, 0, 0},
-                                {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0},
-                                {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0},
-                                {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0},
-                                {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0}};
-        return a[s][j];
-    }
-    static PRR_HD double B4(int s) {
-        const double b[6] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -1.0 / 5.0, 0.0};
-        return b[s];
-    }
-};
-
 // One accepted RKF45 step y -> y + sum b4_s k_s (FP64, step size h frozen).
-PRR_HD void kerr_step(const double y[5], double h, const Consts<double> &cs, double out[5]) {
-    double k[6][5];
+PRR_HD void kerr_step(const double y[6], double h, const Consts<double> &cs, double out[6]) {
+    const KerrCoef<double> kc = Kerr::coef<double>(cs);
+    double k[6][6];
     for (int s = 0; s < 6; ++s) {
-        double ys[5];
-        for (int i = 0; i < 5; ++i) {
+        double ys[6];
+        for (int i = 0; i < 6; ++i) {
             double acc = y[i];
-            for (int j = 0; j < s; ++j) acc += FehlbergT::A(s, j) * k[j][i];
+            for (int j = 0; j < s; ++j) acc += Fehl::a(s, j) * k[j][i];
             ys[i] = acc;
         }
-        double f[5];
-        Kerr::rhs<double>(ys, cs, f);
-        for (int i = 0; i < 5; ++i) k[s][i] = h * f[i];
+        double f[6];
+        Kerr::rhs<double>(ys, kc, f);
+        for (int i = 0; i < 6; ++i) k[s][i] = h * f[i];
     }
-    for (int i = 0; i < 5; ++i) {
+    for (int i = 0; i < 6; ++i) {
         double acc = 0.0;
-        for (int s = 0; s < 6; ++s) acc += FehlbergT::B4(s) * k[s][i];
+        for (int s = 0; s < 6; ++s) acc += Fehl::b4(s) * k[s][i];
         out[i] = y[i] + acc;
     }
 }
 
-// lam[8] = adjoint of (y_{n+1}[5], E, L, a) on entry, of (y_n, E, L, a) on
-// exit, through one accepted step of size h (reverse sweep over the stages).
-PRR_HD void kerr_step_vjp(const double y[5], double h, const Consts<double> &cs, double lam[8]) {
+// lam[9] = adjoint of (y_{n+1}[0..4], E, L, Q, a) on entry, of (y_n, E, L,
+// Q, a) on exit, through one accepted step of size h (reverse sweep over the
+// stages; lambda = y[5] feeds nothing back and is not carried).
+PRR_HD void kerr_step_vjp(const double y[6], double h, const Consts<double> &cs, double lam[9]) {
+    const KerrCoef<double> kc = Kerr::coef<double>(cs);
     double k[6][5];
     for (int s = 0; s < 6; ++s) {
-        double ys[5];
+        double ys[6];
         for (int i = 0; i < 5; ++i) {
             double acc = y[i];
-            for (int j = 0; j < s; ++j) acc += FehlbergT::A(s, j) * k[j][i];
+            for (int j = 0; j < s; ++j) acc += Fehl::a(s, j) * k[j][i];
             ys[i] = acc;
         }
-        double f[5];
-        Kerr::rhs<double>(ys, cs, f);
+        ys[5] = 0.0;
+        double f[6];
+        Kerr::rhs<double>(ys, kc, f);
         for (int i = 0; i < 5; ++i) k[s][i] = h * f[i];
     }
     double lk[6][5];
     for (int s = 0; s < 6; ++s)
-        for (int i = 0; i < 5; ++i) lk[s][i] = FehlbergT::B4(s) * lam[i];
+        for (int i = 0; i < 5; ++i) lk[s][i] = Fehl::b4(s) * lam[i];
     double ly[5] = {lam[0], lam[1], lam[2], lam[3], lam[4]};
-    double lc[3] = {lam[5], lam[6], lam[7]};
+    KerrCoef<double> kb{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int s = 5; s >= 0; --s) {
-        double ys[5], fb[5], yb[5] = {0, 0, 0, 0, 0};
+        double ys[6], fb[6], yb[6] = {0, 0, 0, 0, 0, 0};
         for (int i = 0; i < 5; ++i) {
             double acc = y[i];
-            for (int j = 0; j < s; ++j) acc += FehlbergT::A(s, j) * k[j][i];
+            for (int j = 0; j < s; ++j) acc += Fehl::a(s, j) * k[j][i];
             ys[i] = acc;
             fb[i] = h * lk[s][i];
         }
-        Kerr::rhs_vjp(ys, cs, fb, yb, lc);
+        ys[5] = 0.0;
+        fb[5] = 0.0;
+        Kerr::rhs_vjp(ys, kc, fb, yb, kb);
         for (int i = 0; i < 5; ++i) {
             ly[i] += yb[i];
-            for (int j = 0; j < s; ++j) lk[j][i] += FehlbergT::A(s, j) * yb[i];
+            for (int j = 0; j < s; ++j) lk[j][i] += Fehl::a(s, j) * yb[i];
         }
     }
     for (int i = 0; i < 5; ++i) lam[i] = ly[i];
-    lam[5] = lc[0];
-    lam[6] = lc[1];
-    lam[7] = lc[2];
+    Kerr::coef_vjp(cs, kb, lam + 5);
 }
 
 } // namespace prr

@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "frame_host.hpp"
 #include "kernels.hpp"
 #include "prr_host.hpp"
 
@@ -51,7 +52,7 @@ struct prr_ctx {
     DevBuf frames, counters, color, post, img_scratch, stage_rgbf, stage_rgb8, cls, endpoint, steps, attempts,
         state8, x0u0;
     // adjoint
-    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_seed, adj_seg, adj_grad_ray, adj_grad;
+    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_event, adj_seg, adj_grad_ray, adj_grad;
     hipEvent_t ev[5] = {};
     prr_stats stats{};
 };
@@ -104,12 +105,6 @@ void release(DevBuf &b) {
     b.bytes = 0;
 }
 
-double horizon_radius(int metric, double M, double a) {
-    if (metric == PRR_FLAT) return 0.0;
-    if (metric == PRR_KERR_BL) return M + std::sqrt(std::max(M * M - a * a, 0.0));
-    return 2.0 * M;
-}
-
 prr_status validate_tracer(prr_ctx *ctx, const prr_tracer &t) {
     if (t.solver != PRR_EULER && t.solver != PRR_RKF45)
         return fail(ctx, PRR_EINVAL, "unknown solver %d", t.solver);
@@ -135,22 +130,6 @@ prr_status validate_tracer(prr_ctx *ctx, const prr_tracer &t) {
     return PRR_OK;
 }
 
-TracerDev tracer_dev(const prr_tracer &t) {
-    TracerDev d{};
-    d.solver = t.solver;
-    d.precision = t.precision;
-    d.h = t.h;
-    d.hmin = t.hmin;
-    d.hmax = t.hmax;
-    d.atol = t.atol;
-    d.rtol = t.rtol;
-    d.r_escape = t.r_escape;
-    d.lambda_max = t.lambda_max;
-    d.r_fp64 = t.r_fp64;
-    d.max_steps = t.max_steps;
-    d.max_attempts = t.max_attempts;
-    return d;
-}
 
 prr_status validate_frame(prr_ctx *ctx, const prr_frame &f, int idx) {
     if (f.width <= 0 || f.height <= 0 || f.width > 65536 || f.height > 65536)
@@ -184,40 +163,6 @@ prr_status validate_frame(prr_ctx *ctx, const prr_frame &f, int idx) {
     return PRR_OK;
 }
 
-FrameDev frame_dev(const prr_ctx *ctx, const prr_frame &f, int64_t begin) {
-    FrameDev d{};
-    for (int i = 0; i < 3; ++i) d.pos[i] = f.pos[i];
-    double F[3], U[3], R[3];
-    camera_basis(f.alpha, f.beta, F, U, R);
-    for (int i = 0; i < 3; ++i) {
-        d.F[i] = F[i];
-        d.U[i] = U[i];
-        d.R[i] = R[i];
-    }
-    d.alpha = f.alpha;
-    d.beta = f.beta;
-    d.vfov = f.vfov;
-    d.tan_half = std::tan(f.vfov / 2.0);
-    d.aspect = double(f.width) / double(f.height);
-    d.W = f.width;
-    d.H = f.height;
-    d.s = f.samples;
-    d.n = (f.ssaa == PRR_SSAA_NONE || f.samples == 1) ? 0 : int(std::lround(std::sqrt(double(f.samples))));
-    d.projection = f.projection;
-    d.n_spheres = f.n_spheres;
-    d.M = f.mass;
-    d.a = ctx->metric == PRR_KERR_BL ? f.spin : 0.0;
-    const double rh = horizon_radius(ctx->metric, f.mass, d.a);
-    d.r_hor = ctx->metric == PRR_FLAT ? -1.0
-                                      : rh * (1.0 + ctx->tracer.horizon_eps_rel) + ctx->tracer.horizon_eps_abs;
-    d.ray_begin = begin;
-    for (int k = 0; k < f.n_spheres; ++k) {
-        for (int c = 0; c < 3; ++c) d.sph[k][c] = f.spheres[k].c[c];
-        d.sph[k][3] = f.spheres[k].radius;
-        d.sph_tex[k] = f.spheres[k].tex;
-    }
-    return d;
-}
 
 int blocks_per_cu(const prr_ctx *ctx) {
     if (ctx->launch.blocks_per_cu > 0) return ctx->launch.blocks_per_cu;
@@ -234,24 +179,36 @@ int blocks_per_cu(const prr_ctx *ctx) {
 // FP32 in the FP32-bulk form, the rest always in FP64.  Euler: F_rhs + 2 per
 // component.
 struct FlopModel {
-    double rhs;
-    int n;
+    double rhs;       // one right-hand side
+    int n;            // state components
+    double delta_rhs; // one FP32 difference right-hand side (Kerr)
+    double prep;      // FP64 difference coefficients (Kerr)
 };
 FlopModel flop_model(int metric) {
     switch (metric) {
-    case PRR_FLAT: return {0.0, 8};
-    case PRR_SCHW_KS_CART: return {49.0, 8};
-    case PRR_SCHW_SPH: return {44.0, 8};
-    default: return {73.0, 5};
+    case PRR_FLAT: return {0.0, 8, 0.0, 0.0};
+    case PRR_SCHW_KS_CART: return {49.0, 8, 0.0, 0.0};
+    case PRR_SCHW_SPH: return {44.0, 8, 0.0, 0.0};
+    default: return {35.0, 6, 43.0, 45.0};
     }
 }
 
+// Algorithmic flops per attempt (a division counts as one flop).  RKF45:
+// six right-hand sides, the stage sums (15 multiply-adds per component plus
+// the h scaling), and the 4th-order / error sums, norm and controller.
+// Kerr's FP32-bulk attempt: the FP64 stage 1 and difference coefficients,
+// then five FP32 difference stages with their stage sums.
 void fill_flops(prr_ctx *ctx, const CountersDev &c) {
     const FlopModel fm = flop_model(ctx->metric);
     const bool rkf = ctx->tracer.solver == PRR_RKF45;
     const double f_stage = rkf ? 6.0 * fm.rhs + 36.0 * fm.n : fm.rhs;
     const double f_rest = rkf ? 26.0 * fm.n + 6.0 : 2.0 * fm.n;
     const double a64 = double(c.attempts_fp64), a32 = double(c.attempts) - double(c.attempts_fp64);
+    if (rkf && ctx->metric == PRR_KERR_BL) {
+        ctx->stats.flops_fp64 = a64 * (f_stage + f_rest) + a32 * (fm.rhs + fm.prep + f_rest + 2.0 * fm.n);
+        ctx->stats.flops_fp32 = a32 * (5.0 * fm.delta_rhs + 58.0 * fm.n);
+        return;
+    }
     ctx->stats.flops_fp64 = a64 * (f_stage + f_rest) + a32 * f_rest;
     ctx->stats.flops_fp32 = a32 * f_stage;
 }
@@ -306,7 +263,7 @@ void prr_destroy(prr_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->frames, &ctx->counters, &ctx->color, &ctx->post, &ctx->img_scratch,
                       &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
                       &ctx->attempts, &ctx->state8, &ctx->x0u0, &ctx->adj_img, &ctx->adj_dldi,
-                      &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_seed, &ctx->adj_seg, &ctx->adj_grad_ray,
+                      &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_event, &ctx->adj_seg, &ctx->adj_grad_ray,
                       &ctx->adj_grad};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : ctx->ev)
@@ -423,7 +380,7 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
     for (int i = 0; i < n; ++i) {
         prr_status st = validate_frame(ctx, frames[i], i);
         if (st != PRR_OK) return st;
-        P.fd[i] = frame_dev(ctx, frames[i], P.total);
+        P.fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total);
         const int64_t np = int64_t(frames[i].width) * frames[i].height;
         P.total += np * frames[i].samples;
         P.pixels += np;
@@ -487,12 +444,15 @@ JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
 }
 
 // Enqueue reset -> integrate -> SSAA -> FXAA with timing events.
-prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events) {
+// adjoint_pass1: integrate through the adjoint's shared attempt function.
+prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events,
+                          bool adjoint_pass1 = false) {
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     HIPTRY(ctx, launch_reset(job.ctr, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
     const int grid = ctx->cus * blocks_per_cu(ctx);
-    HIPTRY(ctx, launch_integrate(ctx->metric, job, grid, ctx->launch.block, st));
+    if (adjoint_pass1) HIPTRY(ctx, launch_adjoint(1, ctx->tracer.precision, job, AdjDev{}, grid, ctx->launch.block, st));
+    else HIPTRY(ctx, launch_integrate(ctx->metric, job, grid, ctx->launch.block, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     HIPTRY(ctx, launch_ssaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_pix, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[3], st));
@@ -625,7 +585,7 @@ prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int3
     f.n_spheres = int(ctx->spheres.size());
     for (int k = 0; k < f.n_spheres; ++k) f.spheres[k] = ctx->spheres[k];
     RenderPlan P;
-    P.fd.push_back(frame_dev(ctx, f, 0));
+    P.fd.push_back(frame_dev(ctx->metric, ctx->tracer, f, 0));
     P.total = n;
     prr_status s;
     if ((s = ensure(ctx, ctx->frames, sizeof(FrameDev))) != PRR_OK) return s;
@@ -788,27 +748,30 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if ((s = plan_render(ctx, &f, 1, &im, P, false)) != PRR_OK) return s;
     const int64_t T = P.total;
     if ((s = ensure(ctx, ctx->steps, 4 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->cls, size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_off, 8 * size_t(T))) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->adj_seed, 96 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_event, sizeof(EventDev) * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad_ray, 32 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad, 32)) != PRR_OK) return s;
     JobDev job = make_job(ctx, P);
     job.out.steps = static_cast<int32_t *>(ctx->steps.p);
+    job.out.cls = static_cast<int8_t *>(ctx->cls.p);
     HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
-    // pass 1: the ordinary render (step counts, ray colours, SSAA image)
-    if ((s = enqueue_render(ctx, P, job, st, false)) != PRR_OK) return s;
+    // pass 1: the render through the adjoint's attempt function (step counts,
+    // classes, ray colours, SSAA image)
+    if ((s = enqueue_render(ctx, P, job, st, false, true)) != PRR_OK) return s;
     std::vector<int32_t> steps(T);
     HIPTRY(ctx, hipMemcpyAsync(steps.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
     if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises
     prr_stats fwd = ctx->stats;
-    // checkpoint offsets: ray i keeps checkpoints at steps 0, K, 2K, ... <= N_i
+    // checkpoint offsets: ray i keeps checkpoints at steps 0, K, 2K, ... < N_i
     std::vector<int64_t> off(T);
     int64_t nck = 0, swept = 0;
     for (int64_t i = 0; i < T; ++i) {
         off[i] = nck;
-        nck += steps[i] / K + 1;
+        nck += (steps[i] > 0 ? (steps[i] - 1) / K : 0) + 1;
         swept += steps[i];
     }
     if ((s = ensure(ctx, ctx->adj_ckpt, 64 * size_t(nck))) != PRR_OK) return s;
@@ -825,12 +788,12 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     AdjDev ad{};
     ad.K = K;
     ad.steps = static_cast<const int32_t *>(ctx->steps.p);
+    ad.cls = static_cast<const int8_t *>(ctx->cls.p);
     ad.ckpt_off = static_cast<const int64_t *>(ctx->adj_off.p);
     ad.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
     ad.dLdI = dldi;
     ad.img = static_cast<const float *>(ctx->adj_img.p);
     ad.ray_color = static_cast<const float *>(ctx->color.p);
-    ad.seed = static_cast<double *>(ctx->adj_seed.p);
     ad.seg = static_cast<double *>(ctx->adj_seg.p);
     ad.seg_lanes = lanes;
     ad.grad_ray = static_cast<double *>(ctx->adj_grad_ray.p);
@@ -839,8 +802,9 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     ad.replay_mismatch = &static_cast<CountersDev *>(ctx->counters.p)->mismatch;
     JobDev job2 = job;
     job2.out = OutDev{};
+    job2.out.event = static_cast<EventDev *>(ctx->adj_event.p);
     HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    // pass 2: checkpointed forward, seeds
+    // pass 2: checkpointed forward
     HIPTRY(ctx, launch_reset(job.ctr, st));
     HIPTRY(ctx, launch_adjoint(2, ctx->tracer.precision, job2, ad, grid, ctx->launch.block, st));
     // pass 3: replay + reverse sweep, one ray per lane

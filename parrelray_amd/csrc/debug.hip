@@ -1,13 +1,14 @@
-// debug.hip -- host instantiations of the device metric plug-ins
-// (include/prr_debug.h).  Compiled as HIP so that geodesic.hpp's
-// __host__ __device__ templates are available to host code.
+// debug.hip -- host instantiations of the device metric plug-ins and of the
+// per-ray integrator (include/prr_debug.h).  Compiled as HIP so that the
+// __host__ __device__ templates of geodesic.hpp and lane.hpp are available
+// to host code.
 #include <hip/hip_runtime.h>
 
 #include <vector>
 
 #include "../../include/prr_debug.h"
 #include "adjoint.hpp"
-#include "geodesic.hpp"
+#include "frame_host.hpp"
 #include "prr_host.hpp"
 
 using namespace prr;
@@ -30,44 +31,66 @@ extern "C" {
 
 int prr_debug_state_size(int32_t metric) { return metric == PRR_KERR_BL ? Kerr::N : 8; }
 
-prr_status prr_debug_rhs(int32_t metric, double M, double a, double E, double L, const double *y, int32_t fp32,
-                         double *f) {
+prr_status prr_debug_rhs(int32_t metric, double M, double a, double E, double L, double Q, const double *y,
+                         int32_t fp32, double *f) {
     if (!y || !f) return PRR_EINVAL;
+    const Consts<double> cs{M, a, E, L, Q};
     return with_metric(metric, [&](auto mt) {
         using Mt = decltype(mt);
         if (fp32) {
             float yf[Mt::N], ff[Mt::N];
             for (int i = 0; i < Mt::N; ++i) yf[i] = float(y[i]);
-            Mt::template rhs<float>(yf, Consts<float>{float(M), float(a), float(E), float(L)}, ff);
+            Mt::template rhs<float>(yf, Mt::template coef<float>(cs), ff);
             for (int i = 0; i < Mt::N; ++i) f[i] = ff[i];
         } else {
-            Mt::template rhs<double>(y, Consts<double>{M, a, E, L}, f);
+            Mt::template rhs<double>(y, Mt::template coef<double>(cs), f);
         }
         return PRR_OK;
     });
 }
 
-prr_status prr_debug_rhs_vjp(int32_t metric, double M, double a, double E, double L, const double *y,
-                             const double *fb, double *yb, double *cb) {
-    if (metric != PRR_KERR_BL) {
-        set_global_error("rhs_vjp: Kerr only");
-        return PRR_EINVAL;
-    }
+prr_status prr_debug_kerr_delta_rhs(double M, double a, double E, double L, double Q, const double *y0,
+                                    const double *d, double *g) {
+    if (!y0 || !d || !g) return PRR_EINVAL;
+    double F[6];
+    KerrDelta P;
+    Kerr::delta_prep(y0, Kerr::coef<double>(Consts<double>{M, a, E, L, Q}), F, P);
+    float df[6], gf[6];
+    for (int i = 0; i < 6; ++i) df[i] = float(d[i]);
+    Kerr::delta_rhs(df, P, gf);
+    for (int i = 0; i < 6; ++i) g[i] = gf[i];
+    return PRR_OK;
+}
+
+prr_status prr_debug_kerr_stages(double M, double a, double E, double L, double Q, const double *y, double h,
+                                 int32_t fp32, double *d4, double *e) {
+    if (!y || !d4 || !e) return PRR_EINVAL;
+    const Consts<double> cs{M, a, E, L, Q};
+    if (fp32) kerr_stages_fp32(y, h, cs, d4, e);
+    else rkf_stages<Kerr, double>(y, h, cs, d4, e);
+    return PRR_OK;
+}
+
+prr_status prr_debug_rhs_vjp(double M, double a, double E, double L, double Q, const double *y, const double *fb,
+                             double *yb, double *cb) {
     if (!y || !fb || !yb || !cb) return PRR_EINVAL;
-    Kerr::rhs_vjp(y, Consts<double>{M, a, E, L}, fb, yb, cb);
+    const Consts<double> cs{M, a, E, L, Q};
+    KerrCoef<double> kb{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
+    Kerr::rhs_vjp(y, Kerr::coef<double>(cs), fb, yb, kb);
+    Kerr::coef_vjp(cs, kb, cb);
     return PRR_OK;
 }
 
 prr_status prr_debug_orbit_init(double M, double a, double r_cam, double incl, double cam_phi, double vfov,
-                                double aspect, double u, double v, double z[7], double dz[21]) {
+                                double aspect, double u, double v, double z[8], double dz[24]) {
     if (!z) return PRR_EINVAL;
     using D = Dual<3>;
     const D da = D::var(a, 0), dr = D::var(r_cam, 1), di = D::var(incl, 2);
     V3<D> pos, d;
     orbit_camera_ray<D>(dr, di, cam_phi, std::tan(vfov / 2.0), aspect, u, v, pos, d);
-    D zz[7];
+    D zz[8];
     kerr_init_t<D>(pos, d, M, da, zz);
-    for (int i = 0; i < 7; ++i) {
+    for (int i = 0; i < 8; ++i) {
         z[i] = zz[i].v;
         if (dz)
             for (int j = 0; j < 3; ++j) dz[3 * i + j] = zz[i].d[j];
@@ -75,77 +98,80 @@ prr_status prr_debug_orbit_init(double M, double a, double r_cam, double incl, d
     return PRR_OK;
 }
 
-prr_status prr_debug_kerr_step(double M, double a, double E, double L, const double y[5], double h, double out[5]) {
+prr_status prr_debug_kerr_step(double M, double a, double E, double L, double Q, const double y[6], double h,
+                               double out[6]) {
     if (!y || !out) return PRR_EINVAL;
-    kerr_step(y, h, Consts<double>{M, a, E, L}, out);
+    kerr_step(y, h, Consts<double>{M, a, E, L, Q}, out);
     return PRR_OK;
 }
 
-prr_status prr_debug_kerr_step_vjp(double M, double a, double E, double L, const double y[5], double h,
-                                   double lam[8]) {
+prr_status prr_debug_kerr_step_vjp(double M, double a, double E, double L, double Q, const double y[6], double h,
+                                   double lam[9]) {
     if (!y || !lam) return PRR_EINVAL;
-    kerr_step_vjp(y, h, Consts<double>{M, a, E, L}, lam);
+    kerr_step_vjp(y, h, Consts<double>{M, a, E, L, Q}, lam);
     return PRR_OK;
 }
 
-prr_status prr_debug_escape_color(double M, double a, double E, double L, const double y[5], const uint8_t *rgb8,
-                                  int w, int h, double c[3], double dc[24]) {
+prr_status prr_debug_escape_color(double M, double a, double E, double L, double Q, const double y[5],
+                                  const uint8_t *rgb8, int w, int h, double c[3], double dc[27]) {
     if (!y || !rgb8 || !c || w < 2 || h < 2) return PRR_EINVAL;
     std::vector<float4> tx(size_t(w) * h);
     for (size_t i = 0; i < tx.size(); ++i)
         tx[i] = make_float4(float(rgb8[3 * i]) / 255.0f, float(rgb8[3 * i + 1]) / 255.0f,
                             float(rgb8[3 * i + 2]) / 255.0f, 1.0f);
     TexDev t{tx.data(), w, h, PRR_FILTER_BILINEAR, 0};
-    using D = Dual<8>;
+    using D = Dual<9>;
     D yy[5];
     for (int i = 0; i < 5; ++i) yy[i] = D::var(y[i], i);
-    const D dE = D::var(E, 5), dL = D::var(L, 6), da = D::var(a, 7);
-    const V3<D> n = kerr_escape_dir_t<D>(yy, dE, dL, da, M);
+    const D dE = D::var(E, 5), dL = D::var(L, 6), dQ = D::var(Q, 7), da = D::var(a, 8);
+    const V3<D> n = kerr_escape_dir_t<D>(yy, dE, dL, dQ, da, M);
     D col[3];
     sky_color_t<D>(t, n, col);
     for (int k = 0; k < 3; ++k) {
         c[k] = col[k].v;
         if (dc)
-            for (int j = 0; j < 8; ++j) dc[8 * k + j] = col[k].d[j];
+            for (int j = 0; j < 9; ++j) dc[9 * k + j] = col[k].d[j];
     }
     return PRR_OK;
 }
 
 prr_status prr_debug_state_from_xu(int32_t metric, double M, double a, const double x[4], const double u[4],
-                                   double *y, double EL[2]) {
+                                   double *y, double ELQ[3]) {
     if (!x || !u || !y) return PRR_EINVAL;
     return with_metric(metric, [&](auto mt) {
         using Mt = decltype(mt);
         Consts<double> cs;
         state_from_xu<Mt>(x, u, M, a, y, cs);
-        if (EL) {
-            EL[0] = cs.E;
-            EL[1] = cs.L;
+        if (ELQ) {
+            ELQ[0] = cs.E;
+            ELQ[1] = cs.L;
+            ELQ[2] = cs.Q;
         }
         return PRR_OK;
     });
 }
 
-prr_status prr_debug_xu_from_state(int32_t metric, double M, double a, double E, double L, const double *y,
+prr_status prr_debug_xu_from_state(int32_t metric, double M, double a, double E, double L, double Q, const double *y,
                                    double x[4], double u[4]) {
     if (!x || !u || !y) return PRR_EINVAL;
     return with_metric(metric, [&](auto mt) {
         using Mt = decltype(mt);
-        xu_from_state<Mt>(y, Consts<double>{M, a, E, L}, x, u);
+        xu_from_state<Mt>(y, Consts<double>{M, a, E, L, Q}, x, u);
         return PRR_OK;
     });
 }
 
 prr_status prr_debug_init_ray(int32_t metric, double M, double a, const double pos[3], const double d[3], double *y,
-                              double EL[2]) {
+                              double ELQ[3]) {
     if (!pos || !d || !y) return PRR_EINVAL;
     return with_metric(metric, [&](auto mt) {
         using Mt = decltype(mt);
         Consts<double> cs;
         init_state<Mt>(d3(pos[0], pos[1], pos[2]), d3(d[0], d[1], d[2]), M, a, y, cs);
-        if (EL) {
-            EL[0] = cs.E;
-            EL[1] = cs.L;
+        if (ELQ) {
+            ELQ[0] = cs.E;
+            ELQ[1] = cs.L;
+            ELQ[2] = cs.Q;
         }
         return PRR_OK;
     });
@@ -153,29 +179,101 @@ prr_status prr_debug_init_ray(int32_t metric, double M, double a, const double p
 
 prr_status prr_debug_camera_ray(const prr_frame *f, int64_t local, double d[3]) {
     if (!f || !d) return PRR_EINVAL;
-    FrameDev fr{};
-    double F[3], U[3], R[3];
-    camera_basis(f->alpha, f->beta, F, U, R);
-    for (int i = 0; i < 3; ++i) {
-        fr.F[i] = F[i];
-        fr.U[i] = U[i];
-        fr.R[i] = R[i];
-    }
-    fr.alpha = f->alpha;
-    fr.beta = f->beta;
-    fr.vfov = f->vfov;
-    fr.tan_half = std::tan(f->vfov / 2.0);
-    fr.aspect = double(f->width) / double(f->height);
-    fr.W = f->width;
-    fr.H = f->height;
-    fr.s = f->samples;
-    fr.n = (f->ssaa == PRR_SSAA_NONE || f->samples == 1) ? 0 : int(std::lround(std::sqrt(double(f->samples))));
-    fr.projection = f->projection;
+    prr_tracer tr{};
+    const FrameDev fr = frame_dev(PRR_FLAT, tr, *f, 0);
     const D3 v = camera_ray(fr, local);
     d[0] = v.x;
     d[1] = v.y;
     d[2] = v.z;
     return PRR_OK;
+}
+
+prr_status prr_debug_trace_rays(int32_t metric, const prr_tracer *tracer, const prr_frame *f, const double *x0u0,
+                                const int64_t *ids, int n, int32_t *cls, double *endpoint, int32_t *steps,
+                                int32_t *attempts, double *state8) {
+    if (!tracer || !f || n < 0 || (!x0u0 && !ids)) return PRR_EINVAL;
+    if (tracer->solver == PRR_RKF45 && (!(tracer->hmin > 0.0) || tracer->max_attempts > 100000000LL))
+        return PRR_EINVAL;
+    const FrameDev fr = frame_dev(metric, *tracer, *f, 0);
+    JobDev job{};
+    job.frames = &fr;
+    job.nframes = 1;
+    job.refill = 1;
+    job.total = x0u0 ? n : int64_t(f->width) * f->height * f->samples;
+    job.explicit_x0u0 = x0u0;
+    job.tr = tracer_dev(*tracer);
+    int8_t c1;
+    double e3[3], s8[8];
+    int32_t st1, at1;
+    job.out.cls = &c1;
+    job.out.endpoint = e3;
+    job.out.steps = &st1;
+    job.out.attempts = &at1;
+    job.out.state8 = s8;
+    return with_metric(metric, [&](auto mt) {
+        using Mt = decltype(mt);
+        for (int i = 0; i < n; ++i) {
+            const int64_t id = x0u0 ? i : ids[i];
+            if (id < 0 || id >= job.total) return PRR_EINVAL;
+            Lane<Mt> L = lane_start<Mt>(job, id);
+            int c = finite_state<Mt>(L.y) ? -1 : PRR_NAN;
+            const bool mixed = tracer->precision == PRR_FP32_BULK;
+            while (c < 0) {
+                if (tracer->solver == PRR_EULER)
+                    c = mixed ? lane_attempt<Mt, PRR_EULER, true>(job, L) : lane_attempt<Mt, PRR_EULER, false>(job, L);
+                else
+                    c = mixed ? lane_attempt<Mt, PRR_RKF45, true>(job, L) : lane_attempt<Mt, PRR_RKF45, false>(job, L);
+            }
+            L.rid = 0;
+            lane_finish<Mt>(job, L, c);
+            if (cls) cls[i] = c1;
+            if (steps) steps[i] = st1;
+            if (attempts) attempts[i] = at1;
+            for (int k = 0; k < 3; ++k)
+                if (endpoint) endpoint[3 * i + k] = e3[k];
+            for (int k = 0; k < 8; ++k)
+                if (state8) state8[8 * i + k] = s8[k];
+        }
+        return PRR_OK;
+    });
+}
+
+prr_status prr_debug_trace_log(int32_t metric, const prr_tracer *tracer, const prr_frame *f, int64_t id, int max_n,
+                               double *log, int *n_out) {
+    if (!tracer || !f || !log || !n_out || max_n <= 0) return PRR_EINVAL;
+    const FrameDev fr = frame_dev(metric, *tracer, *f, 0);
+    JobDev job{};
+    job.frames = &fr;
+    job.nframes = 1;
+    job.refill = 1;
+    job.total = int64_t(f->width) * f->height * f->samples;
+    job.tr = tracer_dev(*tracer);
+    if (id < 0 || id >= job.total) return PRR_EINVAL;
+    return with_metric(metric, [&](auto mt) {
+        using Mt = decltype(mt);
+        Lane<Mt> L = lane_start<Mt>(job, id);
+        int c = finite_state<Mt>(L.y) ? -1 : PRR_NAN;
+        const bool mixed = tracer->precision == PRR_FP32_BULK;
+        int n = 0;
+        while (c < 0) {
+            const int32_t s0 = L.steps, a64 = L.att64;
+            const double h_before = L.h;
+            if (tracer->solver == PRR_EULER)
+                c = mixed ? lane_attempt<Mt, PRR_EULER, true>(job, L) : lane_attempt<Mt, PRR_EULER, false>(job, L);
+            else
+                c = mixed ? lane_attempt<Mt, PRR_RKF45, true>(job, L) : lane_attempt<Mt, PRR_RKF45, false>(job, L);
+            if (L.steps != s0 && n < max_n) {
+                double *row = log + 12 * n++;
+                row[0] = L.steps;
+                row[1] = L.attempts;
+                row[2] = h_before;
+                row[3] = L.att64 - a64;
+                for (int i = 0; i < 8; ++i) row[4 + i] = i < Mt::N ? L.y[i] : 0.0;
+            }
+        }
+        *n_out = n;
+        return PRR_OK;
+    });
 }
 
 } // extern "C"

@@ -1,0 +1,69 @@
+// frame_host.cpp -- see frame_host.hpp.
+#include "frame_host.hpp"
+
+#include <algorithm>
+#include <cmath>
+
+#include "prr_host.hpp"
+
+namespace prr {
+
+double horizon_radius(int metric, double M, double a) {
+    if (metric == PRR_FLAT) return 0.0;
+    if (metric == PRR_KERR_BL) return M + std::sqrt(std::max(M * M - a * a, 0.0));
+    return 2.0 * M;
+}
+
+TracerDev tracer_dev(const prr_tracer &t) {
+    TracerDev d{};
+    d.solver = t.solver;
+    d.precision = t.precision;
+    d.h = t.h;
+    d.hmin = t.hmin;
+    d.hmax = t.hmax;
+    d.atol = t.atol;
+    d.rtol = t.rtol;
+    d.r_escape = t.r_escape;
+    d.lambda_max = t.lambda_max;
+    d.r_fp64 = t.r_fp64;
+    d.max_steps = t.max_steps;
+    d.max_attempts = t.max_attempts;
+    return d;
+}
+
+FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t begin) {
+    FrameDev d{};
+    for (int i = 0; i < 3; ++i) d.pos[i] = f.pos[i];
+    double F[3], U[3], R[3];
+    camera_basis(f.alpha, f.beta, F, U, R);
+    for (int i = 0; i < 3; ++i) {
+        d.F[i] = F[i];
+        d.U[i] = U[i];
+        d.R[i] = R[i];
+    }
+    d.alpha = f.alpha;
+    d.beta = f.beta;
+    d.vfov = f.vfov;
+    d.tan_half = std::tan(f.vfov / 2.0);
+    d.aspect = double(f.width) / double(f.height);
+    d.W = f.width;
+    d.H = f.height;
+    d.s = f.samples;
+    d.n = (f.ssaa == PRR_SSAA_NONE || f.samples == 1) ? 0 : int(std::lround(std::sqrt(double(f.samples))));
+    d.projection = f.projection;
+    d.n_spheres = f.n_spheres;
+    d.M = f.mass;
+    d.a = metric == PRR_KERR_BL ? f.spin : 0.0;
+    const double rh = horizon_radius(metric, f.mass, d.a);
+    d.r_hor = metric == PRR_FLAT ? -1.0
+                                      : rh * (1.0 + tr.horizon_eps_rel) + tr.horizon_eps_abs;
+    d.ray_begin = begin;
+    for (int k = 0; k < f.n_spheres; ++k) {
+        for (int c = 0; c < 3; ++c) d.sph[k][c] = f.spheres[k].c[c];
+        d.sph[k][3] = f.spheres[k].radius;
+        d.sph_tex[k] = f.spheres[k].tex;
+    }
+    return d;
+}
+
+} // namespace prr

@@ -11,10 +11,8 @@
 // State vectors:
 //   Flat, SchwKS, SchwSph: y = (x^1, x^2, x^3, t, u^1, u^2, u^3, u^t), the
 //     reference's Ray r/u pair (main/base.h:172-176); f(y) = (u, getGeod(x, u)).
-//   Kerr (Boyer-Lindquist): y = (r, mu = cos theta, phi, p_r, p_mu) with the
-//     conserved E = -p_t and L = p_phi held per ray; f(y) = Hamilton's
-//     equations of H = g^{ab} p_a p_b / 2.  t is not integrated (it enters
-//     nothing observable).
+//   Kerr (Boyer-Lindquist): y = (1/r, d(1/r)/ds, cos th, d(cos th)/ds, phi,
+//     lambda) in Mino time s, with E, L and Carter's Q held per ray.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -39,12 +37,87 @@ PRR_HD double dot3(D3 a, D3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
 PRR_HD D3 normalize3(D3 a) { return (1.0 / sqrt(dot3(a, a))) * a; }
 
 PRR_HD void sincos_t(double x, double *s, double *c) { sincos(x, s, c); }
+
+// Light FP64 sine and cosine for the per-step Cartesian maps: Cody-Waite
+// reduction by pi/2 (three 33-bit parts, exact products for |x| < 2^20 pi/2)
+// and Taylor polynomials on |r| <= pi/4 (truncation < 1e-19).  Register
+// light, unlike a library sincos with a large-argument reduction path.
+PRR_HD void sincos_cw(double x, double *s, double *c) {
+    const double k = rint(x * 0.63661977236758134308);
+    double r = fma(-k, 1.5707963267341256, x);
+    r = fma(-k, 6.077100506303966e-11, r);
+    r = fma(-k, 2.0222662487959506e-21, r);
+    const double r2 = r * r;
+    const double sp =
+        r + r * r2 *
+                (-1.0 / 6.0 +
+                 r2 * (1.0 / 120.0 +
+                       r2 * (-1.0 / 5040.0 +
+                             r2 * (1.0 / 362880.0 +
+                                   r2 * (-1.0 / 39916800.0 +
+                                         r2 * (1.0 / 6227020800.0 +
+                                               r2 * (-1.0 / 1307674368000.0 + r2 * (1.0 / 355687428096000.0))))))));
+    const double cp =
+        1.0 + r2 * (-0.5 +
+                    r2 * (1.0 / 24.0 +
+                          r2 * (-1.0 / 720.0 +
+                                r2 * (1.0 / 40320.0 +
+                                      r2 * (-1.0 / 3628800.0 +
+                                            r2 * (1.0 / 479001600.0 +
+                                                  r2 * (-1.0 / 87178291200.0 + r2 * (1.0 / 20922789888000.0))))))));
+    const int q = int(int64_t(k) & 3);
+    if (q == 0) {
+        *s = sp;
+        *c = cp;
+    } else if (q == 1) {
+        *s = cp;
+        *c = -sp;
+    } else if (q == 2) {
+        *s = -sp;
+        *c = -cp;
+    } else {
+        *s = -cp;
+        *c = sp;
+    }
+}
+
+// FP32 sine and cosine for screening tests: the hardware approximations on
+// the device (|error| ~ 1e-6 on [-pi, pi]); x is reduced to [-pi, pi] first.
+PRR_HD void sincos_screen(double x, float *s, float *c) {
+    const float r = float(x - 6.28318530717958647692 * rint(x * 0.15915494309189533577));
+#if defined(__HIP_DEVICE_COMPILE__)
+    *s = __sinf(r);
+    *c = __cosf(r);
+#else
+    *s = sinf(r);
+    *c = cosf(r);
+#endif
+}
 PRR_HD void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
 PRR_HD double sqrt_t(double x) { return sqrt(x); }
 PRR_HD float sqrt_t(float x) { return sqrtf(x); }
 
+// Reciprocal: exact division in FP64; the hardware reciprocal (1 ulp) for
+// the FP32 bulk stages on the device.
+PRR_HD double rcp_t(double x) { return 1.0 / x; }
+PRR_HD float rcp_t(float x) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __builtin_amdgcn_rcpf(x);
+#else
+    return 1.0f / x;
+#endif
+}
+template <class T> PRR_HD T rcp_t(const T &x) { return T(1.0) / x; }
+
+// Per-ray constants: mass and spin, and for Kerr the conserved energy E,
+// axial angular momentum L and Carter constant Q.
 template <class T> struct Consts {
-    T M, a, E, L;
+    T M, a, E, L, Q;
+};
+
+// Right-hand-side constants of the 8-component metrics.
+template <class T> struct MassCoef {
+    T M;
 };
 
 // ----------------------------------------------------------------- metrics
@@ -53,8 +126,10 @@ struct Flat {
     static constexpr int N = 8;
     static constexpr int kind = PRR_FLAT;
     static constexpr bool spherical = false;
-    static constexpr bool hamiltonian = false;
-    template <class T> PRR_HD static void rhs(const T *y, const Consts<T> &, T *f) {
+    static constexpr int NERR = 8;
+    static constexpr bool mino = false;
+    template <class T> PRR_HD static MassCoef<T> coef(const Consts<double> &c) { return MassCoef<T>{T(c.M)}; }
+    template <class T> PRR_HD static void rhs(const T *y, const MassCoef<T> &, T *f) {
         f[0] = y[4];
         f[1] = y[5];
         f[2] = y[6];
@@ -71,8 +146,10 @@ struct SchwKS {
     static constexpr int N = 8;
     static constexpr int kind = PRR_SCHW_KS_CART;
     static constexpr bool spherical = false;
-    static constexpr bool hamiltonian = false;
-    template <class T> PRR_HD static void rhs(const T *y, const Consts<T> &c, T *f) {
+    static constexpr int NERR = 8;
+    static constexpr bool mino = false;
+    template <class T> PRR_HD static MassCoef<T> coef(const Consts<double> &c) { return MassCoef<T>{T(c.M)}; }
+    template <class T> PRR_HD static void rhs(const T *y, const MassCoef<T> &c, T *f) {
         const T x = y[0], yy = y[1], z = y[2];
         const T ux = y[4], uy = y[5], uz = y[6], ut = y[7];
         const T rs = T(2) * c.M;
@@ -101,8 +178,10 @@ struct SchwSph {
     static constexpr int N = 8;
     static constexpr int kind = PRR_SCHW_SPH;
     static constexpr bool spherical = true;
-    static constexpr bool hamiltonian = false;
-    template <class T> PRR_HD static void rhs(const T *y, const Consts<T> &c, T *f) {
+    static constexpr int NERR = 8;
+    static constexpr bool mino = false;
+    template <class T> PRR_HD static MassCoef<T> coef(const Consts<double> &c) { return MassCoef<T>{T(c.M)}; }
+    template <class T> PRR_HD static void rhs(const T *y, const MassCoef<T> &c, T *f) {
         const T r = y[0];
         const T ur = y[4], uth = y[5], uph = y[6], ut = y[7];
         T s, co;
@@ -128,177 +207,225 @@ struct SchwSph {
 };
 
 // Kerr, Boyer-Lindquist (metric of metrics/kerr.cuh:77-100; the reference's
-// getGeod is a stub), integrated as Hamilton's equations in the canonical
-// pair (mu = cos th, p_mu = -p_th / sin th), so that the right-hand side is
-// rational: no trigonometry in the hot loop.  With s2 = 1 - mu^2,
-// Sigma = r^2 + a^2 mu^2, Delta = r^2 - 2Mr + a^2, W = (r^2 + a^2) E - a L,
-// Lm = L - a E s2 and N = 2 Sigma H
-//   = Delta p_r^2 + s2 p_mu^2 - W^2/Delta + Lm^2 / s2:
-//   r'  = Delta p_r / Sigma,  mu' = s2 p_mu / Sigma,
-//   ph' = (a W / Delta + Lm / s2) / Sigma,
-//   p_r'  = -(dN/dr - 2 r N / Sigma) / (2 Sigma),
-//     dN/dr = Delta' (p_r^2 + (W/Delta)^2) - 4 r E W / Delta,
-//   p_mu' = -mu (L^2 / s2^2 - a^2 E^2 - p_mu^2 - a^2 N / Sigma) / Sigma.
-// The chart is singular only where a ray with L = 0 crosses the axis
-// exactly (s2 = 0); such a ray ends as NAN.
+// getGeod is a stub).  Integrated in Mino time s (d lambda = Sigma ds) with
+// u = 1/r and mu = cos th, where the geodesic equations separate into two
+// polynomial oscillators and two quadratures (Carter 1968):
+//   u'' = P'(u)/2 = u (c1 + u (c2 + u c3)),        P(u) = R(r) / r^4,
+//   mu'' = Th'(mu)/2 = mu (d1 + d2 mu^2),           Th(mu) = (1 - mu^2) Theta,
+//   phi' = u (2 M a E - a^2 L u) / D(u) + L / (1 - mu^2),  D = 1 - 2 M u + a^2 u^2,
+//   lambda' = Sigma = 1/u^2 + a^2 mu^2,
+// with K = (L - a E)^2 + Q, A = a (a E - L):
+//   c1 = 2 E A - K, c2 = 3 M K, c3 = 2 (A^2 - a^2 K),
+//   d1 = -(Q + L^2 - a^2 E^2), d2 = -2 a^2 E^2.
+// State y = (u, u', mu, mu', phi, lambda); E = -p_t, L = p_phi and Carter's
+// Q are fixed per ray.  No term cancels catastrophically, so the stages are
+// well conditioned in FP32, and u' -> -E as r -> infinity (no pole in s).
+// t is not integrated (it enters nothing observable).
+template <class T> struct KerrCoef {
+    T c1, c2, c3, d1, d2, pa, pb, m2, a2, L;
+};
+// FP64-anchored stage data of the FP32 bulk form: f(y0 + d) - f(y0) is a
+// polynomial / rational function of d whose coefficients depend on y0 only.
+struct KerrDelta {
+    float g1u, g2u, c3, g1m, g2m, d2; // u'' and mu'' Taylor coefficients
+    float n1, n2, D0, dD1, a2;        // phi' first term: d (n1 + d n2) / (D0 D)
+    float mu0, s20, Ls, u0, iu02;     // phi' second term and lambda'
+};
+
 struct Kerr {
-    static constexpr int N = 5;
+    static constexpr int N = 6;
+    static constexpr int NERR = 5; // lambda is bookkeeping: outside the error norm
     static constexpr int kind = PRR_KERR_BL;
     static constexpr bool spherical = true;
-    static constexpr bool hamiltonian = true;
-    template <class T> PRR_HD static void rhs(const T *y, const Consts<T> &c, T *f) {
-        const T r = y[0], mu = y[1], pr = y[3], pm = y[4];
-        const T a = c.a, M = c.M, E = c.E, L = c.L;
-        const T r2 = r * r, a2 = a * a;
-        const T s2 = T(1) - mu * mu;
-        const T Sig = r2 + a2 * mu * mu;
-        const T Del = r2 - T(2) * M * r + a2;
-        const T iS = T(1) / Sig, iD = T(1) / Del, is2 = T(1) / s2;
-        const T W = (r2 + a2) * E - a * L;
-        const T Lm = L - a * E * s2;
-        const T WiD = W * iD;
-        const T Lmis2 = Lm * is2;
-        const T Nn = Del * pr * pr + s2 * pm * pm - W * WiD + Lm * Lmis2;
-        const T NiS = Nn * iS;
-        const T dNr = T(2) * (r - M) * (pr * pr + WiD * WiD) - T(4) * r * E * WiD;
-        const T Lis2 = L * is2;
-        f[0] = Del * pr * iS;
-        f[1] = s2 * pm * iS;
-        f[2] = (a * WiD + Lmis2) * iS;
-        f[3] = T(-0.5) * iS * (dNr - T(2) * r * NiS);
-        f[4] = -mu * iS * (Lis2 * Lis2 - a2 * E * E - pm * pm - a2 * NiS);
+    static constexpr bool mino = true;
+
+    // Coefficients in any scalar type (double, or the duals of the adjoint).
+    template <class T>
+    PRR_HD static KerrCoef<T> coef_of(double M, const T &a, const T &E, const T &L, const T &Q) {
+        const T K = (L - a * E) * (L - a * E) + Q, A = a * (a * E - L);
+        KerrCoef<T> k;
+        k.c1 = T(2.0) * E * A - K;
+        k.c2 = T(3.0 * M) * K;
+        k.c3 = T(2.0) * (A * A - a * a * K);
+        k.d1 = T(0.0) - (Q + L * L - a * a * E * E);
+        k.d2 = T(-2.0) * a * a * E * E;
+        k.pa = T(2.0 * M) * a * E;
+        k.pb = a * a * L;
+        k.m2 = T(2.0 * M);
+        k.a2 = a * a;
+        k.L = L;
+        return k;
+    }
+    // Per-attempt coefficients: computed in FP64, then rounded to T.
+    template <class T> PRR_HD static KerrCoef<T> coef(const Consts<double> &c) {
+        const KerrCoef<double> d = coef_of<double>(c.M, c.a, c.E, c.L, c.Q);
+        return KerrCoef<T>{T(d.c1), T(d.c2), T(d.c3), T(d.d1), T(d.d2), T(d.pa), T(d.pb), T(d.m2), T(d.a2), T(d.L)};
     }
 
-    // Reverse mode of rhs: given fb = dL/df, accumulate dL/dy into yb[5]
-    // (yb[2] stays untouched: phi does not enter f) and dL/d(E, L, a) into
-    // cb[0..2].  Written by hand from the forward expression graph above,
-    // in reverse order.
-    PRR_HD static void rhs_vjp(const double *y, const Consts<double> &c, const double *fb, double *yb, double *cb) {
-        const double r = y[0], mu = y[1], pr = y[3], pm = y[4];
-        const double a = c.a, M = c.M, E = c.E, L = c.L;
-        // forward values
-        const double r2 = r * r, a2 = a * a;
-        const double s2 = 1.0 - mu * mu;
-        const double Sig = r2 + a2 * mu * mu;
-        const double Del = r2 - 2.0 * M * r + a2;
-        const double iS = 1.0 / Sig, iD = 1.0 / Del, is2 = 1.0 / s2;
-        const double W = (r2 + a2) * E - a * L;
-        const double Lm = L - a * E * s2;
-        const double WiD = W * iD;
-        const double Lmis2 = Lm * is2;
-        const double Nn = Del * pr * pr + s2 * pm * pm - W * WiD + Lm * Lmis2;
-        const double NiS = Nn * iS;
-        const double Aq = pr * pr + WiD * WiD;
-        const double dNr = 2.0 * (r - M) * Aq - 4.0 * r * E * WiD;
-        const double Lis2 = L * is2;
-        const double g2 = a * WiD + Lmis2;
-        const double g3 = dNr - 2.0 * r * NiS;
-        const double g4 = Lis2 * Lis2 - a2 * E * E - pm * pm - a2 * NiS;
-        // adjoints
-        double rb = 0, mub = 0, prb = 0, pmb = 0, Eb = 0, Lb = 0, ab = 0;
-        double iSb = 0, Delb = 0, s2b = 0, WiDb = 0, Lmis2b = 0, NiSb = 0, dNrb = 0, Lis2b = 0, a2b = 0;
-        // f0 = Del pr iS
-        Delb += fb[0] * pr * iS;
-        prb += fb[0] * Del * iS;
-        iSb += fb[0] * Del * pr;
-        // f1 = s2 pm iS
-        s2b += fb[1] * pm * iS;
-        pmb += fb[1] * s2 * iS;
-        iSb += fb[1] * s2 * pm;
-        // f2 = g2 iS, g2 = a WiD + Lmis2
-        iSb += fb[2] * g2;
+    // One reciprocal for the three denominators D, 1 - mu^2 and u^2.
+    template <class T> PRR_HD static void rhs(const T *y, const KerrCoef<T> &k, T *f) {
+        const T u = y[0], mu = y[2];
+        const T D = T(1) - u * (k.m2 - k.a2 * u);
+        const T s2 = T(1) - mu * mu;
+        const T u2 = u * u;
+        const T q = rcp_t(D * s2 * u2);
+        f[0] = y[1];
+        f[1] = u * (k.c1 + u * (k.c2 + u * k.c3));
+        f[2] = y[3];
+        f[3] = mu * (k.d1 + k.d2 * mu * mu);
+        f[4] = u * (k.pa - k.pb * u) * (q * s2 * u2) + k.L * (q * D * u2);
+        f[5] = q * D * s2 + k.a2 * mu * mu;
+    }
+
+    // FP64 part of an FP32-bulk attempt: F = f(y0) and the difference
+    // coefficients at y0.
+    PRR_HD static void delta_prep(const double *y, const KerrCoef<double> &k, double *F, KerrDelta &P) {
+        const double u0 = y[0], mu0 = y[2];
+        const double D0 = 1.0 - u0 * (k.m2 - k.a2 * u0), s20 = 1.0 - mu0 * mu0, u02 = u0 * u0;
+        const double q = 1.0 / (D0 * s20 * u02);
+        const double iD0 = q * s20 * u02, is20 = q * D0 * u02, iu02 = q * D0 * s20;
+        const double N0 = u0 * (k.pa - k.pb * u0);
+        F[0] = y[1];
+        F[1] = u0 * (k.c1 + u0 * (k.c2 + u0 * k.c3));
+        F[2] = y[3];
+        F[3] = mu0 * (k.d1 + k.d2 * mu0 * mu0);
+        F[4] = N0 * iD0 + k.L * is20;
+        F[5] = iu02 + k.a2 * mu0 * mu0;
+        P.g1u = float(k.c1 + u0 * (2.0 * k.c2 + 3.0 * k.c3 * u0));
+        P.g2u = float(k.c2 + 3.0 * k.c3 * u0);
+        P.c3 = float(k.c3);
+        P.g1m = float(k.d1 + 3.0 * k.d2 * mu0 * mu0);
+        P.g2m = float(3.0 * k.d2 * mu0);
+        P.d2 = float(k.d2);
+        const double dD1 = 2.0 * k.a2 * u0 - k.m2;
+        P.n1 = float((D0 * (k.pa - 2.0 * k.pb * u0) - N0 * dD1) * iD0);
+        P.n2 = float((-k.pb * D0 - k.a2 * N0) * iD0);
+        P.D0 = float(D0);
+        P.dD1 = float(dD1);
+        P.a2 = float(k.a2);
+        P.mu0 = float(mu0);
+        P.s20 = float(s20);
+        P.Ls = float(k.L * is20);
+        P.u0 = float(u0);
+        P.iu02 = float(iu02);
+    }
+
+    // g = f(y0 + d) - f(y0) in FP32.
+    PRR_HD static void delta_rhs(const float *d, const KerrDelta &P, float *g) {
+        const float du = d[0], dm = d[2];
+        const float D = P.D0 + du * (P.dD1 + P.a2 * du);
+        const float tm = dm * (2.0f * P.mu0 + dm); // mu^2 - mu0^2
+        const float s2 = P.s20 - tm;
+        const float u = P.u0 + du;
+        const float u2 = u * u;
+        const float q = rcp_t(D * s2 * u2);
+        g[0] = d[1];
+        g[1] = du * (P.g1u + du * (P.g2u + du * P.c3));
+        g[2] = d[3];
+        g[3] = dm * (P.g1m + dm * (P.g2m + dm * P.d2));
+        g[4] = du * (P.n1 + du * P.n2) * (q * s2 * u2) + P.Ls * tm * (q * D * u2);
+        g[5] = -du * (2.0f * P.u0 + du) * P.iu02 * (q * D * s2) + P.a2 * tm;
+    }
+
+    // FP32-bulk steps re-impose the first integrals u'^2 = P(u) and
+    // mu'^2 = Th(mu) in FP64 (E, L, Q are exact per-ray constants), keeping
+    // the signs, except near turning points where the square roots are
+    // ill-conditioned.  Rounding of the FP32 stages otherwise moves the state
+    // off the constraint surface, and a pass near the polar axis amplifies
+    // that drift (phi jumps by ~pi L / sqrt(K sin^2 th_min)).
+    PRR_HD static void project(double *y, const Consts<double> &c) {
+        const double a = c.a, E = c.E, L = c.L, Q = c.Q, M = c.M;
+        const double K = (L - a * E) * (L - a * E) + Q, A = a * (a * E - L);
+        const double u = y[0], mu = y[2];
+        const double u2 = u * u, m2 = mu * mu;
+        const double t = E + A * u2;
+        const double P = t * t - (1.0 - 2.0 * M * u + a * a * u2) * K * u2;
+        if (P > 1e-4 * E * E && y[1] * y[1] > 1e-4 * E * E) y[1] = copysign(sqrt(P), y[1]);
+        const double Th = Q * (1.0 - m2) - m2 * L * L + a * a * E * E * m2 * (1.0 - m2);
+        const double Kth = fabs(Q) + L * L + a * a * E * E;
+        if (Th > 1e-4 * Kth && y[3] * y[3] > 1e-4 * Kth) y[3] = copysign(sqrt(Th), y[3]);
+    }
+
+    // Reverse mode of rhs: given fb = dL/df, accumulates dL/dy into yb[0..4]
+    // (phi and lambda do not enter f) and dL/d(coefficients) into kb.
+    PRR_HD static void rhs_vjp(const double *y, const KerrCoef<double> &k, const double *fb, double *yb,
+                               KerrCoef<double> &kb) {
+        const double u = y[0], mu = y[2];
+        const double D = 1.0 - u * (k.m2 - k.a2 * u), s2 = 1.0 - mu * mu, u2 = u * u;
+        const double iD = 1.0 / D, is2 = 1.0 / s2, iu2 = 1.0 / u2;
+        const double N = u * (k.pa - k.pb * u);
+        // f0 = w, f2 = nu
+        yb[1] += fb[0];
+        yb[3] += fb[2];
+        // f1 = c1 u + c2 u^2 + c3 u^3
+        yb[0] += fb[1] * (k.c1 + u * (2.0 * k.c2 + 3.0 * k.c3 * u));
+        kb.c1 += fb[1] * u;
+        kb.c2 += fb[1] * u2;
+        kb.c3 += fb[1] * u2 * u;
+        // f3 = d1 mu + d2 mu^3
+        yb[2] += fb[3] * (k.d1 + 3.0 * k.d2 * mu * mu);
+        kb.d1 += fb[3] * mu;
+        kb.d2 += fb[3] * mu * mu * mu;
+        // f4 = N / D + L / s2, N = pa u - pb u^2, D = 1 - m2 u + a2 u^2
         {
-            const double g2b = fb[2] * iS;
-            ab += g2b * WiD;
-            WiDb += g2b * a;
-            Lmis2b += g2b;
+            const double Nb = fb[4] * iD, Db = -fb[4] * N * iD * iD, s2b = -fb[4] * k.L * is2 * is2;
+            kb.L += fb[4] * is2;
+            yb[0] += Nb * (k.pa - 2.0 * k.pb * u) + Db * (2.0 * k.a2 * u - k.m2);
+            kb.pa += Nb * u;
+            kb.pb -= Nb * u2;
+            kb.m2 -= Db * u;
+            kb.a2 += Db * u2;
+            yb[2] += s2b * (-2.0 * mu);
         }
-        // f3 = -0.5 iS g3, g3 = dNr - 2 r NiS
-        iSb += -0.5 * fb[3] * g3;
-        {
-            const double g3b = -0.5 * fb[3] * iS;
-            dNrb += g3b;
-            rb += g3b * (-2.0 * NiS);
-            NiSb += g3b * (-2.0 * r);
-        }
-        // f4 = -mu iS g4, g4 = Lis2^2 - a2 E^2 - pm^2 - a2 NiS
-        mub += -fb[4] * iS * g4;
-        iSb += -fb[4] * mu * g4;
-        {
-            const double g4b = -fb[4] * mu * iS;
-            Lis2b += g4b * 2.0 * Lis2;
-            a2b += g4b * (-E * E - NiS);
-            Eb += g4b * (-2.0 * a2 * E);
-            pmb += g4b * (-2.0 * pm);
-            NiSb += g4b * (-a2);
-        }
-        // Lis2 = L is2
-        Lb += Lis2b * is2;
-        double is2b = Lis2b * L;
-        // dNr = 2 (r - M) Aq - 4 r E WiD, Aq = pr^2 + WiD^2
-        rb += dNrb * (2.0 * Aq - 4.0 * E * WiD);
-        prb += dNrb * 4.0 * (r - M) * pr;
-        WiDb += dNrb * (4.0 * (r - M) * WiD - 4.0 * r * E);
-        Eb += dNrb * (-4.0 * r * WiD);
-        // NiS = Nn iS
-        const double Nnb = NiSb * iS;
-        iSb += NiSb * Nn;
-        // Nn = Del pr^2 + s2 pm^2 - W WiD + Lm Lmis2
-        Delb += Nnb * pr * pr;
-        prb += Nnb * 2.0 * Del * pr;
-        s2b += Nnb * pm * pm;
-        pmb += Nnb * 2.0 * s2 * pm;
-        double Wb = -Nnb * WiD;
-        WiDb += -Nnb * W;
-        double Lmb = Nnb * Lmis2;
-        Lmis2b += Nnb * Lm;
-        // Lmis2 = Lm is2
-        Lmb += Lmis2b * is2;
-        is2b += Lmis2b * Lm;
-        // WiD = W iD
-        Wb += WiDb * iD;
-        const double iDb = WiDb * W;
-        // Lm = L - a E s2
-        Lb += Lmb;
-        ab += Lmb * (-E * s2);
-        Eb += Lmb * (-a * s2);
-        s2b += Lmb * (-a * E);
-        // W = (r2 + a2) E - a L
-        double r2b = Wb * E;
-        a2b += Wb * E;
-        Eb += Wb * (r2 + a2);
-        ab += Wb * (-L);
-        Lb += Wb * (-a);
-        // reciprocals
-        const double Sigb = -iSb * iS * iS;
-        Delb += -iDb * iD * iD;
-        s2b += -is2b * is2 * is2;
-        // Del = r2 - 2 M r + a2
-        r2b += Delb;
-        rb += Delb * (-2.0 * M);
-        a2b += Delb;
-        // Sig = r2 + a2 mu^2
-        r2b += Sigb;
-        a2b += Sigb * mu * mu;
-        mub += Sigb * 2.0 * a2 * mu;
-        // s2 = 1 - mu^2
-        mub += s2b * (-2.0 * mu);
-        // squares
-        ab += a2b * 2.0 * a;
-        rb += r2b * 2.0 * r;
-        yb[0] += rb;
-        yb[1] += mub;
-        yb[3] += prb;
-        yb[4] += pmb;
+        // f5 = 1/u^2 + a2 mu^2
+        yb[0] += fb[5] * (-2.0 * iu2 / u);
+        yb[2] += fb[5] * 2.0 * k.a2 * mu;
+        kb.a2 += fb[5] * mu * mu;
+    }
+
+    // Chain rule of coef(): dL/d(coefficients) -> dL/d(E, L, Q, a) (added to cb).
+    PRR_HD static void coef_vjp(const Consts<double> &c, const KerrCoef<double> &kb, double cb[4]) {
+        const double a = c.a, E = c.E, L = c.L, Q = c.Q, M = c.M;
+        const double K = (L - a * E) * (L - a * E) + Q, A = a * (a * E - L);
+        double Kb = 0.0, Ab = 0.0, Eb = 0.0, Lb = kb.L, Qb = 0.0, ab = 0.0;
+        // c1 = 2 E A - K
+        Eb += kb.c1 * 2.0 * A;
+        Ab += kb.c1 * 2.0 * E;
+        Kb -= kb.c1;
+        // c2 = 3 M K
+        Kb += kb.c2 * 3.0 * M;
+        // c3 = 2 (A^2 - a^2 K)
+        Ab += kb.c3 * 4.0 * A;
+        ab += kb.c3 * (-4.0 * a * K);
+        Kb += kb.c3 * (-2.0 * a * a);
+        // d1 = -(Q + L^2 - a^2 E^2)
+        Qb -= kb.d1;
+        Lb -= kb.d1 * 2.0 * L;
+        ab += kb.d1 * 2.0 * a * E * E;
+        Eb += kb.d1 * 2.0 * a * a * E;
+        // d2 = -2 a^2 E^2
+        ab += kb.d2 * (-4.0 * a * E * E);
+        Eb += kb.d2 * (-4.0 * a * a * E);
+        // pa = 2 M a E, pb = a^2 L, a2 = a^2 (m2 = 2M is fixed)
+        ab += kb.pa * 2.0 * M * E + kb.pb * 2.0 * a * L + kb.a2 * 2.0 * a;
+        Eb += kb.pa * 2.0 * M * a;
+        Lb += kb.pb * a * a;
+        // K = (L - a E)^2 + Q, A = a (a E - L)
+        Lb += Kb * 2.0 * (L - a * E) - Ab * a;
+        Eb += Kb * (-2.0 * a * (L - a * E)) + Ab * a * a;
+        ab += Kb * (-2.0 * E * (L - a * E)) + Ab * (2.0 * a * E - L);
+        Qb += Kb;
         cb[0] += Eb;
         cb[1] += Lb;
-        cb[2] += ab;
+        cb[2] += Qb;
+        cb[3] += ab;
     }
 };
 
 // ----------------------------------------------------------------- geometry
 // Radius of a state (for the horizon, escape and FP64-region tests).
 template <class Mt> PRR_HD double radius_of(const double *y) {
+    if (Mt::mino) return 1.0 / y[0];
     if (Mt::spherical) return y[0];
     return sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
 }
@@ -306,15 +433,42 @@ template <class Mt> PRR_HD double radius_of(const double *y) {
 // Cartesian position (spherical-type coordinates map as flat space).
 template <class Mt> PRR_HD D3 cart_of(const double *y) {
     if (!Mt::spherical) return d3(y[0], y[1], y[2]);
-    double st, ct, sp, cp;
-    if (Mt::hamiltonian) { // y[1] = cos th
-        ct = y[1];
+    double r, st, ct, sp, cp;
+    if (Mt::mino) { // (u, u', mu, mu', phi, lambda)
+        r = 1.0 / y[0];
+        ct = y[2];
         st = sqrt(fmax(1.0 - ct * ct, 0.0));
+        sincos_cw(y[4], &sp, &cp);
     } else {
-        sincos(y[1], &st, &ct);
+        r = y[0];
+        sincos_cw(y[1], &st, &ct);
+        sincos_cw(y[2], &sp, &cp);
     }
-    sincos(y[2], &sp, &cp);
-    return d3(y[0] * st * cp, y[0] * st * sp, y[0] * ct);
+    return d3(r * st * cp, r * st * sp, r * ct);
+}
+
+// FP32 Cartesian position for the sphere screen (accuracy ~1e-6 relative).
+template <class Mt> PRR_HD void cart_of_f32(const double *y, float P[3]) {
+    if (!Mt::spherical) {
+        P[0] = float(y[0]);
+        P[1] = float(y[1]);
+        P[2] = float(y[2]);
+        return;
+    }
+    float r, st, ct, sp, cp;
+    if (Mt::mino) {
+        r = float(1.0 / y[0]);
+        ct = float(y[2]);
+        st = sqrtf(fmaxf(1.0f - ct * ct, 0.0f));
+        sincos_screen(y[4], &sp, &cp);
+    } else {
+        r = float(y[0]);
+        sincos_screen(y[1], &st, &ct);
+        sincos_screen(y[2], &sp, &cp);
+    }
+    P[0] = r * st * cp;
+    P[1] = r * st * sp;
+    P[2] = r * ct;
 }
 
 // pos_cartTOspher / vel_cartTOspher (cudamaths/transform.cuh:15-31) with the
@@ -390,13 +544,16 @@ PRR_HD KerrG kerr_g(double M, double a, double r, double th) {
 }
 
 // (x, u) in the metric's coordinates -> integrator state y and per-ray
-// constants.  For the 8-component metrics y = (x, u).
+// constants.  For the 8-component metrics y = (x, u).  For Kerr (x = (r, th,
+// phi, t), u = dx/dlambda): E = -p_t, L = p_phi, Q = p_th^2 + cos^2 th
+// (L^2 / sin^2 th - a^2 E^2), u' = du/ds = -Sigma u^r / r^2 and
+// mu' = -sin th Sigma u^th.
 template <class Mt> PRR_HD void state_from_xu(const double x[4], const double u[4], double M, double a, double *y,
                                               Consts<double> &cs) {
     cs.M = M;
     cs.a = a;
-    cs.E = cs.L = 0.0;
-    if (!Mt::hamiltonian) {
+    cs.E = cs.L = cs.Q = 0.0;
+    if (!Mt::mino) {
         for (int i = 0; i < 4; ++i) {
             y[i] = x[i];
             y[4 + i] = u[i];
@@ -408,35 +565,43 @@ template <class Mt> PRR_HD void state_from_xu(const double x[4], const double u[
     cs.L = g.phph * u[2] + g.tph * u[3];
     double st, ct;
     sincos(x[1], &st, &ct);
-    y[0] = x[0];
-    y[1] = ct;
-    y[2] = x[2];
-    y[3] = g.rr * u[0];
-    y[4] = -g.thth * u[1] / st; // p_mu = -p_th / sin th
+    const double pth = g.Sig * u[1];
+    cs.Q = pth * pth + ct * ct * (cs.L * cs.L / (st * st) - a * a * cs.E * cs.E);
+    const double ir = 1.0 / x[0];
+    y[0] = ir;
+    y[1] = -g.Sig * u[0] * ir * ir;
+    y[2] = ct;
+    y[3] = -st * pth;
+    y[4] = x[2];
+    y[5] = 0.0;
 }
 
-// Integrator state -> (x, u); for Kerr, t is returned as 0.
+// Integrator state -> (x, u); for Kerr, t is returned as 0 and
+// u = dx/dlambda = (dx/ds) / Sigma.
 template <class Mt> PRR_HD void xu_from_state(const double *y, const Consts<double> &cs, double x[4], double u[4]) {
-    if (!Mt::hamiltonian) {
+    if (!Mt::mino) {
         for (int i = 0; i < 4; ++i) {
             x[i] = y[i];
             u[i] = y[4 + i];
         }
         return;
     }
-    double f[5];
-    Kerr::rhs<double>(y, cs, f);
-    const double th = acos(fmin(fmax(y[1], -1.0), 1.0));
-    const KerrG g = kerr_g(cs.M, cs.a, y[0], th);
-    const double det = g.tt * g.phph - g.tph * g.tph;
-    x[0] = y[0];
+    double f[6];
+    Kerr::rhs<double>(y, Kerr::coef<double>(cs), f);
+    const double r = 1.0 / y[0], mu = y[2];
+    const double th = acos(fmin(fmax(mu, -1.0), 1.0));
+    const double st = sqrt(fmax(1.0 - mu * mu, 0.0));
+    const double a = cs.a, M = cs.M;
+    const double Sig = r * r + a * a * mu * mu, Del = r * r - 2.0 * M * r + a * a;
+    const double W = (r * r + a * a) * cs.E - a * cs.L;
+    x[0] = r;
     x[1] = th;
-    x[2] = y[2];
+    x[2] = y[4];
     x[3] = 0.0;
-    u[0] = f[0];
-    u[1] = -sqrt(fmax(1.0 - y[1] * y[1], 0.0)) * y[4] / g.Sig; // th' = -sin th p_mu / Sigma
-    u[2] = f[2];
-    u[3] = -(g.phph * cs.E + g.tph * cs.L) / det;
+    u[0] = -y[1] * r * r / Sig;
+    u[1] = -y[3] / (st * Sig);
+    u[2] = f[4] / Sig;
+    u[3] = ((r * r + a * a) * W / Del - a * (a * cs.E * st * st - cs.L)) / Sig;
 }
 
 // initRays (kernels/solver.cuh:26-56): camera position and unit direction ->
@@ -507,7 +672,9 @@ PRR_HD void sph_post_step(double *y) {
 // Direction used for the celestial lookup at escape: the spatial velocity for
 // Cartesian metrics (getColorDefault2, main/base.h:119-128), the position for
 // Schwarzschild coordinates (getColorDefault, :68-84), the Cartesian
-// coordinate velocity for Kerr.
+// coordinate velocity for Kerr: with e_r, e_th, e_ph the flat-space frame of
+// (th, phi), dx/ds ~ -u' e_r / u^2 + r th' e_th + r sin th phi' e_ph, scaled
+// by u^2 > 0: -u' e_r - u mu' / sin th e_th + u sin th phi' e_ph.
 template <class Mt> PRR_HD D3 escape_dir(const double *y, const Consts<double> &cs) {
     if (Mt::kind == PRR_SCHW_SPH) {
         double st, ct, sp, cp;
@@ -515,17 +682,14 @@ template <class Mt> PRR_HD D3 escape_dir(const double *y, const Consts<double> &
         sincos(y[2], &sp, &cp);
         return d3(st * cp, st * sp, ct);
     }
-    if (Mt::hamiltonian) {
-        double f[5];
-        Kerr::rhs<double>(y, cs, f);
+    if (Mt::mino) {
+        double f[6];
+        Kerr::rhs<double>(y, Kerr::coef<double>(cs), f);
         double sp, cp;
-        const double ct = y[1], st = sqrt(fmax(1.0 - ct * ct, 0.0));
-        sincos(y[2], &sp, &cp);
-        const double r = y[0];
-        const double Sig = r * r + cs.a * cs.a * ct * ct;
-        const double thdot = -st * y[4] / Sig;
+        const double u = y[0], ct = y[2], st = sqrt(fmax(1.0 - ct * ct, 0.0));
+        sincos(y[4], &sp, &cp);
         const D3 er = d3(st * cp, st * sp, ct), et = d3(ct * cp, ct * sp, -st), ep = d3(-sp, cp, 0.0);
-        return normalize3(f[0] * er + (r * thdot) * et + (r * st * f[2]) * ep);
+        return normalize3((-y[1]) * er + (-u * y[3] / st) * et + (u * st * f[4]) * ep);
     }
     return normalize3(d3(y[4], y[5], y[6]));
 }

@@ -48,6 +48,14 @@ struct FrameDev {
     int32_t sph_tex[PRR_MAX_SPHERES];
 };
 
+// Terminal RKF45 step of a ray whose escape or sphere hit was located on
+// the path: the state at the step's start, its size and the located fraction.
+struct EventDev {
+    double y_prev[8];
+    double h, theta;
+    int32_t cls, hit;
+};
+
 // Per-ray outputs (device pointers; any may be null).
 struct OutDev {
     float *color;     // 3 per ray
@@ -56,6 +64,7 @@ struct OutDev {
     int32_t *steps;
     int32_t *attempts;
     double *state8;   // 8 per ray (prr_trace)
+    EventDev *event;  // 1 per ray (adjoint)
 };
 
 // Device-side counters; `queue` is the ray queue head.
@@ -81,23 +90,23 @@ struct JobDev {
     CountersDev *ctr;
 };
 
-// Kerr adjoint (prr_adjoint): checkpoints, seeds and per-ray gradients.
+// Kerr adjoint (prr_adjoint): checkpoints and per-ray gradients.
 struct AdjDev {
     int32_t K;               // accepted steps between checkpoints
     int32_t pad;
-    const int32_t *steps;    // accepted steps of every ray (from the first forward pass)
+    const int32_t *steps;    // accepted steps of every ray (first forward pass)
+    const int8_t *cls;       // termination class of every ray (first forward pass)
     const int64_t *ckpt_off; // index of each ray's first checkpoint
-    double *ckpt;            // 8 doubles per checkpoint: y[5], h, lambda, attempts
+    double *ckpt;            // 8 doubles per checkpoint: y[6], h, attempts
     const float *dLdI;       // dL/dI, W*H*3
     const float *img;        // forward linear image, W*H*3
     const float *ray_color;  // forward ray colours, 3 per ray
-    double *seed;            // 12 doubles per ray: dL/dz_N[8], dL/dz_{N-1}(r, mu, phi)[3], dL/dphi_s
     double *seg;             // per-lane segment buffers: K * 6 doubles per lane, lane-minor
     int64_t seg_lanes;       // lanes of the backward grid
     double *grad_ray;        // 4 doubles per ray: dL/d(a, r_cam, incl, phi_s)
     double cam_phi;          // fixed orbit parameters
     double sphere_r;
-    unsigned long long *replay_mismatch; // checkpoints the replay did not reproduce bit for bit
+    unsigned long long *replay_mismatch; // segments the replay did not reproduce bit for bit
 };
 
 // Post-processing of one frame (SSAA combine, FXAA, RGB8).

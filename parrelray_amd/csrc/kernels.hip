@@ -15,8 +15,8 @@
 #include <hip/hip_runtime.h>
 
 #include "adjoint.hpp"
-#include "geodesic.hpp"
 #include "kernels.hpp"
+#include "lane.hpp"
 
 // Register budget of the integrator: waves per SIMD the compiler must fit
 // (build-time knob, -DPRR_WAVES_PER_EU=n; unset = the compiler's choice).
@@ -28,270 +28,20 @@
 
 namespace prr {
 
-// Fehlberg 4(5) tableau (kernels/solver.cuh:125-131).
-__device__ constexpr double kA[6][5] = {
-    {0, 0, 0, 0, 0},
-    {1.0 / 4.0, 0, 0, 0, 0},
-    {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0},
-    {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0},
-    {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0},
-    {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0}};
-__device__ constexpr double kB4[6] = {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -1.0 / 5.0, 0.0};
-// b5 - b4: the embedded error estimate.
-__device__ constexpr double kE[6] = {16.0 / 135.0 - 25.0 / 216.0,       0.0,
-                                     6656.0 / 12825.0 - 1408.0 / 2565.0, 28561.0 / 56430.0 - 2197.0 / 4104.0,
-                                     -9.0 / 50.0 + 1.0 / 5.0,            2.0 / 55.0};
-
-// Six RK stages in precision T from the FP64 state y; returns the 4th-order
-// increment d4 and the error vector e in FP64.
-template <class Mt, class T>
-__device__ inline void rkf_stages(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
-    constexpr int N = Mt::N;
-    const Consts<T> c{T(cd.M), T(cd.a), T(cd.E), T(cd.L)};
-    const T hT = T(h);
-    T yT[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) yT[i] = T(y[i]);
-    T k[6][N];
-    Mt::template rhs<T>(yT, c, k[0]);
-#pragma unroll
-    for (int i = 0; i < N; ++i) k[0][i] *= hT;
-#pragma unroll
-    for (int s = 1; s < 6; ++s) {
-        T ys[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) {
-            T acc = yT[i];
-#pragma unroll
-            for (int j = 0; j < s; ++j) acc += T(kA[s][j]) * k[j][i];
-            ys[i] = acc;
-        }
-        Mt::template rhs<T>(ys, c, k[s]);
-#pragma unroll
-        for (int i = 0; i < N; ++i) k[s][i] *= hT;
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        double a4 = 0.0, ae = 0.0;
-#pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            if (kB4[s] != 0.0) a4 += kB4[s] * double(k[s][i]); // zero terms folded at compile time
-            if (kE[s] != 0.0) ae += kE[s] * double(k[s][i]);
-        }
-        d4[i] = a4;
-        e[i] = ae;
-    }
-}
-
-template <class Mt, class T>
-__device__ inline void euler_rhs(const double *y, const Consts<double> &cd, double *f) {
-    constexpr int N = Mt::N;
-    const Consts<T> c{T(cd.M), T(cd.a), T(cd.E), T(cd.L)};
-    T yT[N], fT[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) yT[i] = T(y[i]);
-    Mt::template rhs<T>(yT, c, fT);
-#pragma unroll
-    for (int i = 0; i < N; ++i) f[i] = double(fT[i]);
-}
-
-__device__ inline int find_frame(const JobDev &job, int64_t id) {
-    int lo = 0, hi = job.nframes - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (job.frames[mid].ray_begin <= id) lo = mid;
-        else hi = mid - 1;
-    }
-    return lo;
-}
-
-// Lane state of one ray.
-template <class Mt> struct Lane {
-    double y[Mt::N];
-    Consts<double> cs;
-    double h, lam;
-    int32_t steps, attempts, att64;
-    int32_t frame;
-    int32_t hit; // sphere index of a SPHERE termination
-    int64_t rid;
-    D3 P0; // Cartesian position at the start of the step (sphere chord); the hit normal after a hit
-};
-
-// lane_init and lane_finish run once per ray; they are kept out of line so
-// that their transcendental constants and temporaries do not occupy
-// registers across the integration loop.
-template <class Mt>
-__device__ __noinline__ void lane_init(const JobDev &job, Lane<Mt> &L, int64_t id) {
-    L.rid = id;
-    L.frame = job.explicit_x0u0 ? 0 : find_frame(job, id);
-    const FrameDev &fr = job.frames[L.frame];
-    if (job.explicit_x0u0) {
-        const double *s = job.explicit_x0u0 + 8 * id;
-        state_from_xu<Mt>(s, s + 4, fr.M, fr.a, L.y, L.cs);
-    } else {
-        const D3 d = camera_ray(fr, id - fr.ray_begin);
-        init_state<Mt>(d3(fr.pos[0], fr.pos[1], fr.pos[2]), d, fr.M, fr.a, L.y, L.cs);
-    }
-    L.h = job.tr.h;
-    L.lam = 0.0;
-    L.steps = L.attempts = L.att64 = 0;
-    if (fr.n_spheres > 0) L.P0 = cart_of<Mt>(L.y);
-}
-
-template <class Mt>
-__device__ inline bool finite_state(const double *y) {
-    bool ok = true;
-#pragma unroll
-    for (int i = 0; i < Mt::N; ++i) ok = ok && isfinite(y[i]);
-    return ok;
-}
-
-// Writes the outputs of a finished ray (rayToRGB, metrics/schwarzschild.cuh:151-183).
-template <class Mt>
-__device__ __noinline__ void lane_finish(const JobDev &job, const Lane<Mt> &L, int cls) {
-    const int64_t id = L.rid;
-    D3 endpoint;
-    float3 color;
-    if (cls == PRR_SPHERE) {
-        endpoint = L.P0;
-        const int slot = job.frames[L.frame].sph_tex[L.hit];
-        color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, endpoint);
-    } else if (cls == PRR_ESCAPE) {
-        const D3 n = escape_dir<Mt>(L.y, L.cs);
-        endpoint = n;
-        double th, ph;
-        if (Mt::kind == PRR_SCHW_SPH) {
-            const double two_pi = 6.28318530717958647692, pi = 3.14159265358979323846;
-            th = L.y[1];
-            ph = L.y[2];
-            if (ph == two_pi) ph = 0.0; // main/base.h:86-89
-            if (th == pi) th = 0.0;
-        } else {
-            th = acos(fmin(fmax(n.z, -1.0), 1.0));
-            ph = atan2(n.y, n.x);
-            if (ph < 0.0) ph += 6.28318530717958647692;
-        }
-        color = sky_lookup(job.tex[0], th, ph);
-    } else {
-        endpoint = cart_of<Mt>(L.y);
-        color = make_float3(0.f, 0.f, 0.f);
-    }
-    const OutDev &o = job.out;
-    if (o.color) {
-        o.color[3 * id] = color.x;
-        o.color[3 * id + 1] = color.y;
-        o.color[3 * id + 2] = color.z;
-    }
-    if (o.cls) o.cls[id] = int8_t(cls);
-    if (o.endpoint) {
-        o.endpoint[3 * id] = endpoint.x;
-        o.endpoint[3 * id + 1] = endpoint.y;
-        o.endpoint[3 * id + 2] = endpoint.z;
-    }
-    if (o.steps) o.steps[id] = L.steps;
-    if (o.attempts) o.attempts[id] = L.attempts;
-    if (o.state8) {
-        double x[4], u[4];
-        xu_from_state<Mt>(L.y, L.cs, x, u);
-        for (int c = 0; c < 4; ++c) {
-            o.state8[8 * id + c] = x[c];
-            o.state8[8 * id + 4 + c] = u[c];
-        }
-    }
-}
-
-// Per-step termination tests after an accepted step from yo (User_UPDATE /
-// User_ENDLOOP, metrics/schwarzschild.cuh:55-79): sphere chord, spherical
-// wraps, horizon, escape.  Returns the class or -1 to continue.
-template <class Mt>
-__device__ inline int after_step(const JobDev &job, Lane<Mt> &L) {
-    const FrameDev &fr = job.frames[L.frame];
-    if (fr.n_spheres > 0) {
-        const D3 P1 = cart_of<Mt>(L.y);
-        double t;
-        const int k = chord_hit(fr, L.P0, P1, t);
-        if (k >= 0) {
-            const D3 H = L.P0 + t * (P1 - L.P0);
-            L.P0 = (1.0 / fr.sph[k][3]) * (H - d3(fr.sph[k][0], fr.sph[k][1], fr.sph[k][2]));
-            L.hit = k;
-            return PRR_SPHERE;
-        }
-        L.P0 = P1;
-    }
-    if (Mt::kind == PRR_SCHW_SPH) {
-        sph_post_step(L.y);
-        if (fr.n_spheres > 0) L.P0 = cart_of<Mt>(L.y);
-    }
-    const double r = radius_of<Mt>(L.y);
-    if (r <= fr.r_hor) return PRR_HORIZON;
-    if (r >= job.tr.r_escape) return PRR_ESCAPE;
-    return -1;
-}
-
-// One Euler step or one RKF45 attempt.  Returns the class when the ray ends.
-template <class Mt, int SOLVER, bool MIXED>
-__device__ inline int lane_advance(const JobDev &job, Lane<Mt> &L) {
-    constexpr int N = Mt::N;
-    const TracerDev &tr = job.tr;
-    if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max) return PRR_STEP_CAP;
-    const bool fp64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
-    if (SOLVER == PRR_EULER) {
-        double f[N];
-        if (fp64) euler_rhs<Mt, double>(L.y, L.cs, f);
-        else euler_rhs<Mt, float>(L.y, L.cs, f);
-        bool ok = true;
-#pragma unroll
-        for (int i = 0; i < N; ++i) ok = ok && isfinite(f[i]);
-        if (!ok) return PRR_NAN;
-        const double h = L.h;
-#pragma unroll
-        for (int i = 0; i < N; ++i) L.y[i] += h * f[i];
-        L.steps++;
-        L.attempts++;
-        L.att64 += fp64 ? 1 : 0;
-        L.lam += h;
-        if (!finite_state<Mt>(L.y)) return PRR_NAN;
-        return after_step<Mt>(job, L);
-    }
-    if (L.attempts >= tr.max_attempts) return PRR_STEP_CAP;
-    L.attempts++;
-    L.att64 += fp64 ? 1 : 0;
-    double d4[N], e[N];
-    if (fp64) rkf_stages<Mt, double>(L.y, L.h, L.cs, d4, e);
-    else rkf_stages<Mt, float>(L.y, L.h, L.cs, d4, e);
-    double errn = 0.0;
-    double y4[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) {
-        y4[i] = L.y[i] + d4[i];
-        const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(y4[i]));
-        errn = fmax(errn, fabs(e[i]) / sc);
-    }
-    // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
-    double hn;
-    if (MIXED) hn = 0.9 * L.h * double(powf(float(errn), -0.2f));
-    else hn = 0.9 * L.h * pow(errn, -0.2);
-    hn = fmin(fmax(hn, tr.hmin), tr.hmax);
-    if (errn > 1.0 && L.h != tr.hmin) { // reject (:185-188)
-        L.h = hn;
-        return -1;
-    }
-#pragma unroll
-    for (int i = 0; i < N; ++i) L.y[i] = y4[i];
-    if (!finite_state<Mt>(L.y)) return PRR_NAN;
-    L.steps++;
-    L.lam += L.h;
-    L.h = hn;
-    return after_step<Mt>(job, L);
-}
-
 __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
     const int lo = __shfl(int(v & 0xffffffffull), src);
     const int hi = __shfl(int(v >> 32), src);
     return (unsigned long long)(unsigned(lo)) | ((unsigned long long)(unsigned(hi)) << 32);
 }
 
-template <class Mt, int SOLVER, bool MIXED>
+struct KerrStepOut {
+    Lane<Kerr> L;
+    int cls;
+};
+template <bool MIXED> __device__ __noinline__ KerrStepOut kerr_attempt_shared(const JobDev &job, const Lane<Kerr> L);
+
+// SHARED: call the adjoint's out-of-line attempt function (Kerr, RKF45).
+template <class Mt, int SOLVER, bool MIXED, bool SHARED>
 __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
     const int lane = int(threadIdx.x & 63);
     Lane<Mt> L;
@@ -311,7 +61,7 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                     __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
                 const int64_t id = int64_t(base) + rank;
                 if (id < job.total) {
-                    lane_init<Mt>(job, L, id);
+                    L = lane_start<Mt>(job, id);
                     active = true;
                     if (!finite_state<Mt>(L.y)) {
                         lane_finish<Mt>(job, L, PRR_NAN);
@@ -323,7 +73,13 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
         }
         if (__ballot(active) == 0ull) break;
         if (active) {
-            const int cls = lane_advance<Mt, SOLVER, MIXED>(job, L);
+            int cls;
+            if constexpr (SHARED) {
+                const KerrStepOut r = kerr_attempt_shared<MIXED>(job, L);
+                L = r.L;
+                cls = r.cls;
+            }
+            else cls = lane_attempt<Mt, SOLVER, MIXED>(job, L);
             if (cls >= 0) {
                 lane_finish<Mt>(job, L, cls);
                 acc_att += (unsigned long long)L.attempts;
@@ -489,22 +245,28 @@ __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
 }
 
 // ----------------------------------------------------------------- adjoint (Kerr)
-// Pass 1 is k_integrate itself (step counts and colours).  Pass 2 re-runs
-// the same integration, stores a checkpoint every K accepted steps and the
-// terminal seed dL/dz_N.  Pass 3 walks each ray's segments backwards:
-// replay K steps from a checkpoint (same lane_advance, so the same accepted
-// steps bit for bit), then the reverse sweep kerr_step_vjp over them.
+// Pass 1 renders with step counts and colours, pass 2 re-runs the
+// integration and stores a checkpoint every K accepted steps, pass 3 walks
+// each ray's segments backwards: replay K steps from a checkpoint, then the
+// reverse sweep kerr_step_vjp over them.  All three passes call one
+// out-of-line attempt function, so their accepted steps agree bit for bit.
+// (lane data passed by value, so the callers' lane state stays in registers)
+template <bool MIXED> __device__ __noinline__ KerrStepOut kerr_attempt_shared(const JobDev &job, const Lane<Kerr> L) {
+    KerrStepOut o;
+    o.L = L;
+    o.cls = lane_attempt<Kerr, PRR_RKF45, MIXED>(job, o.L);
+    return o;
+}
 
 // Weights dL/dc of one ray's colour from dL/dI of its pixel through the SSAA
 // combine (kernels/ssaa.cuh:116-130): s = 1, I = clamp(c); s > 1,
 // I = sqrt(mean(clamp(c^2))).
-__device__ inline void color_weights(const JobDev &job, const AdjDev &ad, const FrameDev &fr, int64_t local,
-                                     const float3 c, double w[3]) {
+__device__ inline void color_weights(const AdjDev &ad, const FrameDev &fr, int64_t local, const float *c,
+                                     double w[3]) {
     const int64_t pix = local / fr.s;
-    const float cc[3] = {c.x, c.y, c.z};
     for (int ch = 0; ch < 3; ++ch) {
         const double g = double(ad.dLdI[3 * pix + ch]);
-        const double cv = double(cc[ch]);
+        const double cv = double(c[ch]);
         if (fr.s == 1) {
             w[ch] = (cv > 0.0 && cv < 1.0) ? g : 0.0;
         } else {
@@ -514,77 +276,26 @@ __device__ inline void color_weights(const JobDev &job, const AdjDev &ad, const 
     }
 }
 
-// Terminal seed of a finished ray: dL/dz_N (8), dL/dz_{N-1} (r, mu, phi) and
-// dL/dphi_s, by forward-mode duals through the colour of its class.
-__device__ __noinline__ void adj_seed(const JobDev &job, const AdjDev &ad, const Lane<Kerr> &L, int cls,
-                                      const double yp[3]) {
-    double *s = ad.seed + 12 * L.rid;
-    for (int i = 0; i < 12; ++i) s[i] = 0.0;
-    if (cls != PRR_ESCAPE && cls != PRR_SPHERE) return;
-    const FrameDev &fr = job.frames[L.frame];
-    const float *rc = ad.ray_color + 3 * L.rid;
-    double w[3];
-    color_weights(job, ad, fr, L.rid - fr.ray_begin, make_float3(rc[0], rc[1], rc[2]), w);
-    if (w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0) return;
-    using D = Dual<8>;
-    D col[3];
-    if (cls == PRR_ESCAPE) {
-        D y[5];
-        for (int i = 0; i < 5; ++i) y[i] = D::var(L.y[i], i);
-        const V3<D> n = kerr_escape_dir_t<D>(y, D::var(L.cs.E, 5), D::var(L.cs.L, 6), D::var(L.cs.a, 7), L.cs.M);
-        sky_color_t<D>(job.tex[0], n, col);
-        for (int i = 0; i < 8; ++i) s[i] = w[0] * col[0].d[i] + w[1] * col[1].d[i] + w[2] * col[2].d[i];
-        return;
-    }
-    // SPHERE: directions 0-2 = (r, mu, phi) at step N-1, 3-5 = at step N, 6 = phi_s
-    D y0[5], y1[5];
-    for (int i = 0; i < 5; ++i) {
-        y0[i] = D(0.0);
-        y1[i] = D(L.y[i]);
-    }
-    for (int i = 0; i < 3; ++i) {
-        y0[i] = D::var(yp[i], i);
-        y1[i] = D::var(L.y[i], 3 + i);
-    }
-    const int k = L.hit;
-    V3<D> c = v3t<D>(D(fr.sph[k][0]), D(fr.sph[k][1]), D(fr.sph[k][2]));
-    if (k == 0) { // the orbit's sphere: centre (R_orbit cos phi_s, R_orbit sin phi_s, 0)
-        const double phs = atan2(fr.sph[0][1], fr.sph[0][0]);
-        D sp, cp;
-        sincos(D::var(phs, 6), &sp, &cp);
-        c = v3t<D>(cp * ad.sphere_r, sp * ad.sphere_r, D(0.0));
-    }
-    const int slot = fr.sph_tex[k];
-    sphere_hit_color_t<D>((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, kerr_cart_t<D>(y0),
-                          kerr_cart_t<D>(y1), c, fr.sph[k][3], col);
-    for (int i = 0; i < 3; ++i) {
-        s[8 + i] = w[0] * col[0].d[i] + w[1] * col[1].d[i] + w[2] * col[2].d[i];
-        s[i] = w[0] * col[0].d[3 + i] + w[1] * col[1].d[3 + i] + w[2] * col[2].d[3 + i];
-    }
-    s[11] = w[0] * col[0].d[6] + w[1] * col[1].d[6] + w[2] * col[2].d[6];
-}
-
 __device__ inline void ckpt_store(double *c, const Lane<Kerr> &L) {
-    for (int i = 0; i < 5; ++i) c[i] = L.y[i];
-    c[5] = L.h;
-    c[6] = L.lam;
+    for (int i = 0; i < 6; ++i) c[i] = L.y[i];
+    c[6] = L.h;
     c[7] = double(L.attempts);
 }
 
-__device__ inline void ckpt_load(const double *c, Lane<Kerr> &L, int32_t steps) {
-    for (int i = 0; i < 5; ++i) L.y[i] = c[i];
-    L.h = c[5];
-    L.lam = c[6];
+__device__ inline void ckpt_load(const JobDev &job, const double *c, Lane<Kerr> &L, int32_t steps) {
+    for (int i = 0; i < 6; ++i) L.y[i] = c[i];
+    L.h = c[6];
     L.attempts = int32_t(c[7]);
+    L.lam = L.y[5];
     L.steps = steps;
-    L.P0 = cart_of<Kerr>(L.y);
+    L.hit = -1;
+    if (job.frames[L.frame].n_spheres > 0) cart_of_f32<Kerr>(L.y, L.P0);
 }
 
-template <bool MIXED>
-__global__ void __launch_bounds__(256) k_adj_forward(JobDev job, AdjDev ad) {
+template <bool MIXED> __global__ void __launch_bounds__(256) k_adj_forward(JobDev job, AdjDev ad) {
     const int lane = int(threadIdx.x & 63);
     Lane<Kerr> L;
-    double yp[3] = {0.0, 0.0, 0.0};
+    int32_t N = 0;
     bool active = false, exhausted = false;
     while (true) {
         const unsigned long long idle = __ballot(!active);
@@ -600,38 +311,77 @@ __global__ void __launch_bounds__(256) k_adj_forward(JobDev job, AdjDev ad) {
                     __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
                 const int64_t id = int64_t(base) + rank;
                 if (id < job.total) {
-                    lane_init<Kerr>(job, L, id);
-                    active = true;
+                    L = lane_start<Kerr>(job, id);
+                    N = ad.steps[id];
                     ckpt_store(ad.ckpt + 8 * ad.ckpt_off[id], L);
-                    if (!finite_state<Kerr>(L.y)) {
-                        adj_seed(job, ad, L, PRR_NAN, yp);
-                        active = false;
-                    }
+                    active = N > 0 && finite_state<Kerr>(L.y);
                 }
             }
         }
         if (__ballot(active) == 0ull) break;
         if (active) {
-            const double y0 = L.y[0], y1 = L.y[1], y2 = L.y[2];
             const int32_t before = L.steps;
-            const int cls = lane_advance<Kerr, PRR_RKF45, MIXED>(job, L);
-            if (L.steps != before) {
-                yp[0] = y0;
-                yp[1] = y1;
-                yp[2] = y2;
-                if (L.steps % ad.K == 0 && cls < 0) ckpt_store(ad.ckpt + 8 * (ad.ckpt_off[L.rid] + L.steps / ad.K), L);
-            }
-            if (cls >= 0) {
-                adj_seed(job, ad, L, cls, yp);
-                active = false;
-            }
+            const KerrStepOut r = kerr_attempt_shared<MIXED>(job, L);
+            L = r.L;
+            const int cls = r.cls;
+            if (cls < 0 && L.steps != before && L.steps < N && L.steps % ad.K == 0)
+                ckpt_store(ad.ckpt + 8 * (ad.ckpt_off[L.rid] + L.steps / ad.K), L);
+            if (cls >= 0 || L.steps >= N) active = false;
         }
     }
 }
 
-// Ray set-up Jacobian dz_0/d(a, r_cam, incl) contracted with lambda_0.
-__device__ __noinline__ void adj_init_grad(const JobDev &job, const AdjDev &ad, const Lane<Kerr> &L,
-                                           const double lam[8], double g[4]) {
+// Terminal seed: lambda at y_{N-1} (state and constants) and dL/dphi_s from
+// the located event of the last step (see adjoint.hpp).
+__device__ __noinline__ void adj_terminal(const JobDev &job, const AdjDev &ad, const Lane<Kerr> L,
+                                          const EventDev &ev, const double w[3], double lam[9], double &gphs) {
+    const FrameDev &fr = job.frames[L.frame];
+    const double tau = ev.theta * ev.h;
+    double ys[6];
+    rk_substep<Kerr>(ev.y_prev, tau, L.cs, ys);
+    using D = Dual<10>;
+    D y[5];
+    for (int i = 0; i < 5; ++i) y[i] = D::var(ys[i], i);
+    const D E = D::var(L.cs.E, 5), Lz = D::var(L.cs.L, 6), Q = D::var(L.cs.Q, 7), a = D::var(L.cs.a, 8);
+    D col[3], G;
+    if (ev.cls == PRR_ESCAPE) {
+        const V3<D> n = kerr_escape_dir_t<D>(y, E, Lz, Q, a, L.cs.M);
+        sky_color_t<D>(job.tex[0], n, col);
+        G = D(1.0 / job.tr.r_escape) - y[0];
+    } else {
+        const int k = ev.hit;
+        V3<D> c = v3t<D>(D(fr.sph[k][0]), D(fr.sph[k][1]), D(fr.sph[k][2]));
+        if (k == 0) { // the orbit's sphere: centre (R_orbit cos phi_s, R_orbit sin phi_s, 0)
+            const double phs = atan2(fr.sph[0][1], fr.sph[0][0]);
+            D sp, cp;
+            sincos(D::var(phs, 9), &sp, &cp);
+            c = v3t<D>(cp * ad.sphere_r, sp * ad.sphere_r, D(0.0));
+        }
+        const V3<D> m = vsub(kerr_cart_t<D>(y), c);
+        const int slot = fr.sph_tex[k];
+        sphere_color_t<D>((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, vnormalize(m), col);
+        G = vdot(m, m) - fr.sph[k][3] * fr.sph[k][3];
+    }
+    double ls[9];
+    for (int i = 0; i < 9; ++i) ls[i] = w[0] * col[0].d[i] + w[1] * col[1].d[i] + w[2] * col[2].d[i];
+    gphs = w[0] * col[0].d[9] + w[1] * col[1].d[9] + w[2] * col[2].d[9];
+    double f[6];
+    Kerr::rhs<double>(ys, Kerr::coef<double>(L.cs), f);
+    double num = 0.0, den = 0.0;
+    for (int i = 0; i < 5; ++i) {
+        num += ls[i] * f[i];
+        den += G.d[i] * f[i];
+    }
+    double kappa = num / den;
+    if (!isfinite(kappa)) kappa = 0.0;
+    for (int i = 0; i < 9; ++i) lam[i] = ls[i] - kappa * G.d[i];
+    gphs -= kappa * G.d[9];
+    kerr_step_vjp(ev.y_prev, tau, L.cs, lam);
+}
+
+// Ray set-up Jacobian d(y_0, E, L, Q)/d(a, r_cam, incl) contracted with lambda_0.
+__device__ __noinline__ void adj_init_grad(const JobDev &job, const AdjDev &ad, const Lane<Kerr> L,
+                                           const double lam[9], double g[4]) {
     const FrameDev &fr = job.frames[L.frame];
     double u, v;
     ray_uv(fr, L.rid - fr.ray_begin, u, v);
@@ -640,17 +390,16 @@ __device__ __noinline__ void adj_init_grad(const JobDev &job, const AdjDev &ad, 
     using D = Dual<3>;
     V3<D> pos, d;
     orbit_camera_ray<D>(D::var(r_cam, 1), D::var(incl, 2), ad.cam_phi, fr.tan_half, fr.aspect, u, v, pos, d);
-    D z[7];
+    D z[8];
     kerr_init_t<D>(pos, d, fr.M, D::var(fr.a, 0), z);
     for (int k = 0; k < 3; ++k) {
-        double acc = k == 0 ? lam[7] : 0.0; // a is itself part of z
-        for (int i = 0; i < 7; ++i) acc += lam[i] * z[i].d[k];
+        double acc = k == 0 ? lam[8] : 0.0; // a is itself one of the constants
+        for (int i = 0; i < 8; ++i) acc += lam[i] * z[i].d[k];
         g[k] = acc;
     }
 }
 
-template <bool MIXED>
-__global__ void __launch_bounds__(256) k_adj_backward(JobDev job, AdjDev ad) {
+template <bool MIXED> __global__ void __launch_bounds__(256) k_adj_backward(JobDev job, AdjDev ad) {
     const int lane = int(threadIdx.x & 63);
     const int64_t gl = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; // this lane's segment buffer
     Lane<Kerr> L;
@@ -678,63 +427,76 @@ __global__ void __launch_bounds__(256) k_adj_backward(JobDev job, AdjDev ad) {
         if (!active) continue;
         // one whole ray per lane: replay + reverse every segment
         const int64_t id = L.rid;
-        const double *sd = ad.seed + 12 * id;
-        double g[4] = {0.0, 0.0, 0.0, sd[11]};
-        bool any = false;
-        for (int i = 0; i < 11; ++i) any = any || sd[i] != 0.0;
+        const int cls = ad.cls[id];
         const int32_t N = ad.steps[id];
-        if (any && N > 0) {
-            lane_init<Kerr>(job, L, id); // constants E, L and the frame
-            double lam[8];
-            for (int i = 0; i < 8; ++i) lam[i] = sd[i];
+        double g[4] = {0.0, 0.0, 0.0, 0.0};
+        double w[3] = {0.0, 0.0, 0.0};
+        if ((cls == PRR_ESCAPE || cls == PRR_SPHERE) && N > 0) {
+            L = lane_start<Kerr>(job, id); // constants E, L, Q and the frame
+            const FrameDev &fr = job.frames[L.frame];
+            color_weights(ad, fr, id - fr.ray_begin, ad.ray_color + 3 * id, w);
+        }
+        if (w[0] != 0.0 || w[1] != 0.0 || w[2] != 0.0) {
+            double lam[9];
+            double gphs = 0.0;
             const int64_t c0 = ad.ckpt_off[id];
             bool ok = true;
-            for (int32_t j = (N - 1) / ad.K; j >= 0; --j) {
-                ckpt_load(ad.ckpt + 8 * (c0 + j), L, j * ad.K);
+            for (int32_t j = (N - 1) / ad.K; j >= 0 && ok; --j) {
+                ckpt_load(job, ad.ckpt + 8 * (c0 + j), L, j * ad.K);
                 const int32_t end = min((j + 1) * ad.K, N);
                 // replay: record (y_n, h_n) of every accepted step of the segment
+                int rc = -1;
                 while (L.steps < end) {
                     double ys[5];
                     for (int i = 0; i < 5; ++i) ys[i] = L.y[i];
-                    const double hs = L.h;
+                    const double hb = L.h;
                     const int32_t before = L.steps;
-                    const int cls = lane_advance<Kerr, PRR_RKF45, MIXED>(job, L);
+                    const KerrStepOut r = kerr_attempt_shared<MIXED>(job, L);
+                    L = r.L;
+                    rc = r.cls;
                     if (L.steps != before) {
                         double *slot = ad.seg + (int64_t(before - j * ad.K) * 6) * ad.seg_lanes + gl;
                         for (int i = 0; i < 5; ++i) slot[i * ad.seg_lanes] = ys[i];
-                        slot[5 * ad.seg_lanes] = hs;
+                        bool at_min;
+                        slot[5 * ad.seg_lanes] = rkf_step_size<Kerr>(job.tr, ys, L.cs, hb, at_min);
                     }
-                    // a ray only ends at step N; ending earlier means the replay diverged
-                    if (cls >= 0 && L.steps < end) {
-                        atomicAdd(ad.replay_mismatch, 1ull);
-                        break;
-                    }
+                    if (rc >= 0) break;
                 }
-                if (L.steps < end) { // diverged: this ray contributes nothing
+                // the ray must end exactly at step N with its class, and nowhere earlier
+                const bool last = end == N;
+                if (L.steps != end || (last ? rc != cls : rc >= 0)) {
+                    atomicAdd(ad.replay_mismatch, 1ull);
                     ok = false;
-                    g[3] = 0.0;
                     break;
                 }
-                if (end < N && end % ad.K == 0) { // the replay must land on the next checkpoint
+                if (!last) { // the replay must land on the next checkpoint
                     const double *nx = ad.ckpt + 8 * (c0 + j + 1);
-                    bool same = L.steps == end;
-                    for (int i = 0; i < 5; ++i) same = same && L.y[i] == nx[i];
-                    if (!same) atomicAdd(ad.replay_mismatch, 1ull);
-                }
-                // reverse sweep
-                for (int32_t n = end - 1; n >= j * ad.K; --n) {
-                    const double *slot = ad.seg + (int64_t(n - j * ad.K) * 6) * ad.seg_lanes + gl;
-                    double yn[5];
-                    for (int i = 0; i < 5; ++i) yn[i] = slot[i * ad.seg_lanes];
-                    kerr_step_vjp(yn, slot[5 * ad.seg_lanes], L.cs, lam);
-                    if (n == N - 1) {
-                        lam[0] += sd[8];
-                        lam[1] += sd[9];
-                        lam[2] += sd[10];
+                    bool same = true;
+                    for (int i = 0; i < 6; ++i) same = same && L.y[i] == nx[i];
+                    if (!same) {
+                        atomicAdd(ad.replay_mismatch, 1ull);
+                        ok = false;
+                        break;
                     }
+                } else {
+                    adj_terminal(job, ad, L, job.out.event[id], w, lam, gphs);
+                }
+                // reverse sweep (the terminal step N-1 is the located sub-step above)
+                for (int32_t n = min(end, N - 1) - 1; n >= j * ad.K; --n) {
+                    const double *slot = ad.seg + (int64_t(n - j * ad.K) * 6) * ad.seg_lanes + gl;
+                    double yn[6];
+                    for (int i = 0; i < 5; ++i) yn[i] = slot[i * ad.seg_lanes];
+                    yn[5] = 0.0;
+                    kerr_step_vjp(yn, slot[5 * ad.seg_lanes], L.cs, lam);
                 }
             }
-            if (ok) adj_init_grad(job, ad, L, lam, g);
+            if (ok) {
+                adj_init_grad(job, ad, L, lam, g);
+                g[3] = gphs;
+                bool fin = true;
+                for (int k = 0; k < 4; ++k) fin = fin && isfinite(g[k]);
+                if (!fin) g[0] = g[1] = g[2] = g[3] = 0.0;
+            }
         }
         double *out = ad.grad_ray + 4 * id;
         for (int k = 0; k < 4; ++k) out[k] = g[k];
@@ -763,7 +525,10 @@ __global__ void __launch_bounds__(1024) k_grad_reduce(const double *grad_ray, in
 hipError_t launch_adjoint(int pass, int precision, const JobDev &job, const AdjDev &ad, int grid, int block,
                           hipStream_t st) {
     const bool mixed = precision == PRR_FP32_BULK;
-    if (pass == 2) {
+    if (pass == 1) {
+        if (mixed) hipLaunchKernelGGL((k_integrate<Kerr, PRR_RKF45, true, true>), dim3(grid), dim3(block), 0, st, job);
+        else hipLaunchKernelGGL((k_integrate<Kerr, PRR_RKF45, false, true>), dim3(grid), dim3(block), 0, st, job);
+    } else if (pass == 2) {
         if (mixed) hipLaunchKernelGGL((k_adj_forward<true>), dim3(grid), dim3(block), 0, st, job, ad);
         else hipLaunchKernelGGL((k_adj_forward<false>), dim3(grid), dim3(block), 0, st, job, ad);
     } else {
@@ -784,11 +549,11 @@ static hipError_t launch_metric(int solver, int precision, const JobDev &job, di
                                 hipStream_t st) {
     const bool mixed = precision == PRR_FP32_BULK;
     if (solver == PRR_EULER) {
-        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true>), grid, block, 0, st, job);
-        else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false>), grid, block, 0, st, job);
+        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true, false>), grid, block, 0, st, job);
+        else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false, false>), grid, block, 0, st, job);
     } else {
-        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true>), grid, block, 0, st, job);
-        else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false>), grid, block, 0, st, job);
+        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true, false>), grid, block, 0, st, job);
+        else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false, false>), grid, block, 0, st, job);
     }
     return hipGetLastError();
 }

