@@ -327,10 +327,10 @@ struct Kerr {
         g[5] = -du * (2.0f * P.u0 + du) * P.iu02 * (q * D * s2) + P.a2 * tm;
     }
 
-    // FP32-bulk steps re-impose the first integrals u'^2 = P(u) and
+    // Every accepted step re-imposes the first integrals u'^2 = P(u) and
     // mu'^2 = Th(mu) in FP64 (E, L, Q are exact per-ray constants), keeping
     // the signs, except near turning points where the square roots are
-    // ill-conditioned.  Rounding of the FP32 stages otherwise moves the state
+    // ill-conditioned.  Truncation and FP32 rounding otherwise move the state
     // off the constraint surface, and a pass near the polar axis amplifies
     // that drift (phi jumps by ~pi L / sqrt(K sin^2 th_min)).
     PRR_HD static void project(double *y, const Consts<double> &c) {

@@ -671,6 +671,8 @@ template <class Mt, int SOLVER, bool MIXED> PRR_HD int lane_attempt(const JobDev
             errn = fmax(errn, ei / sc);
         }
     }
+    // Kerr: a step may not reach the polar axis (|cos th| >= 1), where the chart is singular
+    if (Mt::mino && !(fabs(yn[2]) < 1.0)) errn = fmax(errn, 1024.0);
     // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
     double hn;
     if (MIXED) hn = 0.9 * h * double(powf(float(errn), -0.2f));
@@ -681,9 +683,7 @@ template <class Mt, int SOLVER, bool MIXED> PRR_HD int lane_attempt(const JobDev
         return -1;
     }
     if (!finite_state<Mt>(yn)) return PRR_NAN;
-    if constexpr (Mt::mino && MIXED) {
-        if (!fp64) Kerr::project(yn, L.cs);
-    }
+    if constexpr (Mt::mino) Kerr::project(yn, L.cs);
     L.steps++;
     L.lam = Mt::mino ? yn[5] : L.lam + h;
     L.h = hn;
