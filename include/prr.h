@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PRR_ABI_VERSION 1
+#define PRR_ABI_VERSION 2
 #define PRR_MAX_SPHERES 8
 #define PRR_MAX_TEXTURES 9 /* slot 0 = celestial sphere, slots 1..8 = sphere textures */
 
@@ -176,6 +176,8 @@ typedef struct prr_stats {
     double fxaa_bytes;      /* algorithmic HBM bytes of the FXAA pass */
     double adjoint_ms;      /* prr_adjoint: checkpointed forward + backward passes */
     int64_t adjoint_steps;  /* prr_adjoint: accepted steps swept backwards */
+    double init_ms;         /* ray set-up kernel */
+    double shade_ms;        /* shading kernel */
 } prr_stats;
 
 /* Differentiable parameters of an orbiting-camera frame (adjoint). The

@@ -5,7 +5,7 @@ and offsets against the compiled library.
 """
 import ctypes as C
 
-PRR_ABI_VERSION = 1
+PRR_ABI_VERSION = 2
 PRR_MAX_SPHERES = 8
 PRR_MAX_TEXTURES = 9
 
@@ -82,7 +82,7 @@ class Stats(C.Structure):
                 ("integrate_ms", C.c_double), ("ssaa_ms", C.c_double), ("fxaa_ms", C.c_double),
                 ("total_ms", C.c_double), ("flops_fp64", C.c_double), ("flops_fp32", C.c_double),
                 ("ssaa_bytes", C.c_double), ("fxaa_bytes", C.c_double), ("adjoint_ms", C.c_double),
-                ("adjoint_steps", C.c_int64)]
+                ("adjoint_steps", C.c_int64), ("init_ms", C.c_double), ("shade_ms", C.c_double)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -51,9 +51,11 @@ struct prr_ctx {
     // device buffers, grown on demand
     DevBuf frames, counters, color, post, img_scratch, stage_rgbf, stage_rgb8, cls, endpoint, steps, attempts,
         state8, x0u0;
+    // ray-state arrays and per-lane event location contexts
+    DevBuf rs_y, rs_cls, rs_hit, rs_steps, rs_attempts, loc;
     // adjoint
-    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_event, adj_seg, adj_grad_ray, adj_grad;
-    hipEvent_t ev[5] = {};
+    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_event, adj_rays, adj_rec, adj_lam, adj_grad_ray, adj_grad;
+    hipEvent_t ev[6] = {};
     prr_stats stats{};
 };
 
@@ -263,7 +265,9 @@ void prr_destroy(prr_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->frames, &ctx->counters, &ctx->color, &ctx->post, &ctx->img_scratch,
                       &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
                       &ctx->attempts, &ctx->state8, &ctx->x0u0, &ctx->adj_img, &ctx->adj_dldi,
-                      &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_event, &ctx->adj_seg, &ctx->adj_grad_ray,
+                      &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_event, &ctx->adj_rays, &ctx->adj_rec,
+                      &ctx->adj_lam, &ctx->rs_y, &ctx->rs_cls, &ctx->rs_hit, &ctx->rs_steps, &ctx->rs_attempts,
+                      &ctx->loc, &ctx->adj_grad_ray,
                       &ctx->adj_grad};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : ctx->ev)
@@ -371,6 +375,8 @@ struct RenderPlan {
     double ssaa_bytes = 0, fxaa_bytes = 0;
 };
 
+prr_status ensure_pipeline(prr_ctx *ctx, int64_t total);
+
 // Builds the device frame table and post-processing descriptors and makes
 // sure every buffer exists.  Images given on the device are written in
 // place; host images go through staging buffers.
@@ -398,6 +404,7 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
     if ((st = ensure(ctx, ctx->post, sizeof(PostFrameDev) * n)) != PRR_OK) return st;
     if (P.any_fxaa && (st = ensure(ctx, ctx->img_scratch, sizeof(float) * 3 * size_t(P.pixels))) != PRR_OK)
         return st;
+    if ((st = ensure_pipeline(ctx, P.total)) != PRR_OK) return st;
     if (need_host_staging) {
         if ((st = ensure(ctx, ctx->stage_rgbf, sizeof(float) * 3 * size_t(P.pixels))) != PRR_OK) return st;
         if ((st = ensure(ctx, ctx->stage_rgb8, 3 * size_t(P.pixels))) != PRR_OK) return st;
@@ -429,6 +436,23 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
     return PRR_OK;
 }
 
+int rs_width(int metric) { return metric == PRR_KERR_BL ? 9 : 8; }
+
+int integrate_grid(const prr_ctx *ctx) { return ctx->cus * blocks_per_cu(ctx); }
+
+// Ray-state arrays for `total` rays and one event-location context per lane.
+prr_status ensure_pipeline(prr_ctx *ctx, int64_t total) {
+    prr_status s;
+    const size_t T = size_t(total);
+    if ((s = ensure(ctx, ctx->rs_y, 8 * size_t(rs_width(ctx->metric)) * T)) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->rs_cls, T)) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->rs_hit, T)) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->rs_steps, 4 * T)) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->rs_attempts, 4 * T)) != PRR_OK) return s;
+    const size_t lanes = size_t(integrate_grid(ctx)) * size_t(ctx->launch.block);
+    return ensure(ctx, ctx->loc, sizeof(LocateCtx) * lanes);
+}
+
 JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
     JobDev job{};
     job.frames = static_cast<const FrameDev *>(ctx->frames.p);
@@ -439,22 +463,29 @@ JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
     job.tr = tracer_dev(ctx->tracer);
     for (int i = 0; i < PRR_MAX_TEXTURES; ++i) job.tex[i] = ctx->tex[i];
     job.out.color = static_cast<float *>(ctx->color.p);
+    job.rs.y = static_cast<double *>(ctx->rs_y.p);
+    job.rs.cls = static_cast<int8_t *>(ctx->rs_cls.p);
+    job.rs.hit = static_cast<int8_t *>(ctx->rs_hit.p);
+    job.rs.steps = static_cast<int32_t *>(ctx->rs_steps.p);
+    job.rs.attempts = static_cast<int32_t *>(ctx->rs_attempts.p);
+    job.rs_total = P.total;
+    job.loc = static_cast<LocateCtx *>(ctx->loc.p);
     job.ctr = static_cast<CountersDev *>(ctx->counters.p);
     return job;
 }
 
-// Enqueue reset -> integrate -> SSAA -> FXAA with timing events.
-// adjoint_pass1: integrate through the adjoint's shared attempt function.
-prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events,
-                          bool adjoint_pass1 = false) {
+// Enqueue reset -> init -> integrate -> shade -> SSAA -> FXAA, with timing events.
+prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events) {
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     HIPTRY(ctx, launch_reset(job.ctr, st));
+    HIPTRY(ctx, launch_init(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    const int grid = ctx->cus * blocks_per_cu(ctx);
-    if (adjoint_pass1) HIPTRY(ctx, launch_adjoint(1, ctx->tracer.precision, job, AdjDev{}, grid, ctx->launch.block, st));
-    else HIPTRY(ctx, launch_integrate(ctx->metric, job, grid, ctx->launch.block, st));
+    HIPTRY(ctx, launch_integrate(ctx->metric, job, integrate_grid(ctx), ctx->launch.block, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
-    HIPTRY(ctx, launch_ssaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_pix, st));
+    HIPTRY(ctx, launch_shade(ctx->metric, job, st));
+    if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[5], st));
+    if (!P.pf.empty())
+        HIPTRY(ctx, launch_ssaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_pix, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[3], st));
     if (P.any_fxaa)
         HIPTRY(ctx, launch_fxaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_w, P.max_h,
@@ -476,9 +507,13 @@ prr_status read_stats(prr_ctx *ctx, const RenderPlan &P, hipStream_t st, bool ev
     s.pixels = P.pixels;
     if (events) {
         float ms = 0.f;
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[0], ctx->ev[1]));
+        s.init_ms = ms;
         HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[1], ctx->ev[2]));
         s.integrate_ms = ms;
-        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[3]));
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[2], ctx->ev[5]));
+        s.shade_ms = ms;
+        HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[5], ctx->ev[3]));
         s.ssaa_ms = ms;
         HIPTRY(ctx, hipEventElapsedTime(&ms, ctx->ev[3], ctx->ev[4]));
         s.fxaa_ms = ms;
@@ -595,6 +630,7 @@ prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int3
     if ((s = ensure(ctx, ctx->steps, 4 * size_t(n))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->attempts, 4 * size_t(n))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->cls, size_t(n))) != PRR_OK) return s;
+    if ((s = ensure_pipeline(ctx, n)) != PRR_OK) return s;
     JobDev job = make_job(ctx, P);
     job.explicit_x0u0 = static_cast<const double *>(ctx->x0u0.p);
     job.out = OutDev{};
@@ -604,14 +640,7 @@ prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int3
     job.out.cls = static_cast<int8_t *>(ctx->cls.p);
     HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipMemcpyAsync(ctx->x0u0.p, x0u0, 64 * size_t(n), hipMemcpyHostToDevice, st));
-    HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
-    HIPTRY(ctx, launch_reset(job.ctr, st));
-    HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    const int grid = std::min<int64_t>(ctx->cus * blocks_per_cu(ctx), (int64_t(n) + 63) / 64 + 1);
-    HIPTRY(ctx, launch_integrate(ctx->metric, job, grid, ctx->launch.block, st));
-    HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
-    HIPTRY(ctx, hipEventRecord(ctx->ev[3], st));
-    HIPTRY(ctx, hipEventRecord(ctx->ev[4], st));
+    if ((s = enqueue_render(ctx, P, job, st, true)) != PRR_OK) return s;
     std::vector<int8_t> c8(n);
     if (xend) HIPTRY(ctx, hipMemcpyAsync(xend, ctx->state8.p, 64 * size_t(n), hipMemcpyDeviceToHost, st));
     if (steps) HIPTRY(ctx, hipMemcpyAsync(steps, ctx->steps.p, 4 * size_t(n), hipMemcpyDeviceToHost, st));
@@ -736,6 +765,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if (orbit->has_sphere && tmpl->n_spheres > 1)
         return fail(ctx, PRR_EINVAL, "the adjoint's orbit sphere must be the frame's only sphere");
     const int K = ctx->launch.checkpoint;
+    if (K <= 0) return fail(ctx, PRR_EINVAL, "checkpoint interval must be positive");
     HIPTRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = ctx->stream;
     prr_frame f;
@@ -750,66 +780,100 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if ((s = ensure(ctx, ctx->steps, 4 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->cls, size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_off, 8 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_rays, 8 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_event, sizeof(EventDev) * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_lam, 80 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_rec, 48 * size_t(K) * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad_ray, 32 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad, 32)) != PRR_OK) return s;
     JobDev job = make_job(ctx, P);
     job.out.steps = static_cast<int32_t *>(ctx->steps.p);
     job.out.cls = static_cast<int8_t *>(ctx->cls.p);
+    CountersDev *ctr = static_cast<CountersDev *>(ctx->counters.p);
     HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
-    // pass 1: the render through the adjoint's attempt function (step counts,
-    // classes, ray colours, SSAA image)
-    if ((s = enqueue_render(ctx, P, job, st, false, true)) != PRR_OK) return s;
+    // pass 1: the render (step counts, classes, ray colours, SSAA image)
+    if ((s = enqueue_render(ctx, P, job, st, false)) != PRR_OK) return s;
     std::vector<int32_t> steps(T);
     HIPTRY(ctx, hipMemcpyAsync(steps.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
     if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises
     prr_stats fwd = ctx->stats;
-    // checkpoint offsets: ray i keeps checkpoints at steps 0, K, 2K, ... < N_i
-    std::vector<int64_t> off(T);
+    // checkpoints at steps 0, K, 2K, ... < N_i; rays by decreasing segment count
+    std::vector<int64_t> off(T), order(T);
     int64_t nck = 0, swept = 0;
+    int32_t nseg_max = 0;
     for (int64_t i = 0; i < T; ++i) {
         off[i] = nck;
-        nck += (steps[i] > 0 ? (steps[i] - 1) / K : 0) + 1;
+        const int32_t nseg = steps[i] > 0 ? (steps[i] - 1) / K + 1 : 0;
+        nck += std::max(nseg, 1);
         swept += steps[i];
+        nseg_max = std::max(nseg_max, nseg);
+        order[i] = i;
     }
+    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return steps[x] > steps[y]; });
     if ((s = ensure(ctx, ctx->adj_ckpt, 64 * size_t(nck))) != PRR_OK) return s;
-    const int grid = ctx->cus * blocks_per_cu(ctx);
-    const int64_t lanes = int64_t(grid) * ctx->launch.block;
-    if ((s = ensure(ctx, ctx->adj_seg, 48 * size_t(K) * size_t(lanes))) != PRR_OK) return s;
     HIPTRY(ctx, hipMemcpyAsync(ctx->adj_off.p, off.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_rays.p, order.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
     const float *dldi = dL_dimg;
     if (!on_device) {
         if ((s = ensure(ctx, ctx->adj_dldi, 12 * size_t(npix))) != PRR_OK) return s;
         HIPTRY(ctx, hipMemcpyAsync(ctx->adj_dldi.p, dL_dimg, 12 * size_t(npix), hipMemcpyHostToDevice, st));
         dldi = static_cast<const float *>(ctx->adj_dldi.p);
     }
+    HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
+    HIPTRY(ctx, launch_reset(ctr, st, true));
+    // pass 2: the same integration, storing checkpoints
+    JobDev j2 = job;
+    j2.out = OutDev{};
+    j2.adj.mode = ADJ_CHECKPOINT;
+    j2.adj.K = K;
+    j2.adj.nsteps = static_cast<const int32_t *>(ctx->steps.p);
+    j2.adj.ckpt_off = static_cast<const int64_t *>(ctx->adj_off.p);
+    j2.adj.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
+    j2.adj.mismatch = &ctr->mismatch;
+    HIPTRY(ctx, launch_init(ctx->metric, j2, st));
+    HIPTRY(ctx, launch_integrate(ctx->metric, j2, integrate_grid(ctx), ctx->launch.block, st));
+    // pass 3: per segment index, from the last: replay then reverse sweep
+    JobDev j3 = j2;
+    j3.adj.mode = ADJ_REPLAY;
+    j3.adj.rays = static_cast<const int64_t *>(ctx->adj_rays.p);
+    j3.adj.rec = static_cast<double *>(ctx->adj_rec.p);
+    j3.adj.event = static_cast<EventDev *>(ctx->adj_event.p);
     AdjDev ad{};
     ad.K = K;
-    ad.steps = static_cast<const int32_t *>(ctx->steps.p);
+    ad.rays = j3.adj.rays;
+    ad.nsteps = j2.adj.nsteps;
     ad.cls = static_cast<const int8_t *>(ctx->cls.p);
-    ad.ckpt_off = static_cast<const int64_t *>(ctx->adj_off.p);
-    ad.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
+    ad.rec = j3.adj.rec;
+    ad.event = j3.adj.event;
+    ad.lam = static_cast<double *>(ctx->adj_lam.p);
     ad.dLdI = dldi;
     ad.img = static_cast<const float *>(ctx->adj_img.p);
     ad.ray_color = static_cast<const float *>(ctx->color.p);
-    ad.seg = static_cast<double *>(ctx->adj_seg.p);
-    ad.seg_lanes = lanes;
     ad.grad_ray = static_cast<double *>(ctx->adj_grad_ray.p);
     ad.cam_phi = orbit->cam_phi;
     ad.sphere_r = orbit->sphere_r;
-    ad.replay_mismatch = &static_cast<CountersDev *>(ctx->counters.p)->mismatch;
-    JobDev job2 = job;
-    job2.out = OutDev{};
-    job2.out.event = static_cast<EventDev *>(ctx->adj_event.p);
-    HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    // pass 2: checkpointed forward
-    HIPTRY(ctx, launch_reset(job.ctr, st));
-    HIPTRY(ctx, launch_adjoint(2, ctx->tracer.precision, job2, ad, grid, ctx->launch.block, st));
-    // pass 3: replay + reverse sweep, one ray per lane
-    HIPTRY(ctx, launch_reset(job.ctr, st));
-    HIPTRY(ctx, launch_adjoint(3, ctx->tracer.precision, job2, ad, grid, ctx->launch.block, st));
+    ad.mismatch = &ctr->mismatch;
+    // rays with more than j segments: a prefix of the sorted order
+    std::vector<int64_t> count(size_t(nseg_max) + 1, 0);
+    for (int64_t i = 0; i < T; ++i) {
+        const int32_t nseg = steps[i] > 0 ? (steps[i] - 1) / K + 1 : 0;
+        for (int32_t j = 0; j < nseg; ++j) count[j]++;
+    }
+    for (int32_t j = nseg_max - 1; j >= 0; --j) {
+        j3.adj.seg = j;
+        j3.total = count[j];
+        HIPTRY(ctx, launch_reset(ctr, st, false));
+        const int grid = int(std::min<int64_t>(integrate_grid(ctx), (count[j] + 63) / 64 + 1));
+        HIPTRY(ctx, launch_integrate(ctx->metric, j3, grid, ctx->launch.block, st));
+        ad.seg = j;
+        ad.count = count[j];
+        HIPTRY(ctx, launch_adj_reverse(j3, ad, st));
+    }
+    JobDev jg = job;
+    jg.total = T;
+    HIPTRY(ctx, launch_adj_init_grad(jg, ad, st));
     HIPTRY(ctx, launch_grad_reduce(ad.grad_ray, T, static_cast<double *>(ctx->adj_grad.p), st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     CountersDev c{};
@@ -827,7 +891,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     ctx->stats.adjoint_ms = ms_adj;
     ctx->stats.adjoint_steps = swept;
     if (c.mismatch != 0)
-        return fail(ctx, PRR_EHIP, "adjoint replay did not reproduce %llu checkpoint segments", c.mismatch);
+        return fail(ctx, PRR_EHIP, "adjoint replay did not reproduce the forward pass (%llu segments or rays)",
+                    c.mismatch);
     return PRR_OK;
 }
 

@@ -27,6 +27,60 @@ template <class F> prr_status with_metric(int32_t metric, F &&f) {
 
 } // namespace
 
+namespace {
+
+// One ray through the host run of the pipeline: init_ray, lane_iterate until
+// the ray ends, shade_ray; `log` receives one row per accepted step.
+template <class Mt>
+prr_status host_ray(JobDev &job, const prr_tracer &tr, int64_t id, double *log, int max_n, int *n_log) {
+    constexpr int W = rs_width<Mt>();
+    double rsy[W];
+    int8_t cls1 = 0, hit1 = -1;
+    int32_t steps1 = 0, att1 = 0;
+    job.rs = RayStateDev{rsy, &cls1, &hit1, &steps1, &att1};
+    job.rs_total = 1;
+    job.rs_base = id;
+    init_ray<Mt>(job, id);
+    Lane<Mt> L;
+    lane_load<Mt>(job, L, id);
+    LocateCtx C{};
+    int mode = MODE_ACTIVE;
+    const bool mixed = tr.precision == PRR_FP32_BULK;
+    int n = 0;
+    if (!finite_state<Mt>(L.y)) {
+        lane_store<Mt>(job, L, PRR_NAN);
+        mode = MODE_IDLE;
+    }
+    while (mode != MODE_IDLE) {
+        const int32_t s0 = L.steps, a64 = L.att64;
+        const double h_before = L.h;
+        const int was = mode;
+        int c;
+        if (tr.solver == PRR_EULER)
+            c = mixed ? lane_iterate<Mt, PRR_EULER, true>(job, L, C, mode)
+                      : lane_iterate<Mt, PRR_EULER, false>(job, L, C, mode);
+        else
+            c = mixed ? lane_iterate<Mt, PRR_RKF45, true>(job, L, C, mode)
+                      : lane_iterate<Mt, PRR_RKF45, false>(job, L, C, mode);
+        if (log && was == MODE_ACTIVE && L.steps != s0 && n < max_n) {
+            double *row = log + 12 * n++;
+            row[0] = L.steps;
+            row[1] = L.attempts;
+            row[2] = h_before;
+            row[3] = L.att64 - a64;
+            // the accepted state (also while its events are being located)
+            const double *yy = mode == MODE_LOCATE ? C.yn : L.y;
+            for (int i = 0; i < 8; ++i) row[4 + i] = i < Mt::N ? yy[i] : 0.0;
+        }
+        if (c >= 0) lane_store<Mt>(job, L, c);
+    }
+    shade_ray<Mt>(job, id);
+    if (n_log) *n_log = n;
+    return PRR_OK;
+}
+
+} // namespace
+
 extern "C" {
 
 int prr_debug_state_size(int32_t metric) { return metric == PRR_KERR_BL ? Kerr::N : 8; }
@@ -215,17 +269,7 @@ prr_status prr_debug_trace_rays(int32_t metric, const prr_tracer *tracer, const 
         for (int i = 0; i < n; ++i) {
             const int64_t id = x0u0 ? i : ids[i];
             if (id < 0 || id >= job.total) return PRR_EINVAL;
-            Lane<Mt> L = lane_start<Mt>(job, id);
-            int c = finite_state<Mt>(L.y) ? -1 : PRR_NAN;
-            const bool mixed = tracer->precision == PRR_FP32_BULK;
-            while (c < 0) {
-                if (tracer->solver == PRR_EULER)
-                    c = mixed ? lane_attempt<Mt, PRR_EULER, true>(job, L) : lane_attempt<Mt, PRR_EULER, false>(job, L);
-                else
-                    c = mixed ? lane_attempt<Mt, PRR_RKF45, true>(job, L) : lane_attempt<Mt, PRR_RKF45, false>(job, L);
-            }
-            L.rid = 0;
-            lane_finish<Mt>(job, L, c);
+            host_ray<Mt>(job, *tracer, id, nullptr, 0, nullptr);
             if (cls) cls[i] = c1;
             if (steps) steps[i] = st1;
             if (attempts) attempts[i] = at1;
@@ -251,28 +295,7 @@ prr_status prr_debug_trace_log(int32_t metric, const prr_tracer *tracer, const p
     if (id < 0 || id >= job.total) return PRR_EINVAL;
     return with_metric(metric, [&](auto mt) {
         using Mt = decltype(mt);
-        Lane<Mt> L = lane_start<Mt>(job, id);
-        int c = finite_state<Mt>(L.y) ? -1 : PRR_NAN;
-        const bool mixed = tracer->precision == PRR_FP32_BULK;
-        int n = 0;
-        while (c < 0) {
-            const int32_t s0 = L.steps, a64 = L.att64;
-            const double h_before = L.h;
-            if (tracer->solver == PRR_EULER)
-                c = mixed ? lane_attempt<Mt, PRR_EULER, true>(job, L) : lane_attempt<Mt, PRR_EULER, false>(job, L);
-            else
-                c = mixed ? lane_attempt<Mt, PRR_RKF45, true>(job, L) : lane_attempt<Mt, PRR_RKF45, false>(job, L);
-            if (L.steps != s0 && n < max_n) {
-                double *row = log + 12 * n++;
-                row[0] = L.steps;
-                row[1] = L.attempts;
-                row[2] = h_before;
-                row[3] = L.att64 - a64;
-                for (int i = 0; i < 8; ++i) row[4 + i] = i < Mt::N ? L.y[i] : 0.0;
-            }
-        }
-        *n_out = n;
-        return PRR_OK;
+        return host_ray<Mt>(job, *tracer, id, log, max_n, n_out);
     });
 }
 

@@ -64,7 +64,52 @@ struct OutDev {
     int32_t *steps;
     int32_t *attempts;
     double *state8;   // 8 per ray (prr_trace)
-    EventDev *event;  // 1 per ray (adjoint)
+};
+
+// Ray state between the passes, structure of arrays over the job's rays:
+// y[k] at y + k * total (k < N), then E, L, Q for Kerr.  k_init writes the
+// initial state, k_integrate replaces it by the final one, k_shade reads it.
+struct RayStateDev {
+    double *y;
+    int8_t *cls;
+    int8_t *hit;
+    int32_t *steps;
+    int32_t *attempts;
+};
+
+// Event location in progress for one lane (RKF45): the accepted step's end
+// state, the bracket / best states and the search variables.  One per lane
+// of the integrator grid, in device memory; only lanes that locate touch it.
+struct LocateCtx {
+    double yn[8];    // end state of the accepted step
+    double yb[8];    // sphere: state at the inside end of the bracket
+    double ybest[8]; // sphere: state at the best hit so far
+    double h;        // size of the step
+    double th;       // fraction of the step of the pending sub-step
+    double ta, t1, tb, Ga, G1, Gb; // sphere: minimum search
+    double lo, hi, Glo, Ghi;       // sphere: entry bracket
+    double G0, best_th;
+    int32_t kind;  // 0 escape, 1 sphere
+    int32_t phase; // sphere: 0 next sphere, 1 first probe, 2 minimum search, 3 bracket
+    int32_t k, mask, it, side, best_k, pad;
+};
+
+// Adjoint modes of the integrator (AdjRunDev.mode).
+enum { ADJ_NONE = 0, ADJ_CHECKPOINT = 1, ADJ_REPLAY = 2 };
+
+// Kerr adjoint passes run by the integrator kernel itself.
+struct AdjRunDev {
+    int32_t mode;            // ADJ_*
+    int32_t K;               // accepted steps between checkpoints
+    int32_t seg;             // replay: the segment index j
+    int32_t pad;
+    const int32_t *nsteps;   // accepted steps of every ray (first pass)
+    const int64_t *ckpt_off; // index of each ray's first checkpoint
+    double *ckpt;            // 8 doubles per checkpoint: y[6], h, attempts
+    const int64_t *rays;     // replay: ray of each job index
+    double *rec;             // replay: K * 6 doubles per ray, [slot * 6 + c] * rs_total + ray
+    EventDev *event;         // replay: terminal event of each ray
+    unsigned long long *mismatch; // replay: segments that did not reproduce their checkpoint
 };
 
 // Device-side counters; `queue` is the ray queue head.
@@ -74,7 +119,7 @@ struct CountersDev {
     unsigned long long accepted;
     unsigned long long attempts_fp64;
     unsigned long long rays;
-    unsigned long long mismatch; // adjoint: segments whose replay did not reproduce the checkpoint
+    unsigned long long mismatch; // adjoint: replayed segments or rays that did not reproduce pass 1
     unsigned long long pad[2];
 };
 
@@ -82,31 +127,37 @@ struct JobDev {
     const FrameDev *frames;
     int32_t nframes;
     int32_t refill;
-    int64_t total;               // rays over all frames
+    int64_t total;               // rays over all frames (replay: job indices)
     const double *explicit_x0u0; // prr_trace: 8 initial doubles per ray, frame 0 supplies the rest
     TracerDev tr;
     TexDev tex[PRR_MAX_TEXTURES];
     OutDev out;
+    RayStateDev rs;
+    int64_t rs_total;            // rays in the ray-state arrays
+    int64_t rs_base;             // ray id of the arrays' first entry (0 on the device)
+    LocateCtx *loc;              // one per lane of the integrator grid
+    AdjRunDev adj;
     CountersDev *ctr;
 };
 
-// Kerr adjoint (prr_adjoint): checkpoints and per-ray gradients.
+// Kerr adjoint reverse pass (per ray).
 struct AdjDev {
-    int32_t K;               // accepted steps between checkpoints
-    int32_t pad;
-    const int32_t *steps;    // accepted steps of every ray (first forward pass)
-    const int8_t *cls;       // termination class of every ray (first forward pass)
-    const int64_t *ckpt_off; // index of each ray's first checkpoint
-    double *ckpt;            // 8 doubles per checkpoint: y[6], h, attempts
+    int32_t K;
+    int32_t seg;             // segment index j of this launch
+    int64_t count;           // rays in this launch (the first `count` of `rays`)
+    const int64_t *rays;     // rays sorted by decreasing step count
+    const int32_t *nsteps;   // accepted steps of every ray
+    const int8_t *cls;       // termination class of every ray
+    const double *rec;       // replayed steps of segment j (AdjRunDev.rec layout)
+    const EventDev *event;   // terminal events
+    double *lam;             // 10 doubles per ray: lambda (y[0..4], E, L, Q, a), dL/dphi_s
     const float *dLdI;       // dL/dI, W*H*3
     const float *img;        // forward linear image, W*H*3
     const float *ray_color;  // forward ray colours, 3 per ray
-    double *seg;             // per-lane segment buffers: K * 6 doubles per lane, lane-minor
-    int64_t seg_lanes;       // lanes of the backward grid
     double *grad_ray;        // 4 doubles per ray: dL/d(a, r_cam, incl, phi_s)
     double cam_phi;          // fixed orbit parameters
     double sphere_r;
-    unsigned long long *replay_mismatch; // segments the replay did not reproduce bit for bit
+    unsigned long long *mismatch; // rays whose replay did not end with their class
 };
 
 // Post-processing of one frame (SSAA combine, FXAA, RGB8).

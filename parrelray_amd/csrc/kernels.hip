@@ -1,17 +1,20 @@
-// kernels.hip -- gfx950 kernels: the persistent geodesic integrator and the
-// SSAA / FXAA / RGB8 passes.
+// kernels.hip -- gfx950 kernels: the ray pipeline (set-up, persistent
+// integrator, shading), the SSAA / FXAA / RGB8 passes and the Kerr adjoint.
 //
-// Integrator (replaces initRays + trace_Euler/trace_RKF + rayToRGB,
-// kernels/solver.cuh:26-220, metrics/schwarzschild.cuh:151-183, launched by
-// prr.cpp:279-323 as one thread per ray in 64-row bands):
-//   * one persistent grid sized to the CUs; every lane holds one ray's state
-//     in VGPRs and runs one Euler step / RKF45 attempt per loop trip;
-//   * when at least `refill` lanes of a wave are idle the wave claims that
-//     many ray ids from a device queue with a single atomic (ballot + mbcnt
-//     prefix sum) and initialises them in place, so lanes never wait for the
-//     slowest ray of their wave;
-//   * rays are generated from their id (no ray buffer), coloured on
-//     termination, and only the colour (plus optional diagnostics) is written.
+// Replaces initRays + trace_Euler/trace_RKF + rayToRGB (kernels/solver.cuh:
+// 26-220, metrics/schwarzschild.cuh:151-183, launched by prr.cpp:279-323 as
+// one thread per ray in 64-row bands) with three kernels over a ray-state
+// array in HBM:
+//   * k_init: one thread per ray, camera ray and initial state;
+//   * k_integrate: one persistent grid sized to the CUs; every lane holds one
+//     ray's state in VGPRs and runs one Euler step / RKF45 attempt / event
+//     sub-step per loop trip.  When at least `refill` lanes of a wave are
+//     idle, the wave claims that many ray ids with a single atomic (ballot +
+//     mbcnt prefix sum).  The loop body carries no calls and no once-per-ray
+//     code, so the state stays in registers;
+//   * k_shade: one thread per ray, endpoint and colour.
+// The Kerr adjoint's checkpoint and replay passes run k_integrate itself
+// (AdjRunDev), so that every pass takes the same accepted steps bit for bit.
 #include <hip/hip_runtime.h>
 
 #include "adjoint.hpp"
@@ -34,21 +37,31 @@ __device__ inline unsigned long long shfl_u64(unsigned long long v, int src) {
     return (unsigned long long)(unsigned(lo)) | ((unsigned long long)(unsigned(hi)) << 32);
 }
 
-struct KerrStepOut {
-    Lane<Kerr> L;
-    int cls;
-};
-template <bool MIXED> __device__ __noinline__ KerrStepOut kerr_attempt_shared(const JobDev &job, const Lane<Kerr> L);
+template <class Mt> __global__ void __launch_bounds__(256) k_init(JobDev job) {
+    const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id < job.total) init_ray<Mt>(job, id);
+}
 
-// SHARED: call the adjoint's out-of-line attempt function (Kerr, RKF45).
-template <class Mt, int SOLVER, bool MIXED, bool SHARED>
+template <class Mt> __global__ void __launch_bounds__(256) k_shade(JobDev job) {
+    const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id < job.total) shade_ray<Mt>(job, id);
+}
+
+// The persistent integrator.  Adjoint modes: ADJ_CHECKPOINT stores a
+// checkpoint every K accepted steps; ADJ_REPLAY runs segment j of each ray
+// of job.adj.rays from its checkpoint and records its steps.
+template <class Mt, int SOLVER, bool MIXED>
 __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
     const int lane = int(threadIdx.x & 63);
+    LocateCtx &C = job.loc[int64_t(blockIdx.x) * blockDim.x + threadIdx.x];
+    const bool replay = job.adj.mode == ADJ_REPLAY;
     Lane<Mt> L;
-    bool active = false, exhausted = false;
-    unsigned long long acc_att = 0, acc_ok = 0, acc_64 = 0, acc_rays = 0;
+    int mode = MODE_IDLE;
+    int32_t end = 0; // replay: stop at this step count
+    bool exhausted = false;
+    unsigned acc_att = 0, acc_ok = 0, acc_64 = 0, acc_rays = 0;
     while (true) {
-        const unsigned long long idle = __ballot(!active);
+        const unsigned long long idle = __ballot(mode == MODE_IDLE);
         const int nidle = __popcll(idle);
         if (!exhausted && nidle >= job.refill) {
             const int leader = __ffsll((long long)idle) - 1;
@@ -56,65 +69,76 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
             if (lane == leader) base = atomicAdd(&job.ctr->queue, (unsigned long long)nidle);
             base = shfl_u64(base, leader);
             if (base + (unsigned long long)nidle >= (unsigned long long)job.total) exhausted = true;
-            if (!active) {
+            if (mode == MODE_IDLE) {
                 const unsigned rank =
                     __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
-                const int64_t id = int64_t(base) + rank;
-                if (id < job.total) {
-                    L = lane_start<Mt>(job, id);
-                    active = true;
+                const int64_t idx = int64_t(base) + rank;
+                if (idx < job.total) {
+                    if (replay) {
+                        const int64_t rid = job.adj.rays[idx];
+                        lane_load_checkpoint<Mt>(job, L, rid, job.adj.seg);
+                        end = min((job.adj.seg + 1) * job.adj.K, job.adj.nsteps[rid]);
+                    } else {
+                        lane_load<Mt>(job, L, idx);
+                        if (job.adj.mode == ADJ_CHECKPOINT)
+                            store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, 0);
+                    }
+                    mode = MODE_ACTIVE;
                     if (!finite_state<Mt>(L.y)) {
-                        lane_finish<Mt>(job, L, PRR_NAN);
+                        if (!replay) lane_store<Mt>(job, L, PRR_NAN);
                         acc_rays++;
-                        active = false;
+                        mode = MODE_IDLE;
                     }
                 }
             }
         }
-        if (__ballot(active) == 0ull) break;
-        if (active) {
-            int cls;
-            if constexpr (SHARED) {
-                const KerrStepOut r = kerr_attempt_shared<MIXED>(job, L);
-                L = r.L;
-                cls = r.cls;
-            }
-            else cls = lane_attempt<Mt, SOLVER, MIXED>(job, L);
-            if (cls >= 0) {
-                lane_finish<Mt>(job, L, cls);
-                acc_att += (unsigned long long)L.attempts;
-                acc_ok += (unsigned long long)L.steps;
-                acc_64 += (unsigned long long)L.att64;
+        if (__ballot(mode != MODE_IDLE) == 0ull) break;
+        if (mode != MODE_IDLE) {
+            const int cls = lane_iterate<Mt, SOLVER, MIXED>(job, L, C, mode);
+            if (cls >= 0 || (replay && mode == MODE_ACTIVE && L.steps >= end)) {
+                if (!replay) {
+                    lane_store<Mt>(job, L, cls);
+                } else if (cls < 0) { // a segment's replay must land on the next checkpoint bit for bit
+                    const double *nx = job.adj.ckpt + 8 * (job.adj.ckpt_off[L.rid] + job.adj.seg + 1);
+                    bool same = L.steps == end;
+                    for (int i = 0; i < Mt::N; ++i) same = same && L.y[i] == nx[i];
+                    if (!same) atomicAdd(job.adj.mismatch, 1ull);
+                }
+                acc_att += unsigned(L.attempts);
+                acc_ok += unsigned(L.steps);
+                acc_64 += unsigned(L.att64);
                 acc_rays++;
-                active = false;
+                mode = MODE_IDLE;
             }
         }
     }
     // one atomic per wave for the counters
+    unsigned long long t_att = acc_att, t_ok = acc_ok, t_64 = acc_64, t_rays = acc_rays;
     for (int off = 32; off > 0; off >>= 1) {
-        acc_att += shfl_u64(acc_att, lane ^ off);
-        acc_ok += shfl_u64(acc_ok, lane ^ off);
-        acc_64 += shfl_u64(acc_64, lane ^ off);
-        acc_rays += shfl_u64(acc_rays, lane ^ off);
+        t_att += shfl_u64(t_att, lane ^ off);
+        t_ok += shfl_u64(t_ok, lane ^ off);
+        t_64 += shfl_u64(t_64, lane ^ off);
+        t_rays += shfl_u64(t_rays, lane ^ off);
     }
     if (lane == 0) {
-        atomicAdd(&job.ctr->attempts, acc_att);
-        atomicAdd(&job.ctr->accepted, acc_ok);
-        atomicAdd(&job.ctr->attempts_fp64, acc_64);
-        atomicAdd(&job.ctr->rays, acc_rays);
+        atomicAdd(&job.ctr->attempts, t_att);
+        atomicAdd(&job.ctr->accepted, t_ok);
+        atomicAdd(&job.ctr->attempts_fp64, t_64);
+        atomicAdd(&job.ctr->rays, t_rays);
     }
 }
 
 // Counter reset by a kernel (no memset node, so the sequence can be captured
 // in a graph).
-__global__ void k_reset_counters(CountersDev *c) {
+// all = 0 keeps the adjoint's mismatch count across launches.
+__global__ void k_reset_counters(CountersDev *c, int all) {
     if (threadIdx.x == 0) {
         c->queue = 0;
         c->attempts = 0;
         c->accepted = 0;
         c->attempts_fp64 = 0;
         c->rays = 0;
-        c->mismatch = 0;
+        if (all) c->mismatch = 0;
     }
 }
 
@@ -245,19 +269,6 @@ __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
 }
 
 // ----------------------------------------------------------------- adjoint (Kerr)
-// Pass 1 renders with step counts and colours, pass 2 re-runs the
-// integration and stores a checkpoint every K accepted steps, pass 3 walks
-// each ray's segments backwards: replay K steps from a checkpoint, then the
-// reverse sweep kerr_step_vjp over them.  All three passes call one
-// out-of-line attempt function, so their accepted steps agree bit for bit.
-// (lane data passed by value, so the callers' lane state stays in registers)
-template <bool MIXED> __device__ __noinline__ KerrStepOut kerr_attempt_shared(const JobDev &job, const Lane<Kerr> L) {
-    KerrStepOut o;
-    o.L = L;
-    o.cls = lane_attempt<Kerr, PRR_RKF45, MIXED>(job, o.L);
-    return o;
-}
-
 // Weights dL/dc of one ray's colour from dL/dI of its pixel through the SSAA
 // combine (kernels/ssaa.cuh:116-130): s = 1, I = clamp(c); s > 1,
 // I = sqrt(mean(clamp(c^2))).
@@ -276,76 +287,20 @@ __device__ inline void color_weights(const AdjDev &ad, const FrameDev &fr, int64
     }
 }
 
-__device__ inline void ckpt_store(double *c, const Lane<Kerr> &L) {
-    for (int i = 0; i < 6; ++i) c[i] = L.y[i];
-    c[6] = L.h;
-    c[7] = double(L.attempts);
-}
-
-__device__ inline void ckpt_load(const JobDev &job, const double *c, Lane<Kerr> &L, int32_t steps) {
-    for (int i = 0; i < 6; ++i) L.y[i] = c[i];
-    L.h = c[6];
-    L.attempts = int32_t(c[7]);
-    L.lam = L.y[5];
-    L.steps = steps;
-    L.hit = -1;
-    if (job.frames[L.frame].n_spheres > 0) cart_of_f32<Kerr>(L.y, L.P0);
-}
-
-template <bool MIXED> __global__ void __launch_bounds__(256) k_adj_forward(JobDev job, AdjDev ad) {
-    const int lane = int(threadIdx.x & 63);
-    Lane<Kerr> L;
-    int32_t N = 0;
-    bool active = false, exhausted = false;
-    while (true) {
-        const unsigned long long idle = __ballot(!active);
-        const int nidle = __popcll(idle);
-        if (!exhausted && nidle >= job.refill) {
-            const int leader = __ffsll((long long)idle) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&job.ctr->queue, (unsigned long long)nidle);
-            base = shfl_u64(base, leader);
-            if (base + (unsigned long long)nidle >= (unsigned long long)job.total) exhausted = true;
-            if (!active) {
-                const unsigned rank =
-                    __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
-                const int64_t id = int64_t(base) + rank;
-                if (id < job.total) {
-                    L = lane_start<Kerr>(job, id);
-                    N = ad.steps[id];
-                    ckpt_store(ad.ckpt + 8 * ad.ckpt_off[id], L);
-                    active = N > 0 && finite_state<Kerr>(L.y);
-                }
-            }
-        }
-        if (__ballot(active) == 0ull) break;
-        if (active) {
-            const int32_t before = L.steps;
-            const KerrStepOut r = kerr_attempt_shared<MIXED>(job, L);
-            L = r.L;
-            const int cls = r.cls;
-            if (cls < 0 && L.steps != before && L.steps < N && L.steps % ad.K == 0)
-                ckpt_store(ad.ckpt + 8 * (ad.ckpt_off[L.rid] + L.steps / ad.K), L);
-            if (cls >= 0 || L.steps >= N) active = false;
-        }
-    }
-}
-
 // Terminal seed: lambda at y_{N-1} (state and constants) and dL/dphi_s from
 // the located event of the last step (see adjoint.hpp).
-__device__ __noinline__ void adj_terminal(const JobDev &job, const AdjDev &ad, const Lane<Kerr> L,
-                                          const EventDev &ev, const double w[3], double lam[9], double &gphs) {
-    const FrameDev &fr = job.frames[L.frame];
+__device__ void adj_terminal(const JobDev &job, const AdjDev &ad, const FrameDev &fr, const Consts<double> &cs,
+                             const EventDev &ev, const double w[3], double lam[9], double &gphs) {
     const double tau = ev.theta * ev.h;
     double ys[6];
-    rk_substep<Kerr>(ev.y_prev, tau, L.cs, ys);
+    rk_substep<Kerr>(ev.y_prev, tau, cs, ys);
     using D = Dual<10>;
     D y[5];
     for (int i = 0; i < 5; ++i) y[i] = D::var(ys[i], i);
-    const D E = D::var(L.cs.E, 5), Lz = D::var(L.cs.L, 6), Q = D::var(L.cs.Q, 7), a = D::var(L.cs.a, 8);
+    const D E = D::var(cs.E, 5), Lz = D::var(cs.L, 6), Q = D::var(cs.Q, 7), a = D::var(cs.a, 8);
     D col[3], G;
     if (ev.cls == PRR_ESCAPE) {
-        const V3<D> n = kerr_escape_dir_t<D>(y, E, Lz, Q, a, L.cs.M);
+        const V3<D> n = kerr_escape_dir_t<D>(y, E, Lz, Q, a, cs.M);
         sky_color_t<D>(job.tex[0], n, col);
         G = D(1.0 / job.tr.r_escape) - y[0];
     } else {
@@ -366,7 +321,7 @@ __device__ __noinline__ void adj_terminal(const JobDev &job, const AdjDev &ad, c
     for (int i = 0; i < 9; ++i) ls[i] = w[0] * col[0].d[i] + w[1] * col[1].d[i] + w[2] * col[2].d[i];
     gphs = w[0] * col[0].d[9] + w[1] * col[1].d[9] + w[2] * col[2].d[9];
     double f[6];
-    Kerr::rhs<double>(ys, Kerr::coef<double>(L.cs), f);
+    Kerr::rhs<double>(ys, Kerr::coef<double>(cs), f);
     double num = 0.0, den = 0.0;
     for (int i = 0; i < 5; ++i) {
         num += ls[i] * f[i];
@@ -376,15 +331,71 @@ __device__ __noinline__ void adj_terminal(const JobDev &job, const AdjDev &ad, c
     if (!isfinite(kappa)) kappa = 0.0;
     for (int i = 0; i < 9; ++i) lam[i] = ls[i] - kappa * G.d[i];
     gphs -= kappa * G.d[9];
-    kerr_step_vjp(ev.y_prev, tau, L.cs, lam);
+    kerr_step_vjp(ev.y_prev, tau, cs, lam);
 }
 
-// Ray set-up Jacobian d(y_0, E, L, Q)/d(a, r_cam, incl) contracted with lambda_0.
-__device__ __noinline__ void adj_init_grad(const JobDev &job, const AdjDev &ad, const Lane<Kerr> L,
-                                           const double lam[9], double g[4]) {
-    const FrameDev &fr = job.frames[L.frame];
+// Reverse sweep over segment j of every ray of this launch: the terminal seed
+// on the ray's last segment, then lambda_n = lambda_{n+1} dPhi_n/dy_n over
+// the replayed steps.  lambda lives in ad.lam between launches.
+__global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= ad.count) return;
+    const int64_t id = ad.rays[idx];
+    const int32_t N = ad.nsteps[id];
+    const int K = ad.K, j = ad.seg;
+    double *lm = ad.lam + 10 * id;
+    const FrameDev &fr = job.frames[find_frame(job, id)];
+    Consts<double> cs;
+    load_consts<Kerr>(job, id, fr, cs);
+    double lam[9], gphs;
+    if (j == (N - 1) / K) { // the ray's last segment
+        const int cls = ad.cls[id];
+        double w[3] = {0.0, 0.0, 0.0};
+        if (cls == PRR_ESCAPE || cls == PRR_SPHERE)
+            color_weights(ad, fr, id - fr.ray_begin, ad.ray_color + 3 * id, w);
+        if (ad.event[id].cls != cls && (cls == PRR_ESCAPE || cls == PRR_SPHERE)) atomicAdd(ad.mismatch, 1ull);
+        if ((w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0) || ad.event[id].cls != cls) {
+            for (int i = 0; i < 10; ++i) lm[i] = 0.0;
+            return;
+        }
+        adj_terminal(job, ad, fr, cs, ad.event[id], w, lam, gphs);
+    } else {
+        for (int i = 0; i < 9; ++i) lam[i] = lm[i];
+        gphs = lm[9];
+    }
+    bool zero = true;
+    for (int i = 0; i < 9; ++i) zero = zero && lam[i] == 0.0;
+    if (!zero) {
+        const int64_t T = job.rs_total;
+        const int32_t end = min((j + 1) * K, N - 1); // step N-1 is the located sub-step above
+        for (int32_t n = end - 1; n >= j * K; --n) {
+            const double *r = ad.rec + (int64_t(n - j * K) * 6) * T + id;
+            double yn[6];
+            for (int c = 0; c < 5; ++c) yn[c] = r[c * T];
+            yn[5] = 0.0;
+            kerr_step_vjp(yn, r[5 * T], cs, lam);
+        }
+    }
+    for (int i = 0; i < 9; ++i) lm[i] = lam[i];
+    lm[9] = gphs;
+}
+
+// Ray set-up Jacobian d(y_0, E, L, Q)/d(a, r_cam, incl) contracted with
+// lambda_0: the per-ray gradient.
+__global__ void __launch_bounds__(256) k_adj_init_grad(JobDev job, AdjDev ad) {
+    const int64_t id = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (id >= job.total) return;
+    const double *lam = ad.lam + 10 * id;
+    double *g = ad.grad_ray + 4 * id;
+    bool zero = true;
+    for (int i = 0; i < 10; ++i) zero = zero && lam[i] == 0.0;
+    if (zero || ad.nsteps[id] <= 0) {
+        for (int k = 0; k < 4; ++k) g[k] = 0.0;
+        return;
+    }
+    const FrameDev &fr = job.frames[find_frame(job, id)];
     double u, v;
-    ray_uv(fr, L.rid - fr.ray_begin, u, v);
+    ray_uv(fr, id - fr.ray_begin, u, v);
     const double r_cam = sqrt(fr.pos[0] * fr.pos[0] + fr.pos[1] * fr.pos[1] + fr.pos[2] * fr.pos[2]);
     const double incl = acos(fr.pos[2] / r_cam);
     using D = Dual<3>;
@@ -392,116 +403,16 @@ __device__ __noinline__ void adj_init_grad(const JobDev &job, const AdjDev &ad, 
     orbit_camera_ray<D>(D::var(r_cam, 1), D::var(incl, 2), ad.cam_phi, fr.tan_half, fr.aspect, u, v, pos, d);
     D z[8];
     kerr_init_t<D>(pos, d, fr.M, D::var(fr.a, 0), z);
+    double out[4];
     for (int k = 0; k < 3; ++k) {
         double acc = k == 0 ? lam[8] : 0.0; // a is itself one of the constants
         for (int i = 0; i < 8; ++i) acc += lam[i] * z[i].d[k];
-        g[k] = acc;
+        out[k] = acc;
     }
-}
-
-template <bool MIXED> __global__ void __launch_bounds__(256) k_adj_backward(JobDev job, AdjDev ad) {
-    const int lane = int(threadIdx.x & 63);
-    const int64_t gl = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; // this lane's segment buffer
-    Lane<Kerr> L;
-    bool active = false, exhausted = false;
-    while (true) {
-        const unsigned long long idle = __ballot(!active);
-        const int nidle = __popcll(idle);
-        if (!exhausted && nidle >= 1) {
-            const int leader = __ffsll((long long)idle) - 1;
-            unsigned long long base = 0;
-            if (lane == leader) base = atomicAdd(&job.ctr->queue, (unsigned long long)nidle);
-            base = shfl_u64(base, leader);
-            if (base + (unsigned long long)nidle >= (unsigned long long)job.total) exhausted = true;
-            if (!active) {
-                const unsigned rank =
-                    __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
-                const int64_t id = int64_t(base) + rank;
-                if (id < job.total) {
-                    L.rid = id;
-                    active = true;
-                }
-            }
-        }
-        if (__ballot(active) == 0ull) break;
-        if (!active) continue;
-        // one whole ray per lane: replay + reverse every segment
-        const int64_t id = L.rid;
-        const int cls = ad.cls[id];
-        const int32_t N = ad.steps[id];
-        double g[4] = {0.0, 0.0, 0.0, 0.0};
-        double w[3] = {0.0, 0.0, 0.0};
-        if ((cls == PRR_ESCAPE || cls == PRR_SPHERE) && N > 0) {
-            L = lane_start<Kerr>(job, id); // constants E, L, Q and the frame
-            const FrameDev &fr = job.frames[L.frame];
-            color_weights(ad, fr, id - fr.ray_begin, ad.ray_color + 3 * id, w);
-        }
-        if (w[0] != 0.0 || w[1] != 0.0 || w[2] != 0.0) {
-            double lam[9];
-            double gphs = 0.0;
-            const int64_t c0 = ad.ckpt_off[id];
-            bool ok = true;
-            for (int32_t j = (N - 1) / ad.K; j >= 0 && ok; --j) {
-                ckpt_load(job, ad.ckpt + 8 * (c0 + j), L, j * ad.K);
-                const int32_t end = min((j + 1) * ad.K, N);
-                // replay: record (y_n, h_n) of every accepted step of the segment
-                int rc = -1;
-                while (L.steps < end) {
-                    double ys[5];
-                    for (int i = 0; i < 5; ++i) ys[i] = L.y[i];
-                    const double hb = L.h;
-                    const int32_t before = L.steps;
-                    const KerrStepOut r = kerr_attempt_shared<MIXED>(job, L);
-                    L = r.L;
-                    rc = r.cls;
-                    if (L.steps != before) {
-                        double *slot = ad.seg + (int64_t(before - j * ad.K) * 6) * ad.seg_lanes + gl;
-                        for (int i = 0; i < 5; ++i) slot[i * ad.seg_lanes] = ys[i];
-                        bool at_min;
-                        slot[5 * ad.seg_lanes] = rkf_step_size<Kerr>(job.tr, ys, L.cs, hb, at_min);
-                    }
-                    if (rc >= 0) break;
-                }
-                // the ray must end exactly at step N with its class, and nowhere earlier
-                const bool last = end == N;
-                if (L.steps != end || (last ? rc != cls : rc >= 0)) {
-                    atomicAdd(ad.replay_mismatch, 1ull);
-                    ok = false;
-                    break;
-                }
-                if (!last) { // the replay must land on the next checkpoint
-                    const double *nx = ad.ckpt + 8 * (c0 + j + 1);
-                    bool same = true;
-                    for (int i = 0; i < 6; ++i) same = same && L.y[i] == nx[i];
-                    if (!same) {
-                        atomicAdd(ad.replay_mismatch, 1ull);
-                        ok = false;
-                        break;
-                    }
-                } else {
-                    adj_terminal(job, ad, L, job.out.event[id], w, lam, gphs);
-                }
-                // reverse sweep (the terminal step N-1 is the located sub-step above)
-                for (int32_t n = min(end, N - 1) - 1; n >= j * ad.K; --n) {
-                    const double *slot = ad.seg + (int64_t(n - j * ad.K) * 6) * ad.seg_lanes + gl;
-                    double yn[6];
-                    for (int i = 0; i < 5; ++i) yn[i] = slot[i * ad.seg_lanes];
-                    yn[5] = 0.0;
-                    kerr_step_vjp(yn, slot[5 * ad.seg_lanes], L.cs, lam);
-                }
-            }
-            if (ok) {
-                adj_init_grad(job, ad, L, lam, g);
-                g[3] = gphs;
-                bool fin = true;
-                for (int k = 0; k < 4; ++k) fin = fin && isfinite(g[k]);
-                if (!fin) g[0] = g[1] = g[2] = g[3] = 0.0;
-            }
-        }
-        double *out = ad.grad_ray + 4 * id;
-        for (int k = 0; k < 4; ++k) out[k] = g[k];
-        active = false;
-    }
+    out[3] = lam[9];
+    bool fin = true;
+    for (int k = 0; k < 4; ++k) fin = fin && isfinite(out[k]);
+    for (int k = 0; k < 4; ++k) g[k] = fin ? out[k] : 0.0;
 }
 
 // Deterministic sum of the per-ray gradients: one workgroup, fixed order.
@@ -522,19 +433,14 @@ __global__ void __launch_bounds__(1024) k_grad_reduce(const double *grad_ray, in
         for (int k = 0; k < 4; ++k) out[k] = part[k][0];
 }
 
-hipError_t launch_adjoint(int pass, int precision, const JobDev &job, const AdjDev &ad, int grid, int block,
-                          hipStream_t st) {
-    const bool mixed = precision == PRR_FP32_BULK;
-    if (pass == 1) {
-        if (mixed) hipLaunchKernelGGL((k_integrate<Kerr, PRR_RKF45, true, true>), dim3(grid), dim3(block), 0, st, job);
-        else hipLaunchKernelGGL((k_integrate<Kerr, PRR_RKF45, false, true>), dim3(grid), dim3(block), 0, st, job);
-    } else if (pass == 2) {
-        if (mixed) hipLaunchKernelGGL((k_adj_forward<true>), dim3(grid), dim3(block), 0, st, job, ad);
-        else hipLaunchKernelGGL((k_adj_forward<false>), dim3(grid), dim3(block), 0, st, job, ad);
-    } else {
-        if (mixed) hipLaunchKernelGGL((k_adj_backward<true>), dim3(grid), dim3(block), 0, st, job, ad);
-        else hipLaunchKernelGGL((k_adj_backward<false>), dim3(grid), dim3(block), 0, st, job, ad);
-    }
+hipError_t launch_adj_reverse(const JobDev &job, const AdjDev &ad, hipStream_t st) {
+    if (ad.count <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_adj_reverse, dim3(unsigned((ad.count + 255) / 256)), dim3(256), 0, st, job, ad);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_init_grad(const JobDev &job, const AdjDev &ad, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_init_grad, dim3(unsigned((job.total + 255) / 256)), dim3(256), 0, st, job, ad);
     return hipGetLastError();
 }
 
@@ -544,33 +450,51 @@ hipError_t launch_grad_reduce(const double *grad_ray, int64_t n, double *out, hi
 }
 
 // ----------------------------------------------------------------- dispatch
-template <class Mt>
-static hipError_t launch_metric(int solver, int precision, const JobDev &job, dim3 grid, dim3 block,
-                                hipStream_t st) {
-    const bool mixed = precision == PRR_FP32_BULK;
-    if (solver == PRR_EULER) {
-        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true, false>), grid, block, 0, st, job);
-        else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false, false>), grid, block, 0, st, job);
-    } else {
-        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true, false>), grid, block, 0, st, job);
-        else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false, false>), grid, block, 0, st, job);
-    }
-    return hipGetLastError();
-}
-
-hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st) {
-    const dim3 g(grid), b(block);
+template <class F> static hipError_t with_metric(int metric, F &&f) {
     switch (metric) {
-    case PRR_FLAT: return launch_metric<Flat>(job.tr.solver, job.tr.precision, job, g, b, st);
-    case PRR_SCHW_KS_CART: return launch_metric<SchwKS>(job.tr.solver, job.tr.precision, job, g, b, st);
-    case PRR_SCHW_SPH: return launch_metric<SchwSph>(job.tr.solver, job.tr.precision, job, g, b, st);
-    case PRR_KERR_BL: return launch_metric<Kerr>(job.tr.solver, job.tr.precision, job, g, b, st);
+    case PRR_FLAT: return f(Flat{});
+    case PRR_SCHW_KS_CART: return f(SchwKS{});
+    case PRR_SCHW_SPH: return f(SchwSph{});
+    case PRR_KERR_BL: return f(Kerr{});
     default: return hipErrorInvalidValue;
     }
 }
 
-hipError_t launch_reset(CountersDev *c, hipStream_t st) {
-    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c);
+hipError_t launch_init(int metric, const JobDev &job, hipStream_t st) {
+    return with_metric(metric, [&](auto mt) {
+        using Mt = decltype(mt);
+        hipLaunchKernelGGL((k_init<Mt>), dim3(unsigned((job.total + 255) / 256)), dim3(256), 0, st, job);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_shade(int metric, const JobDev &job, hipStream_t st) {
+    return with_metric(metric, [&](auto mt) {
+        using Mt = decltype(mt);
+        hipLaunchKernelGGL((k_shade<Mt>), dim3(unsigned((job.total + 255) / 256)), dim3(256), 0, st, job);
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st) {
+    const dim3 g(grid), b(block);
+    const bool mixed = job.tr.precision == PRR_FP32_BULK;
+    const bool euler = job.tr.solver == PRR_EULER;
+    return with_metric(metric, [&](auto mt) {
+        using Mt = decltype(mt);
+        if (euler) {
+            if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true>), g, b, 0, st, job);
+            else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false>), g, b, 0, st, job);
+        } else {
+            if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true>), g, b, 0, st, job);
+            else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false>), g, b, 0, st, job);
+        }
+        return hipGetLastError();
+    });
+}
+
+hipError_t launch_reset(CountersDev *c, hipStream_t st, bool all) {
+    hipLaunchKernelGGL(k_reset_counters, dim3(1), dim3(64), 0, st, c, int(all));
     return hipGetLastError();
 }
 

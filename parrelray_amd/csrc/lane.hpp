@@ -1,21 +1,25 @@
-// lane.hpp -- the per-ray integrator: set-up, one Euler step / RKF45
-// attempt, termination events and output, for one ray held by one lane.
+// lane.hpp -- the per-ray integrator: ray set-up, one Euler step / RKF45
+// attempt / event-location sub-step per loop trip, and shading, for one ray
+// held by one lane.
 //
-// The persistent kernels (kernels.hip) call these functions once per loop
-// trip; they are __host__ __device__ so that the same code also runs on the
-// host for selected rays (prr_debug_trace_rays).
+// The kernels (kernels.hip) run these as a pipeline over a ray-state array:
+// init_ray (k_init) -> lane_iterate in a persistent loop (k_integrate) ->
+// shade_ray (k_shade), so that the integrator's loop carries no calls and no
+// once-per-ray code.  Everything here is __host__ __device__: the same code
+// runs on the host for selected rays (prr_debug_trace_rays).
 //
-// Replaces trace_Euler / trace_RKF (kernels/solver.cuh:68-220) and the
-// per-step User_UPDATE / User_ENDLOOP hooks (metrics/schwarzschild.cuh:55-79).
+// Replaces initRays, trace_Euler / trace_RKF and rayToRGB
+// (kernels/solver.cuh:26-220, metrics/schwarzschild.cuh:55-79,151-183).
 // For the adaptive solver the two events whose position depends on the step
 // sequence are located on the integrated path instead of at the first step
-// beyond them:
-//   * escape: the state where r crosses r_escape (Newton on the crossing
-//     fraction theta of the last step, each iterate a fresh FP64 RK45 step of
-//     theta h from the step's start);
-//   * sphere hit: the first point of the path inside a sphere (bracketed
-//     Illinois iteration on |P(theta) - c|^2 - R^2 over the same FP64
-//     sub-steps), for steps whose chord passes within a margin of a sphere.
+// beyond them, by FP64 RK45 sub-steps of theta h from the step's start, one
+// sub-step per loop trip (the lane's search state lives in a LocateCtx in
+// device memory):
+//   * escape: the state where r crosses r_escape (Newton on theta);
+//   * sphere hit: the first point of the path inside a sphere, for steps
+//     whose chord passes within a margin of it: the path's end or the minimum
+//     of |P(theta) - c|^2 (successive parabolic interpolation) brackets the
+//     entry, which Illinois regula falsi refines.
 // The fixed-step Euler solver keeps the reference's chord and first-step
 // semantics.
 #pragma once
@@ -51,19 +55,6 @@ struct Fehl {
                                  -9.0 / 50.0 + 1.0 / 5.0,            2.0 / 55.0};
         return E[s];
     }
-};
-
-// Fixed-size state vector passed by value to out-of-line functions (so that
-// the caller's lane state never has its address taken and stays in VGPRs).
-template <int N> struct Vec {
-    double v[N];
-};
-
-// Located event: state, fraction of the step, sphere index.
-template <int N> struct Located {
-    Vec<N> y;
-    double theta;
-    int k;
 };
 
 // 1 - cos^2 th below which Kerr's FP32-bulk attempts run their stages in FP64.
@@ -230,10 +221,8 @@ template <class Mt, class T> PRR_HD void euler_rhs(const double *y, const Consts
     for (int i = 0; i < N; ++i) f[i] = double(fT[i]);
 }
 
-// One FP64 RK45 step (4th-order solution) of size h from y: the sub-steps of
-// the event location.  Out of line: cold code.
-template <class Mt>
-PRR_HD __noinline__ void rk_substep(const double *y, double h, const Consts<double> &cs, double *out) {
+// One FP64 RK45 step (4th-order solution) of size h from y.
+template <class Mt> PRR_HD void rk_substep(const double *y, double h, const Consts<double> &cs, double *out) {
     double d4[Mt::N], e[Mt::N];
     rkf_stages<Mt, double>(y, h, cs, d4, e);
     for (int i = 0; i < Mt::N; ++i) out[i] = y[i] + d4[i];
@@ -253,36 +242,6 @@ template <class Mt> PRR_HD void escape_g(const double *y, double r_esc, double &
         g = r - r_esc;
         dg = (y[0] * y[4] + y[1] * y[5] + y[2] * y[6]) / r;
     }
-}
-
-// State at the r = r_escape crossing inside the step yo -> yn of size h
-// (theta = the crossing fraction).
-template <class Mt>
-PRR_HD __noinline__ Located<Mt::N> escape_locate(const Vec<Mt::N> yo, const Vec<Mt::N> yn, double h,
-                                                 const Consts<double> cs, double r_esc) {
-    Located<Mt::N> out;
-    out.k = -1;
-    double go, gn, dg;
-    escape_g<Mt>(yo.v, r_esc, go, dg);
-    escape_g<Mt>(yn.v, r_esc, gn, dg);
-    if (!(go < 0.0)) { // started outside: nothing to locate
-        out.y = yo;
-        out.theta = 0.0;
-        return out;
-    }
-    double th = go / (go - gn);
-    th = fmin(fmax(th, 0.0), 1.0);
-    const double tol = 1e-13 * (Mt::mino ? 1.0 / r_esc : r_esc);
-    for (int it = 0; it < 6; ++it) {
-        rk_substep<Mt>(yo.v, th * h, cs, out.y.v);
-        double g;
-        escape_g<Mt>(out.y.v, r_esc, g, dg);
-        if (fabs(g) <= tol || !(dg * h != 0.0)) break;
-        th = fmin(fmax(th - g / (dg * h), 0.0), 1.0);
-        if (it == 5) rk_substep<Mt>(yo.v, th * h, cs, out.y.v);
-    }
-    out.theta = th;
-    return out;
 }
 
 // Sphere k: G = |P - c|^2 - R^2 at a state.
@@ -309,164 +268,123 @@ PRR_HD bool sphere_screen(const FrameDev &fr, int k, const float P0[3], const fl
     return dist <= R + 0.5f * sqrtf(A) + 1e-4f * scale;
 }
 
-// First point of the step yo -> yn (size h) inside a screened sphere: the
-// sphere index k (or -1), the state there and the fraction theta of the step.
-template <class Mt>
-PRR_HD __noinline__ Located<Mt::N> sphere_locate(const FrameDev &fr, unsigned mask, const Vec<Mt::N> yo,
-                                                 const Vec<Mt::N> yn, double h, const Consts<double> cs) {
-    constexpr int N = Mt::N;
-    Located<N> out;
-    out.k = -1;
-    out.theta = 2.0;
-    double yt[N];
-    const D3 P0 = cart_of<Mt>(yo.v), P1 = cart_of<Mt>(yn.v);
-    const D3 dP = P1 - P0;
-    for (int k = 0; k < fr.n_spheres; ++k) {
-        if (!((mask >> k) & 1u)) continue;
-        const D3 c = d3(fr.sph[k][0], fr.sph[k][1], fr.sph[k][2]);
-        const double R2 = fr.sph[k][3] * fr.sph[k][3];
-        const D3 m = P0 - c;
-        const double G0 = dot3(m, m) - R2;
-        if (G0 <= 0.0) { // the step starts inside (reference: chord t = 0)
-            if (0.0 < out.theta) {
-                out.theta = 0.0;
-                out.k = k;
-                out.y = yo;
-            }
-            continue;
-        }
-        // the path's entry: a bracket [0, hi] with G(0) > 0 >= G(hi)
-        double hi = -1.0, Ghi = 0.0;
-        const double G1 = sphere_G<Mt>(fr, k, yn.v);
-        if (G1 <= 0.0) {
-            hi = 1.0;
-            Ghi = G1;
-            for (int i = 0; i < N; ++i) yt[i] = yn.v[i];
-        } else {
-            // interior dip: sample the path at the chord's closest point, then
-            // refine the minimum by successive parabolic interpolation
-            const double A = dot3(dP, dP);
-            double t1 = A > 0.0 ? -dot3(m, dP) / A : 0.5;
-            t1 = fmin(fmax(t1, 0.02), 0.98);
-            double ta = 0.0, Ga = G0, tb = 1.0, Gb = G1;
-            rk_substep<Mt>(yo.v, t1 * h, cs, yt);
-            double G1s = sphere_G<Mt>(fr, k, yt);
-            for (int it = 0; it < 4 && G1s > 0.0; ++it) {
-                // vertex of the parabola through (ta, Ga), (t1, G1s), (tb, Gb)
-                const double p = (t1 - ta) * (G1s - Gb), q = (t1 - tb) * (G1s - Ga);
-                const double den = 2.0 * (p - q);
-                if (!(den != 0.0)) break;
-                double tv = t1 - ((t1 - ta) * p - (t1 - tb) * q) / den;
-                if (!(tv > ta && tv < tb) || fabs(tv - t1) < 1e-9) break;
-                double yv[N];
-                rk_substep<Mt>(yo.v, tv * h, cs, yv);
-                const double Gv = sphere_G<Mt>(fr, k, yv);
-                if (Gv <= 0.0) {
-                    t1 = tv;
-                    G1s = Gv;
-                    for (int i = 0; i < N; ++i) yt[i] = yv[i];
-                    break;
-                }
-                if (Gv < G1s) { // tv becomes the middle point
-                    if (tv < t1) {
-                        tb = t1;
-                        Gb = G1s;
-                    } else {
-                        ta = t1;
-                        Ga = G1s;
-                    }
-                    t1 = tv;
-                    G1s = Gv;
-                } else if (tv < t1) {
-                    ta = tv;
-                    Ga = Gv;
-                } else {
-                    tb = tv;
-                    Gb = Gv;
-                }
-            }
-            if (G1s > 0.0) continue; // the path stays outside: a miss
-            hi = t1;
-            Ghi = G1s;
-        }
-        if (hi >= out.theta) continue;
-        // Illinois regula falsi on [0, hi]
-        double lo = 0.0, Glo = G0;
-        int side = 0;
-        double yb[N];
-        for (int i = 0; i < N; ++i) yb[i] = yt[i];
-        const double gtol = 1e-14 * (R2 + dot3(c, c));
-        for (int it = 0; it < 60 && fabs(Ghi) > gtol && hi - lo > 1e-15; ++it) {
-            double t = (lo * Ghi - hi * Glo) / (Ghi - Glo);
-            if (!(t > lo && t < hi)) t = 0.5 * (lo + hi);
-            rk_substep<Mt>(yo.v, t * h, cs, yt);
-            const double Gt = sphere_G<Mt>(fr, k, yt);
-            if (Gt > 0.0) {
-                lo = t;
-                Glo = Gt;
-                if (side == -1) Ghi *= 0.5;
-                side = -1;
-            } else {
-                hi = t;
-                Ghi = Gt;
-                for (int i = 0; i < N; ++i) yb[i] = yt[i];
-                if (side == 1) Glo *= 0.5;
-                side = 1;
-            }
-        }
-        if (hi < out.theta) {
-            out.theta = hi;
-            out.k = k;
-            for (int i = 0; i < N; ++i) out.y.v[i] = yb[i];
-        }
+// ------------------------------------------------------------ ray state I/O
+// Doubles per ray in the ray-state arrays: the state, then E, L, Q for Kerr.
+template <class Mt> constexpr int rs_width() { return Mt::mino ? Mt::N + 3 : Mt::N; }
+
+PRR_HD int64_t rs_index(const JobDev &job, int64_t id) { return id - job.rs_base; }
+
+// k_init: the initial state of ray id (initRays, kernels/solver.cuh:26-56).
+template <class Mt> PRR_HD void init_ray(const JobDev &job, int64_t id) {
+    const int f = job.explicit_x0u0 ? 0 : find_frame(job, id);
+    const FrameDev &fr = job.frames[f];
+    double y[Mt::N];
+    Consts<double> cs;
+    if (job.explicit_x0u0) {
+        const double *s = job.explicit_x0u0 + 8 * id;
+        state_from_xu<Mt>(s, s + 4, fr.M, fr.a, y, cs);
+    } else {
+        const D3 d = camera_ray(fr, id - fr.ray_begin);
+        init_state<Mt>(d3(fr.pos[0], fr.pos[1], fr.pos[2]), d, fr.M, fr.a, y, cs);
     }
-    return out;
+    const int64_t T = job.rs_total, i = rs_index(job, id);
+    for (int k = 0; k < Mt::N; ++k) job.rs.y[k * T + i] = y[k];
+    if (Mt::mino) {
+        job.rs.y[(Mt::N + 0) * T + i] = cs.E;
+        job.rs.y[(Mt::N + 1) * T + i] = cs.L;
+        job.rs.y[(Mt::N + 2) * T + i] = cs.Q;
+    }
 }
 
-// lane_start and lane_finish run once per ray; they are kept out of line so
-// that their transcendental constants and temporaries do not occupy
-// registers across the integration loop.  Out-of-line functions take and
-// return lane data by value.
-template <class Mt> PRR_HD __noinline__ Lane<Mt> lane_start(const JobDev &job, int64_t id) {
-    Lane<Mt> L;
+template <class Mt> PRR_HD void load_consts(const JobDev &job, int64_t id, const FrameDev &fr, Consts<double> &cs) {
+    const int64_t T = job.rs_total, i = rs_index(job, id);
+    cs.M = fr.M;
+    cs.a = fr.a;
+    if (Mt::mino) {
+        cs.E = job.rs.y[(Mt::N + 0) * T + i];
+        cs.L = job.rs.y[(Mt::N + 1) * T + i];
+        cs.Q = job.rs.y[(Mt::N + 2) * T + i];
+    } else {
+        cs.E = cs.L = cs.Q = 0.0;
+    }
+}
+
+// The lane takes ray id from the ray-state arrays.
+template <class Mt> PRR_HD void lane_load(const JobDev &job, Lane<Mt> &L, int64_t id) {
     L.rid = id;
     L.frame = job.explicit_x0u0 ? 0 : find_frame(job, id);
     const FrameDev &fr = job.frames[L.frame];
-    if (job.explicit_x0u0) {
-        const double *s = job.explicit_x0u0 + 8 * id;
-        state_from_xu<Mt>(s, s + 4, fr.M, fr.a, L.y, L.cs);
-    } else {
-        const D3 d = camera_ray(fr, id - fr.ray_begin);
-        init_state<Mt>(d3(fr.pos[0], fr.pos[1], fr.pos[2]), d, fr.M, fr.a, L.y, L.cs);
-    }
+    const int64_t T = job.rs_total, i = rs_index(job, id);
+    for (int k = 0; k < Mt::N; ++k) L.y[k] = job.rs.y[k * T + i];
+    load_consts<Mt>(job, id, fr, L.cs);
     L.h = job.tr.h / lambda_rate<Mt>(L.y, L.cs);
     L.lam = 0.0;
     L.steps = L.attempts = L.att64 = 0;
     L.hit = -1;
     L.P0[0] = L.P0[1] = L.P0[2] = 0.0f;
     if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
-    return L;
 }
 
-// Writes the outputs of a finished ray (rayToRGB, metrics/schwarzschild.cuh:151-183).
-template <class Mt> PRR_HD __noinline__ void lane_finish(const JobDev &job, const Lane<Mt> L, int cls) {
-    const int64_t id = L.rid;
+// The lane takes ray id from its checkpoint j (adjoint replay).
+template <class Mt> PRR_HD void lane_load_checkpoint(const JobDev &job, Lane<Mt> &L, int64_t id, int j) {
+    L.rid = id;
+    L.frame = find_frame(job, id);
     const FrameDev &fr = job.frames[L.frame];
+    const double *c = job.adj.ckpt + 8 * (job.adj.ckpt_off[id] + j);
+    for (int k = 0; k < Mt::N; ++k) L.y[k] = c[k];
+    load_consts<Mt>(job, id, fr, L.cs);
+    L.h = c[6];
+    L.attempts = int32_t(c[7]);
+    L.lam = Mt::mino ? L.y[5] : 0.0;
+    L.steps = j * job.adj.K;
+    L.att64 = 0;
+    L.hit = -1;
+    L.P0[0] = L.P0[1] = L.P0[2] = 0.0f;
+    if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
+}
+
+PRR_HD void store_checkpoint(const JobDev &job, const double *y, int n, double h, int32_t attempts, int64_t id,
+                             int j) {
+    double *c = job.adj.ckpt + 8 * (job.adj.ckpt_off[id] + j);
+    for (int k = 0; k < 6; ++k) c[k] = k < n ? y[k] : 0.0;
+    c[6] = h;
+    c[7] = double(attempts);
+}
+
+// The lane's final state back into the ray-state arrays.
+template <class Mt> PRR_HD void lane_store(const JobDev &job, const Lane<Mt> &L, int cls) {
+    const int64_t T = job.rs_total, i = rs_index(job, L.rid);
+    for (int k = 0; k < Mt::N; ++k) job.rs.y[k * T + i] = L.y[k];
+    job.rs.cls[i] = int8_t(cls);
+    job.rs.hit[i] = int8_t(L.hit);
+    job.rs.steps[i] = L.steps;
+    job.rs.attempts[i] = L.attempts;
+}
+
+// k_shade: outputs of a finished ray (rayToRGB, metrics/schwarzschild.cuh:151-183).
+template <class Mt> PRR_HD void shade_ray(const JobDev &job, int64_t id) {
+    const int f = job.explicit_x0u0 ? 0 : find_frame(job, id);
+    const FrameDev &fr = job.frames[f];
+    const int64_t T = job.rs_total, i = rs_index(job, id);
+    double y[Mt::N];
+    for (int k = 0; k < Mt::N; ++k) y[k] = job.rs.y[k * T + i];
+    Consts<double> cs;
+    load_consts<Mt>(job, id, fr, cs);
+    const int cls = job.rs.cls[i], hit = job.rs.hit[i];
     D3 endpoint;
     float3 color;
     if (cls == PRR_SPHERE) {
-        const D3 m = cart_of<Mt>(L.y) - d3(fr.sph[L.hit][0], fr.sph[L.hit][1], fr.sph[L.hit][2]);
-        endpoint = normalize3(m);
-        const int slot = fr.sph_tex[L.hit];
+        endpoint = normalize3(cart_of<Mt>(y) - d3(fr.sph[hit][0], fr.sph[hit][1], fr.sph[hit][2]));
+        const int slot = fr.sph_tex[hit];
         color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &job.tex[slot] : nullptr, endpoint);
     } else if (cls == PRR_ESCAPE) {
-        const D3 n = escape_dir<Mt>(L.y, L.cs);
+        const D3 n = escape_dir<Mt>(y, cs);
         endpoint = n;
         double th, ph;
         if (Mt::kind == PRR_SCHW_SPH) {
             const double two_pi = 6.28318530717958647692, pi = 3.14159265358979323846;
-            th = L.y[1];
-            ph = L.y[2];
+            th = y[1];
+            ph = y[2];
             if (ph == two_pi) ph = 0.0; // main/base.h:86-89
             if (th == pi) th = 0.0;
         } else {
@@ -476,55 +394,34 @@ template <class Mt> PRR_HD __noinline__ void lane_finish(const JobDev &job, cons
         }
         color = sky_lookup(job.tex[0], th, ph);
     } else {
-        endpoint = cart_of<Mt>(L.y);
+        endpoint = cart_of<Mt>(y);
         color = make_float3(0.f, 0.f, 0.f);
     }
-    const OutDev &o = job.out;
+    const OutDev &o = job.out; // per-ray arrays like the ray state (index i)
     if (o.color) {
-        o.color[3 * id] = color.x;
-        o.color[3 * id + 1] = color.y;
-        o.color[3 * id + 2] = color.z;
+        o.color[3 * i] = color.x;
+        o.color[3 * i + 1] = color.y;
+        o.color[3 * i + 2] = color.z;
     }
-    if (o.cls) o.cls[id] = int8_t(cls);
+    if (o.cls) o.cls[i] = int8_t(cls);
     if (o.endpoint) {
-        o.endpoint[3 * id] = endpoint.x;
-        o.endpoint[3 * id + 1] = endpoint.y;
-        o.endpoint[3 * id + 2] = endpoint.z;
+        o.endpoint[3 * i] = endpoint.x;
+        o.endpoint[3 * i + 1] = endpoint.y;
+        o.endpoint[3 * i + 2] = endpoint.z;
     }
-    if (o.steps) o.steps[id] = L.steps;
-    if (o.attempts) o.attempts[id] = L.attempts;
+    if (o.steps) o.steps[i] = job.rs.steps[i];
+    if (o.attempts) o.attempts[i] = job.rs.attempts[i];
     if (o.state8) {
         double x[4], u[4];
-        xu_from_state<Mt>(L.y, L.cs, x, u);
+        xu_from_state<Mt>(y, cs, x, u);
         for (int c = 0; c < 4; ++c) {
-            o.state8[8 * id + c] = x[c];
-            o.state8[8 * id + 4 + c] = u[c];
+            o.state8[8 * i + c] = x[c];
+            o.state8[8 * i + 4 + c] = u[c];
         }
     }
 }
 
-// Event record of a terminal RKF45 step (the adjoint's last step): the state
-// at the step's start, the step size and the located fraction.
-PRR_HD __noinline__ void record_event(EventDev *ev, const Vec<8> yo, double h, double theta, int cls, int hit) {
-    for (int i = 0; i < 8; ++i) ev->y_prev[i] = yo.v[i];
-    ev->h = h;
-    ev->theta = theta;
-    ev->cls = cls;
-    ev->hit = hit;
-}
-
-template <int N> PRR_HD Vec<8> widen8(const double *y) {
-    Vec<8> o;
-    for (int i = 0; i < 8; ++i) o.v[i] = i < N ? y[i] : 0.0;
-    return o;
-}
-
-template <int N> PRR_HD Vec<N> vec_of(const double *y) {
-    Vec<N> o;
-    for (int i = 0; i < N; ++i) o.v[i] = y[i];
-    return o;
-}
-
+// ------------------------------------------------------------ Euler
 // Termination tests after an accepted Euler step (reference semantics: the
 // chord P(yo) -> P(yn), spherical wraps, horizon, first state beyond
 // r_escape).  yn becomes the lane state.
@@ -566,10 +463,117 @@ template <class Mt> PRR_HD int euler_events(const JobDev &job, Lane<Mt> &L, cons
     return -1;
 }
 
-// Termination tests after an accepted RKF45 step yo -> yn of size h, with
-// path-located sphere hits and escape crossings.
+template <class Mt, bool MIXED> PRR_HD int euler_step(const JobDev &job, Lane<Mt> &L) {
+    constexpr int N = Mt::N;
+    const TracerDev &tr = job.tr;
+    if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max) return PRR_STEP_CAP;
+    const bool fp64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
+    double f[N];
+    if (fp64) euler_rhs<Mt, double>(L.y, L.cs, f);
+    else euler_rhs<Mt, float>(L.y, L.cs, f);
+    bool ok = true;
+#pragma unroll
+    for (int i = 0; i < N; ++i) ok = ok && __builtin_isfinite(f[i]);
+    if (!ok) return PRR_NAN;
+    const double h = Mt::mino ? tr.h / lambda_rate<Mt>(L.y, L.cs) : tr.h;
+    double yn[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) yn[i] = L.y[i] + h * f[i];
+    L.steps++;
+    L.attempts++;
+    L.att64 += fp64 ? 1 : 0;
+    L.lam = Mt::mino ? yn[5] : L.lam + h;
+    if (!finite_state<Mt>(yn)) return PRR_NAN;
+    double yo[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) yo[i] = L.y[i];
+    return euler_events<Mt>(job, L, yo, yn);
+}
+
+// ------------------------------------------------------------ RKF45 events
+enum { MODE_IDLE = 0, MODE_ACTIVE = 1, MODE_LOCATE = 2 };
+enum { LOC_ESCAPE = 0, LOC_SPHERE = 1 };
+enum { SPH_NEXT = 0, SPH_PROBE = 1, SPH_MIN = 2, SPH_BRACKET = 3 };
+constexpr int kNeedSubstep = -2; // locate_prepare: a sub-step of the returned size is due
+constexpr int kResume = -3;      // the step had no event after all: the lane goes on
+
+template <int N> PRR_HD void copy_n(double *d, const double *s) {
+    for (int i = 0; i < N; ++i) d[i] = s[i];
+}
+
+// Adjoint pass 2: checkpoint every K accepted steps while the lane integrates.
+template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<Mt> &L) {
+    if (job.adj.mode != ADJ_CHECKPOINT) return;
+    const int K = job.adj.K;
+    if (L.steps % K == 0 && L.steps < job.adj.nsteps[L.rid])
+        store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, L.steps / K);
+}
+
+// Adjoint replay: the step's start and size of every accepted step of the segment.
+template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &L, double h) {
+    if (job.adj.mode != ADJ_REPLAY) return;
+    const int64_t slot = (L.steps - 1) - int64_t(job.adj.seg) * job.adj.K; // L.steps already counts this step
+    const int64_t T = job.rs_total;
+    double *r = job.adj.rec + (slot * 6) * T + rs_index(job, L.rid);
+    for (int c = 0; c < 5; ++c) r[c * T] = L.y[c];
+    r[5 * T] = h;
+}
+
+// A located event ends the ray: the lane takes the event's state (adjoint
+// replay: the terminal step is recorded for the seed).
 template <class Mt>
-PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, const double *yo, double *yn, double h) {
+PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, const LocateCtx &C, const double *ys, double theta, int cls,
+                          int hit) {
+    if (job.adj.mode == ADJ_REPLAY && job.adj.event) {
+        EventDev &ev = job.adj.event[L.rid];
+        for (int i = 0; i < 8; ++i) ev.y_prev[i] = i < Mt::N ? L.y[i] : 0.0;
+        ev.h = C.h;
+        ev.theta = theta;
+        ev.cls = cls;
+        ev.hit = hit;
+    }
+    copy_n<Mt::N>(L.y, ys);
+    L.hit = hit;
+    if (cls == PRR_ESCAPE && Mt::kind == PRR_SCHW_SPH) sph_post_step(L.y);
+    return cls;
+}
+
+// Escape test of the accepted step L.y -> yn (size h); starts the location
+// when r crosses r_escape.  Then spherical wraps and the horizon.  Returns a
+// class, -1 (go on, L.y = yn) or MODE_LOCATE's kNeedSubstep marker through
+// `mode`.
+template <class Mt>
+PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h,
+                               int &mode) {
+    const FrameDev &fr = job.frames[L.frame];
+    const double r = radius_of<Mt>(yn);
+    if (r >= job.tr.r_escape) {
+        double go, gn, dg;
+        escape_g<Mt>(L.y, job.tr.r_escape, go, dg);
+        if (!(go < 0.0)) return finish_located(job, L, C, L.y, 0.0, PRR_ESCAPE, -1); // started outside
+        escape_g<Mt>(yn, job.tr.r_escape, gn, dg);
+        copy_n<Mt::N>(C.yn, yn);
+        C.h = h;
+        C.kind = LOC_ESCAPE;
+        C.th = fmin(fmax(go / (go - gn), 0.0), 1.0);
+        C.it = 0;
+        mode = MODE_LOCATE;
+        return -1;
+    }
+    double y[Mt::N];
+    copy_n<Mt::N>(y, yn);
+    if (Mt::kind == PRR_SCHW_SPH) sph_post_step(y);
+    copy_n<Mt::N>(L.y, y);
+    if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
+    if (r <= fr.r_hor) return PRR_HORIZON;
+    maybe_checkpoint<Mt>(job, L);
+    return -1;
+}
+
+// Events of an accepted RKF45 step L.y -> yn of size h: a sphere screen that
+// starts the sphere location, or the escape / wrap / horizon tests.
+template <class Mt>
+PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h, int &mode) {
     const FrameDev &fr = job.frames[L.frame];
     if (fr.n_spheres > 0) {
         float P1[3];
@@ -578,119 +582,285 @@ PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, const double *yo, double *
         for (int k = 0; k < fr.n_spheres; ++k)
             if (sphere_screen(fr, k, L.P0, P1)) mask |= 1u << k;
         if (mask) {
-            const Located<Mt::N> ev =
-                sphere_locate<Mt>(fr, mask, vec_of<Mt::N>(yo), vec_of<Mt::N>(yn), h, L.cs);
-            if (ev.k >= 0) {
-                for (int i = 0; i < Mt::N; ++i) L.y[i] = ev.y.v[i];
-                L.hit = ev.k;
-                if (job.out.event)
-                    record_event(job.out.event + L.rid, widen8<Mt::N>(yo), h, ev.theta, PRR_SPHERE, ev.k);
-                return PRR_SPHERE;
-            }
+            copy_n<Mt::N>(C.yn, yn);
+            C.h = h;
+            C.kind = LOC_SPHERE;
+            C.phase = SPH_NEXT;
+            C.mask = int32_t(mask);
+            C.k = -1;
+            C.best_k = -1;
+            C.best_th = 2.0;
+            mode = MODE_LOCATE;
+            return -1;
         }
-        L.P0[0] = P1[0];
-        L.P0[1] = P1[1];
-        L.P0[2] = P1[2];
     }
-    const double r = radius_of<Mt>(yn);
-    if (r >= job.tr.r_escape) {
-        const Located<Mt::N> ev = escape_locate<Mt>(vec_of<Mt::N>(yo), vec_of<Mt::N>(yn), h, L.cs, job.tr.r_escape);
-        for (int i = 0; i < Mt::N; ++i) L.y[i] = ev.y.v[i];
-        if (Mt::kind == PRR_SCHW_SPH) sph_post_step(L.y);
-        if (job.out.event) record_event(job.out.event + L.rid, widen8<Mt::N>(yo), h, ev.theta, PRR_ESCAPE, -1);
-        return PRR_ESCAPE;
+    return escape_wrap_horizon<Mt>(job, L, C, yn, h, mode);
+}
+
+// Event location before the sub-step of this trip: sphere bookkeeping that
+// needs no sub-step is done here.  Returns kNeedSubstep with *hs set, a class
+// (the ray ends), or -1 / kResume with the lane active again.
+template <class Mt>
+PRR_HD int locate_prepare(const JobDev &job, Lane<Mt> &L, LocateCtx &C, double &hs, int &mode) {
+    constexpr int N = Mt::N;
+    if (C.kind == LOC_ESCAPE) {
+        hs = C.th * C.h;
+        return kNeedSubstep;
     }
-    if (Mt::kind == PRR_SCHW_SPH) {
-        sph_post_step(yn);
-        if (fr.n_spheres > 0) cart_of_f32<Mt>(yn, L.P0);
+    const FrameDev &fr = job.frames[L.frame];
+    for (;;) {
+        if (C.phase == SPH_BRACKET) {
+            const double R2 = fr.sph[C.k][3] * fr.sph[C.k][3];
+            const D3 c = d3(fr.sph[C.k][0], fr.sph[C.k][1], fr.sph[C.k][2]);
+            const double gtol = 1e-14 * (R2 + dot3(c, c));
+            if (C.it < 60 && fabs(C.Ghi) > gtol && C.hi - C.lo > 1e-15) {
+                double t = (C.lo * C.Ghi - C.hi * C.Glo) / (C.Ghi - C.Glo);
+                if (!(t > C.lo && t < C.hi)) t = 0.5 * (C.lo + C.hi);
+                C.th = t;
+                hs = t * C.h;
+                return kNeedSubstep;
+            }
+            if (C.hi < C.best_th) {
+                C.best_th = C.hi;
+                C.best_k = C.k;
+                copy_n<N>(C.ybest, C.yb);
+            }
+            C.phase = SPH_NEXT;
+        }
+        if (C.phase == SPH_PROBE || C.phase == SPH_MIN) {
+            hs = C.th * C.h;
+            return kNeedSubstep;
+        }
+        // SPH_NEXT: the next screened sphere
+        int k = C.k + 1;
+        while (k < fr.n_spheres && !((unsigned(C.mask) >> k) & 1u)) ++k;
+        if (k >= fr.n_spheres) {
+            if (C.best_k >= 0) {
+                mode = MODE_IDLE;
+                return finish_located(job, L, C, C.ybest, C.best_th, PRR_SPHERE, C.best_k);
+            }
+            // no hit: the step's other events
+            mode = MODE_ACTIVE;
+            double yn[N];
+            copy_n<N>(yn, C.yn);
+            const int r = escape_wrap_horizon<Mt>(job, L, C, yn, C.h, mode);
+            if (r >= 0) mode = MODE_IDLE;
+            if (mode == MODE_LOCATE) {
+                hs = C.th * C.h;
+                return kNeedSubstep;
+            }
+            return r >= 0 ? r : kResume;
+        }
+        C.k = k;
+        const D3 c = d3(fr.sph[k][0], fr.sph[k][1], fr.sph[k][2]);
+        const double R2 = fr.sph[k][3] * fr.sph[k][3];
+        const D3 P0 = cart_of<Mt>(L.y), P1 = cart_of<Mt>(C.yn);
+        const D3 m = P0 - c;
+        C.G0 = dot3(m, m) - R2;
+        if (C.G0 <= 0.0) { // the step starts inside (reference: chord t = 0)
+            if (0.0 < C.best_th) {
+                C.best_th = 0.0;
+                C.best_k = k;
+                copy_n<N>(C.ybest, L.y);
+            }
+            continue;
+        }
+        const double G1 = dot3(P1 - c, P1 - c) - R2;
+        if (G1 <= 0.0) { // the step ends inside: bracket [0, 1]
+            C.lo = 0.0;
+            C.Glo = C.G0;
+            C.hi = 1.0;
+            C.Ghi = G1;
+            copy_n<N>(C.yb, C.yn);
+            C.side = 0;
+            C.it = 0;
+            C.phase = C.hi < C.best_th ? SPH_BRACKET : SPH_NEXT;
+            continue;
+        }
+        // probe the path at the chord's closest point
+        const D3 dP = P1 - P0;
+        const double A = dot3(dP, dP);
+        double t1 = A > 0.0 ? -dot3(m, dP) / A : 0.5;
+        C.t1 = fmin(fmax(t1, 0.02), 0.98);
+        C.ta = 0.0;
+        C.Ga = C.G0;
+        C.tb = 1.0;
+        C.Gb = G1;
+        C.th = C.t1;
+        C.phase = SPH_PROBE;
     }
-    for (int i = 0; i < Mt::N; ++i) L.y[i] = yn[i];
-    if (r <= fr.r_hor) return PRR_HORIZON;
+}
+
+// Vertex of the parabola through (ta, Ga), (t1, G1), (tb, Gb); false when
+// it is not a new interior point.
+PRR_HD bool parabola_vertex(const LocateCtx &C, double &tv) {
+    const double p = (C.t1 - C.ta) * (C.G1 - C.Gb), q = (C.t1 - C.tb) * (C.G1 - C.Ga);
+    const double den = 2.0 * (p - q);
+    if (!(den != 0.0)) return false;
+    tv = C.t1 - ((C.t1 - C.ta) * p - (C.t1 - C.tb) * q) / den;
+    return tv > C.ta && tv < C.tb && fabs(tv - C.t1) >= 1e-9;
+}
+
+// The sub-step of fraction C.th gave yt: advance the location.  Returns a
+// class (located) or -1 (go on locating).
+template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yt) {
+    constexpr int N = Mt::N;
+    if (C.kind == LOC_ESCAPE) {
+        double g, dg;
+        escape_g<Mt>(yt, job.tr.r_escape, g, dg);
+        const double tol = 1e-13 * (Mt::mino ? 1.0 / job.tr.r_escape : job.tr.r_escape);
+        if (fabs(g) <= tol || !(dg * C.h != 0.0) || C.it >= 6) return finish_located(job, L, C, yt, C.th, PRR_ESCAPE, -1);
+        C.th = fmin(fmax(C.th - g / (dg * C.h), 0.0), 1.0);
+        C.it++;
+        return -1;
+    }
+    const FrameDev &fr = job.frames[L.frame];
+    const double G = sphere_G<Mt>(fr, C.k, yt);
+    if (C.phase == SPH_BRACKET) { // Illinois regula falsi
+        if (G > 0.0) {
+            C.lo = C.th;
+            C.Glo = G;
+            if (C.side == -1) C.Ghi *= 0.5;
+            C.side = -1;
+        } else {
+            C.hi = C.th;
+            C.Ghi = G;
+            copy_n<N>(C.yb, yt);
+            if (C.side == 1) C.Glo *= 0.5;
+            C.side = 1;
+        }
+        C.it++;
+        return -1;
+    }
+    if (G <= 0.0) { // the path is inside at this probe: bracket [0, th]
+        C.lo = 0.0;
+        C.Glo = C.G0;
+        C.hi = C.th;
+        C.Ghi = G;
+        copy_n<N>(C.yb, yt);
+        C.side = 0;
+        C.it = 0;
+        C.phase = C.hi < C.best_th ? SPH_BRACKET : SPH_NEXT;
+        return -1;
+    }
+    if (C.phase == SPH_PROBE) {
+        C.G1 = G;
+        C.it = 0;
+    } else { // SPH_MIN: fold the probe tv = C.th into the three points
+        const double tv = C.th;
+        if (G < C.G1) {
+            if (tv < C.t1) {
+                C.tb = C.t1;
+                C.Gb = C.G1;
+            } else {
+                C.ta = C.t1;
+                C.Ga = C.G1;
+            }
+            C.t1 = tv;
+            C.G1 = G;
+        } else if (tv < C.t1) {
+            C.ta = tv;
+            C.Ga = G;
+        } else {
+            C.tb = tv;
+            C.Gb = G;
+        }
+        C.it++;
+    }
+    double tv;
+    if (C.it < 4 && parabola_vertex(C, tv)) {
+        C.th = tv;
+        C.phase = SPH_MIN;
+    } else {
+        C.phase = SPH_NEXT; // the path stays outside: a miss
+    }
     return -1;
 }
 
-// One Euler step or one RKF45 attempt.  Returns the class when the ray ends,
-// -1 otherwise.
-template <class Mt, int SOLVER, bool MIXED> PRR_HD int lane_attempt(const JobDev &job, Lane<Mt> &L) {
+// One loop trip of a lane: an Euler step, an RKF45 attempt, or one sub-step
+// of an event location.  Returns the class when the ray ends (mode becomes
+// MODE_IDLE), -1 otherwise.
+template <class Mt, int SOLVER, bool MIXED>
+PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode) {
     constexpr int N = Mt::N;
-    const TracerDev &tr = job.tr;
-    if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max) return PRR_STEP_CAP;
-    const double r = radius_of<Mt>(L.y);
     if (SOLVER == PRR_EULER) {
-        const bool fp64 = !MIXED || r < tr.r_fp64;
-        double f[N];
-        if (fp64) euler_rhs<Mt, double>(L.y, L.cs, f);
-        else euler_rhs<Mt, float>(L.y, L.cs, f);
-        bool ok = true;
-#pragma unroll
-        for (int i = 0; i < N; ++i) ok = ok && __builtin_isfinite(f[i]);
-        if (!ok) return PRR_NAN;
-        const double h = Mt::mino ? tr.h / lambda_rate<Mt>(L.y, L.cs) : tr.h;
-        double yn[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) yn[i] = L.y[i] + h * f[i];
-        L.steps++;
+        const int r = euler_step<Mt, MIXED>(job, L);
+        if (r >= 0) mode = MODE_IDLE;
+        return r;
+    }
+    const TracerDev &tr = job.tr;
+    double hs;
+    bool use64 = true, at_min = false;
+    if (mode == MODE_LOCATE) {
+        const int r = locate_prepare<Mt>(job, L, C, hs, mode);
+        if (r == kResume) return -1;
+        if (r != kNeedSubstep) {
+            if (r >= 0) mode = MODE_IDLE;
+            return r;
+        }
+    } else {
+        if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max || L.attempts >= tr.max_attempts) {
+            mode = MODE_IDLE;
+            return PRR_STEP_CAP;
+        }
         L.attempts++;
-        L.att64 += fp64 ? 1 : 0;
-        L.lam = Mt::mino ? yn[5] : L.lam + h;
-        if (!finite_state<Mt>(yn)) return PRR_NAN;
-        double yo[N];
-#pragma unroll
-        for (int i = 0; i < N; ++i) yo[i] = L.y[i];
-        return euler_events<Mt>(job, L, yo, yn);
+        hs = rkf_step_size<Mt>(tr, L.y, L.cs, L.h, at_min);
+        // FP64 stages: the plain form; in the FP32-bulk form inside r_fp64 and,
+        // for Kerr, where the step can come within ~13 degrees of the polar
+        // axis (1 - mu^2 and L / (1 - mu^2) lose their relative precision)
+        use64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
+        if (Mt::mino && !use64) {
+            const double mu_reach = fabs(L.y[2]) + hs * fabs(L.y[3]);
+            use64 = 1.0 - mu_reach * mu_reach < kAxisFP64;
+        }
+        L.att64 += use64 ? 1 : 0;
     }
-    if (L.attempts >= tr.max_attempts) return PRR_STEP_CAP;
-    L.attempts++;
-    bool at_min;
-    const double h = rkf_step_size<Mt>(tr, L.y, L.cs, L.h, at_min);
-    // FP64 stages: the plain form; in the FP32-bulk form inside r_fp64 and,
-    // for Kerr, where the step can come within ~13 degrees of the polar axis
-    // (1 - mu^2 and L / (1 - mu^2) lose their relative precision in FP32)
-    bool fp64 = !MIXED || r < tr.r_fp64;
-    if (Mt::mino && !fp64) {
-        const double mu_reach = fabs(L.y[2]) + h * fabs(L.y[3]);
-        fp64 = 1.0 - mu_reach * mu_reach < kAxisFP64;
-    }
-    L.att64 += fp64 ? 1 : 0;
     double d4[N], e[N];
-    if (fp64) rkf_stages<Mt, double>(L.y, h, L.cs, d4, e);
-    else if (Mt::mino) kerr_stages_fp32(L.y, h, L.cs, d4, e);
-    else rkf_stages<Mt, float>(L.y, h, L.cs, d4, e);
-    double errn = 0.0;
+    if (use64) rkf_stages<Mt, double>(L.y, hs, L.cs, d4, e);
+    else if (Mt::mino) kerr_stages_fp32(L.y, hs, L.cs, d4, e);
+    else rkf_stages<Mt, float>(L.y, hs, L.cs, d4, e);
     double yn[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) yn[i] = L.y[i] + d4[i];
+    if (mode == MODE_LOCATE) {
+        const int r = locate_update<Mt>(job, L, C, yn);
+        if (r >= 0) mode = MODE_IDLE;
+        return r;
+    }
+    double errn = 0.0;
     // Kerr: mu = cos th and mu' are measured in units of th (divided by
     // sin th), so that the tolerance keeps its meaning near the poles, where a
     // fixed error in mu is a large relative error in 1 - mu^2
     const double ist = Mt::mino ? 1.0 / fmax(sqrt(fmax(1.0 - L.y[2] * L.y[2], 0.0)), 1e-12) : 1.0;
 #pragma unroll
-    for (int i = 0; i < N; ++i) {
-        yn[i] = L.y[i] + d4[i];
-        if (i < Mt::NERR) {
-            const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(yn[i]));
-            const double ei = (Mt::mino && (i == 2 || i == 3)) ? fabs(e[i]) * ist : fabs(e[i]);
-            errn = fmax(errn, ei / sc);
-        }
+    for (int i = 0; i < Mt::NERR; ++i) {
+        const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(yn[i]));
+        const double ei = (Mt::mino && (i == 2 || i == 3)) ? fabs(e[i]) * ist : fabs(e[i]);
+        errn = fmax(errn, ei / sc);
     }
     // Kerr: a step may not reach the polar axis (|cos th| >= 1), where the chart is singular
     if (Mt::mino && !(fabs(yn[2]) < 1.0)) errn = fmax(errn, 1024.0);
     // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
     double hn;
-    if (MIXED) hn = 0.9 * h * double(powf(float(errn), -0.2f));
-    else hn = 0.9 * h * pow(errn, -0.2);
+    if (MIXED) hn = 0.9 * hs * double(powf(float(errn), -0.2f));
+    else hn = 0.9 * hs * pow(errn, -0.2);
     if (!Mt::mino) hn = fmin(fmax(hn, tr.hmin), tr.hmax);
     if (errn > 1.0 && !at_min) { // reject (:185-188)
         L.h = hn;
         return -1;
     }
-    if (!finite_state<Mt>(yn)) return PRR_NAN;
+    if (!finite_state<Mt>(yn)) {
+        mode = MODE_IDLE;
+        return PRR_NAN;
+    }
     if constexpr (Mt::mino) Kerr::project(yn, L.cs);
     L.steps++;
-    L.lam = Mt::mino ? yn[5] : L.lam + h;
+    L.lam = Mt::mino ? yn[5] : L.lam + hs;
     L.h = hn;
-    double yo[N];
-#pragma unroll
-    for (int i = 0; i < N; ++i) yo[i] = L.y[i];
-    return rkf_events<Mt>(job, L, yo, yn, h);
+    maybe_record<Mt>(job, L, hs);
+    const int r = rkf_events<Mt>(job, L, C, yn, hs, mode);
+    if (r >= 0) mode = MODE_IDLE;
+    return r;
 }
 
 } // namespace prr
