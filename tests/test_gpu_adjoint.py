@@ -121,7 +121,7 @@ def test_fp32_bulk_gradient_matches_the_fp64_reference_gradient(setup):
 def test_adjoint_single_pixels_match_central_differences(setup):
     """dI/dp of single pixels (dL/dI one-hot on one pixel) against central
     differences of that pixel (h = 1e-6), for escape and sphere pixels whose
-    colour changes with the parameters."""
+    colour changes with the parameters and is differentiable on [p - h, p + h]."""
     tmpl, o0, _, ref = setup
     p0 = np.array([o0.spin, o0.r_cam, o0.incl, o0.sphere_phi])
     I0, c0 = render(ref, tmpl, o0)
@@ -144,8 +144,16 @@ def test_adjoint_single_pixels_match_central_differences(setup):
             continue
         if any(plus[k][1][j, i, 0] != cls or minus[k][1][j, i, 0] != cls for k in range(4)):
             continue
-        fdw = np.array([(plus[k][0][j, i] - minus[k][0][j, i]) / (2 * h) for k in range(4)]) @ wch
+        fwd = np.array([(plus[k][0][j, i] - I0[j, i]) / h for k in range(4)]) @ wch
+        bwd = np.array([(I0[j, i] - minus[k][0][j, i]) / h for k in range(4)]) @ wch
+        fdw = (fwd + bwd) / 2
         if np.abs(fdw).max() < 1.0:  # inside a checker cell: no signal above the float32 rounding
+            continue
+        # A texel edge (a kink of the bilinear lookup) inside [p - h, p + h]: the
+        # one-sided differences disagree by more than the float32 rounding of the
+        # image (an ulp of 0.5 over h is 0.06), and the pixel has no derivative to
+        # compare against there.
+        if np.abs(fwd - bwd).max() > 0.15 + 0.05 * np.abs(fdw).max():
             continue
         dLdI = np.zeros((SIZE, SIZE, 3), np.float32)
         dLdI[j, i] = wch
