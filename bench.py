@@ -56,6 +56,7 @@ def run_form(ctx, frame, precision, steps, warmup, out8, outf, torch):
     f32, f64 = st["flops_fp32"], st["flops_fp64"]
     t_int = integ / 1e3
     return dict(ms_per_step=ms, integrate_ms=integ, ssaa_ms=sum(s["ssaa_ms"] for s in per) / steps,
+                init_ms=sum(s["init_ms"] for s in per) / steps, shade_ms=sum(s["shade_ms"] for s in per) / steps,
                 fxaa_ms=sum(s["fxaa_ms"] for s in per) / steps, rays=st["rays"], attempts=st["attempts"],
                 accepted=st["accepted"], attempts_fp64=st["attempts_fp64"], flops_fp32=f32, flops_fp64=f64,
                 rays_per_s=st["rays"] / (ms / 1e3), attempts_per_s=st["attempts"] / t_int,
@@ -114,6 +115,23 @@ def run_c1_batched(api, A, CF, ctx, steps, warmup, torch):
                 graph_ms=st["total_ms"])
 
 
+def run_copy(torch, nbytes=1 << 30):
+    """A plain device-to-device copy of nbytes: the calibration kernel for the
+    FETCH_SIZE / WRITE_SIZE counters (it reads and writes exactly nbytes)."""
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    dst.copy_(src)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1)
+    del src, dst
+    return dict(bytes_read=nbytes, bytes_written=nbytes, ms=ms, gbs=2 * nbytes / ms / 1e6)
+
+
 def run_batch(api, A, CF, reps, torch):
     """C2: 64 frames (389,520,384 rays) in one persistent launch, replayed from a hipGraph."""
     frames = CF.frames_c2()
@@ -165,7 +183,14 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the C3 adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the C2 64-frame batch leg")
+    ap.add_argument("--legs", default="fp64,fp32,adjoint,c2,batched,cpu",
+                    help="comma-separated legs to run: fp64, fp32, adjoint, c2, batched, cpu, copy "
+                         "(copy: a 1 GiB device-to-device copy, the calibration kernel for HBM byte counters)")
     args = ap.parse_args()
+    legs = set(args.legs.split(","))
+    for flag, leg in ((args.no_cpu, "cpu"), (args.no_adjoint, "adjoint"), (args.no_batch, "c2")):
+        if flag:
+            legs.discard(leg)
     if args.gpus != 1:
         print(f"bench.py measures one GPU: the integrator has no multi-GPU path in this build "
               f"(--gpus {args.gpus} requested).", file=sys.stderr)
@@ -176,6 +201,13 @@ def main():
     from parrelray_amd import api
     from parrelray_amd import configs as CF
 
+    def progress(**kw):
+        print(json.dumps({"progress": kw}), file=sys.stderr, flush=True)
+
+    copy = None
+    if "copy" in legs:
+        copy = run_copy(torch)
+        progress(copy=copy)
     frame = CF.frame_c1()
     ctx = api.Context(0)
     ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.95)
@@ -183,57 +215,75 @@ def main():
     ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
     out8 = torch.empty((frame.height, frame.width, 3), dtype=torch.uint8, device="cuda")
     outf = torch.empty((frame.height, frame.width, 3), dtype=torch.float32, device="cuda")
-    fp64 = run_form(ctx, frame, A.PRR_FP64, args.steps, args.warmup, out8, outf, torch)
-    print(json.dumps({"progress": "fp64 done", "ms_per_step": fp64["ms_per_step"]}), file=sys.stderr, flush=True)
-    fp32 = run_form(ctx, frame, A.PRR_FP32_BULK, args.steps, args.warmup, out8, outf, torch)
-    print(json.dumps({"progress": "fp32_bulk done", "ms_per_step": fp32["ms_per_step"]}), file=sys.stderr,
-          flush=True)
-    adj = None if args.no_adjoint else run_adjoint(api, A, CF, max(1, args.steps // 2))
-    print(json.dumps({"progress": "adjoint done", "adjoint": adj}), file=sys.stderr, flush=True)
-    batch = None if args.no_batch else run_batch(api, A, CF, 2, torch)
-    print(json.dumps({"progress": "batch done", "batch": batch}), file=sys.stderr, flush=True)
-    bat = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch)
-    print(json.dumps({"progress": "c1 batched done", "ms_per_frame": bat["ms_per_frame"]}), file=sys.stderr,
-          flush=True)
-    cpu = None if args.no_cpu else cpu_baseline(frame)
-    t_int = fp32["integrate_ms"] / 1e3
-    f32, f64 = fp32["flops_fp32"], fp32["flops_fp64"]
-    t_ideal = f32 / P32 + f64 / P64
-    t_ideal_np = f32 / P32_NONPACKED + f64 / P64
+    fp64 = fp32 = adj = batch = bat = None
+    if "fp64" in legs:
+        fp64 = run_form(ctx, frame, A.PRR_FP64, args.steps, args.warmup, out8, outf, torch)
+        progress(fp64_ms_per_step=fp64["ms_per_step"])
+    if "fp32" in legs:
+        fp32 = run_form(ctx, frame, A.PRR_FP32_BULK, args.steps, args.warmup, out8, outf, torch)
+        progress(fp32_bulk_ms_per_step=fp32["ms_per_step"])
+    if "adjoint" in legs:
+        adj = run_adjoint(api, A, CF, max(1, args.steps // 2))
+        progress(adjoint=adj)
+    if "c2" in legs:
+        batch = run_batch(api, A, CF, 2, torch)
+        progress(batch=batch)
+    if "batched" in legs:
+        bat = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch)
+        progress(c1_batched_ms_per_frame=bat["ms_per_frame"])
+    cpu = cpu_baseline(frame) if "cpu" in legs else None
+    roof = None
+    if fp32:
+        t_int = fp32["integrate_ms"] / 1e3
+        f32, f64 = fp32["flops_fp32"], fp32["flops_fp64"]
+        t_ideal = f32 / P32 + f64 / P64
+        t_ideal_np = f32 / P32_NONPACKED + f64 / P64
+        roof = {"bound": "valu", "kernel": "k_integrate<Kerr,RKF45,FP32-bulk>, one C1 frame per launch",
+                "achieved": (f32 + f64) / t_int / 1e12, "peak": (f32 + f64) / t_ideal / 1e12,
+                "unit": "TFLOP/s", "frac": t_ideal / t_int, "frac_nonpacked": t_ideal_np / t_int,
+                "traffic": None, "flops_fp32_per_launch": f32, "flops_fp64_per_launch": f64,
+                "launch_ms": fp32["integrate_ms"]}
+    head = bat or fp32 or fp64
     line = {
         "metric": METRIC,
-        "value": bat["rays_per_s"],
+        "value": head["rays_per_s"] if head else None,
         "unit": "rays/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": bat["ms_per_step"],
+        "ms_per_step": head["ms_per_step"] if head else None,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "fp32-bulk/fp64",
         "data": "synthetic (seeded checker sky 512x256, value-noise sphere texture 1024x512)",
-        "config": {"workload": f"C1 x{BATCH} per step: Kerr a=0.95 BL, RKF45 rtol=atol=1e-8, 1920x1080, "
-                               "2x2 SSAA + FXAA, camera r=20M i=80deg, sphere R=1M at r=8M; FP32-bulk, "
-                               "one persistent launch per batch from a hipGraph",
-                   "rays_per_frame": frame.nrays, "frames_per_step": BATCH},
-        "roofline": {"bound": "valu", "kernel": "k_integrate<Kerr,RKF45,FP32-bulk>",
-                     "achieved": (f32 + f64) / t_int / 1e12, "peak": (f32 + f64) / t_ideal / 1e12,
-                     "unit": "TFLOP/s", "frac": t_ideal / t_int, "frac_nonpacked": t_ideal_np / t_int,
-                     "traffic": None, "flops_fp32_per_launch": f32, "flops_fp64_per_launch": f64},
+        "config": {"workload": (f"C1 x{BATCH} per step: " if bat else "C1, one frame per step: ")
+                   + "Kerr a=0.95 BL, RKF45 rtol=atol=1e-8, 1920x1080, 2x2 SSAA + FXAA, camera r=20M i=80deg, "
+                   "sphere R=1M at r=8M; " + ("FP32-bulk, one persistent launch per batch from a hipGraph"
+                                              if bat else ("FP32-bulk" if fp32 else "plain FP64")),
+                   "rays_per_frame": frame.nrays, "frames_per_step": BATCH if bat else 1},
+        "roofline": roof,
         "cpu_baseline": cpu,
-        "fp64_plain": {"rays_per_s": fp64["rays_per_s"], "ms_per_step": fp64["ms_per_step"],
-                       "integrate_ms": fp64["integrate_ms"], "roofline_frac": fp64["roofline_frac"],
-                       "achieved_tflops": fp64["achieved_tflops"], "attempts": fp64["attempts"]},
-        "fp32_bulk_single": {k: fp32[k] for k in ("rays_per_s", "ms_per_step", "integrate_ms", "ssaa_ms", "fxaa_ms",
-                                                   "attempts", "accepted", "attempts_fp64", "attempts_per_s",
-                                                   "achieved_tflops", "roofline_frac", "ssaa_gbs", "fxaa_gbs")},
-        "fp32_bulk_batched": bat,
-        "speedup_batched_fp32_bulk_vs_plain_fp64": bat["rays_per_s"] / fp64["rays_per_s"],
-        "adjoint_rays_per_s": adj["rays_per_s"] if adj else None,
-        "adjoint": adj,
-        "batch_c2": batch,
     }
+    if fp64:
+        line["fp64_plain"] = {k: fp64[k] for k in ("rays_per_s", "ms_per_step", "integrate_ms", "attempts",
+                                                   "attempts_per_s", "achieved_tflops", "roofline_frac")}
+    if fp32:
+        line["fp32_bulk_single"] = {k: fp32[k] for k in ("rays_per_s", "ms_per_step", "integrate_ms", "init_ms",
+                                                         "shade_ms", "ssaa_ms", "fxaa_ms", "attempts", "accepted",
+                                                         "attempts_fp64", "attempts_per_s", "achieved_tflops",
+                                                         "roofline_frac", "ssaa_gbs", "fxaa_gbs")}
+    if bat:
+        line["fp32_bulk_batched"] = bat
+        if fp64:
+            line["speedup_batched_fp32_bulk_vs_plain_fp64"] = bat["rays_per_s"] / fp64["rays_per_s"]
+    if adj:
+        line["adjoint_rays_per_s"] = adj["rays_per_s"]
+        line["adjoint"] = adj
+    if batch:
+        line["batch_c2"] = batch
+    if copy:
+        line["copy_calibration"] = copy
     print(json.dumps(line))
 
 
