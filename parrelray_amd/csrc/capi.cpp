@@ -475,12 +475,15 @@ JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
 }
 
 // Enqueue reset -> init -> integrate -> shade -> SSAA -> FXAA, with timing events.
-prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events) {
+// adj: integrate with the adjoint-capable kernel binary (the adjoint's first
+// pass, whose step counts its checkpoint and replay passes must reproduce).
+prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events,
+                          bool adj = false) {
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     HIPTRY(ctx, launch_reset(job.ctr, st));
     HIPTRY(ctx, launch_init(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    HIPTRY(ctx, launch_integrate(ctx->metric, job, integrate_grid(ctx), ctx->launch.block, st));
+    HIPTRY(ctx, launch_integrate(ctx->metric, job, integrate_grid(ctx), ctx->launch.block, st, adj));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     HIPTRY(ctx, launch_shade(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[5], st));
@@ -794,7 +797,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     // pass 1: the render (step counts, classes, ray colours, SSAA image)
-    if ((s = enqueue_render(ctx, P, job, st, false)) != PRR_OK) return s;
+    if ((s = enqueue_render(ctx, P, job, st, false, true)) != PRR_OK) return s;
     std::vector<int32_t> steps(T);
     HIPTRY(ctx, hipMemcpyAsync(steps.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
     if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises

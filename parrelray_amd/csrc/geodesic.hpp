@@ -123,6 +123,7 @@ template <class T> struct MassCoef {
 // ----------------------------------------------------------------- metrics
 // metrics/flat.cuh: straight lines, getGeod = 0 (:88-90).
 struct Flat {
+    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
     static constexpr int N = 8;
     static constexpr int kind = PRR_FLAT;
     static constexpr bool spherical = false;
@@ -143,6 +144,7 @@ struct Flat {
 //   du^i = x^i r_s/(2 r^3) [3q^2 - 2|u|^2 - (u^t)^2 + w (q + u^t)^2]
 //   du^t = r_s/(2 r^2) [2|u|^2 - 4q^2 - 2 q u^t - w (q + u^t)^2].
 struct SchwKS {
+    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
     static constexpr int N = 8;
     static constexpr int kind = PRR_SCHW_KS_CART;
     static constexpr bool spherical = false;
@@ -175,6 +177,7 @@ struct SchwKS {
 // "metrics/schwarzschild - sph.cuh":94-115, including its tan/sin clamp
 // (|tan th| < 1e-4 -> tan = sin = 1e-4, :98-101).
 struct SchwSph {
+    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
     static constexpr int N = 8;
     static constexpr int kind = PRR_SCHW_SPH;
     static constexpr bool spherical = true;
@@ -233,6 +236,7 @@ struct KerrDelta {
 };
 
 struct Kerr {
+    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
     static constexpr int N = 6;
     static constexpr int NERR = 5; // lambda is bookkeeping: outside the error norm
     static constexpr int kind = PRR_KERR_BL;
@@ -788,5 +792,12 @@ PRR_HD int chord_hit(const FrameDev &fr, D3 P0, D3 P1, double &t_hit) {
     }
     return best;
 }
+
+// A metric whose integrator has no adjoint paths (checkpoints, replay
+// records, event capture): the render instantiation, with their registers
+// and branches compiled out.
+template <class M> struct NoAdj : M {
+    static constexpr bool adj = false;
+};
 
 } // namespace prr

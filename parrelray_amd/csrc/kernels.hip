@@ -54,7 +54,7 @@ template <class Mt, int SOLVER, bool MIXED>
 __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
     const int lane = int(threadIdx.x & 63);
     LocateCtx &C = job.loc[int64_t(blockIdx.x) * blockDim.x + threadIdx.x];
-    const bool replay = job.adj.mode == ADJ_REPLAY;
+    const bool replay = Mt::adj && job.adj.mode == ADJ_REPLAY;
     Lane<Mt> L;
     int mode = MODE_IDLE;
     int32_t end = 0; // replay: stop at this step count
@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                         end = min((job.adj.seg + 1) * job.adj.K, job.adj.nsteps[rid]);
                     } else {
                         lane_load<Mt>(job, L, idx);
-                        if (job.adj.mode == ADJ_CHECKPOINT)
+                        if (Mt::adj && job.adj.mode == ADJ_CHECKPOINT)
                             store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, 0);
                     }
                     mode = MODE_ACTIVE;
@@ -476,19 +476,23 @@ hipError_t launch_shade(int metric, const JobDev &job, hipStream_t st) {
     });
 }
 
-hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st) {
-    const dim3 g(grid), b(block);
+template <class Mt> void launch_integrate_t(const JobDev &job, dim3 g, dim3 b, hipStream_t st) {
     const bool mixed = job.tr.precision == PRR_FP32_BULK;
-    const bool euler = job.tr.solver == PRR_EULER;
+    if (job.tr.solver == PRR_EULER) {
+        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true>), g, b, 0, st, job);
+        else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false>), g, b, 0, st, job);
+    } else {
+        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true>), g, b, 0, st, job);
+        else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false>), g, b, 0, st, job);
+    }
+}
+
+hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st, bool adj) {
+    const dim3 g(grid), b(block);
     return with_metric(metric, [&](auto mt) {
         using Mt = decltype(mt);
-        if (euler) {
-            if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true>), g, b, 0, st, job);
-            else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false>), g, b, 0, st, job);
-        } else {
-            if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true>), g, b, 0, st, job);
-            else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false>), g, b, 0, st, job);
-        }
+        if (adj || job.adj.mode != ADJ_NONE) launch_integrate_t<Mt>(job, g, b, st);
+        else launch_integrate_t<NoAdj<Mt>>(job, g, b, st);
         return hipGetLastError();
     });
 }

@@ -125,6 +125,25 @@ PRR_HD double rkf_step_size(const TracerDev &tr, const double *y, const Consts<d
     return h;
 }
 
+// The RKF45 step factor 0.9 errn^(-1/5) (kernels/solver.cuh:175): an FP32
+// estimate from the hardware log2 / exp2 on the device, refined by one Newton
+// step in FP64 for y^-5 = x, y <- y (6 - x y^5) / 5, which squares the
+// estimate's relative error (~1e-7 -> ~3e-14) at a tenth of the cost of pow.
+// errn is clamped to [1e-30, 1e30] (errn = 0 gives the factor 0.9e6; the
+// step bounds cap it).
+PRR_HD double step_factor(double errn) {
+#pragma clang fp contract(off)
+    const double x = fmin(fmax(errn, 1e-30), 1e30);
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double y = double(__builtin_amdgcn_exp2f(-0.2f * __builtin_amdgcn_logf(float(x))));
+#else
+    const double y = double(exp2f(-0.2f * log2f(float(x))));
+#endif
+    const double y2 = y * y;
+    const double y5 = y2 * y2 * y;
+    return 0.9 * (y * (1.2 - 0.2 * (x * y5)));
+}
+
 // Six RK stages in precision T from the FP64 state y (plain or FP32 form of
 // the 8-component metrics, and Kerr's FP64 form); the 4th-order increment d4
 // and the error vector e in FP64.
@@ -503,7 +522,7 @@ template <int N> PRR_HD void copy_n(double *d, const double *s) {
 
 // Adjoint pass 2: checkpoint every K accepted steps while the lane integrates.
 template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<Mt> &L) {
-    if (job.adj.mode != ADJ_CHECKPOINT) return;
+    if (!Mt::adj || job.adj.mode != ADJ_CHECKPOINT) return;
     const int K = job.adj.K;
     if (L.steps % K == 0 && L.steps < job.adj.nsteps[L.rid])
         store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, L.steps / K);
@@ -511,7 +530,7 @@ template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<M
 
 // Adjoint replay: the step's start and size of every accepted step of the segment.
 template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &L, double h) {
-    if (job.adj.mode != ADJ_REPLAY) return;
+    if (!Mt::adj || job.adj.mode != ADJ_REPLAY) return;
     const int64_t slot = (L.steps - 1) - int64_t(job.adj.seg) * job.adj.K; // L.steps already counts this step
     const int64_t T = job.rs_total;
     double *r = job.adj.rec + (slot * 6) * T + rs_index(job, L.rid);
@@ -524,7 +543,7 @@ template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &
 template <class Mt>
 PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, const LocateCtx &C, const double *ys, double theta, int cls,
                           int hit) {
-    if (job.adj.mode == ADJ_REPLAY && job.adj.event) {
+    if (Mt::adj && job.adj.mode == ADJ_REPLAY && job.adj.event) {
         EventDev &ev = job.adj.event[L.rid];
         for (int i = 0; i < 8; ++i) ev.y_prev[i] = i < Mt::N ? L.y[i] : 0.0;
         ev.h = C.h;
@@ -841,9 +860,8 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
     // Kerr: a step may not reach the polar axis (|cos th| >= 1), where the chart is singular
     if (Mt::mino && !(fabs(yn[2]) < 1.0)) errn = fmax(errn, 1024.0);
     // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
-    double hn;
-    if (MIXED) hn = 0.9 * hs * double(powf(float(errn), -0.2f));
-    else hn = 0.9 * hs * pow(errn, -0.2);
+    // a NaN errn (non-finite stages) is no rejection: the state check below ends the ray
+    double hn = hs * step_factor(errn);
     if (!Mt::mino) hn = fmin(fmax(hn, tr.hmin), tr.hmax);
     if (errn > 1.0 && !at_min) { // reject (:185-188)
         L.h = hn;
