@@ -14,7 +14,8 @@
 //     (metrics/schwarzschild.cuh:90-119); the GPU uses a factored form.
 //   - Kerr uses the second-order geodesic equation with Christoffel symbols
 //     built from analytic metric derivatives on the 8-component (x, u)
-//     state; the GPU integrates Hamilton's equations with conserved E, L.
+//     state in affine parameter; the GPU integrates the separated equations
+//     in Mino time (u = 1/r, mu = cos theta) with E, L, Carter Q fixed.
 //
 // Pinning (what checks each part; the tests live in tests/):
 //   metric / geod      finite-difference Christoffels of the metric
@@ -27,10 +28,16 @@
 //   camera / init      closed forms (centre ray = forward, FOV corners),
 //                      null condition g(u,u) = 0.
 //   sphere             brute-force ray/sphere intersection in flat space.
-//   sky / bilinear     hand-computed 2x2 cases, constant and linear textures.
-//   SSAA / FXAA / RGB8 a separate numpy restatement
-//                      (tests/golden/make_aa_golden.py) on seeded images,
-//                      constant-image identities, exhaustive RGB8 ties.
+//   sky / bilinear     one-texel textures (block RMS, orientation, last
+//                      row / column), linear textures reproduced exactly
+//                      (test_oracle_images).
+//   SSAA / FXAA / RGB8 hand-computed RMS values and sample order, FXAA
+//                      fixed points (constant images, axis-aligned edges)
+//                      and its equivariance under grid flips / transpose,
+//                      RGB8 ties of the FP32 product (test_oracle_images).
+//   escape / sphere    closed-form line intersections in flat space and an
+//   location (D21/22)  independent scipy event search on the Schwarzschild
+//                      right-hand side (test_oracle_events).
 //
 // Defect resolutions (reference file:line -> what is done here):
 //   D2  fx undefined (kernels/solver.cuh:163-168): x' = u.

@@ -9,7 +9,10 @@ excused by the oracle's own perturbations), and the endpoint-angle error
 full-frame comparison of each form against the GPU's own FP64 tol-1e-12
 render.
 
-    python tests/precision_report.py [--out gpurun_out/precision.json] [--rays 4000]
+    python tests/precision_report.py [--out gpurun_out/precision.json] [--rays 4000] [--configs c1,c4,c4full,c2]
+
+c4 is a 512x512 2x2-SSAA crop of the precision-stress frame; c4full is the
+whole 2048x2048 4x4-SSAA frame (67,108,864 rays).
 """
 import argparse
 import json
@@ -89,6 +92,8 @@ def main():
         report["C1"] = run("C1", CF.frame_c1(), 1e-8, args.rays, 101, log)
     if "c4" in cfgs:
         report["C4"] = run("C4", CF.frame_c4(512, 512, ssaa=2), 1e-10, args.rays, 104, log)
+    if "c4full" in cfgs:
+        report["C4_full"] = run("C4 full", CF.frame_c4(), 1e-10, args.rays, 104, log)
     if "c2" in cfgs:
         report["C2_frame10"] = run("C2 frame 10", CF.frames_c2()[10], 1e-8, args.rays, 102, log)
     os.makedirs(os.path.dirname(args.out) or ".", exist_ok=True)
