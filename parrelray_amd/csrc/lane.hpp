@@ -98,7 +98,7 @@ template <class Mt> PRR_HD bool finite_state(const double *y) {
 template <class Mt> PRR_HD double lambda_rate(const double *y, const Consts<double> &cs) {
 #pragma clang fp contract(off)
     if (!Mt::mino) return 1.0;
-    return 1.0 / (y[0] * y[0]) + cs.a * cs.a * y[2] * y[2];
+    return rcp_t(y[0] * y[0]) + cs.a * cs.a * y[2] * y[2];
 }
 
 // Step size of an RKF45 attempt from the proposal h: the affine bounds
@@ -111,15 +111,16 @@ PRR_HD double rkf_step_size(const TracerDev &tr, const double *y, const Consts<d
         at_min = h == tr.hmin;
         return h;
     }
-    const double iS = 1.0 / lambda_rate<Mt>(y, cs);
+    const double iS = rcp_t(lambda_rate<Mt>(y, cs));
     const double lo = tr.hmin * iS;
     // near the polar axis a step may turn the ray by at most kAxisStep times
     // its angular distance from the axis (sin th): the L / sin^2 th spike of
     // phi' is then resolved by several steps, which the embedded error
-    // estimate alone does not guarantee (angular speed sqrt(Q + L^2 + a^2 E^2 mu^2))
+    // estimate alone does not guarantee (angular speed sqrt(Q + L^2 + a^2 E^2 mu^2));
+    // a bound, so FP32 is enough
     const double s2 = 1.0 - y[2] * y[2];
     const double K = cs.Q + cs.L * cs.L + cs.a * cs.a * cs.E * cs.E * y[2] * y[2];
-    const double hax = kAxisStep * sqrt(fmax(s2, 0.0) / K);
+    const double hax = double(float(kAxisStep) * sqrtf(float(fmax(s2, 0.0)) * rcp_t(float(K))));
     h = fmin(fmax(h, lo), fmin(tr.hmax * iS, fmax(hax, lo)));
     at_min = h <= lo;
     return h;
@@ -279,7 +280,7 @@ PRR_HD bool sphere_screen(const FrameDev &fr, int k, const float P0[3], const fl
     const float mx = P0[0] - cx, my = P0[1] - cy, mz = P0[2] - cz;
     const float dx = P1[0] - P0[0], dy = P1[1] - P0[1], dz = P1[2] - P0[2];
     const float A = dx * dx + dy * dy + dz * dz;
-    float t = A > 0.0f ? -(mx * dx + my * dy + mz * dz) / A : 0.0f;
+    float t = A > 0.0f ? -(mx * dx + my * dy + mz * dz) * rcp_t(A) : 0.0f;
     t = fminf(fmaxf(t, 0.0f), 1.0f);
     const float qx = mx + t * dx, qy = my + t * dy, qz = mz + t * dz;
     const float dist = sqrtf(qx * qx + qy * qy + qz * qz);
@@ -846,17 +847,25 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         if (r >= 0) mode = MODE_IDLE;
         return r;
     }
-    double errn = 0.0;
     // Kerr: mu = cos th and mu' are measured in units of th (divided by
     // sin th), so that the tolerance keeps its meaning near the poles, where a
-    // fixed error in mu is a large relative error in 1 - mu^2
-    const double ist = Mt::mino ? 1.0 / fmax(sqrt(fmax(1.0 - L.y[2] * L.y[2], 0.0)), 1e-12) : 1.0;
+    // fixed error in mu is a large relative error in 1 - mu^2 (a scale: FP32)
+    const double ist =
+        Mt::mino ? double(rsqrt_f32(fmaxf(float(fmax(1.0 - L.y[2] * L.y[2], 0.0)), 1e-24f))) : 1.0;
+    // errn = max_i ei / sc_i, kept as the fraction num / den (one reciprocal
+    // instead of one division per component); NaN components are ignored as
+    // by fmax
+    double num = 0.0, den = 1.0;
 #pragma unroll
     for (int i = 0; i < Mt::NERR; ++i) {
         const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(yn[i]));
         const double ei = (Mt::mino && (i == 2 || i == 3)) ? fabs(e[i]) * ist : fabs(e[i]);
-        errn = fmax(errn, ei / sc);
+        if (ei * den > num * sc) {
+            num = ei;
+            den = sc;
+        }
     }
+    double errn = num * rcp_t(den);
     // Kerr: a step may not reach the polar axis (|cos th| >= 1), where the chart is singular
     if (Mt::mino && !(fabs(yn[2]) < 1.0)) errn = fmax(errn, 1024.0);
     // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
