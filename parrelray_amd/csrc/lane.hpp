@@ -120,7 +120,7 @@ PRR_HD double rkf_step_size(const TracerDev &tr, const double *y, const Consts<d
     // a bound, so FP32 is enough
     const double s2 = 1.0 - y[2] * y[2];
     const double K = cs.Q + cs.L * cs.L + cs.a * cs.a * cs.E * cs.E * y[2] * y[2];
-    const double hax = double(float(kAxisStep) * sqrtf(float(fmax(s2, 0.0)) * rcp_t(float(K))));
+    const double hax = double(float(kAxisStep) * sqrt_f32(float(fmax(s2, 0.0)) * rcp_t(float(K))));
     h = fmin(fmax(h, lo), fmin(tr.hmax * iS, fmax(hax, lo)));
     at_min = h <= lo;
     return h;
@@ -283,9 +283,9 @@ PRR_HD bool sphere_screen(const FrameDev &fr, int k, const float P0[3], const fl
     float t = A > 0.0f ? -(mx * dx + my * dy + mz * dz) * rcp_t(A) : 0.0f;
     t = fminf(fmaxf(t, 0.0f), 1.0f);
     const float qx = mx + t * dx, qy = my + t * dy, qz = mz + t * dz;
-    const float dist = sqrtf(qx * qx + qy * qy + qz * qz);
+    const float dist = sqrt_f32(qx * qx + qy * qy + qz * qz);
     const float scale = fabsf(cx) + fabsf(cy) + fabsf(cz) + R + fabsf(P0[0]) + fabsf(P0[1]) + fabsf(P0[2]);
-    return dist <= R + 0.5f * sqrtf(A) + 1e-4f * scale;
+    return dist <= R + 0.5f * sqrt_f32(A) + 1e-4f * scale;
 }
 
 // ------------------------------------------------------------ ray state I/O
