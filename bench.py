@@ -96,8 +96,11 @@ def run_adjoint(api, A, CF, reps):
 
 def run_c1_batched(api, A, CF, ctx, steps, warmup, torch):
     """BATCH C1 frames in one persistent launch from a hipGraph (FP32-bulk)."""
+    from parrelray_amd import autotune
     frames = [CF.frame_c1() for _ in range(BATCH)]
-    ctx.set_tracer(CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    tracer = CF.tracer_kerr(precision=A.PRR_FP32_BULK)
+    ctx.set_tracer(tracer)
+    shape = autotune.tune_launch(ctx, frames[:1], A.PRR_KERR_BL, tracer, use_cache=False)
     rgb8 = [torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
     g = api.Graph(ctx, frames, rgb8=rgb8)
     for _ in range(warmup):
@@ -110,7 +113,8 @@ def run_c1_batched(api, A, CF, ctx, steps, warmup, torch):
     sec = (time.perf_counter() - t0) / steps
     st = ctx.stats()
     g.close()
-    return dict(frames=BATCH, rays=st["rays"], ms_per_step=sec * 1e3, rays_per_s=st["rays"] / sec,
+    ctx.set_launch()  # back to the default shape for the other legs
+    return dict(frames=BATCH, rays=st["rays"], ms_per_step=sec * 1e3, rays_per_s=st["rays"] / sec, launch=shape,
                 ms_per_frame=sec * 1e3 / BATCH, attempts=st["attempts"], attempts_fp64=st["attempts_fp64"],
                 graph_ms=st["total_ms"])
 

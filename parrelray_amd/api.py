@@ -136,7 +136,7 @@ class Context:
             arr[i].tex = tex
         self._check(lib().prr_set_spheres(self.h, arr, len(spheres)))
 
-    def set_launch(self, block=256, blocks_per_cu=0, refill=8, checkpoint=64):
+    def set_launch(self, block=256, blocks_per_cu=0, refill=64, checkpoint=64):
         l = A.Launch(block, blocks_per_cu, refill, checkpoint)
         self._check(lib().prr_set_launch(self.h, C.byref(l)))
 

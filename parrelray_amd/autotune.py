@@ -20,7 +20,7 @@ CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "autotun
 
 BLOCKS = (64, 128, 256)
 BLOCKS_PER_CU = (1, 2, 3, 4, 6, 8)
-REFILLS = (1, 4, 8, 16, 32)
+REFILLS = (8, 16, 32, 48, 56, 64)
 CHECKPOINTS = (16, 32, 64, 128)
 
 
@@ -62,14 +62,15 @@ def tune_launch(ctx, frames, metric, tracer, device_name="gpu0", budget_s=60.0, 
         return best
     start = time.perf_counter()
     results = []
-    # coarse: occupancy-like dimension first at the default refill, then refill at the best shape
+    # coarse: occupancy-like dimension first at the default refill (whole-wave
+    # refills, which keep neighbouring rays together), then refill at the best shape
     for block in BLOCKS:
         for bpc in BLOCKS_PER_CU:
             waves = block // 64 * bpc
             if waves > 32:
                 continue
-            ctx.set_launch(block, bpc, 8, 64)
-            results.append((_time_integrate(ctx, frames), block, bpc, 8))
+            ctx.set_launch(block, bpc, 64, 64)
+            results.append((_time_integrate(ctx, frames), block, bpc, 64))
             if time.perf_counter() - start > budget_s / 2:
                 break
     results.sort()

@@ -45,7 +45,7 @@ struct prr_ctx {
     int32_t metric = PRR_SCHW_KS_CART;
     double M = 1.0, a = 0.0, q = 0.0, p[16] = {};
     prr_tracer tracer{};
-    prr_launch launch{256, 0, 8, 64};
+    prr_launch launch{256, 0, 64, 64}; // refill 64: a wave takes 64 consecutive rays once all its lanes are idle
     TexDev tex[PRR_MAX_TEXTURES] = {};
     std::vector<prr_sphere> spheres;
     // device buffers, grown on demand
