@@ -286,6 +286,16 @@ prr_status prr_params_tracer(const prr_params *p, prr_tracer *out);
  * main/Spacetime.cpp:38-45) for frame i; also the texture path of each sphere. */
 prr_status prr_params_frame(const prr_params *p, int i, prr_frame *out);
 
+/* ---- texture files ------------------------------------------------------ */
+
+/* stbi_load(path, &w, &h, &n, 3) (cudamaths/image.cu:13-21): a binary PPM
+ * (P6), PNG (8/16-bit, any colour type, non-interlaced; alpha dropped) or
+ * baseline / extended-sequential Huffman JPEG (grey or YCbCr, any sampling)
+ * decoded to RGB8, row-major.  Call with rgb8 = NULL to get *w, *h; then with
+ * a w*h*3 buffer and the same *w, *h.  Progressive JPEG is refused
+ * (PRR_EINVAL); a missing file gives PRR_EIO.  Messages via prr_global_error. */
+prr_status prr_image_read(const char *path, uint8_t *rgb8, int *w, int *h);
+
 /* ---- procedural inputs -------------------------------------------------- */
 
 /* Seeded RGB8 textures used in place of the reference's image files:

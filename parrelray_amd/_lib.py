@@ -83,6 +83,7 @@ def lib():
     _sig(L, "prr_params_frame", st, [_vp, C.c_int, C.POINTER(A.Frame)])
     _sig(L, "prr_texture_checker", st, [_vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64])
     _sig(L, "prr_texture_value_noise", st, [_vp, C.c_int, C.c_int, C.c_int, C.c_uint64])
+    _sig(L, "prr_image_read", st, [C.c_char_p, _vp, C.POINTER(C.c_int), C.POINTER(C.c_int)])
     _sig(L, "prr_debug_state_size", C.c_int, [_i32])
     _sig(L, "prr_debug_rhs", st, [_i32, _d, _d, _d, _d, _d, _vp, _i32, _vp])
     _sig(L, "prr_debug_kerr_delta_rhs", st, [_d, _d, _d, _d, _d, _vp, _vp, _vp])

@@ -8,6 +8,7 @@ kernels; numpy arrays carry host inputs and outputs, torch tensors (when
 given) stay on the GPU.
 """
 import ctypes as C
+import os
 
 import numpy as np
 
@@ -29,6 +30,15 @@ def checker_texture(w=512, h=256, cw=16, ch=8, seed=1):
 def noise_texture(w=1024, h=512, octaves=4, seed=2):
     out = np.empty((h, w, 3), np.uint8)
     check(lib().prr_texture_value_noise(_p(out), w, h, octaves, seed))
+    return out
+
+
+def read_image(path):
+    """RGB8 array (H, W, 3) of a binary PPM, PNG or baseline JPEG file (prr_image_read)."""
+    w, h = C.c_int(0), C.c_int(0)
+    check(lib().prr_image_read(os.fsencode(path), None, C.byref(w), C.byref(h)))
+    out = np.empty((h.value, w.value, 3), np.uint8)
+    check(lib().prr_image_read(os.fsencode(path), _p(out), C.byref(w), C.byref(h)))
     return out
 
 

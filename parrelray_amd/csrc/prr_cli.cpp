@@ -3,7 +3,7 @@
 //
 // Reads sim / traj / scene .params, renders every complete frameN block and
 // writes <out>image_<i>.png (prr.cpp:336).  Texture values (bimage1, sphereN
-// tex) are a binary PPM path or a procedural spec:
+// tex) are a PPM / PNG / JPEG path or a procedural spec:
 //   checker:W:H:CW:CH:SEED   (checkerboard, CW x CH cells, hue jitter)
 //   noise:W:H:OCTAVES:SEED   (value noise)
 #include <cstdio>
@@ -52,7 +52,12 @@ bool load_texture(const std::string &spec, std::vector<uint8_t> &rgb, int &w, in
         return prr_texture_value_noise(rgb.data(), w, h, std::atoi(t[3].c_str()),
                                        std::strtoull(t[4].c_str(), nullptr, 10)) == PRR_OK;
     }
-    return prr::read_ppm(spec, rgb, w, h);
+    std::string err;
+    if (!prr::read_image(spec, rgb, w, h, err)) {
+        std::fprintf(stderr, "ERROR: %s\n", err.c_str());
+        return false;
+    }
+    return true;
 }
 
 int die(prr_ctx *ctx, const char *what) {

@@ -26,5 +26,7 @@ void set_global_error(const std::string &msg);
 bool write_png(const std::string &path, const uint8_t *rgb, int w, int h);
 // Binary PPM (P6, maxval 255) reader.
 bool read_ppm(const std::string &path, std::vector<uint8_t> &rgb, int &w, int &h);
+// PPM (P6), PNG or baseline JPEG file -> RGB8 (image_io.cpp); err says why not.
+bool read_image(const std::string &path, std::vector<uint8_t> &rgb, int &w, int &h, std::string &err);
 
 } // namespace prr
