@@ -35,9 +35,12 @@ def run_prr(cwd, sim):
     return r.stdout
 
 
-def library_render(sim_path, i):
-    """The same frame through the C ABI, set up from the same .params files."""
-    p = api.Params(sim=sim_path)
+def library_render(tmp_path, monkeypatch, i):
+    """The same frame through the C ABI, set up from the same .params files
+    (relative paths in them resolve against the working directory, as in the
+    reference's main)."""
+    monkeypatch.chdir(tmp_path)
+    p = api.Params(sim="c0.params")
     st, tr = p.spacetime(), p.tracer()
     ctx = api.Context(0)
     ctx.set_spacetime(st["metric"], st["M"], st["a"], st["q"])
@@ -47,43 +50,56 @@ def library_render(sim_path, i):
     return ctx.render(f, rays=False)[0]["rgb8"]
 
 
-def test_c0_params_render_to_png_like_the_library(tmp_path):
-    for name in ("c0.params", "c0.traj"):
-        shutil.copy(os.path.join(CONFIGS, name), tmp_path / name)
+def prepare(tmp_path, traj=None, sky=None):
+    """configs/c0.params at 128 x 128 with a 40,000-step cap (the C0 cap of
+    10,000 Euler steps of 0.01 M stops rays aimed at the hole before they get
+    back out to r = 100 M, which leaves that image black)."""
+    text = open(os.path.join(CONFIGS, "c0.params")).read()
+    for old, new in (("<max-steps> = <10000>", "<max-steps> = <40000>"), ("<width> = <256>", "<width> = <128>"),
+                     ("<height> = <256>", "<height> = <128>")):
+        assert old in text
+        text = text.replace(old, new)
+    if sky:
+        text = text.replace("checker:512:256:16:8:1592590337", sky)
+    (tmp_path / "c0.params").write_text(text)
+    if traj:
+        (tmp_path / "c0.traj").write_text(traj)
+    else:
+        shutil.copy(os.path.join(CONFIGS, "c0.traj"), tmp_path / "c0.traj")
+
+
+def test_c0_params_render_to_png_like_the_library(tmp_path, monkeypatch):
+    prepare(tmp_path)
     out = run_prr(tmp_path, "c0.params")
     assert "frame 1/1" in out
     img = read_png(tmp_path / "c0-image_0.png")
-    assert img.shape == (256, 256, 3)
-    ref = library_render(str(tmp_path / "c0.params"), 0)
-    assert np.array_equal(img, ref)
+    assert img.shape == (128, 128, 3)
+    assert np.array_equal(img, library_render(tmp_path, monkeypatch, 0))
     # a black hole in the middle, sky around it
-    assert img[128, 128].max() == 0 and img[5, 5].max() > 0
+    assert img[64, 64].max() == 0 and img[3, 3].max() > 0
 
 
-def test_trajectory_frames_are_rendered_in_order(tmp_path):
-    shutil.copy(os.path.join(CONFIGS, "c0.params"), tmp_path / "c0.params")
+def test_trajectory_frames_are_rendered_in_order(tmp_path, monkeypatch):
     # three frames moving the camera outwards along the same line of sight
     blocks = []
     for i, r in enumerate((20.0, 30.0, 45.0)):
         blocks.append(f"_frame{i}_\n<x> = <{r * 0.9961946980917455}>\n<y> = <0.0>\n"
                       f"<z> = <{r * 0.08715574274765817}>\n<a> = <180>\n<b> = <5>\n")
-    (tmp_path / "c0.traj").write_text("three frames\n\n" + "\n".join(blocks))
+    prepare(tmp_path, traj="three frames\n\n" + "\n".join(blocks))
     out = run_prr(tmp_path, "c0.params")
     assert "frame 3/3" in out
     imgs = [read_png(tmp_path / f"c0-image_{i}.png") for i in range(3)]
     # the shadow shrinks as the camera recedes
     dark = [int((im.max(axis=2) == 0).sum()) for im in imgs]
-    assert dark[0] > dark[1] > dark[2] > 0
-    assert np.array_equal(imgs[1], library_render(str(tmp_path / "c0.params"), 1))
+    assert dark[0] > dark[1] > dark[2] > 0, dark
+    assert np.array_equal(imgs[1], library_render(tmp_path, monkeypatch, 1))
 
 
-def test_sky_texture_from_a_png_file_renders_like_the_procedural_one(tmp_path):
+def test_sky_texture_from_a_png_file_renders_like_the_procedural_one(tmp_path, monkeypatch):
     from PIL import Image
     Image.fromarray(api.checker_texture(512, 256, 16, 8, 1592590337)).save(tmp_path / "sky.png")
-    text = open(os.path.join(CONFIGS, "c0.params")).read()
-    assert "<bimage1> = <checker:512:256:16:8:1592590337>" in text
-    (tmp_path / "c0.params").write_text(text.replace("checker:512:256:16:8:1592590337", "sky.png"))
-    shutil.copy(os.path.join(CONFIGS, "c0.traj"), tmp_path / "c0.traj")
+    prepare(tmp_path, sky="sky.png")
     run_prr(tmp_path, "c0.params")
     img = read_png(tmp_path / "c0-image_0.png")
-    assert np.array_equal(img, library_render(str(tmp_path / "c0.params"), 0))
+    assert np.array_equal(img, library_render(tmp_path, monkeypatch, 0))
+    assert img[3, 3].max() > 0
