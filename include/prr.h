@@ -65,7 +65,16 @@ typedef enum prr_precision {
 } prr_precision;
 
 /* Sample placement; values follow SSAA_DISTR (main/base.h:46-53). */
-typedef enum prr_ssaa { PRR_SSAA_NONE = 0, PRR_SSAA_REGULAR = 1 } prr_ssaa;
+/* SSAA sample distributions; values follow the reference's SSAA_DISTR
+ * (main/base.h:46-53).  RANDOM and JITTERED draw their offsets from a
+ * counter-based stream keyed by prr_frame.seed and the ray index (the
+ * reference's curand states, kernels/ssaa.cuh:20-86). */
+typedef enum prr_ssaa {
+    PRR_SSAA_NONE = 0,
+    PRR_SSAA_REGULAR = 1, /* n x n grid at (p + 1) / (n + 1) of the pixel */
+    PRR_SSAA_RANDOM = 2,  /* samples uniform over the pixel */
+    PRR_SSAA_JITTERED = 4 /* one uniform sample in each cell of an n x n grid */
+} prr_ssaa;
 
 typedef enum prr_projection {
     PRR_PROJ_PINHOLE = 0,  /* Camera::rayDir_PinHole (main/Camera.h:30-33) */
@@ -120,13 +129,13 @@ typedef struct prr_frame {
     double vfov;        /* vertical field of view (radians) */
     int32_t width, height;
     int32_t ssaa;       /* prr_ssaa */
-    int32_t samples;    /* samples per pixel: 1 for NONE, n*n for REGULAR */
+    int32_t samples;    /* samples per pixel: 1 for NONE, n*n for REGULAR and JITTERED, any for RANDOM */
     int32_t projection; /* prr_projection */
     int32_t fxaa;       /* 1: FXAA after the SSAA combine */
     double mass;        /* M */
     double spin;        /* Kerr a (ignored by the other metrics) */
     int32_t n_spheres;
-    int32_t pad;
+    uint32_t seed;      /* RANDOM / JITTERED SSAA: key of the sample stream */
     prr_sphere spheres[PRR_MAX_SPHERES];
 } prr_frame;
 

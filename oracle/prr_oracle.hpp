@@ -60,6 +60,9 @@
 //       entry, which Illinois regula falsi refines), not on the step chord;
 //       the hit normal is (P - c) / |P - c|.  Euler keeps the reference's
 //       chord and first-step semantics.
+//   D23 createRays_random adds the uniform offset unscaled, ij/res + duv
+//       (kernels/ssaa.cuh:63), which scatters samples over the whole image:
+//       the offset is duv / res.
 //   D20 vel_cartTOspher's phi rate has the wrong sign
 //       (cudamaths/transform.cuh:29) and the spherical sky lookup mirrors
 //       phi back (main/base.h:84); both are corrected, which leaves images of
@@ -303,13 +306,40 @@ inline Basis camera_basis(double a, double b) {
     return B;
 }
 
+// Counter-based uniform stream of the random / jittered distributions:
+// SplitMix64's output function (Steele, Lea and Flood 2014) applied to the
+// frame's seed and then to 2 * ray + dim; the top 53 bits as a double in [0, 1).
+inline uint64_t splitmix64_out(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+inline double uniform01(uint32_t seed, int64_t ray, int dim) {
+    return double(splitmix64_out(splitmix64_out(seed) + 2ull * uint64_t(ray) + uint64_t(dim)) >> 11) /
+           9007199254740992.0;
+}
+
 // createRays_regular / createRays_none (kernels/ssaa.cuh:36-48, 92-99):
-// sample k = q*n + p of pixel (i, j).
+// sample k = q*n + p of pixel (i, j); createRays_random (:55-66) with its
+// offset scaled to the pixel (D23) and createRays_jittered (:73-86).
 inline void sample_uv(const prr_frame &f, int64_t ray, double &u, double &v) {
     const int s = f.samples;
     const int64_t pix = ray / s;
     const int k = int(ray % s);
     const int i = int(pix % f.width), j = int(pix / f.width);
+    if (f.ssaa == PRR_SSAA_RANDOM) {
+        u = (i + uniform01(f.seed, ray, 0)) / f.width;
+        v = (j + uniform01(f.seed, ray, 1)) / f.height;
+        return;
+    }
+    if (f.ssaa == PRR_SSAA_JITTERED) {
+        const int n = int(std::lround(std::sqrt(double(s))));
+        const int p = k % n, q = k / n;
+        u = (i + (p + uniform01(f.seed, ray, 0)) / n) / f.width;
+        v = (j + (q + uniform01(f.seed, ray, 1)) / n) / f.height;
+        return;
+    }
     if (f.ssaa == PRR_SSAA_NONE || s == 1) {
         u = (i + 0.5) / f.width;
         v = (j + 0.5) / f.height;

@@ -23,7 +23,7 @@ PRR_EULER, PRR_RKF45 = 0, 2
 # prr_precision
 PRR_FP64, PRR_FP32_BULK = 0, 1
 # prr_ssaa
-PRR_SSAA_NONE, PRR_SSAA_REGULAR = 0, 1
+PRR_SSAA_NONE, PRR_SSAA_REGULAR, PRR_SSAA_RANDOM, PRR_SSAA_JITTERED = 0, 1, 2, 4
 # prr_projection
 PRR_PROJ_PINHOLE, PRR_PROJ_SPHERICAL = 0, 1
 # prr_filter
@@ -53,7 +53,7 @@ class Frame(C.Structure):
                 ("vfov", C.c_double), ("width", C.c_int32), ("height", C.c_int32),
                 ("ssaa", C.c_int32), ("samples", C.c_int32), ("projection", C.c_int32),
                 ("fxaa", C.c_int32), ("mass", C.c_double), ("spin", C.c_double),
-                ("n_spheres", C.c_int32), ("pad", C.c_int32),
+                ("n_spheres", C.c_int32), ("seed", C.c_uint32),
                 ("spheres", Sphere * PRR_MAX_SPHERES)]
 
     @property
@@ -107,8 +107,9 @@ def make_tracer(**kw):
 
 
 def make_frame(pos, alpha, beta, vfov, width, height, ssaa=PRR_SSAA_NONE, samples=1,
-               projection=PRR_PROJ_PINHOLE, fxaa=0, mass=1.0, spin=0.0, spheres=()):
+               projection=PRR_PROJ_PINHOLE, fxaa=0, mass=1.0, spin=0.0, spheres=(), seed=0):
     f = Frame()
+    f.seed = seed
     f.pos[:] = list(pos)
     f.alpha, f.beta, f.vfov = alpha, beta, vfov
     f.width, f.height = width, height

@@ -139,11 +139,14 @@ prr_status validate_frame(prr_ctx *ctx, const prr_frame &f, int idx) {
     if (f.ssaa == PRR_SSAA_NONE) {
         if (f.samples != 1)
             return fail(ctx, PRR_EINVAL, "frame %d: SSAA none needs samples = 1 (got %d)", idx, f.samples);
-    } else if (f.ssaa == PRR_SSAA_REGULAR) {
+    } else if (f.ssaa == PRR_SSAA_REGULAR || f.ssaa == PRR_SSAA_JITTERED) {
         const int n = int(std::lround(std::sqrt(double(f.samples))));
         if (f.samples < 1 || n * n != f.samples || n > 16)
-            return fail(ctx, PRR_EINVAL, "frame %d: regular SSAA needs samples = n*n, n <= 16 (got %d)", idx,
-                        f.samples);
+            return fail(ctx, PRR_EINVAL, "frame %d: %s SSAA needs samples = n*n, n <= 16 (got %d)", idx,
+                        f.ssaa == PRR_SSAA_REGULAR ? "regular" : "jittered", f.samples);
+    } else if (f.ssaa == PRR_SSAA_RANDOM) {
+        if (f.samples < 1 || f.samples > 256)
+            return fail(ctx, PRR_EINVAL, "frame %d: random SSAA needs 1 <= samples <= 256 (got %d)", idx, f.samples);
     } else {
         return fail(ctx, PRR_EINVAL, "frame %d: unsupported SSAA distribution %d", idx, f.ssaa);
     }

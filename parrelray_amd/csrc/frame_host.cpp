@@ -49,7 +49,11 @@ FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t
     d.W = f.width;
     d.H = f.height;
     d.s = f.samples;
-    d.n = (f.ssaa == PRR_SSAA_NONE || f.samples == 1) ? 0 : int(std::lround(std::sqrt(double(f.samples))));
+    const bool grid = f.ssaa == PRR_SSAA_REGULAR || f.ssaa == PRR_SSAA_JITTERED;
+    d.n = (!grid || (f.ssaa == PRR_SSAA_REGULAR && f.samples == 1)) ? 0
+                                                                      : int(std::lround(std::sqrt(double(f.samples))));
+    d.ssaa = f.ssaa;
+    d.seed = f.seed;
     d.projection = f.projection;
     d.n_spheres = f.n_spheres;
     d.M = f.mass;

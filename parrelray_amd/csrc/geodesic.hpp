@@ -520,11 +520,40 @@ PRR_HD void cart_to_sph(D3 p, D3 d, double xs[3], double us[3]) {
 
 // createRays_* (kernels/ssaa.cuh:36-48, 92-99) and rayDir_* (main/Camera.h:30-42)
 // for ray `local` of frame fr.
+// SplitMix64's output function (Steele, Lea and Flood, OOPSLA 2014): a
+// bijective 64-bit mix.  Hashing a counter gives the counter-based uniform
+// stream of the RANDOM and JITTERED distributions: no per-pixel generator
+// state (the reference keeps a curand state per pixel, kernels/ssaa.cuh:26-29).
+PRR_HD uint64_t mix64(uint64_t z) {
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+}
+// Uniform in [0, 1) (53 bits) for coordinate dim (0: u, 1: v) of ray `local` of a frame.
+PRR_HD double sample_u01(uint32_t seed, int64_t local, int dim) {
+    return double(mix64(mix64(seed) + 2ull * uint64_t(local) + uint64_t(dim)) >> 11) * 0x1.0p-53;
+}
+
+// Sample position of ray `local` of a frame: ray (j W + i) s + k is sample k
+// of pixel (i, j) (kernels/ssaa.cuh:45).  NONE: the pixel centre
+// (createRays_none, :92-99); REGULAR: grid point ((p + 1) / (n + 1),
+// (q + 1) / (n + 1)) for k = q n + p (createRays_regular, :36-48); RANDOM:
+// uniform over the pixel (createRays_random, :55-66, whose offset is not
+// scaled to the pixel -- D23); JITTERED: uniform inside cell (p, q)
+// (createRays_jittered, :73-86).
 PRR_HD void ray_uv(const FrameDev &fr, int64_t local, double &u, double &v) {
     const int64_t pix = local / fr.s;
     const int k = int(local - pix * fr.s);
     const int j = int(pix / fr.W), i = int(pix - int64_t(j) * fr.W);
-    if (fr.n == 0) {
+    if (fr.ssaa == PRR_SSAA_RANDOM) {
+        u = (i + sample_u01(fr.seed, local, 0)) / fr.W;
+        v = (j + sample_u01(fr.seed, local, 1)) / fr.H;
+    } else if (fr.ssaa == PRR_SSAA_JITTERED) {
+        const int q = k / fr.n, p = k - q * fr.n;
+        u = (i + (p + sample_u01(fr.seed, local, 0)) / fr.n) / fr.W;
+        v = (j + (q + sample_u01(fr.seed, local, 1)) / fr.n) / fr.H;
+    } else if (fr.n == 0) {
         u = (i + 0.5) / fr.W;
         v = (j + 0.5) / fr.H;
     } else {

@@ -381,11 +381,15 @@ prr_status prr_params_frame(const prr_params *p, int i, prr_frame *f) {
     // Parameters::getSSAA (main/Parameters.h:24-29): unknown -> regular
     const std::string sname = par(p, "camera", "SSAA");
     if (sname == "none") f->ssaa = PRR_SSAA_NONE;
-    else if (sname == "regular" || (sname != "random" && sname != "poisson" && sname != "jittered" &&
-                                    sname != "rotated"))
+    else if (sname == "random") f->ssaa = PRR_SSAA_RANDOM;
+    else if (sname == "jittered") f->ssaa = PRR_SSAA_JITTERED;
+    else if (sname == "regular" || (sname != "poisson" && sname != "rotated"))
         f->ssaa = PRR_SSAA_REGULAR;
-    else return perr(PRR_EINVAL, "SSAA distribution '" + sname + "' is declared by the reference but not implemented");
+    else return perr(PRR_EINVAL, "SSAA distribution '" + sname + "' is declared by the reference but has no kernel there");
     f->samples = pari(p, "camera", "SSAAsamples");
+    // key of the RANDOM / JITTERED sample stream (not a reference key: its
+    // randinit seed, kernels/ssaa.cuh:26, is never called)
+    f->seed = uint32_t(pari(p, "camera", "SSAAseed"));
     if (f->ssaa == PRR_SSAA_NONE && f->samples <= 0) f->samples = 1;
     const std::string proj = par(p, "prr", "projection");
     if (proj.empty())

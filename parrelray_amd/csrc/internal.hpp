@@ -41,6 +41,8 @@ struct FrameDev {
     int32_t W, H, s, n;       // image, samples per pixel, sqrt(samples) (0 = NONE)
     int32_t projection;
     int32_t n_spheres;
+    int32_t ssaa;  // prr_ssaa
+    uint32_t seed; // RANDOM / JITTERED sample stream
     double M, a;
     double r_hor; // horizon test radius r_h (1 + eps_rel) + eps_abs (negative: no test)
     int64_t ray_begin;

@@ -57,8 +57,9 @@ def test_ctypes_layout_matches_the_header(tmp_path):
 def test_enum_values_match_the_header():
     text = open(HEADERS[0]).read()
     for name, value in [("PRR_KERR_BL", A.PRR_KERR_BL), ("PRR_RKF45", A.PRR_RKF45), ("PRR_FP32_BULK", A.PRR_FP32_BULK),
-                        ("PRR_SPHERE", A.PRR_SPHERE), ("PRR_NAN", A.PRR_NAN), ("PRR_FILTER_BILINEAR",
-                                                                               A.PRR_FILTER_BILINEAR)]:
+                        ("PRR_SPHERE", A.PRR_SPHERE), ("PRR_NAN", A.PRR_NAN),
+                        ("PRR_FILTER_BILINEAR", A.PRR_FILTER_BILINEAR), ("PRR_SSAA_RANDOM", A.PRR_SSAA_RANDOM),
+                        ("PRR_SSAA_JITTERED", A.PRR_SSAA_JITTERED)]:
         m = re.search(rf"\b{name}\s*=\s*(\d+)", text)
         if m:
             assert int(m.group(1)) == value, name

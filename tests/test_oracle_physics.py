@@ -335,3 +335,39 @@ def test_sample_positions_regular_grid():
             assert np.allclose(uv, [(1 + (p + 1) / 4) / 4, (2 + (q + 1) / 4) / 3], atol=0, rtol=1e-15)
     f1 = A.make_frame([30, 0, 0], math.pi, 0, 1.0, 4, 3)
     assert np.allclose(O.sample_uv(f1, 5), [(1 + 0.5) / 4, (1 + 0.5) / 3])
+
+
+@pytest.mark.parametrize("ssaa,samples", [(A.PRR_SSAA_RANDOM, 5), (A.PRR_SSAA_JITTERED, 9)])
+def test_random_and_jittered_samples_stay_in_their_pixel_and_cell(ssaa, samples):
+    f = A.make_frame([30, 0, 0], math.pi, 0, 1.0, 7, 5, ssaa=ssaa, samples=samples, seed=12345)
+    n = int(round(math.sqrt(samples)))
+    for ray in range(f.nrays):
+        pix, k = divmod(ray, samples)
+        j, i = divmod(pix, f.width)
+        u, v = O.sample_uv(f, ray)
+        x, y = u * f.width - i, v * f.height - j  # position inside the pixel
+        assert 0.0 <= x < 1.0 and 0.0 <= y < 1.0, (ray, x, y)
+        if ssaa == A.PRR_SSAA_JITTERED:
+            q, p = divmod(k, n)
+            assert p <= x * n < p + 1 and q <= y * n < q + 1, (ray, x * n, y * n)
+
+
+def test_random_samples_are_uniform_and_keyed_by_the_seed():
+    f = A.make_frame([30, 0, 0], math.pi, 0, 1.0, 64, 64, ssaa=A.PRR_SSAA_RANDOM, samples=16, seed=7)
+    xy = []
+    for ray in range(0, f.nrays, 7):
+        pix = ray // 16
+        j, i = divmod(pix, f.width)
+        u, v = O.sample_uv(f, ray)
+        xy.append((u * f.width - i, v * f.height - j))
+    xy = np.array(xy)
+    # 10 x 10 histogram of the in-pixel offsets: chi-square near its 99 degrees of freedom
+    hist, _, _ = np.histogram2d(xy[:, 0], xy[:, 1], bins=10, range=[[0, 1], [0, 1]])
+    expected = len(xy) / 100
+    chi2 = ((hist - expected) ** 2 / expected).sum()
+    assert 60 < chi2 < 150, chi2
+    # u and v are not correlated
+    assert abs(np.corrcoef(xy[:, 0], xy[:, 1])[0, 1]) < 0.05
+    g = A.make_frame([30, 0, 0], math.pi, 0, 1.0, 64, 64, ssaa=A.PRR_SSAA_RANDOM, samples=16, seed=8)
+    assert not np.array_equal(O.sample_uv(f, 123), O.sample_uv(g, 123))
+    assert np.array_equal(O.sample_uv(f, 123), O.sample_uv(f, 123))
