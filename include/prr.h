@@ -56,8 +56,12 @@ typedef enum prr_metric {
     PRR_KERR_BL = 3       /* Kerr, Boyer-Lindquist (r, theta, phi, t) */
 } prr_metric;
 
-/* Solvers; values follow the reference's SOLVER enum (main/base.h:26-34). */
-typedef enum prr_solver { PRR_EULER = 0, PRR_RKF45 = 2 } prr_solver;
+/* Solvers; values follow the reference's SOLVER enum (main/base.h:26-34),
+ * which names "rk" and "dormand" without kernels: here RK4 is the classic
+ * fixed-step 4th-order Runge-Kutta (step h) and DORMAND the adaptive
+ * Dormand-Prince 5(4) pair with the RKF45 controller, propagating the 5th
+ * order.  Both use RKF45's located events. */
+typedef enum prr_solver { PRR_EULER = 0, PRR_RK4 = 1, PRR_RKF45 = 2, PRR_DORMAND = 5 } prr_solver;
 
 typedef enum prr_precision {
     PRR_FP64 = 0,     /* plain form: FP64 throughout */

@@ -54,8 +54,8 @@ def lib():
         L.orc_bilinear.argtypes = [C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_void_p]
         L.orc_chord_hit.argtypes = [C.POINTER(A.Sphere), C.c_int, C.c_void_p, C.c_void_p, _dp]
         L.orc_chord_hit.restype = C.c_int
-        L.orc_fehlberg.argtypes = [C.c_int, C.c_int, C.c_int]
-        L.orc_fehlberg.restype = C.c_double
+        L.orc_tableau.argtypes = [C.c_int, C.c_int, C.c_int, C.c_int]
+        L.orc_tableau.restype = C.c_double
         _lib = L
     return _lib
 
@@ -244,9 +244,17 @@ def chord_hit(spheres, P0, P1):
     return k, t.value
 
 
-def fehlberg():
+def tableau(solver):
+    """(A, b propagated, bh embedded) of a Runge-Kutta solver of the oracle."""
     L = lib()
-    Amat = np.array([[L.orc_fehlberg(0, i, j) for j in range(5)] for i in range(6)])
-    b4 = np.array([L.orc_fehlberg(1, i, 0) for i in range(6)])
-    b5 = np.array([L.orc_fehlberg(2, i, 0) for i in range(6)])
-    return Amat, b4, b5
+    S = int(L.orc_tableau(solver, 3, 0, 0))
+    Amat = np.array([[L.orc_tableau(solver, 0, i, j) if j < i else 0.0 for j in range(S)] for i in range(S)])
+    b = np.array([L.orc_tableau(solver, 1, i, 0) for i in range(S)])
+    bh = np.array([L.orc_tableau(solver, 2, i, 0) for i in range(S)])
+    return Amat, b, bh
+
+
+def fehlberg():
+    """Fehlberg's tableau as (A, b4, b5)."""
+    Amat, b4, b5 = tableau(A.PRR_RKF45)
+    return Amat[:, :5], b4, b5

@@ -171,10 +171,12 @@ int orc_chord_hit(const prr_sphere *sph, int n, const double *P0, const double *
     std::vector<prr_sphere> v(sph, sph + n);
     return chord_hit(v, v3(P0[0], P0[1], P0[2]), v3(P1[0], P1[1], P1[2]), *t);
 }
-double orc_fehlberg(int which, int i, int j) {
-    const Tableau &T = fehlberg();
-    if (which == 0) return T.A[i][j];
-    if (which == 1) return T.b4[i];
-    return T.b5[i];
+// Tableau of a solver: which 0 = A[i][j], 1 = propagated b[i], 2 = embedded bh[i], 3 = stages.
+double orc_tableau(int solver, int which, int i, int j) {
+    const Tableau &T = tableau_of(solver);
+    if (which == 0) return (i < 7 && j < 6) ? T.A[i][j] : 0.0;
+    if (which == 1) return T.b[i];
+    if (which == 2) return T.bh[i];
+    return T.S;
 }
 }

@@ -580,36 +580,69 @@ inline bool after_step(const Scene &S, double M, double a, const std::vector<prr
     return false;
 }
 
-// Fehlberg 4(5) tableau (kernels/solver.cuh:125-131; Fehlberg 1969, NASA TR R-315).
+// Runge-Kutta tableaux: S stages, A, the propagated weights b and the
+// embedded weights bh (bh = b for the fixed-step RK4: zero error).
 struct Tableau {
-    double A[6][5];
-    double b4[6], b5[6];
+    int S;
+    double A[7][6];
+    double b[7], bh[7];
 };
+// Fehlberg 4(5) (kernels/solver.cuh:125-131; Fehlberg 1969, NASA TR R-315):
+// the 4th order is propagated, the 5th is the embedded estimate.
 inline const Tableau &fehlberg() {
     static const Tableau T = {
-        {{0, 0, 0, 0, 0},
-         {1.0 / 4.0, 0, 0, 0, 0},
-         {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0},
-         {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0},
-         {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0},
-         {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0}},
-        {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -1.0 / 5.0, 0.0},
-        {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0}};
+        6,
+        {{0, 0, 0, 0, 0, 0},
+         {1.0 / 4.0, 0, 0, 0, 0, 0},
+         {3.0 / 32.0, 9.0 / 32.0, 0, 0, 0, 0},
+         {1932.0 / 2197.0, -7200.0 / 2197.0, 7296.0 / 2197.0, 0, 0, 0},
+         {439.0 / 216.0, -8.0, 3680.0 / 513.0, -845.0 / 4104.0, 0, 0},
+         {-8.0 / 27.0, 2.0, -3544.0 / 2565.0, 1859.0 / 4104.0, -11.0 / 40.0, 0}},
+        {25.0 / 216.0, 0.0, 1408.0 / 2565.0, 2197.0 / 4104.0, -1.0 / 5.0, 0.0, 0.0},
+        {16.0 / 135.0, 0.0, 6656.0 / 12825.0, 28561.0 / 56430.0, -9.0 / 50.0, 2.0 / 55.0, 0.0}};
     return T;
 }
+// Dormand-Prince 5(4) (Dormand and Prince 1980, J. Comput. Appl. Math. 6,
+// 19-26): the 5th order is propagated, the embedded 4th order estimates.
+inline const Tableau &dormand_prince() {
+    static const Tableau T = {
+        7,
+        {{0, 0, 0, 0, 0, 0},
+         {1.0 / 5.0, 0, 0, 0, 0, 0},
+         {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0},
+         {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0},
+         {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0},
+         {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0},
+         {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}},
+        {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0.0},
+        {5179.0 / 57600.0, 0.0, 7571.0 / 16695.0, 393.0 / 640.0, -92097.0 / 339200.0, 187.0 / 2100.0, 1.0 / 40.0}};
+    return T;
+}
+// Classic fourth-order Runge-Kutta (Kutta 1901), fixed step, no estimate.
+inline const Tableau &rk4() {
+    static const Tableau T = {4,
+                              {{0, 0, 0, 0, 0, 0}, {0.5, 0, 0, 0, 0, 0}, {0, 0.5, 0, 0, 0, 0}, {0, 0, 1.0, 0, 0, 0}},
+                              {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 0, 0, 0},
+                              {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0, 0, 0, 0}};
+    return T;
+}
+inline const Tableau &tableau_of(int solver) {
+    return solver == PRR_DORMAND ? dormand_prince() : (solver == PRR_RK4 ? rk4() : fehlberg());
+}
 
-// One RKF45 attempt of size h from y = (x, u): the 4th-order state y4 and
-// the scaled error norm (kernels/solver.cuh:142-184 with D2, D3).
+// One Runge-Kutta attempt of size h from y = (x, u) with the scene's solver:
+// the propagated state y4 and the scaled error norm (kernels/solver.cuh:142-184
+// with D2, D3, for RKF45).
 inline double rkf_attempt(const Scene &S, double M, double a, const double x[4], const double u[4], double h,
                           double y4[8]) {
-    const Tableau &TB = fehlberg();
+    const Tableau &TB = tableau_of(S.tr.solver);
     const prr_tracer &T = S.tr;
-    double y[8], k[6][8];
+    double y[8], k[7][8];
     for (int i = 0; i < 4; ++i) {
         y[i] = x[i];
         y[4 + i] = u[i];
     }
-    for (int st = 0; st < 6; ++st) {
+    for (int st = 0; st < TB.S; ++st) {
         double ys[8];
         for (int i = 0; i < 8; ++i) {
             double acc = y[i];
@@ -626,18 +659,18 @@ inline double rkf_attempt(const Scene &S, double M, double a, const double x[4],
     double errn = 0.0;
     for (int i = 0; i < 8; ++i) {
         double s4 = y[i], s5 = y[i];
-        for (int st = 0; st < 6; ++st) {
-            s4 += TB.b4[st] * k[st][i];
-            s5 += TB.b5[st] * k[st][i];
+        for (int st = 0; st < TB.S; ++st) {
+            s4 += TB.b[st] * k[st][i];
+            s5 += TB.bh[st] * k[st][i];
         }
         y4[i] = s4;
         const double sc = T.atol + T.rtol * std::max(std::fabs(y[i]), std::fabs(s4));
-        errn = std::max(errn, std::fabs(s5 - s4) / sc);
+        if (s5 != s4) errn = std::max(errn, std::fabs(s5 - s4) / sc);
     }
     return errn;
 }
 
-// Sub-step of size h (4th-order solution) for the event location below.
+// Sub-step of size h (the propagated solution) for the event location below.
 inline void rk_substep(const Scene &S, double M, double a, const double x[4], const double u[4], double h,
                        double xs[4], double us[4]) {
     double y4[8];
@@ -890,7 +923,8 @@ inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr
         }
         return finish(PRR_STEP_CAP);
     }
-    // RKF45
+    // RKF45 (and Dormand-Prince 5(4) with the same controller; RK4: fixed h, no rejection)
+    const bool adaptive = T.solver != PRR_RK4;
     double h = T.h;
     while (R.steps < T.max_steps && R.lambda < T.lambda_max) {
         if (R.attempts >= T.max_attempts) return finish(PRR_STEP_CAP);
@@ -898,9 +932,12 @@ inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr
         double y4[8];
         const double errn = rkf_attempt(S, M, a, x, u, h, y4);
         // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
-        double hn = 0.9 * h * std::pow(errn, -0.2);
-        hn = std::fmin(std::fmax(hn, T.hmin), T.hmax);
-        if (errn > 1.0 && h != T.hmin) { // reject (:185-188)
+        double hn = h;
+        if (adaptive) {
+            hn = 0.9 * h * std::pow(errn, -0.2);
+            hn = std::fmin(std::fmax(hn, T.hmin), T.hmax);
+        }
+        if (adaptive && errn > 1.0 && h != T.hmin) { // reject (:185-188)
             h = hn;
             continue;
         }

@@ -19,7 +19,7 @@ PRR_FLAT, PRR_SCHW_KS_CART, PRR_SCHW_SPH, PRR_KERR_BL = 0, 1, 2, 3
 METRIC_NAMES = {"flat": PRR_FLAT, "schwarzschild": PRR_SCHW_KS_CART, "schwarzschild-cart": PRR_SCHW_KS_CART,
                 "schwarzschild-sph": PRR_SCHW_SPH, "kerr": PRR_KERR_BL}
 # prr_solver (values of the reference's SOLVER enum, main/base.h:26-34)
-PRR_EULER, PRR_RKF45 = 0, 2
+PRR_EULER, PRR_RK4, PRR_RKF45, PRR_DORMAND = 0, 1, 2, 5
 # prr_precision
 PRR_FP64, PRR_FP32_BULK = 0, 1
 # prr_ssaa

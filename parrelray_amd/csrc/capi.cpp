@@ -108,16 +108,18 @@ void release(DevBuf &b) {
 }
 
 prr_status validate_tracer(prr_ctx *ctx, const prr_tracer &t) {
-    if (t.solver != PRR_EULER && t.solver != PRR_RKF45)
+    if (t.solver != PRR_EULER && t.solver != PRR_RKF45 && t.solver != PRR_DORMAND && t.solver != PRR_RK4)
         return fail(ctx, PRR_EINVAL, "unknown solver %d", t.solver);
     if (t.precision != PRR_FP64 && t.precision != PRR_FP32_BULK)
         return fail(ctx, PRR_EINVAL, "unknown precision %d", t.precision);
     if (!(t.h > 0.0) || !std::isfinite(t.h)) return fail(ctx, PRR_EINVAL, "step h must be positive");
-    if (t.solver == PRR_RKF45) {
+    if (t.solver == PRR_RKF45 || t.solver == PRR_DORMAND) {
         if (!(t.hmin > 0.0) || !(t.hmax >= t.hmin) || !std::isfinite(t.hmax))
-            return fail(ctx, PRR_EINVAL, "RKF45 needs 0 < hmin <= hmax < inf");
+            return fail(ctx, PRR_EINVAL, "adaptive solvers need 0 < hmin <= hmax < inf");
         if (!(t.atol >= 0.0) || !(t.rtol >= 0.0) || !(t.atol + t.rtol > 0.0))
-            return fail(ctx, PRR_EINVAL, "RKF45 needs atol, rtol >= 0, not both 0");
+            return fail(ctx, PRR_EINVAL, "adaptive solvers need atol, rtol >= 0, not both 0");
+    }
+    if (t.solver != PRR_EULER) {
         // every ray ends within max_attempts loop trips: the persistent kernel's exit bound
         if (t.max_attempts < 0 || t.max_attempts > 100000000LL)
             return fail(ctx, PRR_EINVAL, "max_attempts must be in [0, 1e8]");
@@ -205,13 +207,17 @@ FlopModel flop_model(int metric) {
 // then five FP32 difference stages with their stage sums.
 void fill_flops(prr_ctx *ctx, const CountersDev &c) {
     const FlopModel fm = flop_model(ctx->metric);
-    const bool rkf = ctx->tracer.solver == PRR_RKF45;
-    const double f_stage = rkf ? 6.0 * fm.rhs + 36.0 * fm.n : fm.rhs;
-    const double f_rest = rkf ? 26.0 * fm.n + 6.0 : 2.0 * fm.n;
+    const int sv = ctx->tracer.solver;
+    const bool rkf = sv != PRR_EULER;
+    // stages S, their right-hand sides and stage sums (S (S - 1) / 2 multiply-adds
+    // and one scaling per component), then the update / error / norm / controller
+    const double S = sv == PRR_DORMAND ? 7.0 : (sv == PRR_RK4 ? 4.0 : 6.0);
+    const double f_stage = rkf ? S * fm.rhs + (S * (S - 1.0) + S) * fm.n : fm.rhs;
+    const double f_rest = rkf ? (sv == PRR_RK4 ? 9.0 * fm.n : 26.0 * fm.n + 6.0) : 2.0 * fm.n;
     const double a64 = double(c.attempts_fp64), a32 = double(c.attempts) - double(c.attempts_fp64);
     if (rkf && ctx->metric == PRR_KERR_BL) {
         ctx->stats.flops_fp64 = a64 * (f_stage + f_rest) + a32 * (fm.rhs + fm.prep + f_rest + 2.0 * fm.n);
-        ctx->stats.flops_fp32 = a32 * (5.0 * fm.delta_rhs + 58.0 * fm.n);
+        ctx->stats.flops_fp32 = a32 * ((S - 1.0) * (fm.delta_rhs + 11.6 * fm.n));
         return;
     }
     ctx->stats.flops_fp64 = a64 * (f_stage + f_rest) + a32 * f_rest;

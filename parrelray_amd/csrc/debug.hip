@@ -59,6 +59,12 @@ prr_status host_ray(JobDev &job, const prr_tracer &tr, int64_t id, double *log, 
         if (tr.solver == PRR_EULER)
             c = mixed ? lane_iterate<Mt, PRR_EULER, true>(job, L, C, mode)
                       : lane_iterate<Mt, PRR_EULER, false>(job, L, C, mode);
+        else if (tr.solver == PRR_DORMAND)
+            c = mixed ? lane_iterate<Mt, PRR_DORMAND, true>(job, L, C, mode)
+                      : lane_iterate<Mt, PRR_DORMAND, false>(job, L, C, mode);
+        else if (tr.solver == PRR_RK4)
+            c = mixed ? lane_iterate<Mt, PRR_RK4, true>(job, L, C, mode)
+                      : lane_iterate<Mt, PRR_RK4, false>(job, L, C, mode);
         else
             c = mixed ? lane_iterate<Mt, PRR_RKF45, true>(job, L, C, mode)
                       : lane_iterate<Mt, PRR_RKF45, false>(job, L, C, mode);
@@ -246,7 +252,8 @@ prr_status prr_debug_trace_rays(int32_t metric, const prr_tracer *tracer, const 
                                 const int64_t *ids, int n, int32_t *cls, double *endpoint, int32_t *steps,
                                 int32_t *attempts, double *state8) {
     if (!tracer || !f || n < 0 || (!x0u0 && !ids)) return PRR_EINVAL;
-    if (tracer->solver == PRR_RKF45 && (!(tracer->hmin > 0.0) || tracer->max_attempts > 100000000LL))
+    const bool adaptive = tracer->solver == PRR_RKF45 || tracer->solver == PRR_DORMAND;
+    if ((adaptive && !(tracer->hmin > 0.0)) || (tracer->solver != PRR_EULER && tracer->max_attempts > 100000000LL))
         return PRR_EINVAL;
     const FrameDev fr = frame_dev(metric, *tracer, *f, 0);
     JobDev job{};

@@ -331,10 +331,10 @@ prr_status prr_params_tracer(const prr_params *p, prr_tracer *t) {
     bool is_known = false;
     for (const char *k : known) is_known = is_known || sname == k;
     const std::string solver = is_known ? sname : "euler";
-    if (solver != "euler" && solver != "rkf")
-        return perr(PRR_EINVAL, "solver '" + solver + "' is declared by the reference but not implemented");
+    if (solver != "euler" && solver != "rkf" && solver != "rk" && solver != "dormand")
+        return perr(PRR_EINVAL, "solver '" + solver + "' is declared by the reference but has no kernel there");
     *t = prr_tracer{};
-    t->solver = solver == "rkf" ? PRR_RKF45 : PRR_EULER;
+    t->solver = solver == "rkf" ? PRR_RKF45 : solver == "dormand" ? PRR_DORMAND : solver == "rk" ? PRR_RK4 : PRR_EULER;
     const std::string prec = par(p, "prr", "precision");
     if (prec.empty() || prec == "fp64") t->precision = PRR_FP64;
     else if (prec == "fp32-bulk") t->precision = PRR_FP32_BULK;

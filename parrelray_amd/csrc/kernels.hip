@@ -476,14 +476,17 @@ hipError_t launch_shade(int metric, const JobDev &job, hipStream_t st) {
     });
 }
 
+template <class Mt, int SOLVER> void launch_integrate_s(const JobDev &job, dim3 g, dim3 b, hipStream_t st) {
+    if (job.tr.precision == PRR_FP32_BULK) hipLaunchKernelGGL((k_integrate<Mt, SOLVER, true>), g, b, 0, st, job);
+    else hipLaunchKernelGGL((k_integrate<Mt, SOLVER, false>), g, b, 0, st, job);
+}
+
 template <class Mt> void launch_integrate_t(const JobDev &job, dim3 g, dim3 b, hipStream_t st) {
-    const bool mixed = job.tr.precision == PRR_FP32_BULK;
-    if (job.tr.solver == PRR_EULER) {
-        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, true>), g, b, 0, st, job);
-        else hipLaunchKernelGGL((k_integrate<Mt, PRR_EULER, false>), g, b, 0, st, job);
-    } else {
-        if (mixed) hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, true>), g, b, 0, st, job);
-        else hipLaunchKernelGGL((k_integrate<Mt, PRR_RKF45, false>), g, b, 0, st, job);
+    switch (job.tr.solver) {
+    case PRR_EULER: launch_integrate_s<Mt, PRR_EULER>(job, g, b, st); break;
+    case PRR_RK4: launch_integrate_s<Mt, PRR_RK4>(job, g, b, st); break;
+    case PRR_DORMAND: launch_integrate_s<Mt, PRR_DORMAND>(job, g, b, st); break;
+    default: launch_integrate_s<Mt, PRR_RKF45>(job, g, b, st); break;
     }
 }
 

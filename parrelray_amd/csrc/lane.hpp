@@ -28,8 +28,16 @@
 
 namespace prr {
 
-// Fehlberg 4(5) tableau (kernels/solver.cuh:125-131; Fehlberg 1969, NASA TR R-315).
+// Explicit Runge-Kutta tableaux: S stages, a(s, j), c(s) = sum_j a(s, j), the
+// propagated weights b(s) and the error weights e(s) (propagated minus
+// embedded; all zero for a fixed-step method).
+
+// Fehlberg 4(5) tableau (kernels/solver.cuh:125-131; Fehlberg 1969, NASA TR
+// R-315); the 4th order is propagated, as in the reference.
 struct Fehl {
+    static constexpr int S = 6;
+    static constexpr bool adaptive = true;
+    static PRR_HD constexpr double b(int s) { return b4(s); }
     static PRR_HD constexpr double a(int s, int j) {
         constexpr double A[6][5] = {{0, 0, 0, 0, 0},
                                     {1.0 / 4.0, 0, 0, 0, 0},
@@ -56,6 +64,64 @@ struct Fehl {
         return E[s];
     }
 };
+
+// Dormand-Prince 5(4) (Dormand and Prince 1980, J. Comput. Appl. Math. 6,
+// 19-26), the reference's "dormand" (main/Parameters.h:45): 7 stages, the 5th
+// order is propagated and the embedded 4th order gives the error estimate.
+struct Dopri {
+    static constexpr int S = 7;
+    static constexpr bool adaptive = true;
+    static PRR_HD constexpr double a(int s, int j) {
+        constexpr double A[7][6] = {
+            {0, 0, 0, 0, 0, 0},
+            {1.0 / 5.0, 0, 0, 0, 0, 0},
+            {3.0 / 40.0, 9.0 / 40.0, 0, 0, 0, 0},
+            {44.0 / 45.0, -56.0 / 15.0, 32.0 / 9.0, 0, 0, 0},
+            {19372.0 / 6561.0, -25360.0 / 2187.0, 64448.0 / 6561.0, -212.0 / 729.0, 0, 0},
+            {9017.0 / 3168.0, -355.0 / 33.0, 46732.0 / 5247.0, 49.0 / 176.0, -5103.0 / 18656.0, 0},
+            {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0}};
+        return A[s][j];
+    }
+    static PRR_HD constexpr double c(int s) {
+        constexpr double C[7] = {0.0, 1.0 / 5.0, 3.0 / 10.0, 4.0 / 5.0, 8.0 / 9.0, 1.0, 1.0};
+        return C[s];
+    }
+    static PRR_HD constexpr double b(int s) {
+        constexpr double B[7] = {35.0 / 384.0, 0.0, 500.0 / 1113.0, 125.0 / 192.0, -2187.0 / 6784.0, 11.0 / 84.0, 0.0};
+        return B[s];
+    }
+    // b5 - b4* (the embedded 4th-order weights 5179/57600, 0, 7571/16695,
+    // 393/640, -92097/339200, 187/2100, 1/40)
+    static PRR_HD constexpr double e(int s) {
+        constexpr double E[7] = {35.0 / 384.0 - 5179.0 / 57600.0,     0.0,
+                                 500.0 / 1113.0 - 7571.0 / 16695.0,   125.0 / 192.0 - 393.0 / 640.0,
+                                 -2187.0 / 6784.0 + 92097.0 / 339200.0, 11.0 / 84.0 - 187.0 / 2100.0,
+                                 -1.0 / 40.0};
+        return E[s];
+    }
+};
+
+// Classic 4th-order Runge-Kutta (Kutta 1901), the reference's "rk"
+// (main/Parameters.h:41): fixed step, no error estimate.
+struct Rk4 {
+    static constexpr int S = 4;
+    static constexpr bool adaptive = false;
+    static PRR_HD constexpr double a(int s, int j) { return (j == s - 1) ? (s == 3 ? 1.0 : 0.5) : 0.0; }
+    static PRR_HD constexpr double c(int s) {
+        constexpr double C[4] = {0.0, 0.5, 0.5, 1.0};
+        return C[s];
+    }
+    static PRR_HD constexpr double b(int s) {
+        constexpr double B[4] = {1.0 / 6.0, 1.0 / 3.0, 1.0 / 3.0, 1.0 / 6.0};
+        return B[s];
+    }
+    static PRR_HD constexpr double e(int) { return 0.0; }
+};
+
+// The tableau of an RK solver.
+template <int SOLVER> struct TableauOf { using type = Fehl; };
+template <> struct TableauOf<PRR_DORMAND> { using type = Dopri; };
+template <> struct TableauOf<PRR_RK4> { using type = Rk4; };
 
 // 1 - cos^2 th below which Kerr's FP32-bulk attempts run their stages in FP64.
 constexpr double kAxisFP64 = 0.05;
@@ -101,6 +167,18 @@ template <class Mt> PRR_HD double lambda_rate(const double *y, const Consts<doub
     return rcp_t(y[0] * y[0]) + cs.a * cs.a * y[2] * y[2];
 }
 
+// Near the polar axis a Kerr step may turn the ray by at most kAxisStep
+// times its angular distance from the axis (sin th): the L / sin^2 th spike
+// of phi' is then resolved by several steps, which the embedded error
+// estimate alone does not guarantee (angular speed sqrt(Q + L^2 + a^2 E^2
+// mu^2)).  A bound, so FP32 is enough.
+PRR_HD double axis_step_limit(const double *y, const Consts<double> &cs) {
+#pragma clang fp contract(off)
+    const double s2 = 1.0 - y[2] * y[2];
+    const double K = cs.Q + cs.L * cs.L + cs.a * cs.a * cs.E * cs.E * y[2] * y[2];
+    return double(float(kAxisStep) * sqrt_f32(float(fmax(s2, 0.0)) * rcp_t(float(K))));
+}
+
 // Step size of an RKF45 attempt from the proposal h: the affine bounds
 // [hmin, hmax] map to [hmin, hmax] / Sigma in Mino time.  at_min: no
 // rejection (the reference's h == hmin, kernels/solver.cuh:185).
@@ -113,17 +191,17 @@ PRR_HD double rkf_step_size(const TracerDev &tr, const double *y, const Consts<d
     }
     const double iS = rcp_t(lambda_rate<Mt>(y, cs));
     const double lo = tr.hmin * iS;
-    // near the polar axis a step may turn the ray by at most kAxisStep times
-    // its angular distance from the axis (sin th): the L / sin^2 th spike of
-    // phi' is then resolved by several steps, which the embedded error
-    // estimate alone does not guarantee (angular speed sqrt(Q + L^2 + a^2 E^2 mu^2));
-    // a bound, so FP32 is enough
-    const double s2 = 1.0 - y[2] * y[2];
-    const double K = cs.Q + cs.L * cs.L + cs.a * cs.a * cs.E * cs.E * y[2] * y[2];
-    const double hax = double(float(kAxisStep) * sqrt_f32(float(fmax(s2, 0.0)) * rcp_t(float(K))));
-    h = fmin(fmax(h, lo), fmin(tr.hmax * iS, fmax(hax, lo)));
+    h = fmin(fmax(h, lo), fmin(tr.hmax * iS, fmax(axis_step_limit(y, cs), lo)));
     at_min = h <= lo;
     return h;
+}
+
+// Step of the fixed-step RK4 solver: h, in Mino time h / Sigma, within the
+// polar-axis limit.
+template <class Mt> PRR_HD double fixed_step_size(const TracerDev &tr, const double *y, const Consts<double> &cs) {
+#pragma clang fp contract(off)
+    if (!Mt::mino) return tr.h;
+    return fmin(tr.h * rcp_t(lambda_rate<Mt>(y, cs)), axis_step_limit(y, cs));
 }
 
 // The RKF45 step factor 0.9 errn^(-1/5) (kernels/solver.cuh:175): an FP32
@@ -148,26 +226,27 @@ PRR_HD double step_factor(double errn) {
 // Six RK stages in precision T from the FP64 state y (plain or FP32 form of
 // the 8-component metrics, and Kerr's FP64 form); the 4th-order increment d4
 // and the error vector e in FP64.
-template <class Mt, class T>
+template <class Mt, class T, class Tab = Fehl>
 PRR_HD void rkf_stages(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
-    constexpr int N = Mt::N;
+    constexpr int N = Mt::N, S = Tab::S;
     const auto c = Mt::template coef<T>(cd);
     const T hT = T(h);
     T yT[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) yT[i] = T(y[i]);
-    T k[6][N];
+    T k[S][N];
     Mt::template rhs<T>(yT, c, k[0]);
 #pragma unroll
     for (int i = 0; i < N; ++i) k[0][i] *= hT;
 #pragma unroll
-    for (int s = 1; s < 6; ++s) {
+    for (int s = 1; s < S; ++s) {
         T ys[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
             T acc = yT[i];
 #pragma unroll
-            for (int j = 0; j < s; ++j) acc += T(Fehl::a(s, j)) * k[j][i];
+            for (int j = 0; j < s; ++j)
+                if (Tab::a(s, j) != 0.0) acc += T(Tab::a(s, j)) * k[j][i];
             ys[i] = acc;
         }
         Mt::template rhs<T>(ys, c, k[s]);
@@ -178,9 +257,9 @@ PRR_HD void rkf_stages(const double *y, double h, const Consts<double> &cd, doub
     for (int i = 0; i < N; ++i) {
         double a4 = 0.0, ae = 0.0;
 #pragma unroll
-        for (int s = 0; s < 6; ++s) {
-            if (Fehl::b4(s) != 0.0) a4 += Fehl::b4(s) * double(k[s][i]);
-            if (Fehl::e(s) != 0.0) ae += Fehl::e(s) * double(k[s][i]);
+        for (int s = 0; s < S; ++s) {
+            if (Tab::b(s) != 0.0) a4 += Tab::b(s) * double(k[s][i]);
+            if (Tab::e(s) != 0.0) ae += Tab::e(s) * double(k[s][i]);
         }
         d4[i] = a4;
         e[i] = ae;
@@ -188,13 +267,15 @@ PRR_HD void rkf_stages(const double *y, double h, const Consts<double> &cd, doub
 }
 
 // Kerr's FP32-bulk attempt: stage 1 and the difference coefficients in FP64,
-// stages 2-6 evaluate g_s = f(y + d_s) - f(y) in FP32 with
-// d_s = h (c_s F + sum_j a_sj g_j).  Since sum b4 = 1 and sum (b5 - b4) = 0,
-//   d4 = h F + h sum b4_s g_s,   e = h sum (b5 - b4)_s g_s,
+// stages 2..S evaluate g_s = f(y + d_s) - f(y) in FP32 with
+// d_s = h (c_s F + sum_j a_sj g_j).  Since sum b = 1 and sum e = 0 (for
+// RKF45: b = b4, e = b5 - b4),
+//   d4 = h F + h sum b_s g_s,   e = h sum e_s g_s,
 // so FP32 rounding enters only through the small differences g_s and the
 // error estimate never sees the rounding of f itself.
+template <class Tab = Fehl>
 PRR_HD void kerr_stages_fp32(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
-    constexpr int N = Kerr::N;
+    constexpr int N = Kerr::N, S = Tab::S;
     double F[N];
     KerrDelta P;
     Kerr::delta_prep(y, Kerr::coef<double>(cd), F, P);
@@ -202,17 +283,18 @@ PRR_HD void kerr_stages_fp32(const double *y, double h, const Consts<double> &cd
     float Ff[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) Ff[i] = float(F[i]);
-    float g[6][N];
+    float g[S][N];
 #pragma unroll
     for (int i = 0; i < N; ++i) g[0][i] = 0.0f;
 #pragma unroll
-    for (int s = 1; s < 6; ++s) {
+    for (int s = 1; s < S; ++s) {
         float d[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            float acc = float(Fehl::c(s)) * Ff[i];
+            float acc = float(Tab::c(s)) * Ff[i];
 #pragma unroll
-            for (int j = 1; j < s; ++j) acc += float(Fehl::a(s, j)) * g[j][i];
+            for (int j = 1; j < s; ++j)
+                if (Tab::a(s, j) != 0.0) acc += float(Tab::a(s, j)) * g[j][i];
             d[i] = hf * acc;
         }
         Kerr::delta_rhs(d, P, g[s]);
@@ -221,9 +303,9 @@ PRR_HD void kerr_stages_fp32(const double *y, double h, const Consts<double> &cd
     for (int i = 0; i < N; ++i) {
         float a4 = 0.0f, ae = 0.0f;
 #pragma unroll
-        for (int s = 1; s < 6; ++s) {
-            if (Fehl::b4(s) != 0.0) a4 += float(Fehl::b4(s)) * g[s][i];
-            if (Fehl::e(s) != 0.0) ae += float(Fehl::e(s)) * g[s][i];
+        for (int s = 1; s < S; ++s) {
+            if (Tab::b(s) != 0.0) a4 += float(Tab::b(s)) * g[s][i];
+            if (Tab::e(s) != 0.0) ae += float(Tab::e(s)) * g[s][i];
         }
         d4[i] = h * F[i] + double(hf * a4);
         e[i] = double(hf * ae);
@@ -797,12 +879,14 @@ template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, Loc
     return -1;
 }
 
-// One loop trip of a lane: an Euler step, an RKF45 attempt, or one sub-step
-// of an event location.  Returns the class when the ray ends (mode becomes
-// MODE_IDLE), -1 otherwise.
+// One loop trip of a lane: an Euler step, a Runge-Kutta attempt (RKF45,
+// Dormand-Prince 5(4) or fixed-step RK4), or one sub-step of an event
+// location.  Returns the class when the ray ends (mode becomes MODE_IDLE),
+// -1 otherwise.
 template <class Mt, int SOLVER, bool MIXED>
 PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode) {
     constexpr int N = Mt::N;
+    using Tab = typename TableauOf<SOLVER>::type;
     if (SOLVER == PRR_EULER) {
         const int r = euler_step<Mt, MIXED>(job, L);
         if (r >= 0) mode = MODE_IDLE;
@@ -824,7 +908,12 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
             return PRR_STEP_CAP;
         }
         L.attempts++;
-        hs = rkf_step_size<Mt>(tr, L.y, L.cs, L.h, at_min);
+        if constexpr (Tab::adaptive) {
+            hs = rkf_step_size<Mt>(tr, L.y, L.cs, L.h, at_min);
+        } else {
+            hs = fixed_step_size<Mt>(tr, L.y, L.cs);
+            at_min = true; // no rejection
+        }
         // FP64 stages: the plain form; in the FP32-bulk form inside r_fp64 and,
         // for Kerr, where the step can come within ~13 degrees of the polar
         // axis (1 - mu^2 and L / (1 - mu^2) lose their relative precision)
@@ -836,9 +925,9 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         L.att64 += use64 ? 1 : 0;
     }
     double d4[N], e[N];
-    if (use64) rkf_stages<Mt, double>(L.y, hs, L.cs, d4, e);
-    else if (Mt::mino) kerr_stages_fp32(L.y, hs, L.cs, d4, e);
-    else rkf_stages<Mt, float>(L.y, hs, L.cs, d4, e);
+    if (use64) rkf_stages<Mt, double, Tab>(L.y, hs, L.cs, d4, e);
+    else if (Mt::mino) kerr_stages_fp32<Tab>(L.y, hs, L.cs, d4, e);
+    else rkf_stages<Mt, float, Tab>(L.y, hs, L.cs, d4, e);
     double yn[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) yn[i] = L.y[i] + d4[i];
@@ -847,25 +936,28 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         if (r >= 0) mode = MODE_IDLE;
         return r;
     }
-    // Kerr: mu = cos th and mu' are measured in units of th (divided by
-    // sin th), so that the tolerance keeps its meaning near the poles, where a
-    // fixed error in mu is a large relative error in 1 - mu^2 (a scale: FP32)
-    const double ist =
-        Mt::mino ? double(rsqrt_f32(fmaxf(float(fmax(1.0 - L.y[2] * L.y[2], 0.0)), 1e-24f))) : 1.0;
-    // errn = max_i ei / sc_i, kept as the fraction num / den (one reciprocal
-    // instead of one division per component); NaN components are ignored as
-    // by fmax
-    double num = 0.0, den = 1.0;
+    double errn = 0.0; // fixed-step RK4: no estimate, no rejection
+    if constexpr (Tab::adaptive) {
+        // Kerr: mu = cos th and mu' are measured in units of th (divided by
+        // sin th), so that the tolerance keeps its meaning near the poles, where a
+        // fixed error in mu is a large relative error in 1 - mu^2 (a scale: FP32)
+        const double ist =
+            Mt::mino ? double(rsqrt_f32(fmaxf(float(fmax(1.0 - L.y[2] * L.y[2], 0.0)), 1e-24f))) : 1.0;
+        // errn = max_i ei / sc_i, kept as the fraction num / den (one reciprocal
+        // instead of one division per component); NaN components are ignored as
+        // by fmax
+        double num = 0.0, den = 1.0;
 #pragma unroll
-    for (int i = 0; i < Mt::NERR; ++i) {
-        const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(yn[i]));
-        const double ei = (Mt::mino && (i == 2 || i == 3)) ? fabs(e[i]) * ist : fabs(e[i]);
-        if (ei * den > num * sc) {
-            num = ei;
-            den = sc;
+        for (int i = 0; i < Mt::NERR; ++i) {
+            const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(yn[i]));
+            const double ei = (Mt::mino && (i == 2 || i == 3)) ? fabs(e[i]) * ist : fabs(e[i]);
+            if (ei * den > num * sc) {
+                num = ei;
+                den = sc;
+            }
         }
+        errn = num * rcp_t(den);
     }
-    double errn = num * rcp_t(den);
     // Kerr: a step may not reach the polar axis (|cos th| >= 1), where the chart is singular
     if (Mt::mino && !(fabs(yn[2]) < 1.0)) errn = fmax(errn, 1024.0);
     // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)

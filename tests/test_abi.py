@@ -59,7 +59,8 @@ def test_enum_values_match_the_header():
     for name, value in [("PRR_KERR_BL", A.PRR_KERR_BL), ("PRR_RKF45", A.PRR_RKF45), ("PRR_FP32_BULK", A.PRR_FP32_BULK),
                         ("PRR_SPHERE", A.PRR_SPHERE), ("PRR_NAN", A.PRR_NAN),
                         ("PRR_FILTER_BILINEAR", A.PRR_FILTER_BILINEAR), ("PRR_SSAA_RANDOM", A.PRR_SSAA_RANDOM),
-                        ("PRR_SSAA_JITTERED", A.PRR_SSAA_JITTERED)]:
+                        ("PRR_SSAA_JITTERED", A.PRR_SSAA_JITTERED), ("PRR_RK4", A.PRR_RK4),
+                        ("PRR_DORMAND", A.PRR_DORMAND)]:
         m = re.search(rf"\b{name}\s*=\s*(\d+)", text)
         if m:
             assert int(m.group(1)) == value, name

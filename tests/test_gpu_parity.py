@@ -35,19 +35,22 @@ def trace_tracer(solver):
     if solver == A.PRR_EULER:
         return A.make_tracer(solver=solver, h=0.01, r_escape=60.0, horizon_eps_rel=1e-3, horizon_eps_abs=0.0,
                              lambda_max=1e9, max_steps=3000, max_attempts=3000)
+    if solver == A.PRR_RK4:
+        return A.make_tracer(solver=solver, h=0.02, r_escape=60.0, horizon_eps_rel=1e-3, horizon_eps_abs=0.0,
+                             lambda_max=1e9, max_steps=20000, max_attempts=20000)
     return A.make_tracer(solver=solver, h=0.05, hmin=1e-9, hmax=5.0, atol=1e-10, rtol=1e-10, r_escape=60.0,
                          horizon_eps_rel=1e-3, horizon_eps_abs=0.0, lambda_max=1e9, max_steps=20000,
                          max_attempts=200000)
 
 
 @pytest.mark.parametrize("metric", [A.PRR_FLAT, A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH])
-@pytest.mark.parametrize("solver", [A.PRR_EULER, A.PRR_RKF45])
+@pytest.mark.parametrize("solver", [A.PRR_EULER, A.PRR_RKF45, A.PRR_DORMAND, A.PRR_RK4])
 def test_trace_second_order_metrics_match_oracle(metric, solver):
-    """Same algorithm, same state: classes agree; Euler final states agree to
-    rounding; RKF45 escape states (located at r = r_escape) agree to the
-    tolerance.  RKF45 step sequences may differ by rounding of the error
-    estimate, so other RKF45 end states (first state inside the horizon,
-    step cap) are not compared."""
+    """Same algorithm, same state: classes agree; fixed-step (Euler, RK4)
+    final states agree to rounding; adaptive (RKF45, Dormand-Prince) escape
+    states (located at r = r_escape) agree to the tolerance.  Adaptive step
+    sequences may differ by rounding of the error estimate, so their other
+    end states (first state inside the horizon, step cap) are not compared."""
     T = trace_tracer(solver)
     x0 = random_states(metric, 512)
     ctx = api.Context(0)
@@ -58,10 +61,11 @@ def test_trace_second_order_metrics_match_oracle(metric, solver):
     o = S.trace(x0)
     agree = g["cls"] == o["cls"]
     assert agree.mean() >= 0.99, np.nonzero(~agree)
-    m = agree if solver == A.PRR_EULER else agree & (o["cls"] == A.PRR_ESCAPE)
+    fixed = solver in (A.PRR_EULER, A.PRR_RK4)
+    m = agree if fixed else agree & (o["cls"] == A.PRR_ESCAPE)
     assert m.sum() >= 50
     err = np.abs(g["xend"][m] - o["xend"][m]).max(axis=1) / (np.abs(o["xend"][m]).max(axis=1) + 1)
-    if solver == A.PRR_EULER:
+    if fixed:
         assert np.median(err) < 1e-12
         assert (err < 1e-6).mean() >= 0.99, np.sort(err)[-10:]
         assert (g["steps"][agree] == o["steps"][agree]).mean() >= 0.97
@@ -70,9 +74,10 @@ def test_trace_second_order_metrics_match_oracle(metric, solver):
         assert (err < 1e-6).mean() >= 0.98, np.sort(err)[-10:]
 
 
-@pytest.mark.parametrize("a", [0.0, 0.6, 0.95])
-def test_trace_kerr_conserves_constants_and_matches_oracle(a):
-    T = trace_tracer(A.PRR_RKF45)
+@pytest.mark.parametrize("a,solver", [(0.0, A.PRR_RKF45), (0.6, A.PRR_RKF45), (0.95, A.PRR_RKF45),
+                                      (0.95, A.PRR_DORMAND)])
+def test_trace_kerr_conserves_constants_and_matches_oracle(a, solver):
+    T = trace_tracer(solver)
     x0 = random_states(A.PRR_KERR_BL, 256, a)
     ctx = api.Context(0)
     ctx.set_spacetime(A.PRR_KERR_BL, 1.0, a)
@@ -135,15 +140,19 @@ def test_random_and_jittered_ssaa_match_oracle(ssaa, samples):
     assert not np.array_equal(other, out["rgb8"])
 
 
-@pytest.mark.parametrize("precision", [A.PRR_FP64, A.PRR_FP32_BULK])
-def test_c1_reduced_parity(precision):
+@pytest.mark.parametrize("precision,solver", [(A.PRR_FP64, A.PRR_RKF45), (A.PRR_FP32_BULK, A.PRR_RKF45),
+                                              (A.PRR_FP32_BULK, A.PRR_DORMAND)])
+def test_c1_reduced_parity(precision, solver):
     """C1 at 192x108, 2x2 SSAA + FXAA, tol 1e-8, against the FP64 tol-1e-12 oracle.
 
     Endpoint tolerance at this integration tolerance: median < 1e-6 rad,
     99th percentile < 1e-5 rad (both forms re-impose Kerr's first integrals
-    in FP64 after every step)."""
+    in FP64 after every step).  Dormand-Prince 5(4) with the same controller
+    meets the same bounds."""
     f = CF.frame_c1(192, 108)
-    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=precision))
+    T = CF.tracer_kerr(precision=precision)
+    T.solver = solver
+    ctx = gpu_context(A.PRR_KERR_BL, T)
     out = ctx.render(f, rays=True)[0]
     S = oracle_scene(A.PRR_KERR_BL, CF.tracer_kerr(tol=1e-12))
     col, img, rgb8 = S.render_frame(f)

@@ -371,3 +371,52 @@ def test_random_samples_are_uniform_and_keyed_by_the_seed():
     g = A.make_frame([30, 0, 0], math.pi, 0, 1.0, 64, 64, ssaa=A.PRR_SSAA_RANDOM, samples=16, seed=8)
     assert not np.array_equal(O.sample_uv(f, 123), O.sample_uv(g, 123))
     assert np.array_equal(O.sample_uv(f, 123), O.sample_uv(f, 123))
+
+
+@pytest.mark.parametrize("solver,order", [(A.PRR_DORMAND, 5), (A.PRR_RK4, 4), (A.PRR_RKF45, 4)])
+def test_runge_kutta_order_conditions(solver, order):
+    """Butcher's order conditions up to the propagated order, for the
+    propagated weights; for the adaptive pairs the embedded weights reach the
+    other order (Fehlberg: 5, Dormand-Prince: 4)."""
+    Am, b, bh = O.tableau(solver)
+    c = Am.sum(axis=1)
+
+    def conditions(w, p):
+        out = [w.sum() - 1]
+        if p >= 2:
+            out.append(w @ c - 1 / 2)
+        if p >= 3:
+            out += [w @ c ** 2 - 1 / 3, w @ Am @ c - 1 / 6]
+        if p >= 4:
+            out += [w @ c ** 3 - 1 / 4, w @ (c * (Am @ c)) - 1 / 8, w @ Am @ c ** 2 - 1 / 12, w @ Am @ Am @ c - 1 / 24]
+        if p >= 5:
+            out += [w @ c ** 4 - 1 / 5, w @ (c ** 2 * (Am @ c)) - 1 / 10, w @ ((Am @ c) ** 2) - 1 / 20,
+                    w @ (c * (Am @ c ** 2)) - 1 / 15, w @ Am @ c ** 3 - 1 / 20, w @ (c * (Am @ Am @ c)) - 1 / 30,
+                    w @ Am @ (c * (Am @ c)) - 1 / 40, w @ Am @ Am @ c ** 2 - 1 / 60, w @ Am @ Am @ Am @ c - 1 / 120]
+        return np.abs(out).max()
+
+    assert conditions(b, order) < 1e-14
+    if solver != A.PRR_RK4:
+        assert conditions(bh, 9 - order) < 1e-14
+        if order < 5:  # and not more (conditions above order 5 are not listed here)
+            assert conditions(b, order + 1) > 1e-6
+
+
+@pytest.mark.parametrize("solver,order", [(A.PRR_DORMAND, 5), (A.PRR_RK4, 4)])
+def test_fixed_step_convergence_of_the_added_solvers(solver, order):
+    """Fixed-step integration of a flat-space... with curvature: a Schwarzschild
+    orbit; the error against a much finer run falls as h^order."""
+    pos = np.array([10.0, 0.0, 0.0])
+    x, u = O.init_ray(A.PRR_SCHW_KS_CART, 1.0, 0.0, pos, [-0.3, 0.9539392014169456, 0.0])
+    x0u0 = np.concatenate([x, u])[None]
+
+    def end(h, n):
+        S = O.Scene(A.PRR_SCHW_KS_CART, 1.0, 0.0, tracer(solver=solver, h=h, hmin=h, hmax=h, max_steps=n,
+                                                          r_escape=1e9, atol=1.0, rtol=0.0))
+        return S.trace(x0u0)["xend"][0]
+
+    ref = end(0.01, 2000)
+    e1 = np.abs(end(0.4, 50) - ref).max()
+    e2 = np.abs(end(0.2, 100) - ref).max()
+    rate = np.log2(e1 / e2)
+    assert order - 0.4 < rate < order + 0.6, rate
