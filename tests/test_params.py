@@ -97,8 +97,27 @@ def test_solver_names(tmp_path):
         return api.Params(str(p)).tracer()
     assert tracer_for("euler").solver == A.PRR_EULER
     assert tracer_for("verlet").solver == A.PRR_EULER  # unknown -> euler (main/Parameters.h:18-22)
+    assert tracer_for("rkf").solver == A.PRR_RKF45
+    assert tracer_for("dormand").solver == A.PRR_DORMAND  # declared by the reference without a kernel
+    assert tracer_for("rk").solver == A.PRR_RK4
+    for name in ("adams", "gragg", "taylor"):  # declared there, not implemented there or here
+        with pytest.raises(PrrError):
+            tracer_for(name)
+
+
+def test_ssaa_names(tmp_path):
+    def frame_for(name, samples):
+        p = tmp_path / f"{name}.params"
+        p.write_text("_frame0_\n<x> = <20>\n<y> = <0>\n<z> = <1>\n<a> = <180>\n<b> = <3>\n"
+                     f"_camera_\n<vfov> = <50>\n<width> = <8>\n<height> = <4>\n<SSAA> = <{name}>\n"
+                     f"<SSAAsamples> = <{samples}>\n<SSAAseed> = <77>\n")
+        return api.Params(str(p)).frame(0)
+    assert frame_for("random", 5).ssaa == A.PRR_SSAA_RANDOM
+    f = frame_for("jittered", 9)
+    assert f.ssaa == A.PRR_SSAA_JITTERED and f.samples == 9 and f.seed == 77
+    assert frame_for("regular", 4).ssaa == A.PRR_SSAA_REGULAR
     with pytest.raises(PrrError):
-        tracer_for("dormand")  # declared by the reference, not implemented by it or here
+        frame_for("poisson", 4)
 
 
 def test_new_keys_live_in_their_own_block(tmp_path):
