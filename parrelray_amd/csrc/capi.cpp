@@ -54,7 +54,8 @@ struct prr_ctx {
     // ray-state arrays and per-lane event location contexts
     DevBuf rs_y, rs_cls, rs_hit, rs_steps, rs_attempts, loc;
     // adjoint
-    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_event, adj_rays, adj_rec, adj_lam, adj_grad_ray, adj_grad;
+    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_ckpt2, adj_event, adj_rays, adj_rec, adj_lam, adj_grad_ray,
+        adj_grad;
     hipEvent_t ev[6] = {};
     prr_stats stats{};
 };
@@ -274,7 +275,7 @@ void prr_destroy(prr_ctx *ctx) {
     DevBuf *bufs[] = {&ctx->frames, &ctx->counters, &ctx->color, &ctx->post, &ctx->img_scratch,
                       &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
                       &ctx->attempts, &ctx->state8, &ctx->x0u0, &ctx->adj_img, &ctx->adj_dldi,
-                      &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_event, &ctx->adj_rays, &ctx->adj_rec,
+                      &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_ckpt2, &ctx->adj_event, &ctx->adj_rays, &ctx->adj_rec,
                       &ctx->adj_lam, &ctx->rs_y, &ctx->rs_cls, &ctx->rs_hit, &ctx->rs_steps, &ctx->rs_attempts,
                       &ctx->loc, &ctx->adj_grad_ray,
                       &ctx->adj_grad};
@@ -798,35 +799,53 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if ((s = ensure(ctx, ctx->adj_rec, 48 * size_t(K) * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad_ray, 32 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad, 32)) != PRR_OK) return s;
+    // first pass: the render, which also stores each ray's first `cap`
+    // checkpoints (every K accepted steps) in fixed slots; rays with more
+    // segments are checkpointed again by a pass of their own
+    const int cap = std::max(8, (512 + K - 1) / K);
+    if ((s = ensure(ctx, ctx->adj_ckpt, 64 * size_t(cap) * size_t(T))) != PRR_OK) return s;
     JobDev job = make_job(ctx, P);
     job.out.steps = static_cast<int32_t *>(ctx->steps.p);
     job.out.cls = static_cast<int8_t *>(ctx->cls.p);
+    job.adj.mode = ADJ_CHECKPOINT;
+    job.adj.K = K;
+    job.adj.cap = cap;
+    job.adj.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
     CountersDev *ctr = static_cast<CountersDev *>(ctx->counters.p);
     HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
-    // pass 1: the render (step counts, classes, ray colours, SSAA image)
     if ((s = enqueue_render(ctx, P, job, st, false, true)) != PRR_OK) return s;
     std::vector<int32_t> steps(T);
     HIPTRY(ctx, hipMemcpyAsync(steps.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
     if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises
     prr_stats fwd = ctx->stats;
-    // checkpoints at steps 0, K, 2K, ... < N_i; rays by decreasing segment count
-    std::vector<int64_t> off(T), order(T);
-    int64_t nck = 0, swept = 0;
+    // checkpoints at steps 0, K, 2K, ... < N_i: ray i keeps its first-pass
+    // slots i * cap when it has at most cap segments, else its checkpoints
+    // go to the overflow area at -1 - off[i]; rays by decreasing segment count
+    // (a stable counting sort), so that the rays with more than j segments
+    // are a prefix of the order
+    std::vector<int64_t> off(T), order(T), over;
+    std::vector<int32_t> nseg(T);
+    int64_t nck2 = 0, swept = 0;
     int32_t nseg_max = 0;
     for (int64_t i = 0; i < T; ++i) {
-        off[i] = nck;
-        const int32_t nseg = steps[i] > 0 ? (steps[i] - 1) / K + 1 : 0;
-        nck += std::max(nseg, 1);
+        nseg[i] = steps[i] > 0 ? (steps[i] - 1) / K + 1 : 0;
+        if (nseg[i] <= cap) {
+            off[i] = i * cap;
+        } else {
+            off[i] = -1 - nck2;
+            nck2 += nseg[i];
+            over.push_back(i);
+        }
         swept += steps[i];
-        nseg_max = std::max(nseg_max, nseg);
-        order[i] = i;
+        nseg_max = std::max(nseg_max, nseg[i]);
     }
-    std::stable_sort(order.begin(), order.end(), [&](int64_t x, int64_t y) { return steps[x] > steps[y]; });
-    if ((s = ensure(ctx, ctx->adj_ckpt, 64 * size_t(nck))) != PRR_OK) return s;
+    std::vector<int64_t> hist(size_t(nseg_max) + 1, 0), start(size_t(nseg_max) + 2, 0);
+    for (int64_t i = 0; i < T; ++i) hist[nseg[i]]++;
+    for (int32_t n = nseg_max; n >= 0; --n) start[n] = start[n + 1] + (n + 1 <= nseg_max ? hist[n + 1] : 0);
+    for (int64_t i = 0; i < T; ++i) order[start[nseg[i]]++] = i;
     HIPTRY(ctx, hipMemcpyAsync(ctx->adj_off.p, off.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
-    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_rays.p, order.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
     const float *dldi = dL_dimg;
     if (!on_device) {
         if ((s = ensure(ctx, ctx->adj_dldi, 12 * size_t(npix))) != PRR_OK) return s;
@@ -834,18 +853,27 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         dldi = static_cast<const float *>(ctx->adj_dldi.p);
     }
     HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    HIPTRY(ctx, launch_reset(ctr, st, true));
-    // pass 2: the same integration, storing checkpoints
     JobDev j2 = job;
     j2.out = OutDev{};
-    j2.adj.mode = ADJ_CHECKPOINT;
-    j2.adj.K = K;
     j2.adj.nsteps = static_cast<const int32_t *>(ctx->steps.p);
     j2.adj.ckpt_off = static_cast<const int64_t *>(ctx->adj_off.p);
-    j2.adj.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
-    j2.adj.mismatch = &ctr->mismatch;
-    HIPTRY(ctx, launch_init(ctx->metric, j2, st));
-    HIPTRY(ctx, launch_integrate(ctx->metric, j2, integrate_grid(ctx), ctx->launch.block, st));
+    if (!over.empty()) {
+        // the rays with more than cap segments, from the start again, storing all their checkpoints
+        if ((s = ensure(ctx, ctx->adj_ckpt2, 64 * size_t(nck2))) != PRR_OK) return s;
+        j2.adj.ckpt2 = static_cast<double *>(ctx->adj_ckpt2.p);
+        HIPTRY(ctx, hipMemcpyAsync(ctx->adj_rays.p, over.data(), 8 * over.size(), hipMemcpyHostToDevice, st));
+        j2.adj.rays = static_cast<const int64_t *>(ctx->adj_rays.p);
+        j2.total = int64_t(over.size());
+        HIPTRY(ctx, launch_reset(ctr, st, true));
+        JobDev ji = j2;
+        ji.total = T;
+        HIPTRY(ctx, launch_init(ctx->metric, ji, st));
+        const int grid = int(std::min<int64_t>(integrate_grid(ctx), (j2.total + 63) / 64 + 1));
+        HIPTRY(ctx, launch_integrate(ctx->metric, j2, grid, ctx->launch.block, st, true));
+        j2.total = T;
+    }
+    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_rays.p, order.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, launch_reset(ctr, st, true));
     // pass 3: per segment index, from the last: replay then reverse sweep
     JobDev j3 = j2;
     j3.adj.mode = ADJ_REPLAY;
@@ -869,10 +897,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     ad.mismatch = &ctr->mismatch;
     // rays with more than j segments: a prefix of the sorted order
     std::vector<int64_t> count(size_t(nseg_max) + 1, 0);
-    for (int64_t i = 0; i < T; ++i) {
-        const int32_t nseg = steps[i] > 0 ? (steps[i] - 1) / K + 1 : 0;
-        for (int32_t j = 0; j < nseg; ++j) count[j]++;
-    }
+    for (int32_t j = nseg_max - 1; j >= 0; --j) count[j] = count[j + 1] + hist[j + 1];
     for (int32_t j = nseg_max - 1; j >= 0; --j) {
         j3.adj.seg = j;
         j3.total = count[j];
@@ -884,6 +909,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         HIPTRY(ctx, launch_adj_reverse(j3, ad, st));
     }
     JobDev jg = job;
+    jg.adj = AdjRunDev{};
     jg.total = T;
     HIPTRY(ctx, launch_adj_init_grad(jg, ad, st));
     HIPTRY(ctx, launch_grad_reduce(ad.grad_ray, T, static_cast<double *>(ctx->adj_grad.p), st));

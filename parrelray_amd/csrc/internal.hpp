@@ -104,11 +104,12 @@ struct AdjRunDev {
     int32_t mode;            // ADJ_*
     int32_t K;               // accepted steps between checkpoints
     int32_t seg;             // replay: the segment index j
-    int32_t pad;
-    const int32_t *nsteps;   // accepted steps of every ray (first pass)
-    const int64_t *ckpt_off; // index of each ray's first checkpoint
+    int32_t cap;             // first pass (ckpt_off null): checkpoint slots per ray
+    const int32_t *nsteps;   // accepted steps of every ray (null in the first pass)
+    const int64_t *ckpt_off; // first checkpoint of each ray: >= 0 in ckpt, < 0 at -1 - off in ckpt2
     double *ckpt;            // 8 doubles per checkpoint: y[6], h, attempts
-    const int64_t *rays;     // replay: ray of each job index
+    double *ckpt2;           // checkpoints of the rays with more than cap segments
+    const int64_t *rays;     // replay, and a checkpoint pass over some rays: ray of each job index
     double *rec;             // replay: K * 6 doubles per ray, [slot * 6 + c] * rs_total + ray
     EventDev *event;         // replay: terminal event of each ray
     unsigned long long *mismatch; // replay: segments that did not reproduce their checkpoint

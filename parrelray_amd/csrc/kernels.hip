@@ -79,7 +79,9 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                         lane_load_checkpoint<Mt>(job, L, rid, job.adj.seg);
                         end = min((job.adj.seg + 1) * job.adj.K, job.adj.nsteps[rid]);
                     } else {
-                        lane_load<Mt>(job, L, idx);
+                        // a checkpoint pass over some of the rays takes them from adj.rays
+                        const bool some = Mt::adj && job.adj.mode == ADJ_CHECKPOINT && job.adj.rays;
+                        lane_load<Mt>(job, L, some ? job.adj.rays[idx] : idx);
                         if (Mt::adj && job.adj.mode == ADJ_CHECKPOINT)
                             store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, 0);
                     }
@@ -99,7 +101,7 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                 if (!replay) {
                     lane_store<Mt>(job, L, cls);
                 } else if (cls < 0) { // a segment's replay must land on the next checkpoint bit for bit
-                    const double *nx = job.adj.ckpt + 8 * (job.adj.ckpt_off[L.rid] + job.adj.seg + 1);
+                    const double *nx = ckpt_at(job, L.rid, job.adj.seg + 1);
                     bool same = L.steps == end;
                     for (int i = 0; i < Mt::N; ++i) same = same && L.y[i] == nx[i];
                     if (!same) atomicAdd(job.adj.mismatch, 1ull);

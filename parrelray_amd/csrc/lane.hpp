@@ -427,12 +427,22 @@ template <class Mt> PRR_HD void lane_load(const JobDev &job, Lane<Mt> &L, int64_
     if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
 }
 
+// Checkpoint j of ray id (8 doubles): slot id * cap + j in the first pass,
+// which has no offset table; afterwards ckpt_off[id] + j in ckpt, or
+// (-1 - ckpt_off[id]) + j in ckpt2 for the rays with more than cap segments.
+PRR_HD double *ckpt_at(const JobDev &job, int64_t id, int j) {
+    const AdjRunDev &a = job.adj;
+    if (!a.ckpt_off) return a.ckpt + 8 * (id * a.cap + j);
+    const int64_t o = a.ckpt_off[id];
+    return o >= 0 ? a.ckpt + 8 * (o + j) : a.ckpt2 + 8 * (-1 - o + j);
+}
+
 // The lane takes ray id from its checkpoint j (adjoint replay).
 template <class Mt> PRR_HD void lane_load_checkpoint(const JobDev &job, Lane<Mt> &L, int64_t id, int j) {
     L.rid = id;
     L.frame = find_frame(job, id);
     const FrameDev &fr = job.frames[L.frame];
-    const double *c = job.adj.ckpt + 8 * (job.adj.ckpt_off[id] + j);
+    const double *c = ckpt_at(job, id, j);
     for (int k = 0; k < Mt::N; ++k) L.y[k] = c[k];
     load_consts<Mt>(job, id, fr, L.cs);
     L.h = c[6];
@@ -447,7 +457,7 @@ template <class Mt> PRR_HD void lane_load_checkpoint(const JobDev &job, Lane<Mt>
 
 PRR_HD void store_checkpoint(const JobDev &job, const double *y, int n, double h, int32_t attempts, int64_t id,
                              int j) {
-    double *c = job.adj.ckpt + 8 * (job.adj.ckpt_off[id] + j);
+    double *c = ckpt_at(job, id, j);
     for (int k = 0; k < 6; ++k) c[k] = k < n ? y[k] : 0.0;
     c[6] = h;
     c[7] = double(attempts);
@@ -603,12 +613,15 @@ template <int N> PRR_HD void copy_n(double *d, const double *s) {
     for (int i = 0; i < N; ++i) d[i] = s[i];
 }
 
-// Adjoint pass 2: checkpoint every K accepted steps while the lane integrates.
+// Adjoint checkpoint passes: a checkpoint every K accepted steps while the lane integrates.
 template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<Mt> &L) {
     if (!Mt::adj || job.adj.mode != ADJ_CHECKPOINT) return;
     const int K = job.adj.K;
-    if (L.steps % K == 0 && L.steps < job.adj.nsteps[L.rid])
-        store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, L.steps / K);
+    if (L.steps % K != 0) return;
+    // with known step counts, checkpoints at 0, K, 2K, ... < N_i; in the
+    // first pass, the first cap of them
+    if (job.adj.nsteps ? L.steps >= job.adj.nsteps[L.rid] : L.steps / K >= job.adj.cap) return;
+    store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, L.steps / K);
 }
 
 // Adjoint replay: the step's start and size of every accepted step of the segment.

@@ -182,3 +182,23 @@ def test_autograd_wrapper_backward_calls_the_adjoint(setup):
     dLdI = (2.0 * (img.detach() - target) / img.numel()).cpu().numpy()
     g = ctx.adjoint(tmpl, o0, dLdI)
     assert np.allclose(params.grad.numpy(), g, rtol=1e-5, atol=1e-12)
+
+
+def test_gradient_does_not_depend_on_the_checkpoint_interval(setup):
+    """The same steps are swept backwards whatever the checkpoint interval K:
+    rays with at most max(8, 512 / K) segments keep the render's checkpoints,
+    longer ones are checkpointed by a pass of their own (tol 1e-10 gives rays
+    of over 512 steps), and each replayed segment must land on its next
+    checkpoint bit for bit (the call fails otherwise)."""
+    tmpl, o0, T, ref = setup
+    I, _ = render(ref, tmpl, o0)
+    dLdI = (2.0 * (I - T) / I.size).astype(np.float32)
+    grads = []
+    for K in (16, 64, 128):
+        ctx = context(A.PRR_FP64, 1e-10)
+        ctx.set_launch(checkpoint=K)
+        grads.append(ctx.adjoint(tmpl, o0, dLdI))
+        st = ctx.stats()
+        print("K", K, "grad", grads[-1], "steps swept", st["adjoint_steps"])
+    for g in grads[1:]:
+        assert np.abs(g - grads[0]).max() <= 1e-12 * np.abs(grads[0]).max(), (g.tolist(), grads[0].tolist())
