@@ -121,13 +121,16 @@ inline void metric_ks(const double x[4], double M, double g[16]) {
 }
 
 // "metrics/schwarzschild - sph.cuh":118-136
-inline void metric_sph(const double x[4], double M, double g[16]) {
+// With a charge q (Reissner-Nordstrom, the reference's unused Spacetime
+// charge, main/Spacetime.h:23): f = 1 - r_s/r + q^2/r^2 in place of 1 - r_s/r.
+inline void metric_sph(const double x[4], double M, double q, double g[16]) {
     const double rs = 2.0 * M, r = x[0], s = std::sin(x[1]);
+    const double f = 1.0 - rs / r + q * q / (r * r);
     std::fill(g, g + 16, 0.0);
-    g[0] = 1.0 / (1.0 - rs / r);
+    g[0] = 1.0 / f;
     g[5] = r * r;
     g[10] = r * r * s * s;
-    g[15] = -(1.0 - rs / r);
+    g[15] = -f;
 }
 
 // metrics/kerr.cuh:77-100 (Boyer-Lindquist, order r, theta, phi, t)
@@ -145,7 +148,7 @@ inline void metric_kerr(const double x[4], double M, double a, double g[16]) {
 inline void metric(int kind, double M, double a, const double x[4], double g[16]) {
     switch (kind) {
     case PRR_SCHW_KS_CART: metric_ks(x, M, g); break;
-    case PRR_SCHW_SPH: metric_sph(x, M, g); break;
+    case PRR_SCHW_SPH: metric_sph(x, M, a, g); break; // a: the charge q
     case PRR_KERR_BL: metric_kerr(x, M, a, g); break;
     default: metric_flat(g); break;
     }
@@ -229,12 +232,23 @@ inline void geod_ks(const double p[4], const double u[4], double M, double du[4]
 }
 
 // "metrics/schwarzschild - sph.cuh":94-115, including the tan/sin clamp at :98-101.
-inline void geod_sph(const double x[4], const double u[4], double M, double du[4]) {
+inline void geod_sph(const double x[4], const double u[4], double M, double q, double du[4]) {
     const double rs = 2.0 * M, r = x[0];
     double tanth = std::tan(x[1]), sinth = std::sin(x[1]);
     if (std::fabs(tanth) < 1e-4) {
         tanth = 1e-4;
         sinth = 1e-4;
+    }
+    if (q != 0.0) {
+        // static, spherically symmetric: Gamma^r_tt = f f'/2, Gamma^r_rr = -f'/(2f),
+        // Gamma^r_thth = -r f, Gamma^t_tr = f'/(2f), f' = r_s/r^2 - 2 q^2/r^3
+        const double f = 1.0 - rs / r + q * q / (r * r), fp = rs / (r * r) - 2.0 * q * q / (r * r * r);
+        du[0] = -0.5 * f * fp * u[3] * u[3] + 0.5 * fp / f * u[0] * u[0] +
+                r * f * (u[1] * u[1] + sinth * sinth * u[2] * u[2]);
+        du[1] = -2.0 / r * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
+        du[2] = -2.0 / r * u[0] * u[2] - 2.0 / tanth * u[1] * u[2];
+        du[3] = -fp / f * u[0] * u[3];
+        return;
     }
     du[0] = -0.5 * rs * (r - rs) / (r * r * r) * u[3] * u[3] + 0.5 * rs / (r * r - r * rs) * u[0] * u[0] +
             (r - rs) * (u[1] * u[1] + sinth * sinth * u[2] * u[2]);
@@ -258,15 +272,16 @@ inline void geod_kerr(const double x[4], const double u[4], double M, double a, 
 inline void geod(int kind, double M, double a, const double x[4], const double u[4], double du[4]) {
     switch (kind) {
     case PRR_SCHW_KS_CART: geod_ks(x, u, M, du); break;
-    case PRR_SCHW_SPH: geod_sph(x, u, M, du); break;
+    case PRR_SCHW_SPH: geod_sph(x, u, M, a, du); break; // a: the charge q
     case PRR_KERR_BL: geod_kerr(x, u, M, a, du); break;
     default: du[0] = du[1] = du[2] = du[3] = 0.0; break; // metrics/flat.cuh:88-90
     }
 }
 
+// Outer horizon: Kerr with spin a, Reissner-Nordstrom with charge a.
 inline double horizon_radius(int kind, double M, double a) {
     if (kind == PRR_FLAT) return 0.0;
-    if (kind == PRR_KERR_BL) return M + std::sqrt(std::max(M * M - a * a, 0.0));
+    if (kind == PRR_KERR_BL || kind == PRR_SCHW_SPH) return M + std::sqrt(std::max(M * M - a * a, 0.0));
     return 2.0 * M;
 }
 
@@ -1023,9 +1038,12 @@ inline RayResult render_ray(const Scene &S, const prr_frame &f, const Basis &B, 
     V3 d = ray_dir(f, B, u, v);
     d = perturb_dir(d, B, delta, axis);
     double x0[4], u0[4];
-    init_ray(S.metric, f.mass, f.spin, v3(f.pos[0], f.pos[1], f.pos[2]), d, x0, u0);
+    // the metric's second parameter: the frame's spin (Kerr), or the scene's
+    // charge for the Schwarzschild-coordinate metric (Reissner-Nordstrom)
+    const double a = S.metric == PRR_SCHW_SPH ? S.a : f.spin;
+    init_ray(S.metric, f.mass, a, v3(f.pos[0], f.pos[1], f.pos[2]), d, x0, u0);
     std::vector<prr_sphere> sph(f.spheres, f.spheres + std::min(f.n_spheres, PRR_MAX_SPHERES));
-    RayResult R = trace(S, f.mass, f.spin, sph, x0, u0);
+    RayResult R = trace(S, f.mass, a, sph, x0, u0);
     shade(S, R);
     return R;
 }

@@ -291,6 +291,9 @@ prr_status prr_set_spacetime(prr_ctx *ctx, int32_t metric, double M, double a, d
     if (metric < PRR_FLAT || metric > PRR_KERR_BL) return fail(ctx, PRR_EINVAL, "unknown metric %d", metric);
     if (!(M >= 0.0)) return fail(ctx, PRR_EINVAL, "mass must be >= 0");
     if (metric == PRR_KERR_BL && !(std::fabs(a) <= M)) return fail(ctx, PRR_EINVAL, "Kerr needs |a| <= M");
+    if (q != 0.0 && metric != PRR_SCHW_SPH)
+        return fail(ctx, PRR_EINVAL, "a charge needs the Schwarzschild-coordinate metric (Reissner-Nordstrom)");
+    if (!(std::fabs(q) <= M) && q != 0.0) return fail(ctx, PRR_EINVAL, "Reissner-Nordstrom needs |q| <= M");
     ctx->metric = metric;
     ctx->M = M;
     ctx->a = a;
@@ -396,7 +399,7 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
     for (int i = 0; i < n; ++i) {
         prr_status st = validate_frame(ctx, frames[i], i);
         if (st != PRR_OK) return st;
-        P.fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total);
+        P.fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total, ctx->q);
         const int64_t np = int64_t(frames[i].width) * frames[i].height;
         P.total += np * frames[i].samples;
         P.pixels += np;
@@ -633,7 +636,7 @@ prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int3
     f.n_spheres = int(ctx->spheres.size());
     for (int k = 0; k < f.n_spheres; ++k) f.spheres[k] = ctx->spheres[k];
     RenderPlan P;
-    P.fd.push_back(frame_dev(ctx->metric, ctx->tracer, f, 0));
+    P.fd.push_back(frame_dev(ctx->metric, ctx->tracer, f, 0, ctx->q));
     P.total = n;
     prr_status s;
     if ((s = ensure(ctx, ctx->frames, sizeof(FrameDev))) != PRR_OK) return s;

@@ -8,9 +8,11 @@
 
 namespace prr {
 
+// Outer horizon: Kerr r+ = M + sqrt(M^2 - a^2); Reissner-Nordstrom (the
+// Schwarzschild-coordinate metric with charge a = q) M + sqrt(M^2 - q^2).
 double horizon_radius(int metric, double M, double a) {
     if (metric == PRR_FLAT) return 0.0;
-    if (metric == PRR_KERR_BL) return M + std::sqrt(std::max(M * M - a * a, 0.0));
+    if (metric == PRR_KERR_BL || metric == PRR_SCHW_SPH) return M + std::sqrt(std::max(M * M - a * a, 0.0));
     return 2.0 * M;
 }
 
@@ -31,7 +33,7 @@ TracerDev tracer_dev(const prr_tracer &t) {
     return d;
 }
 
-FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t begin) {
+FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t begin, double charge) {
     FrameDev d{};
     for (int i = 0; i < 3; ++i) d.pos[i] = f.pos[i];
     double F[3], U[3], R[3];
@@ -57,7 +59,9 @@ FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t
     d.projection = f.projection;
     d.n_spheres = f.n_spheres;
     d.M = f.mass;
-    d.a = metric == PRR_KERR_BL ? f.spin : 0.0;
+    // the metric's second parameter: Kerr's spin, or the charge of the
+    // Schwarzschild-coordinate metric (Reissner-Nordstrom)
+    d.a = metric == PRR_KERR_BL ? f.spin : (metric == PRR_SCHW_SPH ? charge : 0.0);
     const double rh = horizon_radius(metric, f.mass, d.a);
     d.r_hor = metric == PRR_FLAT ? -1.0
                                       : rh * (1.0 + tr.horizon_eps_rel) + tr.horizon_eps_abs;

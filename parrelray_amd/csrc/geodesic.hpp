@@ -146,6 +146,7 @@ template <class T> struct Consts {
 // Right-hand-side constants of the 8-component metrics.
 template <class T> struct MassCoef {
     T M;
+    T q2; // Schwarzschild coordinates: the squared charge of Reissner-Nordstrom (0: Schwarzschild)
 };
 
 // ----------------------------------------------------------------- metrics
@@ -211,7 +212,10 @@ struct SchwSph {
     static constexpr bool spherical = true;
     static constexpr int NERR = 8;
     static constexpr bool mino = false;
-    template <class T> PRR_HD static MassCoef<T> coef(const Consts<double> &c) { return MassCoef<T>{T(c.M)}; }
+    // the metric's second parameter (Consts::a, the spin slot) is the charge q
+    template <class T> PRR_HD static MassCoef<T> coef(const Consts<double> &c) {
+        return MassCoef<T>{T(c.M), T(c.a * c.a)};
+    }
     template <class T> PRR_HD static void rhs(const T *y, const MassCoef<T> &c, T *f) {
         const T r = y[0];
         const T ur = y[4], uth = y[5], uph = y[6], ut = y[7];
@@ -222,10 +226,12 @@ struct SchwSph {
             tn = T(1e-4);
             s = T(1e-4);
         }
+        // f = 1 - r_s/r + q^2/r^2 (Reissner-Nordstrom; q = 0: the reference's
+        // Schwarzschild), A = f'/2 = (M - q^2/r)/r^2
         const T rs = T(2) * c.M;
         const T ir = T(1) / r;
-        const T fr = T(1) - rs * ir; // 1 - r_s/r
-        const T A = T(0.5) * rs * ir * ir;
+        const T fr = T(1) - rs * ir + c.q2 * ir * ir;
+        const T A = (c.M - c.q2 * ir) * ir * ir;
         f[0] = ur;
         f[1] = uth;
         f[2] = uph;
@@ -699,8 +705,8 @@ template <class Mt> PRR_HD void init_state(D3 pos, D3 d, double M, double a, dou
         A = -1.0 + rs / r;
         B = 2.0 * rs * xu / (r * r);
         C = u[0] * u[0] + u[1] * u[1] + u[2] * u[2] + rs * xu * xu / (r * r * r);
-    } else if (Mt::kind == PRR_SCHW_SPH) {
-        const double f = 1.0 - 2.0 * M / x[0], s = sin(x[1]);
+    } else if (Mt::kind == PRR_SCHW_SPH) { // a: the charge q
+        const double f = 1.0 - 2.0 * M / x[0] + a * a / (x[0] * x[0]), s = sin(x[1]);
         A = -f;
         B = 0.0;
         C = u[0] * u[0] / f + x[0] * x[0] * (u[1] * u[1] + s * s * u[2] * u[2]);

@@ -43,6 +43,16 @@ def test_second_order_rhs_matches_oracle_getgeod(metric):
         assert np.allclose(f[4:], du, rtol=1e-12, atol=1e-13 * scale), (x, u, f[4:], du)
 
 
+@pytest.mark.parametrize("q", [0.3, 0.95])
+def test_charged_sph_rhs_matches_oracle(q):
+    """Reissner-Nordstrom: the charge rides in the metric's second parameter."""
+    for x, u in points(A.PRR_SCHW_SPH, 50):
+        f = api.rhs(A.PRR_SCHW_SPH, np.concatenate([x, u]), M=1.0, a=q)
+        du = O.geod(A.PRR_SCHW_SPH, 1.0, q, x, u)
+        scale = np.abs(du).max() + 1e-300
+        assert np.allclose(f[4:], du, rtol=1e-12, atol=1e-13 * scale), (x, u, f[4:], du)
+
+
 def test_sph_rhs_keeps_the_tan_clamp():
     # |tan th| < 1e-4 -> tan = sin = 1e-4 ("metrics/schwarzschild - sph.cuh":98-101)
     for th in (1e-6, math.pi - 2e-5, -3e-5):

@@ -74,6 +74,42 @@ def test_trace_second_order_metrics_match_oracle(metric, solver):
         assert (err < 1e-6).mean() >= 0.98, np.sort(err)[-10:]
 
 
+@pytest.mark.parametrize("solver", [A.PRR_EULER, A.PRR_RKF45])
+def test_trace_reissner_nordstrom_matches_oracle(solver):
+    """The Schwarzschild-coordinate metric with charge q = 0.7 (the
+    spacetime's q): classes and end states as for the uncharged metric."""
+    q = 0.7
+    T = trace_tracer(solver)
+    x0 = random_states_charged(256, q)
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_SCHW_SPH, 1.0, 0.0, q)
+    ctx.set_tracer(T)
+    g = ctx.trace(x0)
+    o = O.Scene(A.PRR_SCHW_SPH, 1.0, q, T).trace(x0)
+    agree = g["cls"] == o["cls"]
+    assert agree.mean() >= 0.99
+    assert (o["cls"] == A.PRR_HORIZON).sum() >= 10 and (o["cls"] == A.PRR_ESCAPE).sum() >= 10
+    m = agree if solver == A.PRR_EULER else agree & (o["cls"] == A.PRR_ESCAPE)
+    err = np.abs(g["xend"][m] - o["xend"][m]).max(axis=1) / (np.abs(o["xend"][m]).max(axis=1) + 1)
+    assert np.median(err) < (1e-12 if solver == A.PRR_EULER else 1e-8)
+    # captured rays stop at the charged horizon r+ = M + sqrt(M^2 - q^2), not at 2M
+    hz = agree & (o["cls"] == A.PRR_HORIZON)
+    rp = 1.0 + math.sqrt(1.0 - q * q)
+    assert np.all(g["xend"][hz, 0] <= rp * (1 + 1e-3) + 1e-9) and np.all(g["xend"][hz, 0] > rp * 0.99)
+
+
+def random_states_charged(n, q):
+    out = []
+    for _ in range(n):
+        pos = RNG.normal(size=3)
+        pos = pos / np.linalg.norm(pos) * RNG.uniform(8.0, 30.0)
+        d = -pos / np.linalg.norm(pos) + 0.6 * RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        x, u = O.init_ray(A.PRR_SCHW_SPH, 1.0, q, pos, d)
+        out.append(np.concatenate([x, u]))
+    return np.array(out)
+
+
 @pytest.mark.parametrize("a,solver", [(0.0, A.PRR_RKF45), (0.6, A.PRR_RKF45), (0.95, A.PRR_RKF45),
                                       (0.95, A.PRR_DORMAND)])
 def test_trace_kerr_conserves_constants_and_matches_oracle(a, solver):

@@ -53,8 +53,10 @@ def random_points(kind, n):
 
 
 @pytest.mark.parametrize("kind,a", [(A.PRR_SCHW_KS_CART, 0.0), (A.PRR_SCHW_SPH, 0.0), (A.PRR_KERR_BL, 0.0),
-                                    (A.PRR_KERR_BL, 0.6), (A.PRR_KERR_BL, 0.998)])
+                                    (A.PRR_KERR_BL, 0.6), (A.PRR_KERR_BL, 0.998), (A.PRR_SCHW_SPH, 0.8)])
 def test_geod_equals_minus_christoffel_of_metric(kind, a):
+    """a: Kerr's spin, or the charge of the Schwarzschild-coordinate metric
+    (Reissner-Nordstrom)."""
     for x, u in random_points(kind, 20):
         G = fd_christoffel(kind, 1.0, a, x)
         expect = -np.einsum("amn,m,n->a", G, u, u)
@@ -420,3 +422,29 @@ def test_fixed_step_convergence_of_the_added_solvers(solver, order):
     e2 = np.abs(end(0.2, 100) - ref).max()
     rate = np.log2(e1 / e2)
     assert order - 0.4 < rate < order + 0.6, rate
+
+
+@pytest.mark.parametrize("q", [0.0, 0.5, 0.9, 1.0])
+def test_reissner_nordstrom_photon_sphere_and_horizon(q):
+    """Charge q in the Schwarzschild-coordinate metric: the circular photon
+    orbit at r = (3M + sqrt(9M^2 - 8q^2)) / 2 (geodesic acceleration zero for
+    the null tangential state there), and infall stops at r+ = M + sqrt(M^2 - q^2)."""
+    M = 1.0
+    r = (3 * M + math.sqrt(9 * M * M - 8 * q * q)) / 2
+    f = 1 - 2 * M / r + q * q / (r * r)
+    x = np.array([r, math.pi / 2, 0.0, 0.0])
+    u = np.array([0.0, 0.0, 1.0 / r, 1.0 / math.sqrt(f)])  # f (u^t)^2 = r^2 (u^phi)^2
+    g = O.metric(A.PRR_SCHW_SPH, M, q, x)
+    assert abs(u @ g @ u) < 1e-12
+    assert abs(O.geod(A.PRR_SCHW_SPH, M, q, x, u)[0]) < 1e-13
+    # slightly off the photon radius the orbit is not circular (unstable both ways)
+    x2 = x.copy()
+    x2[0] *= 1.01
+    assert abs(O.geod(A.PRR_SCHW_SPH, M, q, x2, u)[0]) > 1e-5
+    # radial infall from r = 10 ends at the outer horizon
+    S = O.Scene(A.PRR_SCHW_SPH, M, q, tracer(r_escape=100.0, horizon_eps_rel=1e-3, hmax=0.5))
+    x0, u0 = O.init_ray(A.PRR_SCHW_SPH, M, q, [10.0, 0.0, 0.0], [-1.0, 0.0, 0.0])
+    o = S.trace(np.concatenate([x0, u0])[None])
+    rp = M + math.sqrt(M * M - q * q)
+    assert o["cls"][0] == A.PRR_HORIZON
+    assert rp * (1 - 1e-9) < o["xend"][0][0] <= rp * (1 + 1e-3) + 1e-12
