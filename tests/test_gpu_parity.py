@@ -88,7 +88,9 @@ def test_trace_reissner_nordstrom_matches_oracle(solver):
     o = O.Scene(A.PRR_SCHW_SPH, 1.0, q, T).trace(x0)
     agree = g["cls"] == o["cls"]
     assert agree.mean() >= 0.99
-    assert (o["cls"] == A.PRR_HORIZON).sum() >= 10 and (o["cls"] == A.PRR_ESCAPE).sum() >= 10
+    assert (o["cls"] == A.PRR_HORIZON).sum() >= 10
+    if solver != A.PRR_EULER:  # (3000 Euler steps of 0.01 M do not reach r_escape = 60 M)
+        assert (o["cls"] == A.PRR_ESCAPE).sum() >= 10
     m = agree if solver == A.PRR_EULER else agree & (o["cls"] == A.PRR_ESCAPE)
     err = np.abs(g["xend"][m] - o["xend"][m]).max(axis=1) / (np.abs(o["xend"][m]).max(axis=1) + 1)
     assert np.median(err) < (1e-12 if solver == A.PRR_EULER else 1e-8)
