@@ -62,6 +62,9 @@ prr_status prr_debug_init_ray(int32_t metric, double M, double a, const double p
                               double ELQ[3]);
 /* Direction of ray `local` of a frame (createRays_* + rayDir_*). */
 prr_status prr_debug_camera_ray(const prr_frame *f, int64_t local, double d[3]);
+/* Ray ids of queue positions q[0..n) of one frame in the persistent
+ * integrator (tiles of pixels per wave; a permutation of the frame's ids). */
+prr_status prr_debug_queue_rays(const prr_frame *f, const int64_t *q, int64_t n, int64_t *ids);
 /* The device integrator's per-ray code run on the host for n rays: rays ids[i]
  * of frame f (x0u0 = NULL) or the initial states x0u0 (8 per ray), with the
  * kernels' tracer semantics.  Untextured (escape and sphere colours are

@@ -97,6 +97,7 @@ def lib():
     _sig(L, "prr_debug_xu_from_state", st, [_i32, _d, _d, _d, _d, _d, _vp, _vp, _vp])
     _sig(L, "prr_debug_init_ray", st, [_i32, _d, _d, _vp, _vp, _vp, _vp])
     _sig(L, "prr_debug_camera_ray", st, [C.POINTER(A.Frame), _i64, _vp])
+    _sig(L, "prr_debug_queue_rays", st, [C.POINTER(A.Frame), _vp, _i64, _vp])
     _sig(L, "prr_debug_trace_rays", st, [_i32, C.POINTER(A.Tracer), C.POINTER(A.Frame), _vp, _vp, C.c_int, _vp, _vp,
                                          _vp, _vp, _vp])
     _sig(L, "prr_debug_trace_log", st, [_i32, C.POINTER(A.Tracer), C.POINTER(A.Frame), _i64, C.c_int, _vp,

@@ -403,6 +403,14 @@ def trace_log(metric, tracer, frame, ray, max_n=100000):
     return log[:n.value]
 
 
+def queue_rays(frame, q):
+    """Ray ids of the integrator's queue positions q within one frame."""
+    q = np.ascontiguousarray(q, dtype=np.int64)
+    ids = np.empty_like(q)
+    check(lib().prr_debug_queue_rays(C.byref(frame), _p(q), q.size, _p(ids)))
+    return ids
+
+
 def camera_ray(frame, local):
     d = np.empty(3)
     check(lib().prr_debug_camera_ray(C.byref(frame), local, _p(d)))

@@ -237,6 +237,21 @@ prr_status prr_debug_init_ray(int32_t metric, double M, double a, const double p
     });
 }
 
+prr_status prr_debug_queue_rays(const prr_frame *f, const int64_t *q, int64_t n, int64_t *ids) {
+    if (!f || !q || !ids || n < 0) return PRR_EINVAL;
+    prr_tracer tr{};
+    const FrameDev fr = frame_dev(PRR_FLAT, tr, *f, 0);
+    JobDev job{};
+    job.frames = &fr;
+    job.nframes = 1;
+    job.total = int64_t(f->width) * f->height * f->samples;
+    for (int64_t i = 0; i < n; ++i) {
+        if (q[i] < 0 || q[i] >= job.total) return PRR_EINVAL;
+        ids[i] = queue_ray(job, q[i]);
+    }
+    return PRR_OK;
+}
+
 prr_status prr_debug_camera_ray(const prr_frame *f, int64_t local, double d[3]) {
     if (!f || !d) return PRR_EINVAL;
     prr_tracer tr{};

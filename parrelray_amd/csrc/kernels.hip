@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                     } else {
                         // a checkpoint pass over some of the rays takes them from adj.rays
                         const bool some = Mt::adj && job.adj.mode == ADJ_CHECKPOINT && job.adj.rays;
-                        lane_load<Mt>(job, L, some ? job.adj.rays[idx] : idx);
+                        lane_load<Mt>(job, L, some ? job.adj.rays[idx] : queue_ray(job, idx));
                         if (Mt::adj && job.adj.mode == ADJ_CHECKPOINT)
                             store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, 0);
                     }

@@ -151,6 +151,55 @@ PRR_HD int find_frame(const JobDev &job, int64_t id) {
     return lo;
 }
 
+// Queue position -> ray id.  The queue hands a wave 64 consecutive positions;
+// within each frame they map to a tile of pixels (all samples of each pixel)
+// rather than to a run of one image row, so that a wave's rays are close
+// together on the sky and follow similar paths.  Tiles hold 64 / s pixels
+// (s samples per pixel, a divisor of 64), Th x Tw with Th the largest power
+// of two whose square fits; the part of the image that whole tiles do not
+// cover (a right strip, then a bottom strip) keeps row order.  A bijection on
+// each frame's ids (j W + i) s + k; frames whose s does not divide 64 keep
+// the identity.
+PRR_HD int64_t queue_ray(const JobDev &job, int64_t q) {
+    if (job.explicit_x0u0) return q;
+    const FrameDev &fr = job.frames[find_frame(job, q)];
+    const int64_t s = fr.s;
+    if (s <= 0 || 64 % s != 0) return q;
+    const int64_t P = 64 / s;
+    int64_t th = 1;
+    while ((th * 2) * (th * 2) <= P) th *= 2;
+    const int64_t tw = P / th;
+    const int64_t W = fr.W, H = fr.H;
+    const int64_t wc = (W / tw) * tw, hc = (H / th) * th;
+    const int64_t local = q - fr.ray_begin;
+    const int64_t ncore = wc * hc * s;
+    int64_t i, j, k;
+    if (local < ncore) {
+        const int64_t per = P * s, t = local / per, w = local - t * per;
+        const int64_t tiles_x = wc / tw, ty = t / tiles_x, tx = t - ty * tiles_x;
+        const int64_t p = w / s;
+        k = w - p * s;
+        i = tx * tw + p % tw;
+        j = ty * th + p / tw;
+    } else {
+        int64_t r = local - ncore;
+        const int64_t wr = W - wc, nright = wr * hc * s;
+        if (r < nright) { // right strip: x >= wc, y < hc
+            const int64_t p = r / s;
+            k = r - p * s;
+            i = wc + p % wr;
+            j = p / wr;
+        } else { // bottom strip: y >= hc
+            r -= nright;
+            const int64_t p = r / s;
+            k = r - p * s;
+            i = p % W;
+            j = hc + p / W;
+        }
+    }
+    return fr.ray_begin + (j * W + i) * s + k;
+}
+
 template <class Mt> PRR_HD bool finite_state(const double *y) {
     bool ok = true;
 #pragma unroll

@@ -10,6 +10,8 @@ import math
 import os
 import shutil
 
+import numpy as np
+
 import pytest
 
 from parrelray_amd import _abi as A
@@ -179,3 +181,30 @@ def test_config_files_match_the_benchmark_workloads(name, monkeypatch):
     assert list(f.pos) == pytest.approx(list(ref_f.pos), rel=1e-15)
     for s, rs in zip(f.spheres[:f.n_spheres], ref_f.spheres[:ref_f.n_spheres]):
         assert list(s.c) == pytest.approx(list(rs.c), rel=1e-15) and s.radius == rs.radius
+
+
+@pytest.mark.parametrize("w,h,ssaa,s", [(37, 23, A.PRR_SSAA_NONE, 1), (64, 48, A.PRR_SSAA_REGULAR, 4),
+                                        (50, 7, A.PRR_SSAA_REGULAR, 16), (9, 5, A.PRR_SSAA_RANDOM, 2),
+                                        (13, 11, A.PRR_SSAA_REGULAR, 9)])
+def test_queue_order_is_a_permutation_of_tiles(w, h, ssaa, s):
+    """The integrator's queue visits every ray of a frame exactly once; each
+    wave's 64 positions cover a compact tile of whole pixels where 64 / s
+    pixels fit (s = 9 keeps row order)."""
+    f = A.make_frame([30, 0, 0], math.pi, 0, 1.0, w, h, ssaa=ssaa, samples=s)
+    n = w * h * s
+    ids = api.queue_rays(f, np.arange(n))
+    assert np.array_equal(np.sort(ids), np.arange(n))
+    if 64 % s:
+        assert np.array_equal(ids, np.arange(n))
+        return
+    pix = ids // s
+    px, py = pix % w, pix // w
+    for b in range(n // 64):
+        sl = slice(64 * b, 64 * b + 64)
+        # whole pixels: every sample of each pixel in the same wave
+        assert np.all(np.bincount(pix[sl] - pix[sl].min()) % s == 0)
+    # the first wave is a tile: spread in x and y both below 64 / s pixels (a row run's x spread)
+    first = slice(0, 64)
+    if n >= 64 and w >= 8 and h >= 8:
+        assert np.ptp(px[first]) < 64 // s or 64 // s == 1
+        assert np.ptp(py[first]) >= 1 or 64 // s == 1
