@@ -53,10 +53,11 @@ class Params:
         else:
             check(lib().prr_params_read(C.byref(h), path.encode()))
         self.h = h
+        self._free = lib().prr_params_free  # kept: module globals may be gone at interpreter exit
 
     def __del__(self):
         if getattr(self, "h", None):
-            lib().prr_params_free(self.h)
+            self._free(self.h)
             self.h = None
 
     def read_into(self, path):
@@ -109,11 +110,12 @@ class Context:
         h = C.c_void_p()
         check(lib().prr_create(C.byref(h), device))
         self.h = h
+        self._destroy = lib().prr_destroy  # kept: module globals may be gone at interpreter exit
         self.metric = A.PRR_SCHW_KS_CART
 
     def close(self):
         if getattr(self, "h", None):
-            lib().prr_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -255,13 +257,14 @@ class Graph:
         h = C.c_void_p()
         ctx._check(lib().prr_graph_create(ctx.h, self._frames, n, imgs, C.byref(h)))
         self.h = h
+        self._destroy = lib().prr_graph_destroy  # kept: module globals may be gone at interpreter exit
 
     def launch(self, stream=None):
         self.ctx._check(lib().prr_graph_launch(self.h, C.c_void_p(stream) if stream else None))
 
     def close(self):
         if getattr(self, "h", None):
-            lib().prr_graph_destroy(self.h)
+            self._destroy(self.h)
             self.h = None
 
     def __del__(self):
