@@ -178,6 +178,29 @@ def test_random_and_jittered_ssaa_match_oracle(ssaa, samples):
     assert not np.array_equal(other, out["rgb8"])
 
 
+@pytest.mark.parametrize("precision", [A.PRR_FP64, A.PRR_FP32_BULK])
+def test_c4_reduced_parity(precision):
+    """The precision-stress scene (Kerr a = 0.998, camera r = 6M edge-on at
+    89.9 deg, tol 1e-10, photon-ring rays of ~10^5 steps) at 96x96 without
+    SSAA, against the FP64 tol-1e-12 oracle: classes agree outside the
+    excused boundary rays; endpoint angle median < 1e-7 rad and 99th
+    percentile < 1e-5 rad; RGB8 within 1 LSB on >= 99.9 % of pixels."""
+    f = CF.frame_c4(96, 96, ssaa=1)
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_c4(precision=precision))
+    out = ctx.render(f, rays=True)[0]  # the frame carries the spin
+    S = oracle_scene(A.PRR_KERR_BL, CF.tracer_kerr(tol=1e-12, max_steps=400000))
+    orc = S.render_rays(f)
+    cmp = compare_rays(S, f, out, orc)
+    assert cmp["unexcused"] <= 2, cmp["unexcused_ids"][:20]
+    ang = cmp["angles"]
+    print("endpoint angle: median %.2e p99 %.2e max %.2e" % (np.median(ang), np.quantile(ang, 0.99), ang.max()))
+    assert np.median(ang) < 1e-7
+    assert np.quantile(ang, 0.99) < 1e-5
+    _, img, rgb8 = S.render_frame(f)
+    frac, _ = rgb8_agreement(out["rgb8"], rgb8)
+    assert frac >= 0.999
+
+
 @pytest.mark.parametrize("precision,solver", [(A.PRR_FP64, A.PRR_RKF45), (A.PRR_FP32_BULK, A.PRR_RKF45),
                                               (A.PRR_FP32_BULK, A.PRR_DORMAND)])
 def test_c1_reduced_parity(precision, solver):
