@@ -60,6 +60,7 @@ def run_form(ctx, frame, precision, steps, warmup, out8, outf, torch):
                 fxaa_ms=sum(s["fxaa_ms"] for s in per) / steps, rays=st["rays"], attempts=st["attempts"],
                 accepted=st["accepted"], attempts_fp64=st["attempts_fp64"], flops_fp32=f32, flops_fp64=f64,
                 rays_per_s=st["rays"] / (ms / 1e3), attempts_per_s=st["attempts"] / t_int,
+                accepted_per_s=st["accepted"] / t_int,
                 achieved_tflops=(f32 + f64) / t_int / 1e12,
                 roofline_frac=(f32 / P32 + f64 / P64) / t_int,
                 ssaa_gbs=st["ssaa_bytes"] / (sum(s["ssaa_ms"] for s in per) / steps / 1e3) / 1e9,
@@ -137,13 +138,17 @@ def run_copy(torch, nbytes=1 << 30):
 
 
 def run_batch(api, A, CF, reps, torch):
-    """C2: 64 frames (389,520,384 rays) in one persistent launch, replayed from a hipGraph."""
+    """C2: 64 frames (389,520,384 rays) in one persistent launch, replayed from a hipGraph, with the
+    launch shape (workgroup size, workgroups per CU, refill threshold) from the autotuner."""
+    from parrelray_amd import autotune
     frames = CF.frames_c2()
     ctx = api.Context(0)
     ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.0)
-    ctx.set_tracer(CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    ctx.set_tracer(ctx_tracer(A, CF))
     ctx.set_texture(0, CF.sky_texture())
     ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
+    # launch shape from the autotuner, on a quarter-resolution copy of the batch
+    shape = autotune.tune_launch(ctx, CF.frames_c2(scale=0.25), A.PRR_KERR_BL, ctx_tracer(A, CF), use_cache=False)
     rgb8 = [torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
     g = api.Graph(ctx, frames, rgb8=rgb8)
     g.launch()  # warm-up
@@ -156,7 +161,12 @@ def run_batch(api, A, CF, reps, torch):
     st = ctx.stats()
     g.close()
     return dict(frames=len(frames), rays=st["rays"], seconds=sec, rays_per_s=st["rays"] / sec,
-                attempts=st["attempts"], attempts_fp64=st["attempts_fp64"], graph_ms=st["total_ms"])
+                attempts=st["attempts"], attempts_fp64=st["attempts_fp64"], graph_ms=st["total_ms"], launch=shape)
+
+
+def ctx_tracer(A, CF):
+    """The C2 tracer: Kerr RKF45 tol 1e-8, FP32-bulk."""
+    return CF.tracer_kerr(precision=A.PRR_FP32_BULK)
 
 
 def cpu_baseline(frame, row_stride=16):
@@ -271,11 +281,13 @@ def main():
     }
     if fp64:
         line["fp64_plain"] = {k: fp64[k] for k in ("rays_per_s", "ms_per_step", "integrate_ms", "attempts",
-                                                   "attempts_per_s", "achieved_tflops", "roofline_frac")}
+                                                   "attempts_per_s", "accepted_per_s", "achieved_tflops",
+                                                   "roofline_frac")}
     if fp32:
         line["fp32_bulk_single"] = {k: fp32[k] for k in ("rays_per_s", "ms_per_step", "integrate_ms", "init_ms",
                                                          "shade_ms", "ssaa_ms", "fxaa_ms", "attempts", "accepted",
-                                                         "attempts_fp64", "attempts_per_s", "achieved_tflops",
+                                                         "attempts_fp64", "attempts_per_s", "accepted_per_s",
+                                                         "achieved_tflops",
                                                          "roofline_frac", "ssaa_gbs", "fxaa_gbs")}
     if bat:
         line["fp32_bulk_batched"] = bat
