@@ -16,7 +16,7 @@ capi.cpp (FP32 stage flops and FP64 flops kept apart); `achieved` divides by
 the kernel's duration measured with HIP events on its own stream.  frac =
 (F32/P32 + F64/P64) / t against MI355X's published vector peaks
 (157.3 TFLOP/s FP32, 78.6 TFLOP/s FP64); frac_nonpacked uses 78.6 TFLOP/s for
-FP32 as well, the rate of the non-packed FP32 instructions this build uses.
+FP32 as well, the rate of the non-packed FP32 instructions the kernels are compiled to.
 
 cpu_baseline: the CPU FP64 oracle (oracle/, a restatement of the reference
 solver) on a stratified row subset of the same frame, all host cores, OpenMP.
@@ -206,7 +206,7 @@ def main():
         if flag:
             legs.discard(leg)
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: the integrator has no multi-GPU path in this build "
+        print(f"bench.py measures one GPU: the integrator has no multi-GPU path "
               f"(--gpus {args.gpus} requested).", file=sys.stderr)
         sys.exit(2)
     import torch

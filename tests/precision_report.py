@@ -1,4 +1,4 @@
-"""Precision report (run on a GPU box; not collected by pytest).
+"""Precision report (needs a GPU; not collected by pytest).
 
 For each Kerr workload, renders the frame on the GPU in the plain FP64 form
 and in the FP32-bulk form, then integrates a seeded stratified subset of the
@@ -9,7 +9,7 @@ excused by the oracle's own perturbations), and the endpoint-angle error
 full-frame comparison of each form against the GPU's own FP64 tol-1e-12
 render.
 
-    python tests/precision_report.py [--out gpurun_out/precision.json] [--rays 4000] [--configs c1,c4,c4full,c2]
+    python tests/precision_report.py [--out precision.json] [--rays 4000] [--configs c1,c4,c4full,c2]
 
 c4 is a 512x512 2x2-SSAA crop of the precision-stress frame; c4full is the
 whole 2048x2048 4x4-SSAA frame (67,108,864 rays).
@@ -78,7 +78,7 @@ def run(name, frame, tol, nrays, seed, log):
 
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--out", default="gpurun_out/precision.json")
+    ap.add_argument("--out", default="precision.json")
     ap.add_argument("--rays", type=int, default=4000)
     ap.add_argument("--configs", default="c1,c4")
     args = ap.parse_args()
