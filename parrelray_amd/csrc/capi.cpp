@@ -81,6 +81,24 @@ prr_status fail(prr_ctx *ctx, prr_status st, const char *fmt, ...) {
     return st;
 }
 
+} // namespace
+
+// An exception that reached the C ABI (none may cross it): its status and
+// message, on the context when there is one.
+prr_status prr::abi_exception(prr_ctx *ctx) noexcept {
+    try {
+        throw;
+    } catch (const std::bad_alloc &) {
+        return fail(ctx, PRR_ENOMEM, "out of host memory");
+    } catch (const std::exception &e) {
+        return fail(ctx, PRR_EINVAL, "%s", e.what());
+    } catch (...) {
+        return fail(ctx, PRR_EINVAL, "unexpected exception");
+    }
+}
+
+namespace {
+
 #define HIPTRY(ctx, call)                                                                                       \
     do {                                                                                                        \
         hipError_t e_ = (call);                                                                                 \
@@ -234,7 +252,7 @@ int prr_abi_version(void) { return PRR_ABI_VERSION; }
 const char *prr_global_error(void) { return g_error.c_str(); }
 const char *prr_last_error(const prr_ctx *ctx) { return ctx ? ctx->err.c_str() : g_error.c_str(); }
 
-prr_status prr_create(prr_ctx **out, int hip_device) {
+prr_status prr_create(prr_ctx **out, int hip_device) try {
     if (!out) return fail(nullptr, PRR_EINVAL, "prr_create: out is NULL");
     *out = nullptr;
     int n = 0;
@@ -264,6 +282,8 @@ prr_status prr_create(prr_ctx **out, int hip_device) {
     }
     *out = ctx;
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 void prr_destroy(prr_ctx *ctx) {
@@ -327,7 +347,7 @@ prr_status prr_get_launch(const prr_ctx *ctx, prr_launch *l) {
     return PRR_OK;
 }
 
-prr_status prr_set_texture(prr_ctx *ctx, int slot, const uint8_t *rgb8, int w, int h, int32_t filter) {
+prr_status prr_set_texture(prr_ctx *ctx, int slot, const uint8_t *rgb8, int w, int h, int32_t filter) try {
     if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
     if (slot < 0 || slot >= PRR_MAX_TEXTURES) return fail(ctx, PRR_EINVAL, "texture slot %d out of range", slot);
     if (filter != PRR_FILTER_SQAVG && filter != PRR_FILTER_BILINEAR)
@@ -358,15 +378,19 @@ prr_status prr_set_texture(prr_ctx *ctx, int slot, const uint8_t *rgb8, int w, i
     t.h = h;
     t.filter = filter;
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(ctx);
 }
 
-prr_status prr_set_spheres(prr_ctx *ctx, const prr_sphere *s, int n) {
+prr_status prr_set_spheres(prr_ctx *ctx, const prr_sphere *s, int n) try {
     if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
     if (n < 0 || n > PRR_MAX_SPHERES || (n > 0 && !s)) return fail(ctx, PRR_EINVAL, "bad sphere list");
     for (int k = 0; k < n; ++k)
         if (!(s[k].radius > 0.0)) return fail(ctx, PRR_EINVAL, "sphere %d radius must be positive", k);
     ctx->spheres.assign(s, s + n);
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(ctx);
 }
 
 prr_status prr_get_stats(const prr_ctx *ctx, prr_stats *out) {
@@ -549,7 +573,7 @@ prr_status read_stats(prr_ctx *ctx, const RenderPlan &P, hipStream_t st, bool ev
 extern "C" {
 
 prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *images, prr_ray_out *rays,
-                      void *stream) {
+                      void *stream) try {
     if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
     if (!frames || n <= 0) return fail(ctx, PRR_EINVAL, "no frames");
     HIPTRY(ctx, hipSetDevice(ctx->device));
@@ -620,10 +644,12 @@ prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *i
     }
     HIPTRY(ctx, hipStreamSynchronize(st));
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(ctx);
 }
 
 prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int32_t *cls, int32_t *steps,
-                     int32_t *attempts) {
+                     int32_t *attempts) try {
     if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
     if (!x0u0 || n <= 0) return fail(ctx, PRR_EINVAL, "no initial states");
     HIPTRY(ctx, hipSetDevice(ctx->device));
@@ -666,10 +692,12 @@ prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int3
     if (cls)
         for (int i = 0; i < n; ++i) cls[i] = c8[i];
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(ctx);
 }
 
 // ---------------------------------------------------------------- graphs
-prr_status prr_graph_create(prr_ctx *ctx, const prr_frame *frames, int n, const prr_image *images, prr_graph **out) {
+prr_status prr_graph_create(prr_ctx *ctx, const prr_frame *frames, int n, const prr_image *images, prr_graph **out) try {
     if (!ctx || !out) return fail(ctx, PRR_EINVAL, "NULL argument");
     *out = nullptr;
     if (!frames || n <= 0) return fail(ctx, PRR_EINVAL, "no frames");
@@ -710,6 +738,8 @@ prr_status prr_graph_create(prr_ctx *ctx, const prr_frame *frames, int n, const 
     }
     *out = g;
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(ctx);
 }
 
 prr_status prr_graph_launch(prr_graph *g, void *stream) {
@@ -772,7 +802,7 @@ prr_status prr_frame_from_orbit(const prr_frame *tmpl, const prr_orbit *o, prr_f
 }
 
 prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orbit, const float *dL_dimg,
-                       int32_t on_device, double grad[4], float *img_out) {
+                       int32_t on_device, double grad[4], float *img_out) try {
     if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
     if (!tmpl || !orbit || !dL_dimg || !grad) return fail(ctx, PRR_EINVAL, "NULL argument");
     if (ctx->metric != PRR_KERR_BL) return fail(ctx, PRR_EINVAL, "the adjoint supports the Kerr metric only");
@@ -935,6 +965,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         return fail(ctx, PRR_EHIP, "adjoint replay did not reproduce the forward pass (%llu segments or rays)",
                     c.mismatch);
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(ctx);
 }
 
 } // extern "C"

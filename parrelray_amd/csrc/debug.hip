@@ -265,7 +265,7 @@ prr_status prr_debug_camera_ray(const prr_frame *f, int64_t local, double d[3]) 
 
 prr_status prr_debug_trace_rays(int32_t metric, const prr_tracer *tracer, const prr_frame *f, const double *x0u0,
                                 const int64_t *ids, int n, int32_t *cls, double *endpoint, int32_t *steps,
-                                int32_t *attempts, double *state8) {
+                                int32_t *attempts, double *state8) try {
     if (!tracer || !f || n < 0 || (!x0u0 && !ids)) return PRR_EINVAL;
     const bool adaptive = tracer->solver == PRR_RKF45 || tracer->solver == PRR_DORMAND;
     if ((adaptive && !(tracer->hmin > 0.0)) || (tracer->solver != PRR_EULER && tracer->max_attempts > 100000000LL))
@@ -302,10 +302,12 @@ prr_status prr_debug_trace_rays(int32_t metric, const prr_tracer *tracer, const 
         }
         return PRR_OK;
     });
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 prr_status prr_debug_trace_log(int32_t metric, const prr_tracer *tracer, const prr_frame *f, int64_t id, int max_n,
-                               double *log, int *n_out) {
+                               double *log, int *n_out) try {
     if (!tracer || !f || !log || !n_out || max_n <= 0) return PRR_EINVAL;
     const FrameDev fr = frame_dev(metric, *tracer, *f, 0);
     JobDev job{};
@@ -319,6 +321,8 @@ prr_status prr_debug_trace_log(int32_t metric, const prr_tracer *tracer, const p
         using Mt = decltype(mt);
         return host_ray<Mt>(job, *tracer, id, log, max_n, n_out);
     });
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 } // extern "C"

@@ -7,6 +7,7 @@
 #include <cstdio>
 #include <cstring>
 #include <fstream>
+#include <memory>
 #include <stdexcept>
 #include <string>
 #include <vector>
@@ -233,25 +234,26 @@ int metric_from_name(const std::string &s) {
 
 extern "C" {
 
-prr_status prr_params_read(prr_params **out, const char *path) {
+prr_status prr_params_read(prr_params **out, const char *path) try {
     if (!out || !path) return perr(PRR_EINVAL, "prr_params_read: NULL argument");
     *out = nullptr;
-    prr_params *p = new prr_params();
-    if (!prr::read_params(path, p->m)) {
-        delete p;
-        return perr(PRR_EIO, std::string("cannot open ") + path);
-    }
-    *out = p;
+    std::unique_ptr<prr_params> p(new prr_params());
+    if (!prr::read_params(path, p->m)) return perr(PRR_EIO, std::string("cannot open ") + path);
+    *out = p.release();
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
-prr_status prr_params_read_into(prr_params *p, const char *path) {
+prr_status prr_params_read_into(prr_params *p, const char *path) try {
     if (!p || !path) return perr(PRR_EINVAL, "prr_params_read_into: NULL argument");
     if (!prr::read_params(path, p->m)) return perr(PRR_EIO, std::string("cannot open ") + path);
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
-prr_status prr_params_load_sim(prr_params **out, const char *sim_path) {
+prr_status prr_params_load_sim(prr_params **out, const char *sim_path) try {
     prr_status s = prr_params_read(out, sim_path);
     if (s != PRR_OK) return s;
     // init (prr.cpp:207-210): traj and scene paths come from [tracer]
@@ -265,17 +267,23 @@ prr_status prr_params_load_sim(prr_params **out, const char *sim_path) {
         }
     }
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 void prr_params_free(prr_params *p) { delete p; }
 
-int prr_params_header_exists(const prr_params *p, const char *header) {
+int prr_params_header_exists(const prr_params *p, const char *header) try {
     return p && header && header_exists(p, header);
+} catch (...) {
+    return (prr::abi_exception(nullptr), 0);
 }
-int prr_params_par_exists(const prr_params *p, const char *header, const char *key) {
+int prr_params_par_exists(const prr_params *p, const char *header, const char *key) try {
     return p && header && key && par_exists(p, header, key);
+} catch (...) {
+    return (prr::abi_exception(nullptr), 0);
 }
-int prr_params_par(const prr_params *p, const char *header, const char *key, char *buf, int buflen) {
+int prr_params_par(const prr_params *p, const char *header, const char *key, char *buf, int buflen) try {
     const std::string v = (p && header && key) ? par(p, header, key) : std::string();
     if (buf && buflen > 0) {
         const size_t n = std::min<size_t>(v.size(), size_t(buflen - 1));
@@ -283,16 +291,22 @@ int prr_params_par(const prr_params *p, const char *header, const char *key, cha
         buf[n] = 0;
     }
     return int(v.size());
+} catch (...) {
+    return (prr::abi_exception(nullptr), 0);
 }
-int prr_params_pari(const prr_params *p, const char *header, const char *key) {
+int prr_params_pari(const prr_params *p, const char *header, const char *key) try {
     return (p && header && key) ? pari(p, header, key) : 0;
+} catch (...) {
+    return (prr::abi_exception(nullptr), 0);
 }
-double prr_params_parf(const prr_params *p, const char *header, const char *key, double dflt) {
+double prr_params_parf(const prr_params *p, const char *header, const char *key, double dflt) try {
     return (p && header && key) ? parf(p, header, key, dflt) : dflt;
+} catch (...) {
+    return (prr::abi_exception(nullptr), 0.0);
 }
 
 // checkFrame (prr.cpp:139-153)
-int prr_params_frame_count(const prr_params *p) {
+int prr_params_frame_count(const prr_params *p) try {
     if (!p) return 0;
     for (int i = 0; i < 100000; ++i) {
         const std::string h = "frame" + std::to_string(i);
@@ -300,6 +314,8 @@ int prr_params_frame_count(const prr_params *p) {
             if (!par_exists(p, h, k)) return i;
     }
     return 0;
+} catch (...) {
+    return (prr::abi_exception(nullptr), 0);
 }
 
 // createScene's spacetime part (prr.cpp:163-181).  The metric, fixed at
@@ -307,7 +323,7 @@ int prr_params_frame_count(const prr_params *p) {
 // [prr] block's `metric` key (flat | schwarzschild | schwarzschild-cart | schwarzschild-sph | kerr;
 // no underscores: an underscore anywhere makes a .params line a header).
 prr_status prr_params_spacetime(const prr_params *p, int32_t *metric, double *M, double *a, double *q,
-                                double pvals[16]) {
+                                double pvals[16]) try {
     if (!p) return perr(PRR_EINVAL, "NULL params");
     const int mt = metric_from_name(par(p, "prr", "metric"));
     if (mt < 0) return perr(PRR_EINVAL, "unknown metric '" + par(p, "prr", "metric") + "'");
@@ -319,12 +335,14 @@ prr_status prr_params_spacetime(const prr_params *p, int32_t *metric, double *M,
     if (pvals)
         for (int i = 0; i < 16; ++i) pvals[i] = parf(p, "spacetime", "p" + std::to_string(i), 0.0);
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 // createScene's tracer part (prr.cpp:184-190) with Parameters::getSolver
 // (main/Parameters.h:18-22: unknown names fall back to euler).  Keys the
 // reference does not have live in the [prr] block.
-prr_status prr_params_tracer(const prr_params *p, prr_tracer *t) {
+prr_status prr_params_tracer(const prr_params *p, prr_tracer *t) try {
     if (!p || !t) return perr(PRR_EINVAL, "NULL argument");
     const std::string sname = par(p, "tracer", "solver");
     static const char *known[] = {"euler", "rk", "rkf", "adams", "gragg", "dormand", "taylor"};
@@ -355,13 +373,15 @@ prr_status prr_params_tracer(const prr_params *p, prr_tracer *t) {
     t->max_attempts = int64_t(parf(p, "prr", "max-attempts", 1e6));
     t->r_fp64 = parf(p, "prr", "r-fp64", 3.0);
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 // Frame i: renderframe's pose (prr.cpp:245-253), Camera (prr.cpp:194-195)
 // and Spacetime::updateScene (main/Spacetime.cpp:38-45).  Per-frame sphere
 // keys and the frame mass are sticky, as the reference updates its objects
 // in place frame after frame.
-prr_status prr_params_frame(const prr_params *p, int i, prr_frame *f) {
+prr_status prr_params_frame(const prr_params *p, int i, prr_frame *f) try {
     if (!p || !f) return perr(PRR_EINVAL, "NULL argument");
     if (i < 0 || i >= prr_params_frame_count(p)) return perr(PRR_EINVAL, "frame " + std::to_string(i) + " missing");
     int32_t metric;
@@ -422,6 +442,8 @@ prr_status prr_params_frame(const prr_params *p, int i, prr_frame *f) {
         f->spheres[k].tex = k + 1;
     }
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 // ---------------------------------------------------------------- textures
@@ -444,7 +466,7 @@ static void hsv_to_rgb(double h, double s, double v, double rgb[3]) {
 
 static uint8_t q8(double c) { return uint8_t(std::lround(std::min(std::max(c, 0.0), 1.0) * 255.0)); }
 
-prr_status prr_texture_checker(uint8_t *rgb8, int w, int h, int cw, int ch, uint64_t seed) {
+prr_status prr_texture_checker(uint8_t *rgb8, int w, int h, int cw, int ch, uint64_t seed) try {
     if (!rgb8 || w <= 0 || h <= 0 || cw <= 0 || ch <= 0 || w % cw || h % ch)
         return perr(PRR_EINVAL, "checker: size must be a multiple of the cell grid");
     const int sx = w / cw, sy = h / ch;
@@ -458,12 +480,14 @@ prr_status prr_texture_checker(uint8_t *rgb8, int w, int h, int cw, int ch, uint
             for (int c = 0; c < 3; ++c) rgb8[3 * (size_t(y) * w + x) + c] = q8(rgb[c]);
         }
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 // 4-octave value noise: octave o has an 8*2^o x (4*2^o + 1) lattice (wrapped
 // in x), lattice values hashed from (seed, octave, i, j, channel), smoothstep
 // interpolation, amplitude 0.5^o.
-prr_status prr_texture_value_noise(uint8_t *rgb8, int w, int h, int octaves, uint64_t seed) {
+prr_status prr_texture_value_noise(uint8_t *rgb8, int w, int h, int octaves, uint64_t seed) try {
     if (!rgb8 || w <= 0 || h <= 0 || octaves <= 0 || octaves > 12) return perr(PRR_EINVAL, "noise: bad arguments");
     auto lattice = [&](int o, int i, int j, int c) {
         uint64_t k = seed;
@@ -493,6 +517,8 @@ prr_status prr_texture_value_noise(uint8_t *rgb8, int w, int h, int octaves, uin
                 rgb8[3 * (size_t(y) * w + x) + c] = q8(v / norm);
             }
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }
 
 } // extern "C"

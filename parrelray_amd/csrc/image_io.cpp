@@ -366,7 +366,7 @@ bool decode_jpeg(const std::vector<uint8_t> &f, std::vector<uint8_t> &rgb, int &
         if (m == 0xDB) { // DQT
             for (int i = 0; i < n;) {
                 const int pq = d[i] >> 4, tq = d[i] & 15;
-                if (tq > 3) return err = "bad DQT", false;
+                if (tq > 3 || i + 1 + (pq ? 128 : 64) > n) return err = "bad DQT", false;
                 ++i;
                 for (int k = 0; k < 64; ++k) {
                     qt[tq][k] = pq ? uint16_t(be16(&d[i + 2 * k])) : d[i + k];
@@ -385,12 +385,15 @@ bool decode_jpeg(const std::vector<uint8_t> &f, std::vector<uint8_t> &rgb, int &
                 i += 17 + nsym;
             }
         } else if (m == 0xDD) { // DRI
+            if (n < 2) return err = "bad DRI", false;
             restart = be16(d);
         } else if (m == 0xC0 || m == 0xC1) { // baseline / extended sequential, Huffman
+            if (n < 6) return err = "bad JPEG frame header", false;
             if (d[0] != 8) return err = "only 8-bit JPEG is supported", false;
             H = be16(d + 1);
             W = be16(d + 3);
             const int nc = d[5];
+            if (n < 6 + 3 * nc) return err = "bad JPEG frame header", false;
             if (W <= 0 || H <= 0 || !(nc == 1 || nc == 3)) return err = "JPEG with 1 or 3 components expected", false;
             comp.resize(size_t(nc));
             for (int c = 0; c < nc; ++c) {
@@ -415,7 +418,8 @@ bool decode_jpeg(const std::vector<uint8_t> &f, std::vector<uint8_t> &rgb, int &
                    false;
         } else if (m == 0xDA) { // SOS: entropy-coded data follows the header
             if (!frame) return err = "JPEG scan before frame header", false;
-            const int ns = d[0];
+            const int ns = n >= 1 ? d[0] : 0;
+            if (ns < 1 || ns > 4 || n < 1 + 2 * ns + 3) return err = "bad JPEG scan header", false;
             std::vector<Component *> sc;
             for (int i = 0; i < ns; ++i) {
                 Component *c = nullptr;
@@ -512,7 +516,7 @@ bool read_image(const std::string &path, std::vector<uint8_t> &rgb, int &w, int 
 
 } // namespace prr
 
-extern "C" prr_status prr_image_read(const char *path, uint8_t *rgb8, int *w, int *h) {
+extern "C" prr_status prr_image_read(const char *path, uint8_t *rgb8, int *w, int *h) try {
     if (!path || !w || !h) {
         prr::set_global_error("prr_image_read: NULL argument");
         return PRR_EINVAL;
@@ -535,4 +539,6 @@ extern "C" prr_status prr_image_read(const char *path, uint8_t *rgb8, int *w, in
     *w = iw;
     *h = ih;
     return PRR_OK;
+} catch (...) {
+    return prr::abi_exception(nullptr);
 }

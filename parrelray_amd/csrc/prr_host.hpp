@@ -22,6 +22,11 @@ bool read_params(const std::string &path, ParamMap &m);
 // Message returned by prr_global_error() (defined in capi.cpp).
 void set_global_error(const std::string &msg);
 
+// Inside a catch handler of a C-ABI entry point: the status of the exception
+// being handled (PRR_ENOMEM for std::bad_alloc), with its message on ctx (or
+// the global message when ctx is null).  Defined in capi.cpp.
+prr_status abi_exception(prr_ctx *ctx) noexcept;
+
 // Minimal PNG encoder (stored deflate blocks) for RGB8 images.
 bool write_png(const std::string &path, const uint8_t *rgb, int w, int h);
 // Binary PPM (P6, maxval 255) reader.

@@ -146,3 +146,26 @@ def test_progressive_jpeg_and_missing_files_are_refused(tmp_path):
     (tmp_path / "x.txt").write_text("hello")
     with pytest.raises(PrrError, match="not a PPM"):
         api.read_image(tmp_path / "x.txt")
+
+
+@pytest.mark.parametrize("fmt", ["jpg", "png"])
+def test_truncated_and_corrupted_files_fail_cleanly(tmp_path, fmt):
+    """Damaged files give an error status (or, for a cut inside padding, an
+    image): the decoder never reads past the file."""
+    Image.fromarray(smooth_image(40, 50)).save(tmp_path / f"a.{fmt}", **({"quality": 90} if fmt == "jpg" else {}))
+    data = (tmp_path / f"a.{fmt}").read_bytes()
+    rng = np.random.default_rng(1)
+    cases = [data[:n] for n in (3, 10, 40, 100, 200, len(data) // 2, len(data) - 3)]
+    for _ in range(20):  # random byte flips after the signature
+        b = bytearray(data)
+        for i in rng.integers(8, len(b), size=4):
+            b[i] = int(rng.integers(0, 256))
+        cases.append(bytes(b))
+    for k, blob in enumerate(cases):
+        p = tmp_path / f"bad{k}.{fmt}"
+        p.write_bytes(blob)
+        try:
+            img = api.read_image(p)
+            assert img.ndim == 3 and img.shape[2] == 3
+        except PrrError:
+            pass
