@@ -31,7 +31,8 @@ extern "C" {
 
 #define PRR_ABI_VERSION 2
 #define PRR_MAX_SPHERES 8
-#define PRR_MAX_TEXTURES 9 /* slot 0 = celestial sphere, slots 1..8 = sphere textures */
+#define PRR_MAX_TEXTURES 10 /* slot 0 = celestial sphere, slots 1..8 = sphere textures, 9 = second sky */
+#define PRR_SKY2_SLOT 9     /* celestial sphere seen through a wormhole's throat (bimage2) */
 
 typedef struct prr_ctx prr_ctx;
 typedef struct prr_params prr_params;
@@ -53,7 +54,10 @@ typedef enum prr_metric {
     PRR_FLAT = 0,         /* Minkowski, Cartesian (x, y, z, t) */
     PRR_SCHW_KS_CART = 1, /* Schwarzschild, Kerr-Schild Cartesian (x, y, z, t) */
     PRR_SCHW_SPH = 2,     /* Schwarzschild, Schwarzschild coordinates (r, theta, phi, t) */
-    PRR_KERR_BL = 3       /* Kerr, Boyer-Lindquist (r, theta, phi, t) */
+    PRR_KERR_BL = 3,      /* Kerr, Boyer-Lindquist (r, theta, phi, t) */
+    PRR_ELLIS = 4         /* Ellis wormhole (l, theta, phi, t), throat radius p[0], escape radius
+                           * beyond the throat p[1] (0: r_escape; rout2): reaches the reference's
+                           * second sky (bimage2, "l < 0 if wormhole", main/base.h:168) */
 } prr_metric;
 
 /* Solvers; values follow the reference's SOLVER enum (main/base.h:26-34),
@@ -218,7 +222,8 @@ int prr_abi_version(void);
 
 /* Spacetime::setBhMass/Spin/Charge/setBhP (main/Spacetime.h:21-26) plus the
  * compile-time metric choice (main/metric.h:12).  M and a are used by
- * prr_trace; frames carry their own mass and spin. */
+ * prr_trace; frames carry their own mass and spin.  q: the charge of
+ * PRR_SCHW_SPH; p: PRR_ELLIS's throat and far-side escape radii (p[0], p[1]). */
 prr_status prr_set_spacetime(prr_ctx *ctx, int32_t metric, double M, double a, double q, const double p[16]);
 /* Tracer fields (prr.cpp:184-190). */
 prr_status prr_set_tracer(prr_ctx *ctx, const prr_tracer *tracer);

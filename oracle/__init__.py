@@ -29,6 +29,7 @@ def lib():
         L.orc_scene_new.restype = C.c_void_p
         L.orc_scene_new.argtypes = [C.c_int, C.c_double, C.c_double, C.POINTER(A.Tracer)]
         L.orc_scene_free.argtypes = [C.c_void_p]
+        L.orc_scene_set_far_escape.argtypes = [C.c_void_p, C.c_double]
         L.orc_scene_set_tracer.argtypes = [C.c_void_p, C.POINTER(A.Tracer)]
         L.orc_scene_set_texture.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_scene_set_spheres.argtypes = [C.c_void_p, C.POINTER(A.Sphere), C.c_int]
@@ -84,6 +85,10 @@ class Scene:
         if getattr(self, "h", None) and _lib is not None:
             _lib.orc_scene_free(self.h)
             self.h = None
+
+    def set_far_escape(self, r):
+        """Escape radius beyond a wormhole's throat (rout2); 0: the tracer's r_escape."""
+        lib().orc_scene_set_far_escape(self.h, r)
 
     def set_tracer(self, tracer):
         self.tracer = tracer

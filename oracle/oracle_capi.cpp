@@ -26,6 +26,8 @@ void *orc_scene_new(int metric, double M, double a, const prr_tracer *tr) {
 
 void orc_scene_free(void *s) { delete static_cast<Scene *>(s); }
 
+void orc_scene_set_far_escape(void *s, double r) { static_cast<Scene *>(s)->r_esc2 = r; }
+
 void orc_scene_set_tracer(void *s, const prr_tracer *tr) { static_cast<Scene *>(s)->tr = *tr; }
 
 void orc_scene_set_texture(void *s, int slot, const uint8_t *rgb8, int w, int h, int filter) {

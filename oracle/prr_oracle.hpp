@@ -38,6 +38,11 @@
 //   escape / sphere    closed-form line intersections in flat space and an
 //   location (D21/22)  independent scipy event search on the Schwarzschild
 //                      right-hand side (test_oracle_events).
+//   Ellis wormhole     finite-difference Christoffels of metric_ellis, the
+//                      null condition and the conserved b_imp = R^2 phi'
+//                      along rays, and the closed-form criterion for crossing
+//                      the throat (impact parameter below the throat radius
+//                      b, test_oracle_wormhole).
 //
 // Defect resolutions (reference file:line -> what is done here):
 //   D2  fx undefined (kernels/solver.cuh:163-168): x' = u.
@@ -99,7 +104,12 @@ struct RGB {
 // ------------------------------------------------------------------ metrics
 // Component order (x^1, x^2, x^3, t) as the reference's f4 (x, y, z, w);
 // g is row-major 4x4 (mat4, cudamaths/mat4.cuh:35).
-inline bool spherical_coords(int metric) { return metric == PRR_SCHW_SPH || metric == PRR_KERR_BL; }
+inline bool spherical_coords(int metric) {
+    return metric == PRR_SCHW_SPH || metric == PRR_KERR_BL || metric == PRR_ELLIS;
+}
+// Metrics whose state carries the reference's spherical-metric wraps and
+// position-based sky lookup ("metrics/schwarzschild - sph.cuh":69-78).
+inline bool sph_chart(int metric) { return metric == PRR_SCHW_SPH || metric == PRR_ELLIS; }
 
 // metrics/flat.cuh:94-99
 inline void metric_flat(double g[16]) {
@@ -133,6 +143,18 @@ inline void metric_sph(const double x[4], double M, double q, double g[16]) {
     g[15] = -f;
 }
 
+// Ellis wormhole (Ellis 1973; Morris and Thorne 1988) in (l, theta, phi, t):
+// ds^2 = -dt^2 + dl^2 + (b^2 + l^2) (dtheta^2 + sin^2 theta dphi^2).  No
+// reference metric; it reaches the reference's second sky (main/base.h:168).
+inline void metric_ellis(const double x[4], double b, double g[16]) {
+    const double R2 = b * b + x[0] * x[0], s = std::sin(x[1]);
+    std::fill(g, g + 16, 0.0);
+    g[0] = 1.0;
+    g[5] = R2;
+    g[10] = R2 * s * s;
+    g[15] = -1.0;
+}
+
 // metrics/kerr.cuh:77-100 (Boyer-Lindquist, order r, theta, phi, t)
 inline void metric_kerr(const double x[4], double M, double a, double g[16]) {
     const double rs = 2.0 * M, r = x[0], s = std::sin(x[1]), c = std::cos(x[1]);
@@ -150,6 +172,7 @@ inline void metric(int kind, double M, double a, const double x[4], double g[16]
     case PRR_SCHW_KS_CART: metric_ks(x, M, g); break;
     case PRR_SCHW_SPH: metric_sph(x, M, a, g); break; // a: the charge q
     case PRR_KERR_BL: metric_kerr(x, M, a, g); break;
+    case PRR_ELLIS: metric_ellis(x, a, g); break; // a: the throat radius b
     default: metric_flat(g); break;
     }
 }
@@ -257,6 +280,25 @@ inline void geod_sph(const double x[4], const double u[4], double M, double q, d
     du[3] = -rs / (r * r - r * rs) * u[0] * u[3];
 }
 
+// Geodesic equation of metric_ellis: with R^2 = b^2 + l^2 the non-zero
+// symbols are Gamma^l_thth = -l, Gamma^l_phph = -l sin^2 th,
+// Gamma^th_lth = Gamma^ph_lph = l / R^2, Gamma^th_phph = -sin th cos th,
+// Gamma^ph_thph = cot th; tan and sin are clamped to magnitude 1e-4 near the
+// axis with their signs kept (rays cross the axis, axis_post_step).
+inline void geod_ellis(const double x[4], const double u[4], double b, double du[4]) {
+    const double l = x[0];
+    double tanth = std::tan(x[1]), sinth = std::sin(x[1]);
+    if (std::fabs(tanth) < 1e-4) {
+        tanth = std::copysign(1e-4, tanth);
+        sinth = std::copysign(1e-4, sinth);
+    }
+    const double k = l / (b * b + l * l);
+    du[0] = l * (u[1] * u[1] + sinth * sinth * u[2] * u[2]);
+    du[1] = -2.0 * k * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
+    du[2] = -2.0 * k * u[0] * u[2] - 2.0 / tanth * u[1] * u[2];
+    du[3] = 0.0;
+}
+
 inline void geod_kerr(const double x[4], const double u[4], double M, double a, double du[4]) {
     double G[4][4][4];
     christoffel_kerr(x, M, a, G);
@@ -274,19 +316,22 @@ inline void geod(int kind, double M, double a, const double x[4], const double u
     case PRR_SCHW_KS_CART: geod_ks(x, u, M, du); break;
     case PRR_SCHW_SPH: geod_sph(x, u, M, a, du); break; // a: the charge q
     case PRR_KERR_BL: geod_kerr(x, u, M, a, du); break;
+    case PRR_ELLIS: geod_ellis(x, u, a, du); break; // a: the throat radius b
     default: du[0] = du[1] = du[2] = du[3] = 0.0; break; // metrics/flat.cuh:88-90
     }
 }
 
 // Outer horizon: Kerr with spin a, Reissner-Nordstrom with charge a.
+inline bool has_horizon(int kind) { return kind != PRR_FLAT && kind != PRR_ELLIS; }
 inline double horizon_radius(int kind, double M, double a) {
-    if (kind == PRR_FLAT) return 0.0;
+    if (!has_horizon(kind)) return 0.0;
     if (kind == PRR_KERR_BL || kind == PRR_SCHW_SPH) return M + std::sqrt(std::max(M * M - a * a, 0.0));
     return 2.0 * M;
 }
 
 // Position radius and Cartesian position of a state.
 inline double radius_of(int kind, const double x[4]) {
+    if (kind == PRR_ELLIS) return std::fabs(x[0]); // distance from the throat, either side
     return spherical_coords(kind) ? x[0] : std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]);
 }
 inline V3 cart_of(int kind, const double x[4]) {
@@ -398,6 +443,10 @@ inline void init_ray(int kind, double M, double a, V3 pos, V3 d, double x[4], do
         cart_to_sph(pos, d, xs, us);
         x[0] = xs[0]; x[1] = xs[1]; x[2] = xs[2];
         u[0] = us[0]; u[1] = us[1]; u[2] = us[2];
+        if (kind == PRR_ELLIS) { // areal radius r > b on the l > 0 side: l = sqrt(r^2 - b^2), dl = r dr / l
+            x[0] = std::sqrt(std::max(xs[0] * xs[0] - a * a, 0.0));
+            u[0] = us[0] * xs[0] / x[0];
+        }
     } else {
         x[0] = pos.x; x[1] = pos.y; x[2] = pos.z;
         u[0] = d.x; u[1] = d.y; u[2] = d.z;
@@ -497,10 +546,17 @@ inline RGB sphere_color(const Texture *t, V3 n) {
 struct Scene {
     int metric = PRR_SCHW_KS_CART;
     double M = 1.0, a = 0.0;
+    double r_esc2 = 0.0; // escape radius beyond a wormhole's throat (rout2); <= 0: tr.r_escape
     prr_tracer tr{};
     std::vector<prr_sphere> spheres;
     Texture tex[PRR_MAX_TEXTURES];
 };
+
+// Escape radius on the state's side of a wormhole's throat: the reference's
+// sph1.rout and sph2.rout (main/base.h:166-170, prr.cpp:185-190).
+inline double escape_radius(const Scene &S, const double x[4]) {
+    return (S.metric == PRR_ELLIS && x[0] < 0.0 && S.r_esc2 > 0.0) ? S.r_esc2 : S.tr.r_escape;
+}
 
 struct RayResult {
     int cls = PRR_STEP_CAP;
@@ -553,6 +609,29 @@ inline void sph_post_step(double x[4], double u[4]) {
     if (x[1] > M_PI && x[1] <= 2.0 * M_PI) x[1] = 2.0 * M_PI - x[1];
 }
 
+// The Ellis wormhole's chart map across the polar axis (no reference
+// counterpart): theta < 0 or > pi is reflected with phi + pi and the sign of
+// u^theta flipped, which names the same point and direction.
+inline void axis_post_step(double x[4], double u[4]) {
+    if (x[1] < 0.0) {
+        x[1] = -x[1];
+        x[2] += M_PI;
+        u[1] = -u[1];
+    } else if (x[1] > M_PI) {
+        x[1] = 2.0 * M_PI - x[1];
+        x[2] += M_PI;
+        u[1] = -u[1];
+    }
+    x[2] = std::fmod(x[2], 2.0 * M_PI);
+    if (x[2] < 0.0) x[2] += 2.0 * M_PI;
+    if (x[2] >= 2.0 * M_PI) x[2] = 0.0;
+}
+
+inline void chart_post_step(int metric, double x[4], double u[4]) {
+    if (metric == PRR_SCHW_SPH) sph_post_step(x, u);
+    if (metric == PRR_ELLIS) axis_post_step(x, u);
+}
+
 inline bool finite8(const double x[4], const double u[4]) {
     for (int i = 0; i < 4; ++i)
         if (!std::isfinite(x[i]) || !std::isfinite(u[i])) return false;
@@ -579,16 +658,16 @@ inline bool after_step(const Scene &S, double M, double a, const std::vector<prr
             return true;
         }
     }
-    if (S.metric == PRR_SCHW_SPH) sph_post_step(x, u);
+    chart_post_step(S.metric, x, u);
     const double r = radius_of(S.metric, x);
-    if (S.metric != PRR_FLAT) {
+    if (has_horizon(S.metric)) {
         const double rh = horizon_radius(S.metric, M, a);
         if (r <= rh * (1.0 + S.tr.horizon_eps_rel) + S.tr.horizon_eps_abs) {
             R.cls = PRR_HORIZON;
             return true;
         }
     }
-    if (r >= S.tr.r_escape) {
+    if (r >= escape_radius(S, x)) {
         R.cls = PRR_ESCAPE;
         return true;
     }
@@ -701,7 +780,10 @@ inline void rk_substep(const Scene &S, double M, double a, const double x[4], co
 // theta h from the step's start), not the first state beyond r_escape,
 // which depends on the step sequence.
 inline void escape_g(int kind, const double x[4], const double u[4], double r_esc, double &g, double &dg) {
-    if (spherical_coords(kind)) {
+    if (kind == PRR_ELLIS) { // |l| = r_escape on either side of the throat
+        g = std::fabs(x[0]) - r_esc;
+        dg = x[0] < 0.0 ? -u[0] : u[0];
+    } else if (spherical_coords(kind)) {
         g = x[0] - r_esc;
         dg = u[0];
     } else {
@@ -713,7 +795,7 @@ inline void escape_g(int kind, const double x[4], const double u[4], double r_es
 
 inline void escape_locate(const Scene &S, double M, double a, const double xo[4], const double uo[4],
                           const double xn[4], const double un[4], double h, double xs[4], double us[4]) {
-    const double r_esc = S.tr.r_escape;
+    const double r_esc = escape_radius(S, xn);
     double go, gn, dg;
     escape_g(S.metric, xo, uo, r_esc, go, dg);
     escape_g(S.metric, xn, un, r_esc, gn, dg);
@@ -877,17 +959,17 @@ inline bool rkf_events(const Scene &S, double M, double a, const std::vector<prr
         }
     }
     const double r = radius_of(S.metric, x);
-    if (r >= S.tr.r_escape) {
+    if (r >= escape_radius(S, x)) {
         double xs[4], us[4];
         escape_locate(S, M, a, xo, uo, x, u, h, xs, us);
         std::memcpy(x, xs, sizeof xs);
         std::memcpy(u, us, sizeof us);
-        if (S.metric == PRR_SCHW_SPH) sph_post_step(x, u);
+        chart_post_step(S.metric, x, u);
         R.cls = PRR_ESCAPE;
         return true;
     }
-    if (S.metric == PRR_SCHW_SPH) sph_post_step(x, u);
-    if (S.metric != PRR_FLAT) {
+    chart_post_step(S.metric, x, u);
+    if (has_horizon(S.metric)) {
         const double rh = horizon_radius(S.metric, M, a);
         if (r <= rh * (1.0 + S.tr.horizon_eps_rel) + S.tr.horizon_eps_abs) {
             R.cls = PRR_HORIZON;
@@ -982,7 +1064,7 @@ inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr
 // Schwarzschild-coordinate metric (getColorDefault, main/base.h:68-84), and
 // the Cartesian coordinate velocity for Kerr BL.
 inline V3 escape_dir(int kind, const double x[4], const double u[4]) {
-    if (kind == PRR_SCHW_SPH) {
+    if (sph_chart(kind)) {
         const double st = std::sin(x[1]), ct = std::cos(x[1]), sp = std::sin(x[2]), cp = std::cos(x[2]);
         return v3(st * cp, st * sp, ct);
     }
@@ -1001,7 +1083,7 @@ inline void shade(const Scene &S, RayResult &R) {
         const V3 n = escape_dir(S.metric, R.x, R.u);
         R.endpoint = n;
         double th, ph;
-        if (S.metric == PRR_SCHW_SPH) {
+        if (sph_chart(S.metric)) {
             th = R.x[1];
             ph = R.x[2];
             if (ph == 2.0 * M_PI) ph = 0.0; // main/base.h:86-89
@@ -1009,7 +1091,9 @@ inline void shade(const Scene &S, RayResult &R) {
         } else {
             dir_angles(n, th, ph);
         }
-        R.color = sky_lookup(S.tex[0], th, ph);
+        // through a wormhole's throat (l < 0): the second sky (bimage2, main/base.h:168)
+        const bool far_side = S.metric == PRR_ELLIS && R.x[0] < 0.0;
+        R.color = sky_lookup(S.tex[far_side ? PRR_SKY2_SLOT : 0], th, ph);
         break;
     }
     case PRR_SPHERE: break; // colour set at the hit
@@ -1039,8 +1123,9 @@ inline RayResult render_ray(const Scene &S, const prr_frame &f, const Basis &B, 
     d = perturb_dir(d, B, delta, axis);
     double x0[4], u0[4];
     // the metric's second parameter: the frame's spin (Kerr), or the scene's
-    // charge for the Schwarzschild-coordinate metric (Reissner-Nordstrom)
-    const double a = S.metric == PRR_SCHW_SPH ? S.a : f.spin;
+    // charge for the Schwarzschild-coordinate metric (Reissner-Nordstrom), or
+    // the scene's throat radius for the wormhole
+    const double a = (S.metric == PRR_SCHW_SPH || S.metric == PRR_ELLIS) ? S.a : f.spin;
     init_ray(S.metric, f.mass, a, v3(f.pos[0], f.pos[1], f.pos[2]), d, x0, u0);
     std::vector<prr_sphere> sph(f.spheres, f.spheres + std::min(f.n_spheres, PRR_MAX_SPHERES));
     RayResult R = trace(S, f.mass, a, sph, x0, u0);

@@ -7,7 +7,8 @@ import ctypes as C
 
 PRR_ABI_VERSION = 2
 PRR_MAX_SPHERES = 8
-PRR_MAX_TEXTURES = 9
+PRR_MAX_TEXTURES = 10
+PRR_SKY2_SLOT = 9
 
 # prr_status
 PRR_OK, PRR_EINVAL, PRR_ENOMEM, PRR_EHIP, PRR_EIO, PRR_EBUDGET, PRR_ENODEV = range(7)
@@ -15,9 +16,9 @@ STATUS_NAMES = {0: "PRR_OK", 1: "PRR_EINVAL", 2: "PRR_ENOMEM", 3: "PRR_EHIP", 4:
                 5: "PRR_EBUDGET", 6: "PRR_ENODEV"}
 
 # prr_metric
-PRR_FLAT, PRR_SCHW_KS_CART, PRR_SCHW_SPH, PRR_KERR_BL = 0, 1, 2, 3
+PRR_FLAT, PRR_SCHW_KS_CART, PRR_SCHW_SPH, PRR_KERR_BL, PRR_ELLIS = 0, 1, 2, 3, 4
 METRIC_NAMES = {"flat": PRR_FLAT, "schwarzschild": PRR_SCHW_KS_CART, "schwarzschild-cart": PRR_SCHW_KS_CART,
-                "schwarzschild-sph": PRR_SCHW_SPH, "kerr": PRR_KERR_BL}
+                "schwarzschild-sph": PRR_SCHW_SPH, "kerr": PRR_KERR_BL, "wormhole": PRR_ELLIS}
 # prr_solver (values of the reference's SOLVER enum, main/base.h:26-34)
 PRR_EULER, PRR_RK4, PRR_RKF45, PRR_DORMAND = 0, 1, 2, 5
 # prr_precision

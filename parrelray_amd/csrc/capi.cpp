@@ -182,13 +182,23 @@ prr_status validate_frame(prr_ctx *ctx, const prr_frame &f, int idx) {
     if (ctx->metric == PRR_KERR_BL && !(std::fabs(f.spin) <= f.mass))
         return fail(ctx, PRR_EINVAL, "frame %d: Kerr needs |a| <= M", idx);
     const double r = std::sqrt(f.pos[0] * f.pos[0] + f.pos[1] * f.pos[1] + f.pos[2] * f.pos[2]);
-    if ((ctx->metric == PRR_SCHW_SPH || ctx->metric == PRR_KERR_BL) &&
+    if ((ctx->metric == PRR_SCHW_SPH || ctx->metric == PRR_KERR_BL || ctx->metric == PRR_ELLIS) &&
         !(f.pos[0] * f.pos[0] + f.pos[1] * f.pos[1] > 0.0))
         return fail(ctx, PRR_EINVAL, "frame %d: camera on the polar axis of a spherical chart", idx);
     if (!(r < ctx->tracer.r_escape)) return fail(ctx, PRR_EINVAL, "frame %d: camera outside r_escape", idx);
+    if (ctx->metric == PRR_ELLIS) {
+        if (!(r > ctx->p[0])) return fail(ctx, PRR_EINVAL, "frame %d: camera inside the wormhole's throat radius", idx);
+        if (f.n_spheres != 0) return fail(ctx, PRR_EINVAL, "frame %d: the wormhole scene holds no spheres", idx);
+    }
     return PRR_OK;
 }
 
+
+// The metric's parameter that the frame does not carry: the charge of the
+// Schwarzschild-coordinate metric, the Ellis throat radius.
+double metric_param(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx->p[0] : ctx->q; }
+// Escape radius beyond the wormhole's throat: p[1] (the reference's rout2), else r_escape.
+double far_escape(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx->p[1] : 0.0; }
 
 int blocks_per_cu(const prr_ctx *ctx) {
     if (ctx->launch.blocks_per_cu > 0) return ctx->launch.blocks_per_cu;
@@ -308,7 +318,11 @@ void prr_destroy(prr_ctx *ctx) {
 
 prr_status prr_set_spacetime(prr_ctx *ctx, int32_t metric, double M, double a, double q, const double p[16]) {
     if (!ctx) return fail(nullptr, PRR_EINVAL, "NULL context");
-    if (metric < PRR_FLAT || metric > PRR_KERR_BL) return fail(ctx, PRR_EINVAL, "unknown metric %d", metric);
+    if (metric < PRR_FLAT || metric > PRR_ELLIS) return fail(ctx, PRR_EINVAL, "unknown metric %d", metric);
+    if (metric == PRR_ELLIS && !(p && p[0] > 0.0 && std::isfinite(p[0])))
+        return fail(ctx, PRR_EINVAL, "the Ellis wormhole needs a throat radius p[0] > 0");
+    if (metric == PRR_ELLIS && !(p[1] >= 0.0 && std::isfinite(p[1])))
+        return fail(ctx, PRR_EINVAL, "the wormhole's far-side escape radius p[1] must be >= 0 (0: r_escape)");
     if (!(M >= 0.0)) return fail(ctx, PRR_EINVAL, "mass must be >= 0");
     if (metric == PRR_KERR_BL && !(std::fabs(a) <= M)) return fail(ctx, PRR_EINVAL, "Kerr needs |a| <= M");
     if (q != 0.0 && metric != PRR_SCHW_SPH)
@@ -423,7 +437,7 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
     for (int i = 0; i < n; ++i) {
         prr_status st = validate_frame(ctx, frames[i], i);
         if (st != PRR_OK) return st;
-        P.fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total, ctx->q);
+        P.fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total, metric_param(ctx), far_escape(ctx));
         const int64_t np = int64_t(frames[i].width) * frames[i].height;
         P.total += np * frames[i].samples;
         P.pixels += np;
@@ -662,7 +676,7 @@ prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int3
     f.n_spheres = int(ctx->spheres.size());
     for (int k = 0; k < f.n_spheres; ++k) f.spheres[k] = ctx->spheres[k];
     RenderPlan P;
-    P.fd.push_back(frame_dev(ctx->metric, ctx->tracer, f, 0, ctx->q));
+    P.fd.push_back(frame_dev(ctx->metric, ctx->tracer, f, 0, metric_param(ctx), far_escape(ctx)));
     P.total = n;
     prr_status s;
     if ((s = ensure(ctx, ctx->frames, sizeof(FrameDev))) != PRR_OK) return s;

@@ -21,6 +21,7 @@ template <class F> prr_status with_metric(int32_t metric, F &&f) {
     case PRR_SCHW_KS_CART: return f(SchwKS{});
     case PRR_SCHW_SPH: return f(SchwSph{});
     case PRR_KERR_BL: return f(Kerr{});
+    case PRR_ELLIS: return f(Ellis{});
     default: set_global_error("unknown metric"); return PRR_EINVAL;
     }
 }

@@ -11,7 +11,7 @@ namespace prr {
 // Outer horizon: Kerr r+ = M + sqrt(M^2 - a^2); Reissner-Nordstrom (the
 // Schwarzschild-coordinate metric with charge a = q) M + sqrt(M^2 - q^2).
 double horizon_radius(int metric, double M, double a) {
-    if (metric == PRR_FLAT) return 0.0;
+    if (metric == PRR_FLAT || metric == PRR_ELLIS) return 0.0;
     if (metric == PRR_KERR_BL || metric == PRR_SCHW_SPH) return M + std::sqrt(std::max(M * M - a * a, 0.0));
     return 2.0 * M;
 }
@@ -33,7 +33,8 @@ TracerDev tracer_dev(const prr_tracer &t) {
     return d;
 }
 
-FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t begin, double charge) {
+FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t begin, double param,
+                   double r_esc2) {
     FrameDev d{};
     for (int i = 0; i < 3; ++i) d.pos[i] = f.pos[i];
     double F[3], U[3], R[3];
@@ -59,12 +60,13 @@ FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t
     d.projection = f.projection;
     d.n_spheres = f.n_spheres;
     d.M = f.mass;
-    // the metric's second parameter: Kerr's spin, or the charge of the
-    // Schwarzschild-coordinate metric (Reissner-Nordstrom)
-    d.a = metric == PRR_KERR_BL ? f.spin : (metric == PRR_SCHW_SPH ? charge : 0.0);
+    // the metric's second parameter: Kerr's spin, the charge of the
+    // Schwarzschild-coordinate metric (Reissner-Nordstrom), the Ellis throat radius
+    d.a = metric == PRR_KERR_BL ? f.spin : ((metric == PRR_SCHW_SPH || metric == PRR_ELLIS) ? param : 0.0);
     const double rh = horizon_radius(metric, f.mass, d.a);
-    d.r_hor = metric == PRR_FLAT ? -1.0
+    d.r_hor = (metric == PRR_FLAT || metric == PRR_ELLIS) ? -1.0
                                       : rh * (1.0 + tr.horizon_eps_rel) + tr.horizon_eps_abs;
+    d.r_esc2 = r_esc2 > 0.0 ? r_esc2 : tr.r_escape;
     d.ray_begin = begin;
     for (int k = 0; k < f.n_spheres; ++k) {
         for (int c = 0; c < 3; ++c) d.sph[k][c] = f.spheres[k].c[c];

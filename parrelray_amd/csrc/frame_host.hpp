@@ -14,7 +14,10 @@ double horizon_radius(int metric, double M, double a);
 TracerDev tracer_dev(const prr_tracer &t);
 // prr_frame -> FrameDev (Camera::computeVectors, main/Camera.cpp:19-28, and
 // the horizon test radius r_h (1 + eps_rel) + eps_abs); begin = first ray id.
-// charge: the spacetime's q, used by the Schwarzschild-coordinate metric (Reissner-Nordstrom).
-FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t begin, double charge = 0.0);
+// param: the metric's second parameter when it is not the frame's spin: the
+// charge of the Schwarzschild-coordinate metric, the Ellis throat radius.
+// r_esc2: the escape radius beyond the wormhole's throat (<= 0: r_escape).
+FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t begin, double param = 0.0,
+                   double r_esc2 = 0.0);
 
 } // namespace prr

@@ -243,6 +243,47 @@ struct SchwSph {
     }
 };
 
+// Ellis wormhole, ds^2 = -dt^2 + dl^2 + (b^2 + l^2) (dth^2 + sin^2 th dph^2)
+// (Ellis 1973; Morris and Thorne 1988), in (l, th, ph, t) with l on both
+// sides of the throat l = 0 of radius b (the metric's second parameter,
+// Consts::a).  The reference keeps a second sky for this case ("l < 0 if
+// wormhole", main/base.h:168) without a metric that reaches it.  With
+// R^2 = b^2 + l^2: Gamma^l_thth = -l, Gamma^l_phph = -l sin^2 th,
+// Gamma^th_lth = Gamma^ph_lph = l / R^2, and the angular symbols of flat
+// space; tan and sin are clamped to 1e-4 in magnitude near the axis, with
+// their signs (the path crosses the axis: axis_post_step).
+struct Ellis {
+    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
+    static constexpr int N = 8;
+    static constexpr int kind = PRR_ELLIS;
+    static constexpr bool spherical = true;
+    static constexpr int NERR = 8;
+    static constexpr bool mino = false;
+    template <class T> PRR_HD static MassCoef<T> coef(const Consts<double> &c) {
+        return MassCoef<T>{T(0), T(c.a * c.a)}; // q2: b^2
+    }
+    template <class T> PRR_HD static void rhs(const T *y, const MassCoef<T> &c, T *f) {
+        const T l = y[0];
+        const T ul = y[4], uth = y[5], uph = y[6], ut = y[7];
+        T s, co;
+        sincos_t(y[1], &s, &co);
+        T tn = s / co;
+        if (fabs(tn) < T(1e-4)) { // within 1e-4 of the axis, keeping the signs
+            tn = copysign(T(1e-4), tn);
+            s = copysign(T(1e-4), s);
+        }
+        const T k = l / (c.q2 + l * l); // l / R^2
+        f[0] = ul;
+        f[1] = uth;
+        f[2] = uph;
+        f[3] = ut;
+        f[4] = l * (uth * uth + s * s * uph * uph);
+        f[5] = T(-2) * k * ul * uth + s * co * uph * uph;
+        f[6] = T(-2) * k * ul * uph - T(2) * uth * uph / tn;
+        f[7] = T(0);
+    }
+};
+
 // Kerr, Boyer-Lindquist (metric of metrics/kerr.cuh:77-100; the reference's
 // getGeod is a stub).  Integrated in Mino time s (d lambda = Sigma ds) with
 // u = 1/r and mu = cos th, where the geodesic equations separate into two
@@ -467,6 +508,7 @@ struct Kerr {
 // Radius of a state (for the horizon, escape and FP64-region tests).
 template <class Mt> PRR_HD double radius_of(const double *y) {
     if (Mt::mino) return rcp_t(y[0]);
+    if (Mt::kind == PRR_ELLIS) return fabs(y[0]); // distance from the throat on either side
     if (Mt::spherical) return y[0];
     return sqrt(y[0] * y[0] + y[1] * y[1] + y[2] * y[2]);
 }
@@ -688,6 +730,11 @@ template <class Mt> PRR_HD void init_state(D3 pos, D3 d, double M, double a, dou
         u[0] = us[0];
         u[1] = us[1];
         u[2] = us[2];
+        if (Mt::kind == PRR_ELLIS) { // a camera at areal radius r > b, on the l > 0 side: l = sqrt(r^2 - b^2)
+            const double r = xs[0];
+            x[0] = sqrt(fmax(r * r - a * a, 0.0));
+            u[0] = us[0] * r / x[0];
+        }
     } else {
         x[0] = pos.x;
         x[1] = pos.y;
@@ -710,6 +757,11 @@ template <class Mt> PRR_HD void init_state(D3 pos, D3 d, double M, double a, dou
         A = -f;
         B = 0.0;
         C = u[0] * u[0] / f + x[0] * x[0] * (u[1] * u[1] + s * s * u[2] * u[2]);
+    } else if (Mt::kind == PRR_ELLIS) { // a: the throat radius b
+        const double R2 = a * a + x[0] * x[0], s = sin(x[1]);
+        A = -1.0;
+        B = 0.0;
+        C = u[0] * u[0] + R2 * (u[1] * u[1] + s * s * u[2] * u[2]);
     } else if (Mt::kind == PRR_KERR_BL) {
         const KerrG g = kerr_g(M, a, x[0], x[1]);
         A = g.tt;
@@ -739,6 +791,28 @@ PRR_HD void sph_post_step(double *y) {
     if (y[1] > pi && y[1] <= two_pi) y[1] = two_pi - y[1];
 }
 
+// Chart map of the wormhole's (theta, phi) across the polar axis: a state
+// at theta < 0 (or > pi) is the same point and direction as the reflected
+// theta with phi + pi and -theta', so the path stays continuous (the
+// reference's flip within 0.04 rad of the axis, sph_post_step, moves the
+// point); then phi in [0, 2pi).
+PRR_HD void axis_post_step(double *y) {
+    const double pi = 3.14159265358979323846, two_pi = 6.28318530717958647692;
+    if (y[1] < 0.0 || y[1] > pi) {
+        y[1] = y[1] < 0.0 ? -y[1] : two_pi - y[1];
+        y[2] += pi;
+        y[5] = -y[5];
+    }
+    y[2] -= two_pi * floor(y[2] / two_pi);
+    if (y[2] >= two_pi) y[2] = 0.0;
+}
+
+// The chart's step-end map for the metrics in (r or l, theta, phi, t).
+template <class Mt> PRR_HD void chart_post_step(double *y) {
+    if (Mt::kind == PRR_SCHW_SPH) sph_post_step(y);
+    if (Mt::kind == PRR_ELLIS) axis_post_step(y);
+}
+
 // Direction used for the celestial lookup at escape: the spatial velocity for
 // Cartesian metrics (getColorDefault2, main/base.h:119-128), the position for
 // Schwarzschild coordinates (getColorDefault, :68-84), the Cartesian
@@ -746,7 +820,7 @@ PRR_HD void sph_post_step(double *y) {
 // (th, phi), dx/ds ~ -u' e_r / u^2 + r th' e_th + r sin th phi' e_ph, scaled
 // by u^2 > 0: -u' e_r - u mu' / sin th e_th + u sin th phi' e_ph.
 template <class Mt> PRR_HD D3 escape_dir(const double *y, const Consts<double> &cs) {
-    if (Mt::kind == PRR_SCHW_SPH) {
+    if (Mt::kind == PRR_SCHW_SPH || Mt::kind == PRR_ELLIS) {
         double st, ct, sp, cp;
         sincos(y[1], &st, &ct);
         sincos(y[2], &sp, &cp);

@@ -228,6 +228,7 @@ int metric_from_name(const std::string &s) {
     if (s == "schwarzschild" || s == "schwarzschild-cart" || s.empty()) return PRR_SCHW_KS_CART;
     if (s == "schwarzschild-sph") return PRR_SCHW_SPH;
     if (s == "kerr") return PRR_KERR_BL;
+    if (s == "wormhole") return PRR_ELLIS;
     return -1;
 }
 } // namespace
@@ -320,7 +321,9 @@ int prr_params_frame_count(const prr_params *p) try {
 
 // createScene's spacetime part (prr.cpp:163-181).  The metric, fixed at
 // compile time in the reference (main/metric.h:12), is read from the new
-// [prr] block's `metric` key (flat | schwarzschild | schwarzschild-cart | schwarzschild-sph | kerr;
+// [prr] block's `metric` key (flat | schwarzschild | schwarzschild-cart | schwarzschild-sph | kerr |
+// wormhole, whose throat radius is the [spacetime] block's p0 and whose
+// far-side escape radius is p1 or the tracer's rout2;
 // no underscores: an underscore anywhere makes a .params line a header).
 prr_status prr_params_spacetime(const prr_params *p, int32_t *metric, double *M, double *a, double *q,
                                 double pvals[16]) try {
@@ -332,8 +335,12 @@ prr_status prr_params_spacetime(const prr_params *p, int32_t *metric, double *M,
     if (M) *M = bh ? parf(p, "spacetime", "mass", 0.0) : 0.0;
     if (a) *a = bh ? parf(p, "spacetime", "spin", 0.0) : 0.0;
     if (q) *q = bh ? parf(p, "spacetime", "charge", 0.0) : 0.0;
-    if (pvals)
+    if (pvals) {
         for (int i = 0; i < 16; ++i) pvals[i] = parf(p, "spacetime", "p" + std::to_string(i), 0.0);
+        // the wormhole's far-side escape radius: the tracer block's rout2
+        // (prr.cpp:190) unless p1 sets it
+        if (mt == PRR_ELLIS && pvals[1] == 0.0) pvals[1] = parf(p, "tracer", "rout2", 0.0);
+    }
     return PRR_OK;
 } catch (...) {
     return prr::abi_exception(nullptr);

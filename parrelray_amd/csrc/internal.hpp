@@ -45,6 +45,7 @@ struct FrameDev {
     uint32_t seed; // RANDOM / JITTERED sample stream
     double M, a;
     double r_hor; // horizon test radius r_h (1 + eps_rel) + eps_abs (negative: no test)
+    double r_esc2; // escape radius beyond a wormhole's throat (l < 0; rout2)
     int64_t ray_begin;
     double sph[PRR_MAX_SPHERES][4]; // centre xyz, radius
     int32_t sph_tex[PRR_MAX_SPHERES];

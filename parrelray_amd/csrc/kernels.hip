@@ -458,6 +458,7 @@ template <class F> static hipError_t with_metric(int metric, F &&f) {
     case PRR_SCHW_KS_CART: return f(SchwKS{});
     case PRR_SCHW_SPH: return f(SchwSph{});
     case PRR_KERR_BL: return f(Kerr{});
+    case PRR_ELLIS: return f(Ellis{});
     default: return hipErrorInvalidValue;
     }
 }

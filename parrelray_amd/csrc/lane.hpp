@@ -379,12 +379,21 @@ template <class Mt> PRR_HD void rk_substep(const double *y, double h, const Cons
     for (int i = 0; i < Mt::N; ++i) out[i] = y[i] + d4[i];
 }
 
+// Escape radius on the state's side: beyond a wormhole's throat (l < 0)
+// rays escape at the second sky's radius (rout2).
+template <class Mt> PRR_HD double escape_radius(const JobDev &job, const FrameDev &fr, const double *y) {
+    return (Mt::kind == PRR_ELLIS && y[0] < 0.0) ? fr.r_esc2 : job.tr.r_escape;
+}
+
 // Escape crossing function g (g < 0 inside r_escape) and dg/dt along the
 // integration variable.
 template <class Mt> PRR_HD void escape_g(const double *y, double r_esc, double &g, double &dg) {
     if (Mt::mino) {
         g = 1.0 / r_esc - y[0];
         dg = -y[1];
+    } else if (Mt::kind == PRR_ELLIS) { // |l| = r_escape on either side of the throat
+        g = fabs(y[0]) - r_esc;
+        dg = y[0] < 0.0 ? -y[4] : y[4];
     } else if (Mt::spherical) {
         g = y[0] - r_esc;
         dg = y[4];
@@ -542,7 +551,7 @@ template <class Mt> PRR_HD void shade_ray(const JobDev &job, int64_t id) {
         const D3 n = escape_dir<Mt>(y, cs);
         endpoint = n;
         double th, ph;
-        if (Mt::kind == PRR_SCHW_SPH) {
+        if (Mt::kind == PRR_SCHW_SPH || Mt::kind == PRR_ELLIS) {
             const double two_pi = 6.28318530717958647692, pi = 3.14159265358979323846;
             th = y[1];
             ph = y[2];
@@ -553,7 +562,9 @@ template <class Mt> PRR_HD void shade_ray(const JobDev &job, int64_t id) {
             ph = atan2(n.y, n.x);
             if (ph < 0.0) ph += 6.28318530717958647692;
         }
-        color = sky_lookup(job.tex[0], th, ph);
+        // through a wormhole's throat (l < 0): the second sky (bimage2)
+        const bool far_side = Mt::kind == PRR_ELLIS && y[0] < 0.0;
+        color = sky_lookup(job.tex[far_side ? PRR_SKY2_SLOT : 0], th, ph);
     } else {
         endpoint = cart_of<Mt>(y);
         color = make_float3(0.f, 0.f, 0.f);
@@ -616,11 +627,11 @@ template <class Mt> PRR_HD int euler_events(const JobDev &job, Lane<Mt> &L, cons
             return PRR_SPHERE;
         }
     }
-    if (Mt::kind == PRR_SCHW_SPH) sph_post_step(yn);
+    chart_post_step<Mt>(yn);
     for (int i = 0; i < Mt::N; ++i) L.y[i] = yn[i];
     const double r = radius_of<Mt>(yn);
     if (r <= fr.r_hor) return PRR_HORIZON;
-    if (r >= job.tr.r_escape) return PRR_ESCAPE;
+    if (r >= escape_radius<Mt>(job, fr, yn)) return PRR_ESCAPE;
     return -1;
 }
 
@@ -698,7 +709,7 @@ PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, const LocateCtx &C, co
     }
     copy_n<Mt::N>(L.y, ys);
     L.hit = hit;
-    if (cls == PRR_ESCAPE && Mt::kind == PRR_SCHW_SPH) sph_post_step(L.y);
+    if (cls == PRR_ESCAPE) chart_post_step<Mt>(L.y);
     return cls;
 }
 
@@ -711,11 +722,12 @@ PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, con
                                int &mode) {
     const FrameDev &fr = job.frames[L.frame];
     const double r = radius_of<Mt>(yn);
-    if (r >= job.tr.r_escape) {
+    const double r_esc = escape_radius<Mt>(job, fr, yn);
+    if (r >= r_esc) {
         double go, gn, dg;
-        escape_g<Mt>(L.y, job.tr.r_escape, go, dg);
+        escape_g<Mt>(L.y, r_esc, go, dg);
         if (!(go < 0.0)) return finish_located(job, L, C, L.y, 0.0, PRR_ESCAPE, -1); // started outside
-        escape_g<Mt>(yn, job.tr.r_escape, gn, dg);
+        escape_g<Mt>(yn, r_esc, gn, dg);
         copy_n<Mt::N>(C.yn, yn);
         C.h = h;
         C.kind = LOC_ESCAPE;
@@ -726,7 +738,7 @@ PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, con
     }
     double y[Mt::N];
     copy_n<Mt::N>(y, yn);
-    if (Mt::kind == PRR_SCHW_SPH) sph_post_step(y);
+    chart_post_step<Mt>(y);
     copy_n<Mt::N>(L.y, y);
     if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
     if (r <= fr.r_hor) return PRR_HORIZON;
@@ -870,9 +882,10 @@ PRR_HD bool parabola_vertex(const LocateCtx &C, double &tv) {
 template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yt) {
     constexpr int N = Mt::N;
     if (C.kind == LOC_ESCAPE) {
+        const double r_esc = escape_radius<Mt>(job, job.frames[L.frame], yt);
         double g, dg;
-        escape_g<Mt>(yt, job.tr.r_escape, g, dg);
-        const double tol = 1e-13 * (Mt::mino ? 1.0 / job.tr.r_escape : job.tr.r_escape);
+        escape_g<Mt>(yt, r_esc, g, dg);
+        const double tol = 1e-13 * (Mt::mino ? 1.0 / r_esc : r_esc);
         if (fabs(g) <= tol || !(dg * C.h != 0.0) || C.it >= 6) return finish_located(job, L, C, yt, C.th, PRR_ESCAPE, -1);
         C.th = fmin(fmax(C.th - g / (dg * C.h), 0.0), 1.0);
         C.it++;

@@ -116,6 +116,13 @@ int main(int argc, char **argv) {
         if (!load_texture(buf, rgb, w, h)) return die(ctx, (std::string("texture ") + buf).c_str());
         if (prr_set_texture(ctx, 0, rgb.data(), w, h, PRR_FILTER_SQAVG) != PRR_OK) return die(ctx, "texture");
     }
+    // the sky beyond a wormhole's throat (main/base.h:168)
+    prr_params_par(p, "tracer", "bimage2", buf, sizeof buf);
+    if (buf[0]) {
+        if (!load_texture(buf, rgb, w, h)) return die(ctx, (std::string("texture ") + buf).c_str());
+        if (prr_set_texture(ctx, PRR_SKY2_SLOT, rgb.data(), w, h, PRR_FILTER_SQAVG) != PRR_OK)
+            return die(ctx, "texture");
+    }
     for (int k = 0; k < PRR_MAX_SPHERES; ++k) {
         const std::string hdr = "sphere" + std::to_string(k);
         if (!prr_params_header_exists(p, hdr.c_str())) break;

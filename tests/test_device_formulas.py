@@ -53,6 +53,30 @@ def test_charged_sph_rhs_matches_oracle(q):
         assert np.allclose(f[4:], du, rtol=1e-12, atol=1e-13 * scale), (x, u, f[4:], du)
 
 
+@pytest.mark.parametrize("b", [0.5, 2.0])
+def test_ellis_rhs_matches_oracle(b):
+    """Ellis wormhole: the throat radius rides in the metric's second
+    parameter; l takes both signs."""
+    for x, u in points(A.PRR_SCHW_SPH, 50):
+        x[0] *= RNG.choice([-1.0, 1.0]) * RNG.uniform(0.0, 0.5)
+        f = api.rhs(A.PRR_ELLIS, np.concatenate([x, u]), M=0.0, a=b)
+        assert np.array_equal(f[:4], u)
+        du = O.geod(A.PRR_ELLIS, 0.0, b, x, u)
+        scale = np.abs(du).max() + 1e-300
+        assert np.allclose(f[4:], du, rtol=1e-12, atol=1e-13 * scale), (x, u, f[4:], du)
+
+
+def test_ellis_init_ray_matches_oracle():
+    for _ in range(30):
+        pos = RNG.normal(size=3)
+        pos = pos / np.linalg.norm(pos) * RNG.uniform(4, 50)
+        d = RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        xo, uo = O.init_ray(A.PRR_ELLIS, 0.0, 1.5, pos, d)
+        y, _ = api.init_ray(A.PRR_ELLIS, pos, d, 0.0, 1.5)
+        assert np.allclose(y, np.concatenate([xo, uo]), rtol=1e-13, atol=1e-14)
+
+
 def test_sph_rhs_keeps_the_tan_clamp():
     # |tan th| < 1e-4 -> tan = sin = 1e-4 ("metrics/schwarzschild - sph.cuh":98-101)
     for th in (1e-6, math.pi - 2e-5, -3e-5):
@@ -131,10 +155,16 @@ def test_kerr_delta_form_equals_the_difference_of_rhs():
             for scale in (1e-1, 1e-3, 1e-6):
                 d = RNG.normal(size=6) * scale * np.maximum(np.abs(y), 1e-3)
                 d[0] = max(d[0], -0.5 * y[0])
+                # stay in the chart: |mu| < 1 (a state past the pole has 1 - mu^2 < 0)
+                d[2] = np.clip(y[2] + d[2], -0.999, 0.999) - y[2]
                 g = api.kerr_delta_rhs(y, d, a, E, L, Q)
                 ref = api.rhs(A.PRR_KERR_BL, y + d, 1.0, a, E, L, Q) - f0
                 # FP32 relative rounding of the difference, plus FP64 rounding of the reference
                 tol = 3e-6 * np.abs(ref) + 1e-15 * np.abs(f0) + 1e-30
+                # 1 - mu^2 is formed in FP32 as s20 - (mu^2 - mu0^2): its relative error
+                # grows by (s20 + |mu^2 - mu0^2|) / s2 near the axis, and L / s2 with it
+                s20, s2 = 1 - y[2] ** 2, 1 - (y[2] + d[2]) ** 2
+                tol[4] += 1.2e-7 * (s20 + abs(s2 - s20)) / s2 * abs(L) / s2
                 assert np.all(np.abs(g - ref) <= tol + 2e-7 * np.abs(g)), (a, scale, g, ref)
 
 

@@ -103,3 +103,34 @@ def test_sky_texture_from_a_png_file_renders_like_the_procedural_one(tmp_path, m
     img = read_png(tmp_path / "c0-image_0.png")
     assert np.array_equal(img, library_render(tmp_path, monkeypatch, 0))
     assert img[3, 3].max() > 0
+
+
+def test_wormhole_with_a_second_sky(tmp_path, monkeypatch):
+    """metric = wormhole with throat radius p0 = 8: the C0 camera at r = 30
+    (its centre ray passes the centre at impact parameter ~5) looks into the
+    throat, whose disc shows bimage2 (the sky beyond it)."""
+    prepare(tmp_path)
+    text = (tmp_path / "c0.params").read_text()
+    text = text.replace("<metric> = <schwarzschild-sph>", "<metric> = <wormhole>")
+    text = text.replace("<bimage1> = <checker:512:256:16:8:1592590337>",
+                        "<bimage1> = <checker:512:256:16:8:1592590337>\n<bimage2> = <checker:256:128:4:2:77>")
+    text = text.replace("<spin> = <0.0>", "<spin> = <0.0>\n<p0> = <8.0>")
+    text = text.replace("<rout1> = <100.0>", "<rout1> = <100.0>\n<rout2> = <70.0>")
+    (tmp_path / "c0.params").write_text(text)
+    run_prr(tmp_path, "c0.params")
+    img = read_png(tmp_path / "c0-image_0.png")
+    monkeypatch.chdir(tmp_path)
+    p = api.Params(sim="c0.params")
+    st, tr = p.spacetime(), p.tracer()
+    assert st["metric"] == A.PRR_ELLIS and st["p"][0] == 8.0 and st["p"][1] == 70.0
+    ctx = api.Context(0)
+    ctx.set_spacetime(st["metric"], st["M"], st["a"], st["q"], p=st["p"])
+    ctx.set_tracer(tr)
+    ctx.set_texture(0, api.checker_texture(512, 256, 16, 8, 1592590337), A.PRR_FILTER_SQAVG)
+    ctx.set_texture(A.PRR_SKY2_SLOT, api.checker_texture(256, 128, 4, 2, 77), A.PRR_FILTER_SQAVG)
+    f = p.frame(0)
+    assert np.array_equal(img, ctx.render(f, rays=False)[0]["rgb8"])
+    # the throat's disc is lit by the second sky; without it, it is black
+    assert img[64, 64].max() > 0
+    ctx.set_texture(A.PRR_SKY2_SLOT, None)
+    assert ctx.render(f, rays=False)[0]["rgb8"][64, 64].max() == 0

@@ -73,3 +73,25 @@ def test_invalid_inputs_are_rejected_without_touching_the_gpu_state():
     # the context still works
     out = ctx.render(f)[0]
     assert out["rgb8"].shape == (18, 32, 3)
+
+
+def test_wormhole_inputs_are_checked():
+    """Ellis wormhole: throat radius p[0] > 0, far escape radius p[1] >= 0,
+    the camera outside the throat's areal radius, no spheres."""
+    from parrelray_amd._lib import PrrError
+    ctx = api.Context(0)
+    for p in ([0.0] * 16, [-1.0] + [0.0] * 15, [2.0, -5.0] + [0.0] * 14):
+        with pytest.raises(PrrError):
+            ctx.set_spacetime(A.PRR_ELLIS, 0.0, 0.0, 0.0, p=p)
+    with pytest.raises(PrrError):  # a charge belongs to the Schwarzschild-coordinate metric
+        ctx.set_spacetime(A.PRR_ELLIS, 0.0, 0.0, 0.5, p=[2.0] + [0.0] * 15)
+    ctx.set_spacetime(A.PRR_ELLIS, 0.0, 0.0, 0.0, p=[2.0] + [0.0] * 15)
+    ctx.set_tracer(CF.tracer_kerr())
+    inside = A.make_frame([1.0, 1.0, 0.5], 3.9, 0.0, 1.0, 8, 8, mass=0.0)
+    with pytest.raises(PrrError, match="throat"):
+        ctx.render(inside)
+    sph = A.make_frame([20.0, 1.0, 0.5], 3.1, 0.0, 1.0, 8, 8, mass=0.0, spheres=[([5.0, 0.0, 0.0], 1.0, 1)])
+    with pytest.raises(PrrError, match="spheres"):
+        ctx.render(sph)
+    ok = A.make_frame([20.0, 1.0, 0.5], 3.1, 0.0, 1.0, 8, 8, mass=0.0)
+    assert ctx.render(ok)[0]["rgb8"].shape == (8, 8, 3)

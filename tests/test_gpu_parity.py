@@ -112,6 +112,96 @@ def random_states_charged(n, q):
     return np.array(out)
 
 
+ELLIS_B = 2.0  # throat radius
+
+
+def wormhole_states(n):
+    """Cameras at areal radius 8-30 looking roughly at the throat: a mix of
+    rays through it (impact parameter < b) and rays deflected back."""
+    out = []
+    for _ in range(n):
+        pos = RNG.normal(size=3)
+        pos = pos / np.linalg.norm(pos) * RNG.uniform(8.0, 30.0)
+        d = -pos / np.linalg.norm(pos) + 0.15 * RNG.normal(size=3)
+        d /= np.linalg.norm(d)
+        x, u = O.init_ray(A.PRR_ELLIS, 0.0, ELLIS_B, pos, d)
+        out.append(np.concatenate([x, u]))
+    return np.array(out)
+
+
+def wormhole_context(T, r_esc2=0.0):
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_ELLIS, 0.0, 0.0, 0.0, p=[ELLIS_B, r_esc2] + [0.0] * 14)
+    ctx.set_tracer(T)
+    return ctx
+
+
+@pytest.mark.parametrize("solver", [A.PRR_EULER, A.PRR_RKF45])
+def test_trace_wormhole_matches_oracle(solver):
+    """Ellis wormhole (throat radius p[0] = 2): classes (escape only, on
+    either side of the throat) and end states as for the other metrics."""
+    T = trace_tracer(solver)
+    if solver == A.PRR_EULER:  # long enough to reach |l| = 60 from l <= 30
+        T.max_steps = T.max_attempts = 12000
+    x0 = wormhole_states(256)
+    g = wormhole_context(T, r_esc2=45.0).trace(x0)  # beyond the throat: escape at |l| = 45 (rout2)
+    S = O.Scene(A.PRR_ELLIS, 0.0, ELLIS_B, T)
+    S.set_far_escape(45.0)
+    o = S.trace(x0)
+    agree = g["cls"] == o["cls"]
+    assert agree.mean() >= 0.99
+    esc = agree & (o["cls"] == A.PRR_ESCAPE)
+    assert esc.mean() >= 0.95
+    far = esc & (o["xend"][:, 0] < 0)
+    assert far.sum() >= 30 and (esc & ~far).sum() >= 30
+    # the same side of the throat, and each side's escape radius
+    assert np.all((g["xend"][esc, 0] < 0) == (o["xend"][esc, 0] < 0))
+    if solver != A.PRR_EULER:  # located escapes
+        assert np.allclose(g["xend"][far, 0], -45.0, rtol=1e-12)
+        assert np.allclose(g["xend"][esc & ~far, 0], 60.0, rtol=1e-12)
+    err = np.abs(g["xend"][esc] - o["xend"][esc]).max(axis=1) / (np.abs(o["xend"][esc]).max(axis=1) + 1)
+    assert np.median(err) < (1e-12 if solver == A.PRR_EULER else 1e-8), np.sort(err)[-10:]
+    assert (err < 1e-6).mean() >= 0.98, np.sort(err)[-10:]
+
+
+def test_wormhole_render_shows_the_second_sky():
+    """A frame looking into the throat, 2x2 SSAA + FXAA, RKF45 tol 1e-8: rays
+    escaping at l < 0 take the second sky (slot PRR_SKY2_SLOT); per-ray
+    classes and endpoints against the oracle, RGB8 within 1 LSB."""
+    pos, al, be = CF.orbit_pose(12.0, 75.0, 0.4)
+    f = A.make_frame(pos, al, be, math.radians(30.0), 96, 64, ssaa=A.PRR_SSAA_REGULAR, samples=4, fxaa=1,
+                     mass=0.0)
+    T = CF.tracer_kerr()
+    T.r_escape = 60.0
+    ctx = wormhole_context(T)
+    ctx.set_texture(0, CF.sky_texture(), A.PRR_FILTER_SQAVG)
+    ctx.set_texture(A.PRR_SKY2_SLOT, CF.sphere_texture(), A.PRR_FILTER_SQAVG)
+    out = ctx.render(f, rays=True)[0]
+    # the same tracer (tol 1e-8): nothing re-imposes first integrals here, so
+    # the GPU's endpoints are compared with the oracle's at the same tolerance
+    S = O.Scene(A.PRR_ELLIS, 0.0, ELLIS_B, T)
+    S.set_texture(0, CF.sky_texture(), A.PRR_FILTER_SQAVG)
+    S.set_texture(A.PRR_SKY2_SLOT, CF.sphere_texture(), A.PRR_FILTER_SQAVG)
+    orc = S.render_rays(f, want_state=True)
+    assert np.all(orc["cls"] == A.PRR_ESCAPE)
+    far = orc["state8"][:, 0] < 0
+    assert 0.05 < far.mean() < 0.95
+    cmp = compare_rays(S, f, out, orc)
+    assert cmp["unexcused"] == 0, cmp["unexcused_ids"][:20]
+    ang = cmp["angles"]
+    print("endpoint angle: median %.2e p99 %.2e max %.2e" % (np.median(ang), np.quantile(ang, 0.99), ang.max()))
+    assert np.median(ang) < 1e-9 and np.quantile(ang, 0.99) < 1e-7
+    # the colour of each ray comes from the sky of its own side
+    assert np.abs(out["color"] - orc["color"]).max(axis=1).mean() < 1e-3
+    _, img, rgb8 = S.render_frame(f)
+    frac, _ = rgb8_agreement(out["rgb8"], rgb8)
+    assert frac >= 0.999
+    # without the second texture the far side is black
+    ctx.set_texture(A.PRR_SKY2_SLOT, None)
+    dark = ctx.render(f, rays=True)[0]
+    assert np.all(dark["color"][far] == 0) and np.array_equal(dark["color"][~far], out["color"][~far])
+
+
 @pytest.mark.parametrize("a,solver", [(0.0, A.PRR_RKF45), (0.6, A.PRR_RKF45), (0.95, A.PRR_RKF45),
                                       (0.95, A.PRR_DORMAND)])
 def test_trace_kerr_conserves_constants_and_matches_oracle(a, solver):

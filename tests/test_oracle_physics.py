@@ -44,6 +44,8 @@ def random_points(kind, n):
     for _ in range(n):
         if kind in (A.PRR_SCHW_SPH, A.PRR_KERR_BL):
             x = [RNG.uniform(3.0, 40.0), RNG.uniform(0.3, math.pi - 0.3), RNG.uniform(0, 2 * math.pi), 0.0]
+        elif kind == A.PRR_ELLIS:  # l on both sides of the throat
+            x = [RNG.uniform(-10.0, 10.0), RNG.uniform(0.3, math.pi - 0.3), RNG.uniform(0, 2 * math.pi), 0.0]
         else:
             v = RNG.normal(size=3)
             x = list(v / np.linalg.norm(v) * RNG.uniform(3.0, 40.0)) + [0.0]
@@ -53,10 +55,11 @@ def random_points(kind, n):
 
 
 @pytest.mark.parametrize("kind,a", [(A.PRR_SCHW_KS_CART, 0.0), (A.PRR_SCHW_SPH, 0.0), (A.PRR_KERR_BL, 0.0),
-                                    (A.PRR_KERR_BL, 0.6), (A.PRR_KERR_BL, 0.998), (A.PRR_SCHW_SPH, 0.8)])
+                                    (A.PRR_KERR_BL, 0.6), (A.PRR_KERR_BL, 0.998), (A.PRR_SCHW_SPH, 0.8),
+                                    (A.PRR_ELLIS, 0.5), (A.PRR_ELLIS, 2.0)])
 def test_geod_equals_minus_christoffel_of_metric(kind, a):
-    """a: Kerr's spin, or the charge of the Schwarzschild-coordinate metric
-    (Reissner-Nordstrom)."""
+    """a: Kerr's spin, the charge of the Schwarzschild-coordinate metric
+    (Reissner-Nordstrom), or the Ellis wormhole's throat radius."""
     for x, u in random_points(kind, 20):
         G = fd_christoffel(kind, 1.0, a, x)
         expect = -np.einsum("amn,m,n->a", G, u, u)
@@ -104,14 +107,14 @@ def test_kerr_metric_known_values():
     assert abs(O.metric(A.PRR_KERR_BL, 1.0, a, [re, th, 0.0, 0.0])[3, 3]) < 1e-13
 
 
-@pytest.mark.parametrize("kind", [A.PRR_FLAT, A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH, A.PRR_KERR_BL])
+@pytest.mark.parametrize("kind", [A.PRR_FLAT, A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH, A.PRR_KERR_BL, A.PRR_ELLIS])
 def test_initial_state_is_null(kind):
     for _ in range(20):
         pos = RNG.normal(size=3)
         pos = pos / np.linalg.norm(pos) * RNG.uniform(5, 50)
         d = RNG.normal(size=3)
         d /= np.linalg.norm(d)
-        a = 0.9 if kind == A.PRR_KERR_BL else 0.0
+        a = 0.9 if kind == A.PRR_KERR_BL else (2.0 if kind == A.PRR_ELLIS else 0.0)
         x, u = O.init_ray(kind, 1.0, a, pos, d)
         g = O.metric(kind, 1.0, a, x)
         norm = u @ g @ u

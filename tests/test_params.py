@@ -136,6 +136,20 @@ def test_new_keys_live_in_their_own_block(tmp_path):
     assert f.projection == A.PRR_PROJ_PINHOLE and f.samples == 1 and f.spin == 0.9
 
 
+def test_wormhole_reads_throat_and_far_escape_radius(tmp_path):
+    """metric = wormhole: p0 is the throat radius; p1, the escape radius
+    beyond the throat, defaults to the tracer block's rout2 (prr.cpp:190)."""
+    def spacetime(extra_tracer, extra_st):
+        p = tmp_path / "w.params"
+        p.write_text("_tracer_\n<rout1> = <60>\n" + extra_tracer + "_prr_\n<metric> = <wormhole>\n"
+                     "_spacetime_\n<hasBlackHole> = <0>\n<p0> = <2.5>\n" + extra_st)
+        return api.Params(str(p)).spacetime()
+    st = spacetime("<rout2> = <45>\n", "")
+    assert st["metric"] == A.PRR_ELLIS and st["p"][0] == 2.5 and st["p"][1] == 45.0
+    assert spacetime("<rout2> = <45>\n", "<p1> = <30>\n")["p"][1] == 30.0
+    assert spacetime("", "")["p"][1] == 0.0  # 0: the near side's r_escape
+
+
 def test_missing_file_is_an_io_error():
     with pytest.raises(PrrError) as e:
         api.Params("/nonexistent/sim.params")
