@@ -358,6 +358,43 @@ def test_launch_shape_does_not_change_results(refill, block):
     assert np.array_equal(ref["steps"], out["steps"])
 
 
+@pytest.mark.parametrize("metric", [A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH, A.PRR_ELLIS])
+def test_fp32_bulk_second_order_metrics_track_fp64(metric):
+    """The 8-component metrics in FP32-bulk: FP32 stages of the full
+    right-hand side (no difference form), FP64 state, error norm and
+    controller.  At RKF45 tol 1e-8 (FP32 rounding of the stages, ~6e-8 of
+    h f, stays below the tolerance for steps up to ~0.1 of the state's
+    scale), classes agree with the FP64 trace and escape end states agree to
+    the tolerance's scale."""
+    T64 = trace_tracer(A.PRR_RKF45)
+    T64.atol = T64.rtol = 1e-8
+    T32 = trace_tracer(A.PRR_RKF45)
+    T32.atol = T32.rtol = 1e-8
+    T32.precision = A.PRR_FP32_BULK
+    T32.r_fp64 = 0.0  # FP32 stages everywhere
+    if metric == A.PRR_ELLIS:
+        x0 = wormhole_states(256)
+        g64, g32 = (wormhole_context(T).trace(x0) for T in (T64, T32))
+    else:
+        x0 = random_states(metric, 256)
+        out = []
+        for T in (T64, T32):
+            ctx = api.Context(0)
+            ctx.set_spacetime(metric, 1.0, 0.0)
+            ctx.set_tracer(T)
+            out.append(ctx.trace(x0))
+        g64, g32 = out
+    agree = g64["cls"] == g32["cls"]
+    assert agree.mean() >= 0.98
+    esc = agree & (g64["cls"] == A.PRR_ESCAPE)
+    assert esc.sum() >= 50
+    err = np.abs(g32["xend"][esc] - g64["xend"][esc]).max(axis=1) / (np.abs(g64["xend"][esc]).max(axis=1) + 1)
+    print("fp32-bulk vs fp64 escape states: median %.2e p99 %.2e" % (np.median(err), np.quantile(err, 0.99)))
+    assert np.median(err) < 1e-6 and np.quantile(err, 0.9) < 1e-5
+    # FP32 stage rounding costs rejected attempts, not a collapse of the step size
+    assert g32["attempts"][esc].sum() < 3 * g64["attempts"][esc].sum()
+
+
 def test_fp32_bulk_close_to_fp64():
     f = CF.frame_c1(192, 108, ssaa=1, fxaa=0)
     c64 = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP64)).render(f, rays=True)[0]
