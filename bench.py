@@ -169,6 +169,18 @@ def ctx_tracer(A, CF):
     return CF.tracer_kerr(precision=A.PRR_FP32_BULK)
 
 
+def cpu_model():
+    """The host CPU's model name (first processor in /proc/cpuinfo)."""
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(frame, row_stride=16):
     """Oracle (FP64, Christoffel form) on every row_stride-th image row, all cores."""
     import numpy as np
@@ -184,7 +196,7 @@ def cpu_baseline(frame, row_stride=16):
     ids = (rows[:, None] * per_row + np.arange(per_row)[None, :]).ravel()
     cores = len(os.sched_getaffinity(0))
     r = S.render_rays(frame, ids, nthreads=cores)
-    return dict(value=ids.size / r["seconds"], unit="rays/s", cores=cores, kind="port",
+    return dict(value=ids.size / r["seconds"], unit="rays/s", cores=cores, kind="port", model=cpu_model(),
                 sample=f"C1 rows j % {row_stride} == 0 ({ids.size} rays of {frame.nrays}), FP64 RKF45 tol 1e-8, "
                        f"{r['seconds']:.1f} s")
 
