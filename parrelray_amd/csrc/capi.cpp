@@ -55,6 +55,7 @@ struct prr_ctx {
     DevBuf rs_y, rs_cls, rs_hit, rs_steps, rs_attempts, loc;
     // adjoint
     DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_ckpt2, adj_event, adj_rays, adj_rec, adj_lam, adj_grad_ray,
+        adj_task_ray, adj_task_seg, adj_levelstart, adj_nseg, adj_over, adj_plan,
         adj_grad;
     hipEvent_t ev[6] = {};
     prr_stats stats{};
@@ -198,6 +199,13 @@ prr_status validate_frame(prr_ctx *ctx, const prr_frame &f, int idx) {
 // Schwarzschild-coordinate metric, the Ellis throat radius.
 double metric_param(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx->p[0] : ctx->q; }
 // Escape radius beyond the wormhole's throat: p[1] (the reference's rout2), else r_escape.
+// Device memory for the replayed steps of one chunk of adjoint tasks (48 B
+// per step, K steps per task); larger replays run in several chunks.
+constexpr size_t kRecBudget = size_t(8) << 30;
+// Segment-count histogram bins of the device-side adjoint plan; tracers whose
+// step cap allows more segments per ray plan on the host.
+constexpr int32_t kPlanBins = 4096;
+
 double far_escape(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx->p[1] : 0.0; }
 
 int blocks_per_cu(const prr_ctx *ctx) {
@@ -306,6 +314,8 @@ void prr_destroy(prr_ctx *ctx) {
                       &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
                       &ctx->attempts, &ctx->state8, &ctx->x0u0, &ctx->adj_img, &ctx->adj_dldi,
                       &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_ckpt2, &ctx->adj_event, &ctx->adj_rays, &ctx->adj_rec,
+                      &ctx->adj_task_ray, &ctx->adj_task_seg, &ctx->adj_levelstart, &ctx->adj_nseg,
+                      &ctx->adj_over, &ctx->adj_plan,
                       &ctx->adj_lam, &ctx->rs_y, &ctx->rs_cls, &ctx->rs_hit, &ctx->rs_steps, &ctx->rs_attempts,
                       &ctx->loc, &ctx->adj_grad_ray,
                       &ctx->adj_grad};
@@ -843,7 +853,6 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if ((s = ensure(ctx, ctx->adj_rays, 8 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_event, sizeof(EventDev) * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_lam, 80 * size_t(T))) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->adj_rec, 48 * size_t(K) * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad_ray, 32 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad, 32)) != PRR_OK) return s;
     // first pass: the render, which also stores each ray's first `cap`
@@ -863,54 +872,105 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     if ((s = enqueue_render(ctx, P, job, st, false, true)) != PRR_OK) return s;
-    std::vector<int32_t> steps(T);
-    HIPTRY(ctx, hipMemcpyAsync(steps.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
-    if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises
-    prr_stats fwd = ctx->stats;
-    // checkpoints at steps 0, K, 2K, ... < N_i: ray i keeps its first-pass
-    // slots i * cap when it has at most cap segments, else its checkpoints
-    // go to the overflow area at -1 - off[i]; rays by decreasing segment count
-    // (a stable counting sort), so that the rays with more than j segments
-    // are a prefix of the order
-    std::vector<int64_t> off(T), order(T), over;
-    std::vector<int32_t> nseg(T);
-    int64_t nck2 = 0, swept = 0;
-    int32_t nseg_max = 0;
-    for (int64_t i = 0; i < T; ++i) {
-        nseg[i] = steps[i] > 0 ? (steps[i] - 1) / K + 1 : 0;
-        if (nseg[i] <= cap) {
-            off[i] = i * cap;
-        } else {
-            off[i] = -1 - nck2;
-            nck2 += nseg[i];
-            over.push_back(i);
-        }
-        swept += steps[i];
-        nseg_max = std::max(nseg_max, nseg[i]);
-    }
-    std::vector<int64_t> hist(size_t(nseg_max) + 1, 0), start(size_t(nseg_max) + 2, 0);
-    for (int64_t i = 0; i < T; ++i) hist[nseg[i]]++;
-    for (int32_t n = nseg_max; n >= 0; --n) start[n] = start[n + 1] + (n + 1 <= nseg_max ? hist[n + 1] : 0);
-    for (int64_t i = 0; i < T; ++i) order[start[nseg[i]]++] = i;
-    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_off.p, off.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
+    // dL/dI from the host: staged while the first pass runs
     const float *dldi = dL_dimg;
     if (!on_device) {
         if ((s = ensure(ctx, ctx->adj_dldi, 12 * size_t(npix))) != PRR_OK) return s;
         HIPTRY(ctx, hipMemcpyAsync(ctx->adj_dldi.p, dL_dimg, 12 * size_t(npix), hipMemcpyHostToDevice, st));
         dldi = static_cast<const float *>(ctx->adj_dldi.p);
     }
+    // segments per ray, checkpoint offsets and the rays by decreasing segment
+    // count (so that the rays with more than j segments are a prefix of the
+    // order): on the device when every segment count fits kPlanBins
+    // histogram bins, else on the host (a stable counting sort).  A ray keeps
+    // its first-pass checkpoint slots i * cap when it has at most cap
+    // segments; else its checkpoints go to the overflow area at -1 - off[i].
+    const int64_t seg_bound = (ctx->tracer.max_steps - 1) / K + 2; // exceeds every segment count
+    const bool plan_dev = seg_bound <= kPlanBins;
+    std::vector<unsigned long long> hist; // rays per segment count 0..nseg_max
+    unsigned long long pctr[4] = {0, 0, 0, 0};
+    int64_t nck2 = 0, swept = 0, n_over = 0;
+    int32_t nseg_max = 0;
+    if ((s = ensure(ctx, ctx->adj_nseg, 4 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_over, 8 * size_t(T))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_plan, 8 * size_t(2 * kPlanBins + 4))) != PRR_OK) return s;
+    AdjPlanDev plan{};
+    plan.steps = static_cast<const int32_t *>(ctx->steps.p);
+    plan.T = T;
+    plan.K = K;
+    plan.cap = cap;
+    plan.nbins = kPlanBins;
+    plan.nseg = static_cast<int32_t *>(ctx->adj_nseg.p);
+    plan.off = static_cast<int64_t *>(ctx->adj_off.p);
+    plan.over = static_cast<int64_t *>(ctx->adj_over.p);
+    plan.hist = static_cast<unsigned long long *>(ctx->adj_plan.p);
+    plan.cursor = plan.hist + kPlanBins;
+    plan.ctr = plan.cursor + kPlanBins;
+    plan.order = static_cast<int64_t *>(ctx->adj_rays.p);
+    // host buffers of the copies below live until the final synchronisation
+    std::vector<int64_t> order_h, off_h, over_h;
+    std::vector<int32_t> steps_h;
+    std::vector<unsigned long long> cursor_h;
+    if (plan_dev) {
+        HIPTRY(ctx, launch_adj_plan_count(plan, st));
+        hist.assign(size_t(kPlanBins), 0ull);
+        HIPTRY(ctx, hipMemcpyAsync(hist.data(), plan.hist, 8 * size_t(kPlanBins), hipMemcpyDeviceToHost, st));
+        HIPTRY(ctx, hipMemcpyAsync(pctr, plan.ctr, sizeof pctr, hipMemcpyDeviceToHost, st));
+    } else {
+        steps_h.resize(size_t(T));
+        HIPTRY(ctx, hipMemcpyAsync(steps_h.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
+    }
+    if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises
+    prr_stats fwd = ctx->stats;
+    if (plan_dev) {
+        nck2 = int64_t(pctr[0]);
+        n_over = int64_t(pctr[1]);
+        swept = int64_t(pctr[2]);
+        nseg_max = int32_t(pctr[3]);
+        hist.resize(size_t(nseg_max) + 1);
+        // start of each segment count in the order (decreasing counts first)
+        cursor_h.assign(size_t(nseg_max) + 1, 0ull);
+        for (int32_t n = nseg_max - 1; n >= 0; --n) cursor_h[n] = cursor_h[n + 1] + hist[n + 1];
+        HIPTRY(ctx, hipMemcpyAsync(plan.cursor, cursor_h.data(), 8 * cursor_h.size(), hipMemcpyHostToDevice, st));
+        HIPTRY(ctx, launch_adj_plan_scatter(plan, st));
+    } else {
+        std::vector<int32_t> nseg(static_cast<size_t>(T));
+        off_h.resize(size_t(T));
+        order_h.resize(size_t(T));
+        for (int64_t i = 0; i < T; ++i) {
+            nseg[i] = steps_h[i] > 0 ? (steps_h[i] - 1) / K + 1 : 0;
+            if (nseg[i] <= cap) {
+                off_h[i] = i * cap;
+            } else {
+                off_h[i] = -1 - nck2;
+                nck2 += nseg[i];
+                over_h.push_back(i);
+            }
+            swept += steps_h[i];
+            nseg_max = std::max(nseg_max, nseg[i]);
+        }
+        n_over = int64_t(over_h.size());
+        hist.assign(size_t(nseg_max) + 1, 0ull);
+        for (int64_t i = 0; i < T; ++i) hist[nseg[i]]++;
+        std::vector<int64_t> start(size_t(nseg_max) + 2, 0);
+        for (int32_t n = nseg_max; n >= 0; --n) start[n] = start[n + 1] + (n + 1 <= nseg_max ? int64_t(hist[n + 1]) : 0);
+        for (int64_t i = 0; i < T; ++i) order_h[start[nseg[i]]++] = i;
+        HIPTRY(ctx, hipMemcpyAsync(plan.off, off_h.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
+        HIPTRY(ctx, hipMemcpyAsync(plan.order, order_h.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
+        if (n_over > 0)
+            HIPTRY(ctx, hipMemcpyAsync(plan.over, over_h.data(), 8 * size_t(n_over), hipMemcpyHostToDevice, st));
+    }
     HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
     JobDev j2 = job;
     j2.out = OutDev{};
     j2.adj.nsteps = static_cast<const int32_t *>(ctx->steps.p);
     j2.adj.ckpt_off = static_cast<const int64_t *>(ctx->adj_off.p);
-    if (!over.empty()) {
+    if (n_over > 0) {
         // the rays with more than cap segments, from the start again, storing all their checkpoints
         if ((s = ensure(ctx, ctx->adj_ckpt2, 64 * size_t(nck2))) != PRR_OK) return s;
         j2.adj.ckpt2 = static_cast<double *>(ctx->adj_ckpt2.p);
-        HIPTRY(ctx, hipMemcpyAsync(ctx->adj_rays.p, over.data(), 8 * over.size(), hipMemcpyHostToDevice, st));
-        j2.adj.rays = static_cast<const int64_t *>(ctx->adj_rays.p);
-        j2.total = int64_t(over.size());
+        j2.adj.rays = plan.over;
+        j2.total = n_over;
         HIPTRY(ctx, launch_reset(ctr, st, true));
         JobDev ji = j2;
         ji.total = T;
@@ -919,17 +979,53 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         HIPTRY(ctx, launch_integrate(ctx->metric, j2, grid, ctx->launch.block, st, true));
         j2.total = T;
     }
-    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_rays.p, order.data(), 8 * size_t(T), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, launch_reset(ctr, st, true));
-    // pass 3: per segment index, from the last: replay then reverse sweep
+    // pass 3: replay, then the reverse sweep.  A task is one segment j of one
+    // ray; level j holds the rays with more than j segments (a prefix of the
+    // order), and levels run from the last.  Levels are replayed in chunks
+    // whose records fit kRecBudget (all of them at once when they fit), each
+    // chunk in one launch, and each chunk's reverse sweep is one launch over
+    // its rays: the replays of different segments run concurrently instead
+    // of one launch per segment index.
+    std::vector<int64_t> count(size_t(nseg_max) + 1, 0), levelstart(size_t(nseg_max) + 1, 0);
+    for (int32_t j = nseg_max - 1; j >= 0; --j) count[j] = count[j + 1] + int64_t(hist[j + 1]);
+    int64_t ntask = 0;
+    for (int32_t j = nseg_max - 1; j >= 0; --j) {
+        levelstart[j] = ntask;
+        ntask += count[j];
+    }
+    if ((s = ensure(ctx, ctx->adj_task_ray, 8 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_task_seg, 4 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_levelstart, 8 * levelstart.size())) != PRR_OK) return s;
+    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_levelstart.p, levelstart.data(), 8 * levelstart.size(), hipMemcpyHostToDevice,
+                               st));
+    plan.levelstart = static_cast<const int64_t *>(ctx->adj_levelstart.p);
+    plan.nlevels = nseg_max;
+    plan.ntask = ntask;
+    plan.task_ray = static_cast<int64_t *>(ctx->adj_task_ray.p);
+    plan.task_seg = static_cast<int32_t *>(ctx->adj_task_seg.p);
+    HIPTRY(ctx, launch_adj_plan_tasks(plan, st));
+    const size_t rec_task = 48 * size_t(K); // bytes of records per task
+    const int64_t chunk_cap = std::max<int64_t>(1, int64_t(kRecBudget / rec_task));
+    std::vector<std::pair<int32_t, int32_t>> chunks; // (jhi, jlo)
+    for (int32_t j = nseg_max - 1; j >= 0;) {
+        int32_t lo = j;
+        int64_t n = count[j];
+        while (lo > 0 && n + count[lo - 1] <= chunk_cap) n += count[--lo];
+        chunks.emplace_back(j, lo);
+        j = lo - 1;
+    }
+    int64_t chunk_max = 0;
+    for (const auto &c : chunks) chunk_max = std::max(chunk_max, levelstart[c.second] + count[c.second] - levelstart[c.first]);
+    if ((s = ensure(ctx, ctx->adj_rec, rec_task * size_t(std::max<int64_t>(chunk_max, 1)))) != PRR_OK) return s;
     JobDev j3 = j2;
     j3.adj.mode = ADJ_REPLAY;
-    j3.adj.rays = static_cast<const int64_t *>(ctx->adj_rays.p);
     j3.adj.rec = static_cast<double *>(ctx->adj_rec.p);
     j3.adj.event = static_cast<EventDev *>(ctx->adj_event.p);
     AdjDev ad{};
     ad.K = K;
-    ad.rays = j3.adj.rays;
+    ad.rays = static_cast<const int64_t *>(ctx->adj_rays.p);
+    ad.levelstart = static_cast<const int64_t *>(ctx->adj_levelstart.p);
     ad.nsteps = j2.adj.nsteps;
     ad.cls = static_cast<const int8_t *>(ctx->cls.p);
     ad.rec = j3.adj.rec;
@@ -942,17 +1038,20 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     ad.cam_phi = orbit->cam_phi;
     ad.sphere_r = orbit->sphere_r;
     ad.mismatch = &ctr->mismatch;
-    // rays with more than j segments: a prefix of the sorted order
-    std::vector<int64_t> count(size_t(nseg_max) + 1, 0);
-    for (int32_t j = nseg_max - 1; j >= 0; --j) count[j] = count[j + 1] + hist[j + 1];
-    for (int32_t j = nseg_max - 1; j >= 0; --j) {
-        j3.adj.seg = j;
-        j3.total = count[j];
+    for (const auto &c : chunks) {
+        const int64_t t0 = levelstart[c.first], t1 = levelstart[c.second] + count[c.second];
+        j3.adj.rays = static_cast<const int64_t *>(ctx->adj_task_ray.p) + t0;
+        j3.adj.seg = static_cast<const int32_t *>(ctx->adj_task_seg.p) + t0;
+        j3.adj.rec_stride = t1 - t0;
+        j3.total = t1 - t0;
         HIPTRY(ctx, launch_reset(ctr, st, false));
-        const int grid = int(std::min<int64_t>(integrate_grid(ctx), (count[j] + 63) / 64 + 1));
+        const int grid = int(std::min<int64_t>(integrate_grid(ctx), (j3.total + 63) / 64 + 1));
         HIPTRY(ctx, launch_integrate(ctx->metric, j3, grid, ctx->launch.block, st));
-        ad.seg = j;
-        ad.count = count[j];
+        ad.jhi = c.first;
+        ad.jlo = c.second;
+        ad.t0 = t0;
+        ad.rec_stride = t1 - t0;
+        ad.count = count[c.second];
         HIPTRY(ctx, launch_adj_reverse(j3, ad, st));
     }
     JobDev jg = job;

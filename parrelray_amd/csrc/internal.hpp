@@ -104,14 +104,16 @@ enum { ADJ_NONE = 0, ADJ_CHECKPOINT = 1, ADJ_REPLAY = 2 };
 struct AdjRunDev {
     int32_t mode;            // ADJ_*
     int32_t K;               // accepted steps between checkpoints
-    int32_t seg;             // replay: the segment index j
+    int32_t pad;
     int32_t cap;             // first pass (ckpt_off null): checkpoint slots per ray
     const int32_t *nsteps;   // accepted steps of every ray (null in the first pass)
     const int64_t *ckpt_off; // first checkpoint of each ray: >= 0 in ckpt, < 0 at -1 - off in ckpt2
     double *ckpt;            // 8 doubles per checkpoint: y[6], h, attempts
     double *ckpt2;           // checkpoints of the rays with more than cap segments
     const int64_t *rays;     // replay, and a checkpoint pass over some rays: ray of each job index
-    double *rec;             // replay: K * 6 doubles per ray, [slot * 6 + c] * rs_total + ray
+    const int32_t *seg;      // replay: segment index j of each job index (a task is one segment of one ray)
+    double *rec;             // replay: K * 6 doubles per task, [slot * 6 + c] * rec_stride + task
+    int64_t rec_stride;      // replay: tasks of this launch
     EventDev *event;         // replay: terminal event of each ray
     unsigned long long *mismatch; // replay: segments that did not reproduce their checkpoint
 };
@@ -147,12 +149,15 @@ struct JobDev {
 // Kerr adjoint reverse pass (per ray).
 struct AdjDev {
     int32_t K;
-    int32_t seg;             // segment index j of this launch
-    int64_t count;           // rays in this launch (the first `count` of `rays`)
-    const int64_t *rays;     // rays sorted by decreasing step count
+    int32_t jlo, jhi;        // segment levels of this launch's replayed chunk, jhi >= jlo
+    int64_t count;           // rays in this launch (the first `count` of `rays`: those with a segment jlo)
+    const int64_t *rays;     // rays sorted by decreasing segment count
+    const int64_t *levelstart; // task index of (level j, rank 0); level j holds the ranks < count of level j
+    int64_t t0;              // first task of the chunk (its records start there)
+    int64_t rec_stride;      // tasks in the chunk
     const int32_t *nsteps;   // accepted steps of every ray
     const int8_t *cls;       // termination class of every ray
-    const double *rec;       // replayed steps of segment j (AdjRunDev.rec layout)
+    const double *rec;       // replayed steps of the chunk (AdjRunDev.rec layout)
     const EventDev *event;   // terminal events
     double *lam;             // 10 doubles per ray: lambda (y[0..4], E, L, Q, a), dL/dphi_s
     const float *dLdI;       // dL/dI, W*H*3
@@ -162,6 +167,26 @@ struct AdjDev {
     double cam_phi;          // fixed orbit parameters
     double sphere_r;
     unsigned long long *mismatch; // rays whose replay did not end with their class
+};
+
+// Device-side planning of the adjoint's replay (after the checkpointing pass).
+struct AdjPlanDev {
+    const int32_t *steps;        // accepted steps of every ray
+    int64_t T;                   // rays
+    int32_t K, cap;              // checkpoint interval; first-pass checkpoint slots per ray
+    int32_t nbins;               // histogram bins (> every segment count)
+    int32_t nlevels;             // tasks: segment levels (the largest segment count)
+    int32_t *nseg;               // segments of every ray
+    int64_t *off;                // checkpoint offsets (AdjRunDev.ckpt_off)
+    int64_t *over;               // the rays with more than cap segments, any order
+    unsigned long long *hist;    // rays per segment count
+    unsigned long long *ctr;     // [0] overflow checkpoints, [1] overflow rays, [2] accepted steps, [3] max segments
+    unsigned long long *cursor;  // order: next position of each segment count (start positions from the host)
+    int64_t *order;              // rays by decreasing segment count
+    const int64_t *levelstart;   // tasks: first task of level j (levels descending: level nlevels-1 first)
+    int64_t ntask;
+    int64_t *task_ray;
+    int32_t *task_seg;
 };
 
 // Post-processing of one frame (SSAA combine, FXAA, RGB8).

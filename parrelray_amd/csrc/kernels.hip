@@ -74,10 +74,12 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                     __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
                 const int64_t idx = int64_t(base) + rank;
                 if (idx < job.total) {
-                    if (replay) {
+                    if (replay) { // task idx: segment adj.seg[idx] of ray adj.rays[idx]
                         const int64_t rid = job.adj.rays[idx];
-                        lane_load_checkpoint<Mt>(job, L, rid, job.adj.seg);
-                        end = min((job.adj.seg + 1) * job.adj.K, job.adj.nsteps[rid]);
+                        const int32_t seg = job.adj.seg[idx];
+                        lane_load_checkpoint<Mt>(job, L, rid, seg);
+                        L.rec_i = idx;
+                        end = min((seg + 1) * job.adj.K, job.adj.nsteps[rid]);
                     } else {
                         // a checkpoint pass over some of the rays takes them from adj.rays
                         const bool some = Mt::adj && job.adj.mode == ADJ_CHECKPOINT && job.adj.rays;
@@ -101,7 +103,7 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                 if (!replay) {
                     lane_store<Mt>(job, L, cls);
                 } else if (cls < 0) { // a segment's replay must land on the next checkpoint bit for bit
-                    const double *nx = ckpt_at(job, L.rid, job.adj.seg + 1);
+                    const double *nx = ckpt_at(job, L.rid, L.seg + 1);
                     bool same = L.steps == end;
                     for (int i = 0; i < Mt::N; ++i) same = same && L.y[i] == nx[i];
                     if (!same) atomicAdd(job.adj.mismatch, 1ull);
@@ -339,18 +341,26 @@ __device__ void adj_terminal(const JobDev &job, const AdjDev &ad, const FrameDev
 // Reverse sweep over segment j of every ray of this launch: the terminal seed
 // on the ray's last segment, then lambda_n = lambda_{n+1} dPhi_n/dy_n over
 // the replayed steps.  lambda lives in ad.lam between launches.
+// Reverse sweep over the segment levels jhi..jlo of one replayed chunk: one
+// thread per ray (rank idx in the decreasing-segment-count order), from its
+// highest segment in the chunk down to jlo, reading segment j's steps from
+// task levelstart[j] + idx of the chunk's records.  lambda enters from the
+// terminal event (the ray's last segment) or from ad.lam (the previous
+// chunk) and leaves through ad.lam.
 __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= ad.count) return;
     const int64_t id = ad.rays[idx];
     const int32_t N = ad.nsteps[id];
-    const int K = ad.K, j = ad.seg;
+    const int K = ad.K;
+    const int32_t last = (N - 1) / K; // the ray's last segment (the rank order puts it at >= jlo)
     double *lm = ad.lam + 10 * id;
     const FrameDev &fr = job.frames[find_frame(job, id)];
     Consts<double> cs;
     load_consts<Kerr>(job, id, fr, cs);
     double lam[9], gphs;
-    if (j == (N - 1) / K) { // the ray's last segment
+    const int32_t jtop = min(last, ad.jhi);
+    if (jtop == last) {
         const int cls = ad.cls[id];
         double w[3] = {0.0, 0.0, 0.0};
         if (cls == PRR_ESCAPE || cls == PRR_SPHERE)
@@ -368,14 +378,17 @@ __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
     bool zero = true;
     for (int i = 0; i < 9; ++i) zero = zero && lam[i] == 0.0;
     if (!zero) {
-        const int64_t T = job.rs_total;
-        const int32_t end = min((j + 1) * K, N - 1); // step N-1 is the located sub-step above
-        for (int32_t n = end - 1; n >= j * K; --n) {
-            const double *r = ad.rec + (int64_t(n - j * K) * 6) * T + id;
-            double yn[6];
-            for (int c = 0; c < 5; ++c) yn[c] = r[c * T];
-            yn[5] = 0.0;
-            kerr_step_vjp(yn, r[5 * T], cs, lam);
+        const int64_t T = ad.rec_stride;
+        for (int32_t j = jtop; j >= ad.jlo; --j) {
+            const int64_t col = ad.levelstart[j] - ad.t0 + idx;
+            const int32_t end = min((j + 1) * K, N - 1); // step N-1 is the located sub-step above
+            for (int32_t n = end - 1; n >= j * K; --n) {
+                const double *r = ad.rec + (int64_t(n - j * K) * 6) * T + col;
+                double yn[6];
+                for (int c = 0; c < 5; ++c) yn[c] = r[c * T];
+                yn[5] = 0.0;
+                kerr_step_vjp(yn, r[5 * T], cs, lam);
+            }
         }
     }
     for (int i = 0; i < 9; ++i) lm[i] = lam[i];
@@ -415,6 +428,102 @@ __global__ void __launch_bounds__(256) k_adj_init_grad(JobDev job, AdjDev ad) {
     bool fin = true;
     for (int k = 0; k < 4; ++k) fin = fin && isfinite(out[k]);
     for (int k = 0; k < 4; ++k) g[k] = fin ? out[k] : 0.0;
+}
+
+// ------------------------------------------------------------ adjoint plan
+// Wave-aggregated claim of one slot of cnt[key] per lane with `on`: one
+// atomic per distinct key in the wave; lanes of one key get consecutive
+// slots in lane order.  Every lane of the wave must call it.
+__device__ unsigned long long wave_claim(unsigned long long *cnt, int key, bool on) {
+    const int lane = int(threadIdx.x & 63);
+    unsigned long long rem = __ballot(on), res = 0;
+    while (rem) {
+        const int leader = __ffsll((long long)rem) - 1;
+        const int k = __shfl(key, leader);
+        const unsigned long long m = rem & __ballot(key == k);
+        unsigned long long base = 0;
+        if (lane == leader) base = atomicAdd(&cnt[k], (unsigned long long)__popcll(m));
+        base = shfl_u64(base, leader);
+        if ((m >> lane) & 1ull) res = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
+        rem &= ~m;
+    }
+    return res;
+}
+
+__device__ unsigned long long wave_sum(unsigned long long v) {
+    const int lane = int(threadIdx.x & 63);
+    for (int o = 32; o > 0; o >>= 1) v += shfl_u64(v, lane ^ o);
+    return v;
+}
+
+__global__ void __launch_bounds__(256) k_adj_plan_clear(AdjPlanDev p) {
+    const int i = int(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (i < p.nbins) p.hist[i] = 0ull;
+    if (i < 4) p.ctr[i] = 0ull;
+}
+
+__global__ void __launch_bounds__(256) k_adj_plan_count(AdjPlanDev p) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool on = i < p.T;
+    const int32_t st = on ? p.steps[i] : 0;
+    const int32_t n = st > 0 ? (st - 1) / p.K + 1 : 0;
+    if (on) {
+        p.nseg[i] = n;
+        if (n <= p.cap) {
+            p.off[i] = i * p.cap;
+        } else { // rare: checkpoints in the overflow area
+            const unsigned long long base = atomicAdd(&p.ctr[0], (unsigned long long)n);
+            p.off[i] = -1 - int64_t(base);
+            p.over[atomicAdd(&p.ctr[1], 1ull)] = i;
+        }
+    }
+    (void)wave_claim(p.hist, min(n, p.nbins - 1), on);
+    const unsigned long long sum = wave_sum((unsigned long long)st);
+    unsigned mx = unsigned(n);
+    for (int o = 32; o > 0; o >>= 1) mx = max(mx, unsigned(__shfl_xor(int(mx), o)));
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&p.ctr[2], sum);
+        atomicMax(&p.ctr[3], (unsigned long long)mx);
+    }
+}
+
+__global__ void __launch_bounds__(256) k_adj_plan_scatter(AdjPlanDev p) {
+    const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    const bool on = i < p.T;
+    const int32_t n = on ? p.nseg[i] : 0;
+    const unsigned long long pos = wave_claim(p.cursor, n, on);
+    if (on) p.order[pos] = i;
+}
+
+__global__ void __launch_bounds__(256) k_adj_plan_tasks(AdjPlanDev p) {
+    const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (t >= p.ntask) return;
+    // level j of task t: the smallest j with levelstart[j] <= t (levelstart falls with j)
+    int lo = 0, hi = p.nlevels - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (p.levelstart[mid] <= t) hi = mid;
+        else lo = mid + 1;
+    }
+    p.task_ray[t] = p.order[t - p.levelstart[lo]];
+    p.task_seg[t] = lo;
+}
+
+hipError_t launch_adj_plan_count(const AdjPlanDev &p, hipStream_t st) {
+    hipLaunchKernelGGL(k_adj_plan_clear, dim3(unsigned((p.nbins + 255) / 256)), dim3(256), 0, st, p);
+    if (p.T > 0) hipLaunchKernelGGL(k_adj_plan_count, dim3(unsigned((p.T + 255) / 256)), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_plan_scatter(const AdjPlanDev &p, hipStream_t st) {
+    if (p.T > 0) hipLaunchKernelGGL(k_adj_plan_scatter, dim3(unsigned((p.T + 255) / 256)), dim3(256), 0, st, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_adj_plan_tasks(const AdjPlanDev &p, hipStream_t st) {
+    if (p.ntask > 0)
+        hipLaunchKernelGGL(k_adj_plan_tasks, dim3(unsigned((p.ntask + 255) / 256)), dim3(256), 0, st, p);
+    return hipGetLastError();
 }
 
 // Deterministic sum of the per-ray gradients: one workgroup, fixed order.

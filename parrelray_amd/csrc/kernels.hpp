@@ -23,6 +23,13 @@ hipError_t launch_fxaa(const PostFrameDev *pfs, int nframes, int max_w, int max_
 // Kerr adjoint: reverse sweep over one segment index, and the set-up Jacobian.
 hipError_t launch_adj_reverse(const JobDev &job, const AdjDev &ad, hipStream_t st);
 hipError_t launch_adj_init_grad(const JobDev &job, const AdjDev &ad, hipStream_t st);
+// Adjoint planning: segment counts, checkpoint offsets, the overflow rays and
+// the segment-count histogram (clear + count); the order by decreasing
+// segment count (scatter, from the host's start positions in cursor); the
+// replay tasks of every level (tasks).
+hipError_t launch_adj_plan_count(const AdjPlanDev &p, hipStream_t st);
+hipError_t launch_adj_plan_scatter(const AdjPlanDev &p, hipStream_t st);
+hipError_t launch_adj_plan_tasks(const AdjPlanDev &p, hipStream_t st);
 // Deterministic sum of n per-ray gradients (4 doubles each) into out[4].
 hipError_t launch_grad_reduce(const double *grad_ray, int64_t n, double *out, hipStream_t st);
 

@@ -139,6 +139,8 @@ template <class Mt> struct Lane {
     int32_t hit; // sphere index of a SPHERE termination
     int64_t rid;
     float P0[3]; // FP32 Cartesian position at the start of the step (sphere screen)
+    int32_t seg;   // adjoint replay: the segment being replayed
+    int64_t rec_i; // adjoint replay: the task's record column
 };
 
 PRR_HD int find_frame(const JobDev &job, int64_t id) {
@@ -483,6 +485,8 @@ template <class Mt> PRR_HD void lane_load(const JobDev &job, Lane<Mt> &L, int64_
     L.hit = -1;
     L.P0[0] = L.P0[1] = L.P0[2] = 0.0f;
     if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
+    L.seg = 0;
+    L.rec_i = 0;
 }
 
 // Checkpoint j of ray id (8 doubles): slot id * cap + j in the first pass,
@@ -507,6 +511,7 @@ template <class Mt> PRR_HD void lane_load_checkpoint(const JobDev &job, Lane<Mt>
     L.attempts = int32_t(c[7]);
     L.lam = Mt::mino ? L.y[5] : 0.0;
     L.steps = j * job.adj.K;
+    L.seg = j;
     L.att64 = 0;
     L.hit = -1;
     L.P0[0] = L.P0[1] = L.P0[2] = 0.0f;
@@ -687,9 +692,9 @@ template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<M
 // Adjoint replay: the step's start and size of every accepted step of the segment.
 template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &L, double h) {
     if (!Mt::adj || job.adj.mode != ADJ_REPLAY) return;
-    const int64_t slot = (L.steps - 1) - int64_t(job.adj.seg) * job.adj.K; // L.steps already counts this step
-    const int64_t T = job.rs_total;
-    double *r = job.adj.rec + (slot * 6) * T + rs_index(job, L.rid);
+    const int64_t slot = (L.steps - 1) - int64_t(L.seg) * job.adj.K; // L.steps already counts this step
+    const int64_t T = job.adj.rec_stride;
+    double *r = job.adj.rec + (slot * 6) * T + L.rec_i;
     for (int c = 0; c < 5; ++c) r[c * T] = L.y[c];
     r[5 * T] = h;
 }
