@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -16,6 +17,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "../../include/prr_debug.h"
 #include "frame_host.hpp"
 #include "kernels.hpp"
 #include "prr_host.hpp"
@@ -202,6 +204,7 @@ double metric_param(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx-
 // Device memory for the replayed steps of one chunk of adjoint tasks (48 B
 // per step, K steps per task); larger replays run in several chunks.
 constexpr size_t kRecBudget = size_t(8) << 30;
+std::atomic<uint64_t> g_rec_budget{0}; // prr_debug_adjoint_record_budget (0: kRecBudget)
 // Segment-count histogram bins of the device-side adjoint plan; tracers whose
 // step cap allows more segments per ray plan on the host.
 constexpr int32_t kPlanBins = 4096;
@@ -1006,7 +1009,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     plan.task_seg = static_cast<int32_t *>(ctx->adj_task_seg.p);
     HIPTRY(ctx, launch_adj_plan_tasks(plan, st));
     const size_t rec_task = 48 * size_t(K); // bytes of records per task
-    const int64_t chunk_cap = std::max<int64_t>(1, int64_t(kRecBudget / rec_task));
+    const uint64_t budget = g_rec_budget.load() ? g_rec_budget.load() : uint64_t(kRecBudget);
+    const int64_t chunk_cap = std::max<int64_t>(1, int64_t(budget / rec_task));
     std::vector<std::pair<int32_t, int32_t>> chunks; // (jhi, jlo)
     for (int32_t j = nseg_max - 1; j >= 0;) {
         int32_t lo = j;
@@ -1080,6 +1084,11 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     return PRR_OK;
 } catch (...) {
     return prr::abi_exception(ctx);
+}
+
+prr_status prr_debug_adjoint_record_budget(uint64_t bytes) {
+    g_rec_budget.store(bytes);
+    return PRR_OK;
 }
 
 } // extern "C"

@@ -202,3 +202,31 @@ def test_gradient_does_not_depend_on_the_checkpoint_interval(setup):
         print("K", K, "grad", grads[-1], "steps swept", st["adjoint_steps"])
     for g in grads[1:]:
         assert np.abs(g - grads[0]).max() <= 1e-12 * np.abs(grads[0]).max(), (g.tolist(), grads[0].tolist())
+
+
+def test_gradient_does_not_depend_on_replay_chunks_or_where_rays_are_binned(setup):
+    """The replay runs in chunks of segment levels when its records exceed the
+    record budget (lambda passes between chunks through memory), and rays are
+    binned by segment count on the host when the tracer's step cap allows
+    more than 4096 segments per ray: neither changes the gradient."""
+    from parrelray_amd._lib import check, lib
+    tmpl, o0, T, ref = setup
+    I, _ = render(ref, tmpl, o0)
+    dLdI = (2.0 * (I - T) / I.size).astype(np.float32)
+    K = 16
+    ctx = context(A.PRR_FP64, 1e-10)
+    ctx.set_launch(checkpoint=K)
+    g0 = ctx.adjoint(tmpl, o0, dLdI)
+    try:
+        check(lib().prr_debug_adjoint_record_budget(48 * K * 300))  # about 300 tasks per chunk
+        g1 = ctx.adjoint(tmpl, o0, dLdI)
+    finally:
+        check(lib().prr_debug_adjoint_record_budget(0))
+    tr = CF.tracer_kerr(precision=A.PRR_FP64, tol=1e-10)
+    tr.max_steps = 4096 * K + 64  # no ray comes near it: same steps, host-side binning
+    tr.max_attempts = 10 * tr.max_steps
+    ctx.set_tracer(tr)
+    g2 = ctx.adjoint(tmpl, o0, dLdI)
+    print("grads", g0.tolist(), g1.tolist(), g2.tolist())
+    for g in (g1, g2):
+        assert np.abs(g - g0).max() <= 1e-12 * np.abs(g0).max(), (g.tolist(), g0.tolist())
