@@ -43,6 +43,8 @@ struct prr_ctx {
     int device = 0;
     int cus = 0;
     hipStream_t stream = nullptr;
+    hipStream_t copy_stream = nullptr; // the adjoint's host-to-device copy of dL/dI (created on first use)
+    hipEvent_t copy_done = nullptr;
     std::string err;
     int32_t metric = PRR_SCHW_KS_CART;
     double M = 1.0, a = 0.0, q = 0.0, p[16] = {};
@@ -205,9 +207,6 @@ double metric_param(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx-
 // per step, K steps per task); larger replays run in several chunks.
 constexpr size_t kRecBudget = size_t(8) << 30;
 std::atomic<uint64_t> g_rec_budget{0}; // prr_debug_adjoint_record_budget (0: kRecBudget)
-// Segment-count histogram bins of the device-side adjoint plan; tracers whose
-// step cap allows more segments per ray plan on the host.
-constexpr int32_t kPlanBins = 4096;
 
 double far_escape(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx->p[1] : 0.0; }
 
@@ -326,6 +325,8 @@ void prr_destroy(prr_ctx *ctx) {
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    if (ctx->copy_done) (void)hipEventDestroy(ctx->copy_done);
+    if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx;
 }
 
@@ -875,40 +876,49 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     if ((s = enqueue_render(ctx, P, job, st, false, true)) != PRR_OK) return s;
-    // dL/dI from the host: staged while the first pass runs
+    // dL/dI from the host: copied on a second stream while the first pass
+    // runs; the reverse sweep waits for it
     const float *dldi = dL_dimg;
     if (!on_device) {
+        if (!ctx->copy_stream) {
+            HIPTRY(ctx, hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking));
+            HIPTRY(ctx, hipEventCreateWithFlags(&ctx->copy_done, hipEventDisableTiming));
+        }
         if ((s = ensure(ctx, ctx->adj_dldi, 12 * size_t(npix))) != PRR_OK) return s;
-        HIPTRY(ctx, hipMemcpyAsync(ctx->adj_dldi.p, dL_dimg, 12 * size_t(npix), hipMemcpyHostToDevice, st));
+        HIPTRY(ctx, hipMemcpyAsync(ctx->adj_dldi.p, dL_dimg, 12 * size_t(npix), hipMemcpyHostToDevice,
+                                   ctx->copy_stream));
+        HIPTRY(ctx, hipEventRecord(ctx->copy_done, ctx->copy_stream));
         dldi = static_cast<const float *>(ctx->adj_dldi.p);
     }
     // segments per ray, checkpoint offsets and the rays by decreasing segment
     // count (so that the rays with more than j segments are a prefix of the
-    // order): on the device when every segment count fits kPlanBins
+    // order): on the device when every segment count fits kAdjPlanBins
     // histogram bins, else on the host (a stable counting sort).  A ray keeps
     // its first-pass checkpoint slots i * cap when it has at most cap
     // segments; else its checkpoints go to the overflow area at -1 - off[i].
     const int64_t seg_bound = (ctx->tracer.max_steps - 1) / K + 2; // exceeds every segment count
-    const bool plan_dev = seg_bound <= kPlanBins;
+    const bool plan_dev = seg_bound <= kAdjPlanBins;
     std::vector<unsigned long long> hist; // rays per segment count 0..nseg_max
     unsigned long long pctr[4] = {0, 0, 0, 0};
     int64_t nck2 = 0, swept = 0, n_over = 0;
     int32_t nseg_max = 0;
     if ((s = ensure(ctx, ctx->adj_nseg, 4 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_over, 8 * size_t(T))) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->adj_plan, 8 * size_t(2 * kPlanBins + 4))) != PRR_OK) return s;
+    // plan buffer: histogram, counters, then the bin starts and level starts (one copy from the host)
+    if ((s = ensure(ctx, ctx->adj_plan, 8 * size_t(3 * kAdjPlanBins + 4))) != PRR_OK) return s;
     AdjPlanDev plan{};
     plan.steps = static_cast<const int32_t *>(ctx->steps.p);
     plan.T = T;
     plan.K = K;
     plan.cap = cap;
-    plan.nbins = kPlanBins;
+    plan.nbins = kAdjPlanBins;
     plan.nseg = static_cast<int32_t *>(ctx->adj_nseg.p);
     plan.off = static_cast<int64_t *>(ctx->adj_off.p);
     plan.over = static_cast<int64_t *>(ctx->adj_over.p);
     plan.hist = static_cast<unsigned long long *>(ctx->adj_plan.p);
-    plan.cursor = plan.hist + kPlanBins;
-    plan.ctr = plan.cursor + kPlanBins;
+    plan.ctr = plan.hist + kAdjPlanBins;
+    plan.cursor = plan.ctr + 4;
+    int64_t *levelstart_d = reinterpret_cast<int64_t *>(plan.cursor + kAdjPlanBins);
     plan.order = static_cast<int64_t *>(ctx->adj_rays.p);
     // host buffers of the copies below live until the final synchronisation
     std::vector<int64_t> order_h, off_h, over_h;
@@ -916,8 +926,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     std::vector<unsigned long long> cursor_h;
     if (plan_dev) {
         HIPTRY(ctx, launch_adj_plan_count(plan, st));
-        hist.assign(size_t(kPlanBins), 0ull);
-        HIPTRY(ctx, hipMemcpyAsync(hist.data(), plan.hist, 8 * size_t(kPlanBins), hipMemcpyDeviceToHost, st));
+        hist.assign(size_t(kAdjPlanBins), 0ull);
+        HIPTRY(ctx, hipMemcpyAsync(hist.data(), plan.hist, 8 * size_t(kAdjPlanBins), hipMemcpyDeviceToHost, st));
         HIPTRY(ctx, hipMemcpyAsync(pctr, plan.ctr, sizeof pctr, hipMemcpyDeviceToHost, st));
     } else {
         steps_h.resize(size_t(T));
@@ -931,11 +941,6 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         swept = int64_t(pctr[2]);
         nseg_max = int32_t(pctr[3]);
         hist.resize(size_t(nseg_max) + 1);
-        // start of each segment count in the order (decreasing counts first)
-        cursor_h.assign(size_t(nseg_max) + 1, 0ull);
-        for (int32_t n = nseg_max - 1; n >= 0; --n) cursor_h[n] = cursor_h[n + 1] + hist[n + 1];
-        HIPTRY(ctx, hipMemcpyAsync(plan.cursor, cursor_h.data(), 8 * cursor_h.size(), hipMemcpyHostToDevice, st));
-        HIPTRY(ctx, launch_adj_plan_scatter(plan, st));
     } else {
         std::vector<int32_t> nseg(static_cast<size_t>(T));
         off_h.resize(size_t(T));
@@ -963,6 +968,27 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         if (n_over > 0)
             HIPTRY(ctx, hipMemcpyAsync(plan.over, over_h.data(), 8 * size_t(n_over), hipMemcpyHostToDevice, st));
     }
+    // rays with more than j segments (count[j], a prefix of the order) and
+    // the first task of each level j (levels from the last); device plan:
+    // the start of each segment count in the order (decreasing counts first)
+    std::vector<int64_t> count(size_t(nseg_max) + 1, 0), levelstart(size_t(nseg_max) + 1, 0);
+    for (int32_t j = nseg_max - 1; j >= 0; --j) count[j] = count[j + 1] + int64_t(hist[j + 1]);
+    int64_t ntask = 0;
+    for (int32_t j = nseg_max - 1; j >= 0; --j) {
+        levelstart[j] = ntask;
+        ntask += count[j];
+    }
+    if (nseg_max < kAdjPlanBins) { // bin starts and level starts in one copy
+        cursor_h.assign(size_t(2 * kAdjPlanBins), 0ull);
+        for (int32_t n = nseg_max - 1; n >= 0; --n) cursor_h[n] = cursor_h[n + 1] + hist[n + 1];
+        for (int32_t j = 0; j <= nseg_max; ++j) cursor_h[kAdjPlanBins + j] = uint64_t(levelstart[j]);
+        HIPTRY(ctx, hipMemcpyAsync(plan.cursor, cursor_h.data(), 8 * cursor_h.size(), hipMemcpyHostToDevice, st));
+    } else { // host plan with more levels than the plan buffer holds
+        if ((s = ensure(ctx, ctx->adj_levelstart, 8 * levelstart.size())) != PRR_OK) return s;
+        levelstart_d = static_cast<int64_t *>(ctx->adj_levelstart.p);
+        HIPTRY(ctx, hipMemcpyAsync(levelstart_d, levelstart.data(), 8 * levelstart.size(), hipMemcpyHostToDevice, st));
+    }
+    if (plan_dev) HIPTRY(ctx, launch_adj_plan_scatter(plan, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
     JobDev j2 = job;
     j2.out = OutDev{};
@@ -990,19 +1016,9 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     // chunk in one launch, and each chunk's reverse sweep is one launch over
     // its rays: the replays of different segments run concurrently instead
     // of one launch per segment index.
-    std::vector<int64_t> count(size_t(nseg_max) + 1, 0), levelstart(size_t(nseg_max) + 1, 0);
-    for (int32_t j = nseg_max - 1; j >= 0; --j) count[j] = count[j + 1] + int64_t(hist[j + 1]);
-    int64_t ntask = 0;
-    for (int32_t j = nseg_max - 1; j >= 0; --j) {
-        levelstart[j] = ntask;
-        ntask += count[j];
-    }
     if ((s = ensure(ctx, ctx->adj_task_ray, 8 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_task_seg, 4 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->adj_levelstart, 8 * levelstart.size())) != PRR_OK) return s;
-    HIPTRY(ctx, hipMemcpyAsync(ctx->adj_levelstart.p, levelstart.data(), 8 * levelstart.size(), hipMemcpyHostToDevice,
-                               st));
-    plan.levelstart = static_cast<const int64_t *>(ctx->adj_levelstart.p);
+    plan.levelstart = levelstart_d;
     plan.nlevels = nseg_max;
     plan.ntask = ntask;
     plan.task_ray = static_cast<int64_t *>(ctx->adj_task_ray.p);
@@ -1029,7 +1045,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     AdjDev ad{};
     ad.K = K;
     ad.rays = static_cast<const int64_t *>(ctx->adj_rays.p);
-    ad.levelstart = static_cast<const int64_t *>(ctx->adj_levelstart.p);
+    ad.levelstart = levelstart_d;
     ad.nsteps = j2.adj.nsteps;
     ad.cls = static_cast<const int8_t *>(ctx->cls.p);
     ad.rec = j3.adj.rec;
@@ -1042,6 +1058,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     ad.cam_phi = orbit->cam_phi;
     ad.sphere_r = orbit->sphere_r;
     ad.mismatch = &ctr->mismatch;
+    if (!on_device) HIPTRY(ctx, hipStreamWaitEvent(st, ctx->copy_done, 0)); // dL/dI is on the device
     for (const auto &c : chunks) {
         const int64_t t0 = levelstart[c.first], t1 = levelstart[c.second] + count[c.second];
         j3.adj.rays = static_cast<const int64_t *>(ctx->adj_task_ray.p) + t0;

@@ -169,12 +169,16 @@ struct AdjDev {
     unsigned long long *mismatch; // rays whose replay did not end with their class
 };
 
+// Segment-count histogram bins of the device-side adjoint plan; tracers whose
+// step cap allows more segments per ray plan on the host.
+constexpr int32_t kAdjPlanBins = 4096;
+
 // Device-side planning of the adjoint's replay (after the checkpointing pass).
 struct AdjPlanDev {
     const int32_t *steps;        // accepted steps of every ray
     int64_t T;                   // rays
     int32_t K, cap;              // checkpoint interval; first-pass checkpoint slots per ray
-    int32_t nbins;               // histogram bins (> every segment count)
+    int32_t nbins;               // histogram bins (> every segment count), at most kAdjPlanBins
     int32_t nlevels;             // tasks: segment levels (the largest segment count)
     int32_t *nseg;               // segments of every ray
     int64_t *off;                // checkpoint offsets (AdjRunDev.ckpt_off)

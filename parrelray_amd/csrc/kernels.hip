@@ -431,25 +431,6 @@ __global__ void __launch_bounds__(256) k_adj_init_grad(JobDev job, AdjDev ad) {
 }
 
 // ------------------------------------------------------------ adjoint plan
-// Wave-aggregated claim of one slot of cnt[key] per lane with `on`: one
-// atomic per distinct key in the wave; lanes of one key get consecutive
-// slots in lane order.  Every lane of the wave must call it.
-__device__ unsigned long long wave_claim(unsigned long long *cnt, int key, bool on) {
-    const int lane = int(threadIdx.x & 63);
-    unsigned long long rem = __ballot(on), res = 0;
-    while (rem) {
-        const int leader = __ffsll((long long)rem) - 1;
-        const int k = __shfl(key, leader);
-        const unsigned long long m = rem & __ballot(key == k);
-        unsigned long long base = 0;
-        if (lane == leader) base = atomicAdd(&cnt[k], (unsigned long long)__popcll(m));
-        base = shfl_u64(base, leader);
-        if ((m >> lane) & 1ull) res = base + (unsigned long long)__popcll(m & ((1ull << lane) - 1ull));
-        rem &= ~m;
-    }
-    return res;
-}
-
 __device__ unsigned long long wave_sum(unsigned long long v) {
     const int lane = int(threadIdx.x & 63);
     for (int o = 32; o > 0; o >>= 1) v += shfl_u64(v, lane ^ o);
@@ -462,7 +443,14 @@ __global__ void __launch_bounds__(256) k_adj_plan_clear(AdjPlanDev p) {
     if (i < 4) p.ctr[i] = 0ull;
 }
 
+// Counts are gathered per workgroup in LDS and flushed with one global atomic
+// per non-empty bin, so that the few bins do not serialise every wave.
 __global__ void __launch_bounds__(256) k_adj_plan_count(AdjPlanDev p) {
+    __shared__ unsigned lcount[kAdjPlanBins];
+    __shared__ unsigned long long lsum[4];
+    __shared__ unsigned lmax[4];
+    for (int b = threadIdx.x; b < p.nbins; b += blockDim.x) lcount[b] = 0u;
+    __syncthreads();
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool on = i < p.T;
     const int32_t st = on ? p.steps[i] : 0;
@@ -476,23 +464,47 @@ __global__ void __launch_bounds__(256) k_adj_plan_count(AdjPlanDev p) {
             p.off[i] = -1 - int64_t(base);
             p.over[atomicAdd(&p.ctr[1], 1ull)] = i;
         }
+        atomicAdd(&lcount[min(n, p.nbins - 1)], 1u);
     }
-    (void)wave_claim(p.hist, min(n, p.nbins - 1), on);
     const unsigned long long sum = wave_sum((unsigned long long)st);
     unsigned mx = unsigned(n);
     for (int o = 32; o > 0; o >>= 1) mx = max(mx, unsigned(__shfl_xor(int(mx), o)));
+    const int w = int(threadIdx.x >> 6);
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&p.ctr[2], sum);
-        atomicMax(&p.ctr[3], (unsigned long long)mx);
+        lsum[w] = sum;
+        lmax[w] = mx;
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < p.nbins; b += blockDim.x)
+        if (lcount[b]) atomicAdd(&p.hist[b], (unsigned long long)lcount[b]);
+    if (threadIdx.x == 0) {
+        unsigned long long s = 0;
+        unsigned m = 0;
+        for (int k = 0; k < int(blockDim.x >> 6); ++k) {
+            s += lsum[k];
+            m = max(m, lmax[k]);
+        }
+        atomicAdd(&p.ctr[2], s);
+        atomicMax(&p.ctr[3], (unsigned long long)m);
     }
 }
 
+// Each workgroup claims one run of positions per segment count it holds;
+// its rays take consecutive positions of that run.
 __global__ void __launch_bounds__(256) k_adj_plan_scatter(AdjPlanDev p) {
+    __shared__ unsigned lcount[kAdjPlanBins];
+    __shared__ unsigned long long lbase[kAdjPlanBins];
+    for (int b = threadIdx.x; b < p.nbins; b += blockDim.x) lcount[b] = 0u;
+    __syncthreads();
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool on = i < p.T;
-    const int32_t n = on ? p.nseg[i] : 0;
-    const unsigned long long pos = wave_claim(p.cursor, n, on);
-    if (on) p.order[pos] = i;
+    const int32_t n = on ? min(p.nseg[i], p.nbins - 1) : 0;
+    const unsigned rank = on ? atomicAdd(&lcount[n], 1u) : 0u;
+    __syncthreads();
+    for (int b = threadIdx.x; b < p.nbins; b += blockDim.x)
+        if (lcount[b]) lbase[b] = atomicAdd(&p.cursor[b], (unsigned long long)lcount[b]);
+    __syncthreads();
+    if (on) p.order[lbase[n] + rank] = i;
 }
 
 __global__ void __launch_bounds__(256) k_adj_plan_tasks(AdjPlanDev p) {
