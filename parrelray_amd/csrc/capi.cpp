@@ -858,7 +858,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if ((s = ensure(ctx, ctx->adj_event, sizeof(EventDev) * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_lam, 80 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_grad_ray, 32 * size_t(T))) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->adj_grad, 32)) != PRR_OK) return s;
+    if ((s = ensure(ctx, ctx->adj_grad, 32 + 32 * 256)) != PRR_OK) return s; // the gradient, then partial sums
     // first pass: the render, which also stores each ray's first `cap`
     // checkpoints (every K accepted steps) in fixed slots; rays with more
     // segments are checkpointed again by a pass of their own
@@ -1079,7 +1079,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     jg.adj = AdjRunDev{};
     jg.total = T;
     HIPTRY(ctx, launch_adj_init_grad(jg, ad, st));
-    HIPTRY(ctx, launch_grad_reduce(ad.grad_ray, T, static_cast<double *>(ctx->adj_grad.p), st));
+    HIPTRY(ctx, launch_grad_reduce(ad.grad_ray, T, static_cast<double *>(ctx->adj_grad.p) + 4,
+                                   static_cast<double *>(ctx->adj_grad.p), st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     CountersDev c{};
     HIPTRY(ctx, hipMemcpyAsync(&c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost, st));

@@ -539,20 +539,41 @@ hipError_t launch_adj_plan_tasks(const AdjPlanDev &p, hipStream_t st) {
 }
 
 // Deterministic sum of the per-ray gradients: one workgroup, fixed order.
-__global__ void __launch_bounds__(1024) k_grad_reduce(const double *grad_ray, int64_t n, double *out) {
-    __shared__ double part[4][1024];
+// Two levels: kGradParts workgroups each sum a fixed contiguous range of
+// rays into partial[b], then one workgroup sums the partials; every sum has a
+// fixed order, whatever the hardware schedule.
+constexpr int kGradParts = 256;
+
+__device__ void block_sum4(double acc[4], double (*part)[256]) {
     const int t = int(threadIdx.x);
-    double acc[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int64_t i = t; i < n; i += 1024)
-        for (int k = 0; k < 4; ++k) acc[k] += grad_ray[4 * i + k];
     for (int k = 0; k < 4; ++k) part[k][t] = acc[k];
     __syncthreads();
-    for (int s = 512; s > 0; s >>= 1) {
+    for (int s = 128; s > 0; s >>= 1) {
         if (t < s)
             for (int k = 0; k < 4; ++k) part[k][t] += part[k][t + s];
         __syncthreads();
     }
-    if (t == 0)
+}
+
+__global__ void __launch_bounds__(256) k_grad_reduce_part(const double *grad_ray, int64_t n, double *partial) {
+    __shared__ double part[4][256];
+    const int64_t chunk = (n + kGradParts - 1) / kGradParts;
+    const int64_t lo = int64_t(blockIdx.x) * chunk, hi = min(n, lo + chunk);
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int64_t i = lo + threadIdx.x; i < hi; i += 256)
+        for (int k = 0; k < 4; ++k) acc[k] += grad_ray[4 * i + k];
+    block_sum4(acc, part);
+    if (threadIdx.x == 0)
+        for (int k = 0; k < 4; ++k) partial[4 * blockIdx.x + k] = part[k][0];
+}
+
+__global__ void __launch_bounds__(256) k_grad_reduce_final(const double *partial, double *out) {
+    __shared__ double part[4][256];
+    double acc[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int b = threadIdx.x; b < kGradParts; b += 256)
+        for (int k = 0; k < 4; ++k) acc[k] += partial[4 * b + k];
+    block_sum4(acc, part);
+    if (threadIdx.x == 0)
         for (int k = 0; k < 4; ++k) out[k] = part[k][0];
 }
 
@@ -567,8 +588,9 @@ hipError_t launch_adj_init_grad(const JobDev &job, const AdjDev &ad, hipStream_t
     return hipGetLastError();
 }
 
-hipError_t launch_grad_reduce(const double *grad_ray, int64_t n, double *out, hipStream_t st) {
-    hipLaunchKernelGGL(k_grad_reduce, dim3(1), dim3(1024), 0, st, grad_ray, n, out);
+hipError_t launch_grad_reduce(const double *grad_ray, int64_t n, double *partial, double *out, hipStream_t st) {
+    hipLaunchKernelGGL(k_grad_reduce_part, dim3(kGradParts), dim3(256), 0, st, grad_ray, n, partial);
+    hipLaunchKernelGGL(k_grad_reduce_final, dim3(1), dim3(256), 0, st, partial, out);
     return hipGetLastError();
 }
 

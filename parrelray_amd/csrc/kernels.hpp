@@ -30,7 +30,8 @@ hipError_t launch_adj_init_grad(const JobDev &job, const AdjDev &ad, hipStream_t
 hipError_t launch_adj_plan_count(const AdjPlanDev &p, hipStream_t st);
 hipError_t launch_adj_plan_scatter(const AdjPlanDev &p, hipStream_t st);
 hipError_t launch_adj_plan_tasks(const AdjPlanDev &p, hipStream_t st);
-// Deterministic sum of n per-ray gradients (4 doubles each) into out[4].
-hipError_t launch_grad_reduce(const double *grad_ray, int64_t n, double *out, hipStream_t st);
+// Deterministic sum of n per-ray gradients (4 doubles each) into out[4];
+// partial: 4 * 256 doubles of scratch.
+hipError_t launch_grad_reduce(const double *grad_ray, int64_t n, double *partial, double *out, hipStream_t st);
 
 } // namespace prr
