@@ -95,3 +95,6 @@ def test_wormhole_inputs_are_checked():
         ctx.render(sph)
     ok = A.make_frame([20.0, 1.0, 0.5], 3.1, 0.0, 1.0, 8, 8, mass=0.0)
     assert ctx.render(ok)[0]["rgb8"].shape == (8, 8, 3)
+    # the adjoint differentiates Kerr only
+    with pytest.raises(PrrError, match="Kerr"):
+        ctx.adjoint(CF.frame_c3(8, 8), CF.c3_orbit(), np.zeros((8, 8, 3), np.float32))
