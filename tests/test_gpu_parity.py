@@ -318,6 +318,64 @@ def test_c1_reduced_parity(precision, solver):
     assert frac >= 0.999
 
 
+def multi_sphere_frame(W=128, H=72, spin=0.9):
+    """Four spheres: two overlapping on the sky (one behind the other), an
+    untextured one, one close to the hole; camera r = 20M, inclination 80 deg."""
+    pos, al, be = CF.orbit_pose(20.0, 80.0)
+    spheres = [CF.equatorial_sphere(8.0, 0.3, 1.0, 1), CF.equatorial_sphere(11.0, 0.22, 1.2, 1),
+               CF.equatorial_sphere(6.0, -0.45, 0.8, 0), ([-2.0, 4.5, 0.8], 0.7, 1)]
+    return A.make_frame(pos, al, be, math.radians(50.0), W, H, mass=1.0, spin=spin, spheres=spheres)
+
+
+@pytest.mark.parametrize("precision", [A.PRR_FP64, A.PRR_FP32_BULK])
+def test_multiple_spheres_match_oracle(precision):
+    """Several spheres in one scene (the reference's sphereN blocks): the
+    nearest located hit along the path wins, per-sphere textures (slot 0: white);
+    classes, endpoints and RGB8 against the oracle (Kerr a = 0.9, RKF45)."""
+    f = multi_sphere_frame()
+    T = CF.tracer_kerr(precision=precision)
+    ctx = gpu_context(A.PRR_KERR_BL, T)
+    out = ctx.render(f, rays=True)[0]
+    S = oracle_scene(A.PRR_KERR_BL, CF.tracer_kerr(tol=1e-12))
+    orc = S.render_rays(f)
+    hits = orc["cls"] == A.PRR_SPHERE
+    assert hits.sum() >= 300
+    cmp = compare_rays(S, f, out, orc)
+    assert cmp["unexcused"] <= 1, cmp["unexcused_ids"][:20]
+    ang = cmp["angles"]
+    print("endpoint angle: median %.2e p99 %.2e max %.2e" % (np.median(ang), np.quantile(ang, 0.99), ang.max()))
+    assert np.median(ang) < 1e-6 and np.quantile(ang, 0.99) < 1e-5
+    # every sphere is seen, by the colour of its texture or its white
+    same = cmp["same"] & hits
+    assert np.abs(out["color"][same] - orc["color"][same]).max(axis=1).mean() < 1e-3
+    _, img, rgb8 = S.render_frame(f)
+    frac, _ = rgb8_agreement(out["rgb8"], rgb8)
+    assert frac >= 0.999
+
+
+def test_multiple_spheres_euler_chords_match_oracle():
+    """The Euler solver's chord test over several spheres (the first sphere
+    that a step chord enters, scene/sphere.h:34-50 with D11): Schwarzschild
+    coordinates, fixed steps, so classes and end points agree to rounding."""
+    pos, al, be = CF.orbit_pose(20.0, 80.0)
+    spheres = [CF.equatorial_sphere(8.0, 0.3, 1.0, 1), CF.equatorial_sphere(11.0, 0.22, 1.2, 1),
+               CF.equatorial_sphere(6.0, -0.45, 0.8, 0)]
+    f = A.make_frame(pos, al, be, math.radians(50.0), 96, 54, mass=1.0, spin=0.0, spheres=spheres)
+    T = A.make_tracer(solver=A.PRR_EULER, h=0.01, r_escape=60.0, horizon_eps_rel=1e-3, horizon_eps_abs=0.0,
+                      lambda_max=1e9, max_steps=12000, max_attempts=12000)
+    ctx = gpu_context(A.PRR_SCHW_SPH, T)
+    out = ctx.render(f, rays=True)[0]
+    S = oracle_scene(A.PRR_SCHW_SPH, T)
+    orc = S.render_rays(f)
+    assert (orc["cls"] == A.PRR_SPHERE).sum() >= 200
+    cmp = compare_rays(S, f, out, orc)
+    assert cmp["unexcused"] == 0, cmp["unexcused_ids"][:20]
+    assert np.median(cmp["angles"]) < 1e-9
+    _, img, rgb8 = S.render_frame(f)
+    frac, _ = rgb8_agreement(out["rgb8"], rgb8)
+    assert frac >= 0.999
+
+
 def test_aa_passes_bit_exact():
     """SSAA combine, FXAA and RGB8 on the GPU's own ray colours equal the oracle's passes bit for bit."""
     f = CF.frame_c1(160, 90)
