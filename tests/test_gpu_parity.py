@@ -376,6 +376,50 @@ def test_multiple_spheres_euler_chords_match_oracle():
     assert frac >= 0.999
 
 
+RENDER_VARIANTS = {
+    # metric, solver, projection, samples per pixel, fxaa, spin
+    "ks_cart_rkf45_ssaa2": (A.PRR_SCHW_KS_CART, A.PRR_RKF45, A.PRR_PROJ_PINHOLE, 4, 0, 0.0),
+    "sph_euler_spherical_projection": (A.PRR_SCHW_SPH, A.PRR_EULER, A.PRR_PROJ_SPHERICAL, 1, 0, 0.0),
+    "kerr_rkf45_ssaa3_fxaa": (A.PRR_KERR_BL, A.PRR_RKF45, A.PRR_PROJ_PINHOLE, 9, 1, 0.6),
+    "flat_dormand_spherical_projection": (A.PRR_FLAT, A.PRR_DORMAND, A.PRR_PROJ_SPHERICAL, 4, 1, 0.0),
+}
+
+
+@pytest.mark.parametrize("name", sorted(RENDER_VARIANTS))
+def test_render_variants_match_oracle(name):
+    """Frames over the other metrics, solvers, projections and SSAA factors
+    (camera r = 20M, inclination 80 deg, the C1 sphere): classes, endpoints
+    and RGB8 against the oracle on the same frame."""
+    metric, solver, proj, s, fxaa, spin = RENDER_VARIANTS[name]
+    pos, al, be = CF.orbit_pose(20.0, 80.0)
+    n = int(round(math.sqrt(s)))
+    f = A.make_frame(pos, al, be, math.radians(50.0), 64, 40, ssaa=A.PRR_SSAA_REGULAR if s > 1 else A.PRR_SSAA_NONE,
+                     samples=s, projection=proj, fxaa=fxaa, mass=1.0, spin=spin,
+                     spheres=[CF.equatorial_sphere(8.0, 0.3)])
+    assert n * n == s
+    T = trace_tracer(solver)
+    if solver == A.PRR_EULER:
+        T.max_steps = T.max_attempts = 12000
+    ctx = gpu_context(metric, T)
+    out = ctx.render(f, rays=True)[0]
+    Tq = trace_tracer(solver)
+    if solver == A.PRR_EULER:
+        Tq.max_steps = Tq.max_attempts = 12000
+    else:
+        Tq.atol = Tq.rtol = 1e-12
+    S = oracle_scene(metric, Tq)
+    orc = S.render_rays(f)
+    cmp = compare_rays(S, f, out, orc)
+    assert cmp["unexcused"] <= 1, cmp["unexcused_ids"][:20]
+    ang = cmp["angles"]
+    print("%s: classes %s, endpoint angle median %.2e p99 %.2e" % (name, np.bincount(orc["cls"], minlength=5).tolist(),
+                                                                    np.median(ang), np.quantile(ang, 0.99)))
+    assert np.median(ang) < (1e-9 if solver == A.PRR_EULER else 1e-6)
+    _, img, rgb8 = S.render_frame(f)
+    frac, _ = rgb8_agreement(out["rgb8"], rgb8)
+    assert frac >= 0.999
+
+
 def test_aa_passes_bit_exact():
     """SSAA combine, FXAA and RGB8 on the GPU's own ray colours equal the oracle's passes bit for bit."""
     f = CF.frame_c1(160, 90)
