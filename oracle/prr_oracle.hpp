@@ -22,7 +22,8 @@
 //                      (test_oracle_physics), E, L, Carter Q conservation,
 //                      Kerr a -> 0 equals the Schwarzschild form, Darwin's
 //                      exact deflection angle (tests/golden/darwin_*.json),
-//                      photon-orbit radii.
+//                      photon-orbit radii, Kerr spherical photon orbits
+//                      (Teo 2003) keeping their radius.
 //   RKF45 / Euler      order tests on ODEs with closed-form solutions, the
 //                      Fehlberg tableau consistency conditions.
 //   camera / init      closed forms (centre ray = forward, FOV corners),

@@ -134,6 +134,32 @@ def test_kerr_mino_rhs_matches_christoffel_form(a):
         assert abs(f[3] - expect) <= 1e-10 * scale_m + 1e-300, (a, x, u, f[3], expect)
 
 
+@pytest.mark.parametrize("a", [0.95, 0.998])
+def test_kerr_mino_rhs_vanishes_on_spherical_photon_orbits(a):
+    """On a spherical photon orbit r_p (Teo 2003) P(u) has a double root at
+    u_p = 1/r_p: the state has u' = 0 and the Mino-time RHS u'' = P'(u)/2 = 0,
+    which ties c1, c2 and c3 to E, L, Q and a."""
+    r_pro = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(-a)))
+    r_ret = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(a)))
+    for f in (0.15, 0.5, 0.85):
+        r = r_pro + f * (r_ret - r_pro)
+        xi = (r * r * (3 - r) - a * a * (r + 1)) / (a * (r - 1))
+        eta = r ** 3 * (4 * a * a - r * (r - 3) ** 2) / (a * a * (r - 1) ** 2)
+        g = O.metric(A.PRR_KERR_BL, 1.0, a, [r, math.pi / 2, 0.0, 0.0])
+        gi = np.linalg.inv(g)
+        u = np.array([0.0, math.sqrt(eta) / (r * r), gi[2, 3] * -1.0 + gi[2, 2] * xi, gi[3, 3] * -1.0 + gi[3, 2] * xi])
+        x = np.array([r, math.pi / 2, 0.0, 0.0])
+        y, (E, L, Q) = api.state_from_xu(A.PRR_KERR_BL, x, u, 1.0, a)
+        assert abs(L / E - xi) < 1e-12 * abs(xi) and abs(Q / E ** 2 - eta) < 1e-11 * eta
+        f_ = api.rhs(A.PRR_KERR_BL, y, 1.0, a, E, L, Q)
+        scale = E * E * y[0]
+        assert abs(y[1]) < 1e-14 and abs(f_[1]) < 1e-12 * scale, (r, y[1], f_[1])
+        # off the orbit radius the radial acceleration is not zero
+        y2 = y.copy()
+        y2[0] *= 1.01
+        assert abs(api.rhs(A.PRR_KERR_BL, y2, 1.0, a, E, L, Q)[1]) > 1e-5 * scale
+
+
 def test_kerr_first_integrals_hold_at_set_up():
     """u'^2 = P(u) and mu'^2 = Theta_mu(mu) for the E, L, Q of a null ray."""
     for a in (0.0, 0.6, 0.95):

@@ -226,6 +226,46 @@ def test_trace_kerr_conserves_constants_and_matches_oracle(a, solver):
         assert abs(c1[2] - c0[2]) < 1e-6 * (abs(c0[2]) + 1)
 
 
+@pytest.mark.parametrize("a", [0.95, 0.998])
+def test_kerr_spherical_photon_orbits_keep_their_radius_on_the_gpu(a):
+    """The shadow-defining spherical photon orbits (Teo 2003) through the
+    GPU's Mino-time integrator: started on the equator with u^r = 0 and the
+    orbit's constants, the ray is still at r_p when it reaches the affine cap
+    lambda = 10 (the orbit is unstable; the oracle at tol 1e-12 drifts ~3e-8),
+    with theta between the analytic turning points of Theta."""
+    T = A.make_tracer(solver=A.PRR_RKF45, h=0.01, hmin=1e-12, hmax=0.05, atol=1e-12, rtol=1e-12, r_escape=200.0,
+                      horizon_eps_rel=1e-3, horizon_eps_abs=0.0, lambda_max=10.0, max_steps=10**6,
+                      max_attempts=10**7)
+    r_pro = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(-a)))
+    r_ret = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(a)))
+    rows, radii, th_min = [], [], []
+    for f in (0.2, 0.4, 0.6, 0.8):
+        r = r_pro + f * (r_ret - r_pro)
+        xi = (r * r * (3 - r) - a * a * (r + 1)) / (a * (r - 1))
+        eta = r ** 3 * (4 * a * a - r * (r - 3) ** 2) / (a * a * (r - 1) ** 2)
+        # theta turning points: Theta = eta + a^2 c^2 - xi^2 c^2 / (1 - c^2) = 0, c = cos th
+        B2 = a * a - eta - xi * xi
+        c2 = (-B2 - math.sqrt(B2 * B2 + 4 * a * a * eta)) / (-2 * a * a)
+        th_min.append(math.acos(math.sqrt(c2)))
+        g = O.metric(A.PRR_KERR_BL, 1.0, a, [r, math.pi / 2, 0.0, 0.0])
+        gi = np.linalg.inv(g)
+        u = [0.0, math.sqrt(eta) / (r * r), gi[2, 3] * -1.0 + gi[2, 2] * xi, gi[3, 3] * -1.0 + gi[3, 2] * xi]
+        rows.append([r, math.pi / 2, 0.0, 0.0] + u)
+        radii.append(r)
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_KERR_BL, 1.0, a)
+    ctx.set_tracer(T)
+    g = ctx.trace(np.array(rows))
+    assert np.all(g["cls"] == A.PRR_STEP_CAP)
+    dr = np.abs(g["xend"][:, 0] - np.array(radii)) / np.array(radii)
+    print("a", a, "relative radius drift", dr, "theta", g["xend"][:, 1], "theta_min", th_min)
+    assert dr.max() < 1e-6
+    # theta stays between its turning points (and has left the equator)
+    th = g["xend"][:, 1]
+    assert np.all(th >= np.array(th_min) - 1e-9) and np.all(th <= math.pi - np.array(th_min) + 1e-9)
+    assert np.all(np.abs(th - math.pi / 2) > 1e-3)
+
+
 def test_c0_parity_full_frame():
     T = CF.tracer_c0()
     f = CF.frame_c0()

@@ -184,6 +184,52 @@ def test_kerr_equatorial_photon_orbits():
     assert r_isco_ph < 3.0
 
 
+def spherical_photon_orbit(a, r, M=1.0):
+    """Constants of the Kerr spherical photon orbit at Boyer-Lindquist radius r
+    (Teo 2003, with E = 1): xi = L/E, eta = Q/E^2; the orbits exist where
+    eta >= 0, between the prograde and retrograde equatorial photon orbits."""
+    xi = (r * r * (3 * M - r) - a * a * (r + M)) / (a * (r - M))
+    eta = r ** 3 * (4 * a * a * M - r * (r - 3 * M) ** 2) / (a * a * (r - M) ** 2)
+    return xi, eta
+
+
+@pytest.mark.parametrize("a", [0.95, 0.998])
+def test_kerr_spherical_photon_orbits_keep_their_radius(a):
+    """Non-equatorial spherical photon orbits (the rays that outline the
+    shadow, Bardeen's critical curve): started on the equator with u^r = 0 and
+    the orbit's E, L, Q, the ray keeps r = r_p while theta oscillates between
+    the zeros of Theta = eta + a^2 cos^2 th - xi^2 cot^2 th.  Every term of the
+    Kerr geodesic equation enters: a dropped or mis-signed one moves r off r_p
+    within an orbit."""
+    r_pro = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(-a)))
+    r_ret = 2 * (1 + math.cos(2.0 / 3.0 * math.acos(a)))
+    checked = 0
+    for f in (0.2, 0.4, 0.6, 0.8):
+        r = r_pro + f * (r_ret - r_pro)
+        xi, eta = spherical_photon_orbit(a, r)
+        assert eta > 0
+        uth = math.sqrt(eta) / (r * r)  # p_theta = +sqrt(Q) at the equator, u^th = p_th / Sigma
+        x, u = kerr_state_from_constants(1.0, a, r, math.pi / 2, 1.0, xi, 0.0, uth)
+        g = O.metric(A.PRR_KERR_BL, 1.0, a, x)
+        assert abs(u @ g @ u) < 1e-12 * (u[3] ** 2)
+        du = O.geod(A.PRR_KERR_BL, 1.0, a, x, u)
+        assert abs(du[0]) < 1e-11 * (u[3] ** 2)
+        # theta turning points: cos^2 th solves a^2 c^4 + (xi^2 + eta - a^2) c^2 ... (Theta = 0)
+        A2, B2, C2 = -a * a, a * a - eta - xi * xi, eta  # in c = cos^2 th: A2 c^2 + B2 c + C2 = 0
+        c_max = (-B2 - math.sqrt(B2 * B2 - 4 * A2 * C2)) / (2 * A2)
+        th_min = math.acos(math.sqrt(c_max))
+        # the orbit is unstable: the tol-1e-12 error grows ~e^lambda near the
+        # horizon, to ~3e-8 at lambda = 10 (and ~1e-3 at 25)
+        S = O.Scene(A.PRR_KERR_BL, 1.0, a, tracer(lambda_max=10.0, hmax=0.05))
+        res = S.trace(np.concatenate([x, u])[None])
+        assert res["cls"][0] == A.PRR_STEP_CAP  # still orbiting at the affine cap
+        xe = res["xend"][0]
+        assert abs(xe[0] - r) < 1e-6 * r, (a, r, xe[0])
+        assert th_min - 1e-6 <= xe[1] <= math.pi - th_min + 1e-6
+        checked += 1
+    assert checked == 4
+
+
 def test_schwarzschild_photon_sphere_is_circular():
     x = np.array([3.0, math.pi / 2, 0.0, 0.0])
     L = 3.0 * math.sqrt(3.0)  # b_c = 3 sqrt(3) M
