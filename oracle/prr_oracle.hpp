@@ -28,7 +28,11 @@
 //                      Fehlberg tableau consistency conditions.
 //   camera / init      closed forms (centre ray = forward, FOV corners),
 //                      null condition g(u,u) = 0.
-//   sphere             brute-force ray/sphere intersection in flat space.
+//   escape direction   straight lines keep their direction (flat); Kerr at
+//                      a = 0 gives the Kerr-Schild velocity direction.
+//   sphere             brute-force ray/sphere intersection in flat space;
+//                      the sphere colour (texture at the normal's angles
+//                      times the shade) against hits of known normal.
 //   sky / bilinear     one-texel textures (block RMS, orientation, last
 //                      row / column), linear textures reproduced exactly
 //                      (test_oracle_images).

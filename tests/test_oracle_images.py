@@ -178,3 +178,40 @@ def test_bilinear_reproduces_a_linear_texture():
     assert got[1] == pytest.approx((0.25 * 5 * (W - 1) + 0.75 * 0) / 255, abs=2e-6)
     # v clamps to the last row
     assert S.bilinear(1, 3.0 / W, 1.7)[0] == pytest.approx((3 * 3 + 7 * (H - 1)) / 255, abs=2e-6)
+
+
+def test_sphere_colour_is_the_texture_at_the_normal_times_the_shade():
+    """Sphere::getColor (scene/sphere.h:59-64): the texture at
+    u = (atan2(n_y, n_x) + pi) / 2 pi, v = acos(n_z) / pi of the unit normal n
+    at the hit, times clamp(0.5 + 0.5 n_z, 0, 1); untextured spheres are white.
+    Flat space, a camera at r = 20 looking at two spheres: the hit point (pinned
+    by the ray/sphere tests) names the sphere and its normal, the texture
+    filter is pinned by the bilinear test above."""
+    tex = RNG.integers(0, 256, size=(32, 64, 3), dtype=np.uint8)
+    tracer = A.make_tracer(solver=A.PRR_RKF45, h=0.05, hmin=1e-9, hmax=2.0, atol=1e-12, rtol=1e-12, r_escape=60.0,
+                           horizon_eps_rel=1e-3, horizon_eps_abs=0.0, lambda_max=1e9, max_steps=10**6,
+                           max_attempts=10**7)
+    S = O.Scene(A.PRR_FLAT, 0.0, 0.0, tracer)
+    S.set_texture(1, tex, A.PRR_FILTER_BILINEAR)
+    spheres = [(np.array([0.0, -3.0, 0.5]), 2.5, 1), (np.array([0.0, 3.5, -0.5]), 2.0, 0)]
+    f = A.make_frame([20.0, 0.0, 0.0], math.pi, 0.0, math.radians(20.0), 48, 32, mass=0.0,
+                     spheres=[(list(c), R, t) for c, R, t in spheres])
+    r = S.render_rays(f, want_state=True)
+    hit = r["cls"] == A.PRR_SPHERE
+    assert hit.sum() > 200
+    counts = [0, 0]
+    for P, n, col in zip(r["state8"][hit, :3], r["endpoint"][hit], r["color"][hit]):
+        k = int(np.argmin([abs(np.linalg.norm(P - c) - R) for c, R, _ in spheres]))
+        c, R, t = spheres[k]
+        m = (P - c) / R
+        assert np.abs(n - m).max() < 1e-9
+        shade = np.float32(min(max(np.float32(0.5) + np.float32(0.5) * np.float32(m[2]), 0.0), 1.0))
+        if t:
+            u = (math.atan2(m[1], m[0]) + math.pi) / (2 * math.pi)
+            v = math.acos(max(-1.0, min(1.0, m[2]))) / math.pi
+            expect = S.bilinear(t, u, v) * shade
+        else:
+            expect = np.full(3, shade, np.float32)
+        assert np.abs(col - expect).max() < 1e-6, (k, m, col, expect)
+        counts[k] += 1
+    assert min(counts) > 100

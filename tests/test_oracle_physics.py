@@ -14,6 +14,7 @@ from scipy import integrate
 
 import oracle as O
 from parrelray_amd import _abi as A
+from parrelray_amd import configs as CF
 
 GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
 RNG = np.random.default_rng(20240531)
@@ -310,6 +311,26 @@ def test_ks_and_schwarzschild_coordinates_trace_the_same_orbit():
         r, th, ph = rs["xend"][0, :3]
         ps = r * np.array([math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
         assert np.allclose(pk, ps, atol=1e-8), (pk, ps)
+
+
+def test_kerr_escape_direction_at_zero_spin_is_the_ks_velocity():
+    """The escape direction of a Kerr ray is the Cartesian direction of its
+    coordinate velocity (u^r e_r + r u^th e_th + r sin th u^ph e_ph, in the
+    flat-space frame of (r, th, ph)); the Cartesian metrics use their spatial
+    velocity (getColorDefault2, main/base.h:119-128).  At a = 0 the Boyer-
+    Lindquist and Kerr-Schild charts share r, th, ph, so the same frame gives
+    the same classes and, for escaping rays located on r = r_escape (D21), the
+    same directions."""
+    T = tracer(r_escape=60.0)
+    pos, al, be = CF.orbit_pose(15.0, 70.0, 0.4)
+    f = A.make_frame(pos, al, be, math.radians(60.0), 32, 24, mass=1.0, spin=0.0)
+    rk = O.Scene(A.PRR_SCHW_KS_CART, 1.0, 0.0, T).render_rays(f)
+    rb = O.Scene(A.PRR_KERR_BL, 1.0, 0.0, T).render_rays(f)
+    assert np.array_equal(rk["cls"], rb["cls"])
+    esc = rk["cls"] == A.PRR_ESCAPE
+    assert esc.sum() > 300 and (rk["cls"] == A.PRR_HORIZON).sum() > 20
+    cosang = np.clip(np.sum(rk["endpoint"][esc] * rb["endpoint"][esc], axis=1), -1.0, 1.0)
+    assert np.arccos(cosang).max() < 1e-7
 
 
 def _fixed_step_end(solver, h, N, x, u):
