@@ -33,6 +33,8 @@ sys.path.insert(0, ROOT)
 P32, P64 = 157.3e12, 78.6e12
 P32_NONPACKED = 78.6e12
 BATCH = 4
+# k_integrate<Kerr, RKF45, FP32-bulk> on one C1 frame: 1849.9 MB read + 1473.7 MB written
+TRAFFIC_FP32_BULK_C1 = (1849.9 + 1473.7) * 1e6
 METRIC = "Kerr RKF45 1080p rays/s + adjoint rays/s, FP32-bulk vs FP64 at ≤1e-6 rad"
 
 
@@ -267,7 +269,12 @@ def main():
         roof = {"bound": "valu", "kernel": "k_integrate<Kerr,RKF45,FP32-bulk>, one C1 frame per launch",
                 "achieved": (f32 + f64) / t_int / 1e12, "peak": (f32 + f64) / t_ideal / 1e12,
                 "unit": "TFLOP/s", "frac": t_ideal / t_int, "frac_nonpacked": t_ideal_np / t_int,
-                "traffic": None, "flops_fp32_per_launch": f32, "flops_fp64_per_launch": f64,
+                # HBM bytes per launch of this kernel on this frame: FETCH_SIZE x 2 + WRITE_SIZE
+                # (calibrated on a 1 GiB copy), from the rocprofv3 --pmc passes of
+                # profiles/k_integrate_kerr_c1_1080p.md; a constant of the kernel and workload
+                "traffic": TRAFFIC_FP32_BULK_C1, "traffic_unit": "bytes",
+                "traffic_source": "rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, profiles/k_integrate_kerr_c1_1080p.md",
+                "flops_fp32_per_launch": f32, "flops_fp64_per_launch": f64,
                 "launch_ms": fp32["integrate_ms"]}
     head = bat or fp32 or fp64
     line = {
