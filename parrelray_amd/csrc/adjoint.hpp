@@ -199,41 +199,44 @@ PRR_HD void kerr_step(const double y[6], double h, const Consts<double> &cs, dou
 // stages; lambda = y[5] feeds nothing back and is not carried).
 PRR_HD void kerr_step_vjp(const double y[6], double h, const Consts<double> &cs, double lam[9]) {
     const KerrCoef<double> kc = Kerr::coef<double>(cs);
-    double k[6][5];
-    for (int s = 0; s < 6; ++s) {
-        double ys[6];
-        for (int i = 0; i < 5; ++i) {
-            double acc = y[i];
-            for (int j = 0; j < s; ++j) acc += Fehl::a(s, j) * k[j][i];
-            ys[i] = acc;
+    // forward: the stage states; the right-hand side reads components 0..3
+    // only, so phi's and lambda's stage values are not formed, and the
+    // adjoint of phi passes through unchanged (d f / d phi = 0)
+    double Y[6][4];
+    {
+        double k[6][4];
+        for (int s = 0; s < 6; ++s) {
+            double ys[6];
+            for (int i = 0; i < 4; ++i) {
+                double acc = y[i];
+                for (int j = 0; j < s; ++j) acc += Fehl::a(s, j) * k[j][i];
+                ys[i] = acc;
+                Y[s][i] = acc;
+            }
+            ys[4] = ys[5] = 0.0;
+            double f[6];
+            Kerr::rhs<double>(ys, kc, f);
+            for (int i = 0; i < 4; ++i) k[s][i] = h * f[i];
         }
-        ys[5] = 0.0;
-        double f[6];
-        Kerr::rhs<double>(ys, kc, f);
-        for (int i = 0; i < 5; ++i) k[s][i] = h * f[i];
     }
-    double lk[6][5];
+    double lk[6][4];
     for (int s = 0; s < 6; ++s)
-        for (int i = 0; i < 5; ++i) lk[s][i] = Fehl::b4(s) * lam[i];
-    double ly[5] = {lam[0], lam[1], lam[2], lam[3], lam[4]};
+        for (int i = 0; i < 4; ++i) lk[s][i] = Fehl::b4(s) * lam[i];
+    double ly[4] = {lam[0], lam[1], lam[2], lam[3]};
     KerrCoef<double> kb{0, 0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int s = 5; s >= 0; --s) {
-        double ys[6], fb[6], yb[6] = {0, 0, 0, 0, 0, 0};
-        for (int i = 0; i < 5; ++i) {
-            double acc = y[i];
-            for (int j = 0; j < s; ++j) acc += Fehl::a(s, j) * k[j][i];
-            ys[i] = acc;
-            fb[i] = h * lk[s][i];
-        }
-        ys[5] = 0.0;
+        double ys[6] = {Y[s][0], Y[s][1], Y[s][2], Y[s][3], 0.0, 0.0};
+        double fb[6], yb[6] = {0, 0, 0, 0, 0, 0};
+        for (int i = 0; i < 4; ++i) fb[i] = h * lk[s][i];
+        fb[4] = h * (Fehl::b4(s) * lam[4]);
         fb[5] = 0.0;
         Kerr::rhs_vjp(ys, kc, fb, yb, kb);
-        for (int i = 0; i < 5; ++i) {
+        for (int i = 0; i < 4; ++i) {
             ly[i] += yb[i];
             for (int j = 0; j < s; ++j) lk[j][i] += Fehl::a(s, j) * yb[i];
         }
     }
-    for (int i = 0; i < 5; ++i) lam[i] = ly[i];
+    for (int i = 0; i < 4; ++i) lam[i] = ly[i];
     Kerr::coef_vjp(cs, kb, lam + 5);
 }
 

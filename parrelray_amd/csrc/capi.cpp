@@ -1073,6 +1073,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         ad.t0 = t0;
         ad.rec_stride = t1 - t0;
         ad.count = count[c.second];
+        ad.seed_lo = count[c.first + 1]; // rays whose last segment is in this chunk: nseg - 1 in [jlo, jhi]
         HIPTRY(ctx, launch_adj_reverse(j3, ad, st));
     }
     JobDev jg = job;

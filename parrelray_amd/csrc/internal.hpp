@@ -151,6 +151,7 @@ struct AdjDev {
     int32_t K;
     int32_t jlo, jhi;        // segment levels of this launch's replayed chunk, jhi >= jlo
     int64_t count;           // rays in this launch (the first `count` of `rays`: those with a segment jlo)
+    int64_t seed_lo;         // seeding: ranks seed_lo..count-1 end in this chunk (their last segment is in it)
     const int64_t *rays;     // rays sorted by decreasing segment count
     const int64_t *levelstart; // task index of (level j, rank 0); level j holds the ranks < count of level j
     int64_t t0;              // first task of the chunk (its records start there)

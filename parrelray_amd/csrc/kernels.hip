@@ -341,12 +341,39 @@ __device__ void adj_terminal(const JobDev &job, const AdjDev &ad, const FrameDev
 // Reverse sweep over segment j of every ray of this launch: the terminal seed
 // on the ray's last segment, then lambda_n = lambda_{n+1} dPhi_n/dy_n over
 // the replayed steps.  lambda lives in ad.lam between launches.
+// Seed of the reverse sweep at each ray's terminal event: lambda from the
+// pixel weights of its colour (forward-mode duals through the escape
+// direction and sky lookup, or the sphere's texture and normal, plus the
+// located event's fraction) into ad.lam, for the rays whose last segment lies
+// in this chunk (ranks seed_lo..count-1).  A kernel of its own, so that the
+// duals' registers do not weigh on the sweep.
+__global__ void __launch_bounds__(256) k_adj_seed(JobDev job, AdjDev ad) {
+    const int64_t idx = ad.seed_lo + int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= ad.count) return;
+    const int64_t id = ad.rays[idx];
+    double *lm = ad.lam + 10 * id;
+    const FrameDev &fr = job.frames[find_frame(job, id)];
+    Consts<double> cs;
+    load_consts<Kerr>(job, id, fr, cs);
+    const int cls = ad.cls[id];
+    double w[3] = {0.0, 0.0, 0.0};
+    if (cls == PRR_ESCAPE || cls == PRR_SPHERE) color_weights(ad, fr, id - fr.ray_begin, ad.ray_color + 3 * id, w);
+    if (ad.event[id].cls != cls && (cls == PRR_ESCAPE || cls == PRR_SPHERE)) atomicAdd(ad.mismatch, 1ull);
+    if ((w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0) || ad.event[id].cls != cls) {
+        for (int i = 0; i < 10; ++i) lm[i] = 0.0;
+        return;
+    }
+    double lam[9], gphs;
+    adj_terminal(job, ad, fr, cs, ad.event[id], w, lam, gphs);
+    for (int i = 0; i < 9; ++i) lm[i] = lam[i];
+    lm[9] = gphs;
+}
+
 // Reverse sweep over the segment levels jhi..jlo of one replayed chunk: one
 // thread per ray (rank idx in the decreasing-segment-count order), from its
 // highest segment in the chunk down to jlo, reading segment j's steps from
-// task levelstart[j] + idx of the chunk's records.  lambda enters from the
-// terminal event (the ray's last segment) or from ad.lam (the previous
-// chunk) and leaves through ad.lam.
+// task levelstart[j] + idx of the chunk's records.  lambda enters from ad.lam
+// (the seed, or the previous chunk) and leaves through it.
 __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= ad.count) return;
@@ -355,29 +382,16 @@ __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
     const int K = ad.K;
     const int32_t last = (N - 1) / K; // the ray's last segment (the rank order puts it at >= jlo)
     double *lm = ad.lam + 10 * id;
+    double lam[9];
+    for (int i = 0; i < 9; ++i) lam[i] = lm[i];
+    bool zero = true;
+    for (int i = 0; i < 9; ++i) zero = zero && lam[i] == 0.0;
+    if (zero) return; // nothing flows back (lm stays zero)
     const FrameDev &fr = job.frames[find_frame(job, id)];
     Consts<double> cs;
     load_consts<Kerr>(job, id, fr, cs);
-    double lam[9], gphs;
     const int32_t jtop = min(last, ad.jhi);
-    if (jtop == last) {
-        const int cls = ad.cls[id];
-        double w[3] = {0.0, 0.0, 0.0};
-        if (cls == PRR_ESCAPE || cls == PRR_SPHERE)
-            color_weights(ad, fr, id - fr.ray_begin, ad.ray_color + 3 * id, w);
-        if (ad.event[id].cls != cls && (cls == PRR_ESCAPE || cls == PRR_SPHERE)) atomicAdd(ad.mismatch, 1ull);
-        if ((w[0] == 0.0 && w[1] == 0.0 && w[2] == 0.0) || ad.event[id].cls != cls) {
-            for (int i = 0; i < 10; ++i) lm[i] = 0.0;
-            return;
-        }
-        adj_terminal(job, ad, fr, cs, ad.event[id], w, lam, gphs);
-    } else {
-        for (int i = 0; i < 9; ++i) lam[i] = lm[i];
-        gphs = lm[9];
-    }
-    bool zero = true;
-    for (int i = 0; i < 9; ++i) zero = zero && lam[i] == 0.0;
-    if (!zero) {
+    {
         const int64_t T = ad.rec_stride;
         for (int32_t j = jtop; j >= ad.jlo; --j) {
             const int64_t col = ad.levelstart[j] - ad.t0 + idx;
@@ -392,7 +406,6 @@ __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
         }
     }
     for (int i = 0; i < 9; ++i) lm[i] = lam[i];
-    lm[9] = gphs;
 }
 
 // Ray set-up Jacobian d(y_0, E, L, Q)/d(a, r_cam, incl) contracted with
@@ -579,6 +592,9 @@ __global__ void __launch_bounds__(256) k_grad_reduce_final(const double *partial
 
 hipError_t launch_adj_reverse(const JobDev &job, const AdjDev &ad, hipStream_t st) {
     if (ad.count <= 0) return hipSuccess;
+    if (ad.count > ad.seed_lo)
+        hipLaunchKernelGGL(k_adj_seed, dim3(unsigned((ad.count - ad.seed_lo + 255) / 256)), dim3(256), 0, st, job,
+                           ad);
     hipLaunchKernelGGL(k_adj_reverse, dim3(unsigned((ad.count + 255) / 256)), dim3(256), 0, st, job, ad);
     return hipGetLastError();
 }

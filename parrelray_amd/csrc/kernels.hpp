@@ -20,7 +20,8 @@ hipError_t launch_reset(CountersDev *c, hipStream_t st, bool all = true);
 hipError_t launch_ssaa(const PostFrameDev *pfs, int nframes, int64_t max_pixels, hipStream_t st);
 // FXAA + RGB8 for the frames whose fxaa flag is set.
 hipError_t launch_fxaa(const PostFrameDev *pfs, int nframes, int max_w, int max_h, hipStream_t st);
-// Kerr adjoint: reverse sweep over one segment index, and the set-up Jacobian.
+// Kerr adjoint: seed at the terminal events and reverse sweep over one chunk of
+// segment levels; the set-up Jacobian.
 hipError_t launch_adj_reverse(const JobDev &job, const AdjDev &ad, hipStream_t st);
 hipError_t launch_adj_init_grad(const JobDev &job, const AdjDev &ad, hipStream_t st);
 // Adjoint planning: segment counts, checkpoint offsets, the overflow rays and
