@@ -434,7 +434,9 @@ struct Kerr {
                                KerrCoef<double> &kb) {
         const double u = y[0], mu = y[2];
         const double D = 1.0 - u * (k.m2 - k.a2 * u), s2 = 1.0 - mu * mu, u2 = u * u;
-        const double iD = 1.0 / D, is2 = 1.0 / s2, iu2 = 1.0 / u2;
+        // one reciprocal for the three denominators, as in rhs
+        const double q = rcp_t(D * s2 * u2);
+        const double iD = q * s2 * u2, is2 = q * D * u2, iu2 = q * D * s2;
         const double N = u * (k.pa - k.pb * u);
         // f0 = w, f2 = nu
         yb[1] += fb[0];
