@@ -540,15 +540,14 @@ JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
 }
 
 // Enqueue reset -> init -> integrate -> shade -> SSAA -> FXAA, with timing events.
-// adj: integrate with the adjoint-capable kernel binary (the adjoint's first
-// pass, whose step counts its checkpoint and replay passes must reproduce).
-prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events,
-                          bool adj = false) {
+// job.adj.mode picks the integrator binary: the render one, or (the adjoint's
+// first pass) the checkpointing one, whose steps the replay must reproduce.
+prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events) {
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     HIPTRY(ctx, launch_reset(job.ctr, st));
     HIPTRY(ctx, launch_init(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    HIPTRY(ctx, launch_integrate(ctx->metric, job, integrate_grid(ctx), ctx->launch.block, st, adj));
+    HIPTRY(ctx, launch_integrate(ctx->metric, job, integrate_grid(ctx), ctx->launch.block, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     HIPTRY(ctx, launch_shade(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[5], st));
@@ -875,7 +874,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
-    if ((s = enqueue_render(ctx, P, job, st, false, true)) != PRR_OK) return s;
+    if ((s = enqueue_render(ctx, P, job, st, false)) != PRR_OK) return s;
     // dL/dI from the host: copied on a second stream while the first pass
     // runs; the reverse sweep waits for it
     const float *dldi = dL_dimg;
@@ -1005,7 +1004,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         ji.total = T;
         HIPTRY(ctx, launch_init(ctx->metric, ji, st));
         const int grid = int(std::min<int64_t>(integrate_grid(ctx), (j2.total + 63) / 64 + 1));
-        HIPTRY(ctx, launch_integrate(ctx->metric, j2, grid, ctx->launch.block, st, true));
+        HIPTRY(ctx, launch_integrate(ctx->metric, j2, grid, ctx->launch.block, st));
         j2.total = T;
     }
     HIPTRY(ctx, launch_reset(ctr, st, true));

@@ -152,7 +152,8 @@ template <class T> struct MassCoef {
 // ----------------------------------------------------------------- metrics
 // metrics/flat.cuh: straight lines, getGeod = 0 (:88-90).
 struct Flat {
-    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
+    // the integrator carries the adjoint's checkpoint and replay paths
+    static constexpr bool adj = true, ckpt = true, replay = true;
     static constexpr int N = 8;
     static constexpr int kind = PRR_FLAT;
     static constexpr bool spherical = false;
@@ -173,7 +174,8 @@ struct Flat {
 //   du^i = x^i r_s/(2 r^3) [3q^2 - 2|u|^2 - (u^t)^2 + w (q + u^t)^2]
 //   du^t = r_s/(2 r^2) [2|u|^2 - 4q^2 - 2 q u^t - w (q + u^t)^2].
 struct SchwKS {
-    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
+    // the integrator carries the adjoint's checkpoint and replay paths
+    static constexpr bool adj = true, ckpt = true, replay = true;
     static constexpr int N = 8;
     static constexpr int kind = PRR_SCHW_KS_CART;
     static constexpr bool spherical = false;
@@ -206,7 +208,8 @@ struct SchwKS {
 // "metrics/schwarzschild - sph.cuh":94-115, including its tan/sin clamp
 // (|tan th| < 1e-4 -> tan = sin = 1e-4, :98-101).
 struct SchwSph {
-    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
+    // the integrator carries the adjoint's checkpoint and replay paths
+    static constexpr bool adj = true, ckpt = true, replay = true;
     static constexpr int N = 8;
     static constexpr int kind = PRR_SCHW_SPH;
     static constexpr bool spherical = true;
@@ -253,7 +256,8 @@ struct SchwSph {
 // space; tan and sin are clamped to 1e-4 in magnitude near the axis, with
 // their signs (the path crosses the axis: axis_post_step).
 struct Ellis {
-    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
+    // the integrator carries the adjoint's checkpoint and replay paths
+    static constexpr bool adj = true, ckpt = true, replay = true;
     static constexpr int N = 8;
     static constexpr int kind = PRR_ELLIS;
     static constexpr bool spherical = true;
@@ -311,7 +315,8 @@ struct KerrDelta {
 };
 
 struct Kerr {
-    static constexpr bool adj = true; // the integrator carries the adjoint's checkpoint/replay paths
+    // the integrator carries the adjoint's checkpoint and replay paths
+    static constexpr bool adj = true, ckpt = true, replay = true;
     static constexpr int N = 6;
     static constexpr int NERR = 5; // lambda is bookkeeping: outside the error norm
     static constexpr int kind = PRR_KERR_BL;
@@ -938,8 +943,17 @@ PRR_HD int chord_hit(const FrameDev &fr, D3 P0, D3 P1, double &t_hit) {
 // A metric whose integrator has no adjoint paths (checkpoints, replay
 // records, event capture): the render instantiation, with their registers
 // and branches compiled out.
+// Instantiations of the integrator: renders carry no adjoint code (NoAdj);
+// the Kerr adjoint's checkpointing passes and its replay each carry only
+// their own path (AdjCkpt, AdjReplay).
 template <class M> struct NoAdj : M {
-    static constexpr bool adj = false;
+    static constexpr bool adj = false, ckpt = false, replay = false;
+};
+template <class M> struct AdjCkpt : M {
+    static constexpr bool adj = true, ckpt = true, replay = false;
+};
+template <class M> struct AdjReplay : M {
+    static constexpr bool adj = true, ckpt = false, replay = true;
 };
 
 } // namespace prr

@@ -54,7 +54,7 @@ template <class Mt, int SOLVER, bool MIXED>
 __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
     const int lane = int(threadIdx.x & 63);
     LocateCtx &C = job.loc[int64_t(blockIdx.x) * blockDim.x + threadIdx.x];
-    const bool replay = Mt::adj && job.adj.mode == ADJ_REPLAY;
+    const bool replay = Mt::replay && job.adj.mode == ADJ_REPLAY;
     Lane<Mt> L;
     int mode = MODE_IDLE;
     int32_t end = 0; // replay: stop at this step count
@@ -82,9 +82,9 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
                         end = min((seg + 1) * job.adj.K, job.adj.nsteps[rid]);
                     } else {
                         // a checkpoint pass over some of the rays takes them from adj.rays
-                        const bool some = Mt::adj && job.adj.mode == ADJ_CHECKPOINT && job.adj.rays;
+                        const bool some = Mt::ckpt && job.adj.mode == ADJ_CHECKPOINT && job.adj.rays;
                         lane_load<Mt>(job, L, some ? job.adj.rays[idx] : queue_ray(job, idx));
-                        if (Mt::adj && job.adj.mode == ADJ_CHECKPOINT)
+                        if (Mt::ckpt && job.adj.mode == ADJ_CHECKPOINT)
                             store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, 0);
                     }
                     mode = MODE_ACTIVE;
@@ -652,12 +652,18 @@ template <class Mt> void launch_integrate_t(const JobDev &job, dim3 g, dim3 b, h
     }
 }
 
-hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st, bool adj) {
+hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st) {
     const dim3 g(grid), b(block);
     return with_metric(metric, [&](auto mt) {
         using Mt = decltype(mt);
-        if (adj || job.adj.mode != ADJ_NONE) launch_integrate_t<Mt>(job, g, b, st);
-        else launch_integrate_t<NoAdj<Mt>>(job, g, b, st);
+        if (job.adj.mode == ADJ_NONE) {
+            launch_integrate_t<NoAdj<Mt>>(job, g, b, st);
+        } else if constexpr (Mt::kind == PRR_KERR_BL) { // the adjoint differentiates Kerr only
+            if (job.adj.mode == ADJ_CHECKPOINT) launch_integrate_t<AdjCkpt<Mt>>(job, g, b, st);
+            else launch_integrate_t<AdjReplay<Mt>>(job, g, b, st);
+        } else {
+            return hipErrorInvalidValue;
+        }
         return hipGetLastError();
     });
 }

@@ -10,10 +10,11 @@ namespace prr {
 // Ray set-up (k_init) and shading (k_shade), one thread per ray of the job.
 hipError_t launch_init(int metric, const JobDev &job, hipStream_t st);
 hipError_t launch_shade(int metric, const JobDev &job, hipStream_t st);
-// Persistent integrator over job.total rays (or replay indices); grid/block in workgroups/threads.
-// adj: the instantiation with the adjoint's checkpoint/replay paths (always
-// taken when job.adj.mode is not ADJ_NONE); renders use the one without.
-hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st, bool adj = false);
+// Persistent integrator over job.total rays (or replay tasks); grid/block in workgroups/threads.
+// job.adj.mode picks the instantiation: renders (ADJ_NONE) carry no adjoint
+// code; the Kerr adjoint's checkpointing passes and its replay each carry
+// their own path (other metrics have no adjoint: hipErrorInvalidValue).
+hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st);
 // Zero the device counters and the ray-queue head (all: also the adjoint's mismatch count).
 hipError_t launch_reset(CountersDev *c, hipStream_t st, bool all = true);
 // SSAA combine for nframes frames (pfs is a device array).

@@ -680,7 +680,7 @@ template <int N> PRR_HD void copy_n(double *d, const double *s) {
 
 // Adjoint checkpoint passes: a checkpoint every K accepted steps while the lane integrates.
 template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<Mt> &L) {
-    if (!Mt::adj || job.adj.mode != ADJ_CHECKPOINT) return;
+    if (!Mt::ckpt || job.adj.mode != ADJ_CHECKPOINT) return;
     const int K = job.adj.K;
     if (L.steps % K != 0) return;
     // with known step counts, checkpoints at 0, K, 2K, ... < N_i; in the
@@ -691,7 +691,7 @@ template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<M
 
 // Adjoint replay: the step's start and size of every accepted step of the segment.
 template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &L, double h) {
-    if (!Mt::adj || job.adj.mode != ADJ_REPLAY) return;
+    if (!Mt::replay || job.adj.mode != ADJ_REPLAY) return;
     const int64_t slot = (L.steps - 1) - int64_t(L.seg) * job.adj.K; // L.steps already counts this step
     const int64_t T = job.adj.rec_stride;
     double *r = job.adj.rec + (slot * 6) * T + L.rec_i;
@@ -704,7 +704,7 @@ template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &
 template <class Mt>
 PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, const LocateCtx &C, const double *ys, double theta, int cls,
                           int hit) {
-    if (Mt::adj && job.adj.mode == ADJ_REPLAY && job.adj.event) {
+    if (Mt::replay && job.adj.mode == ADJ_REPLAY && job.adj.event) {
         EventDev &ev = job.adj.event[L.rid];
         for (int i = 0; i < 8; ++i) ev.y_prev[i] = i < Mt::N ? L.y[i] : 0.0;
         ev.h = C.h;
