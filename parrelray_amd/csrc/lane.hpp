@@ -156,7 +156,8 @@ PRR_HD int find_frame(const JobDev &job, int64_t id) {
 // Queue position -> ray id.  The queue hands a wave 64 consecutive positions;
 // within each frame they map to a tile of pixels (all samples of each pixel)
 // rather than to a run of one image row, so that a wave's rays are close
-// together on the sky and follow similar paths.  Tiles hold 64 / s pixels
+// together on the sky and follow similar paths; tiles alternate between the
+// top and the bottom of the image.  Tiles hold 64 / s pixels
 // (s samples per pixel, a divisor of 64), Th x Tw with Th the largest power
 // of two whose square fits; the part of the image that whole tiles do not
 // cover (a right strip, then a bottom strip) keeps row order.  A bijection on
@@ -177,8 +178,15 @@ PRR_HD int64_t queue_ray(const JobDev &job, int64_t q) {
     const int64_t ncore = wc * hc * s;
     int64_t i, j, k;
     if (local < ncore) {
-        const int64_t per = P * s, t = local / per, w = local - t * per;
-        const int64_t tiles_x = wc / tw, ty = t / tiles_x, tx = t - ty * tiles_x;
+        const int64_t per = P * s, q_t = local / per, w = local - q_t * per;
+        // tiles are handed out alternately from the first and the last tile
+        // row inwards, so that features mirrored about the middle of the image
+        // (the photon ring above and below the shadow, whose rays take the
+        // most steps) start early in launches whose rays fit the chip a few
+        // times over, instead of half of them starting last
+        const int64_t tiles_x = wc / tw, ntiles = tiles_x * (hc / th);
+        const int64_t t = (q_t & 1) ? ntiles - 1 - (q_t >> 1) : (q_t >> 1);
+        const int64_t ty = t / tiles_x, tx = t - ty * tiles_x;
         const int64_t p = w / s;
         k = w - p * s;
         i = tx * tw + p % tw;
