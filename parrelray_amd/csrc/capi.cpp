@@ -434,7 +434,7 @@ namespace {
 struct RenderPlan {
     std::vector<FrameDev> fd;
     std::vector<PostFrameDev> pf;
-    int64_t total = 0, pixels = 0, max_pix = 0;
+    int64_t total = 0, qtotal = 0, pixels = 0, max_pix = 0;
     int max_w = 0, max_h = 0;
     bool any_fxaa = false;
     double ssaa_bytes = 0, fxaa_bytes = 0;
@@ -453,6 +453,11 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
         if (st != PRR_OK) return st;
         P.fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total, metric_param(ctx), far_escape(ctx));
         const int64_t np = int64_t(frames[i].width) * frames[i].height;
+        // each frame's queue starts at a multiple of 64, so that a whole-wave
+        // refill never mixes two frames and a frame's waves are the same
+        // whether it is rendered alone or in a batch
+        P.fd[i].queue_begin = (P.qtotal + 63) / 64 * 64;
+        P.qtotal = P.fd[i].queue_begin + np * frames[i].samples;
         P.total += np * frames[i].samples;
         P.pixels += np;
         P.max_pix = std::max(P.max_pix, np);
@@ -522,8 +527,12 @@ JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
     JobDev job{};
     job.frames = static_cast<const FrameDev *>(ctx->frames.p);
     job.nframes = int(P.fd.size());
-    job.refill = ctx->launch.refill;
+    // FP32-bulk always refills whole waves: its choice of stage precision is
+    // made per wave (k_integrate), and whole-wave refills keep it independent
+    // of the launch shape
+    job.refill = ctx->tracer.precision == PRR_FP32_BULK ? 64 : ctx->launch.refill;
     job.total = P.total;
+    job.qtotal = P.qtotal > 0 ? P.qtotal : P.total;
     job.explicit_x0u0 = nullptr;
     job.tr = tracer_dev(ctx->tracer);
     for (int i = 0; i < PRR_MAX_TEXTURES; ++i) job.tex[i] = ctx->tex[i];

@@ -246,6 +246,7 @@ prr_status prr_debug_queue_rays(const prr_frame *f, const int64_t *q, int64_t n,
     job.frames = &fr;
     job.nframes = 1;
     job.total = int64_t(f->width) * f->height * f->samples;
+    job.qtotal = job.total;
     for (int64_t i = 0; i < n; ++i) {
         if (q[i] < 0 || q[i] >= job.total) return PRR_EINVAL;
         ids[i] = queue_ray(job, q[i]);

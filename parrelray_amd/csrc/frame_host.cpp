@@ -68,6 +68,7 @@ FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t
                                       : rh * (1.0 + tr.horizon_eps_rel) + tr.horizon_eps_abs;
     d.r_esc2 = r_esc2 > 0.0 ? r_esc2 : tr.r_escape;
     d.ray_begin = begin;
+    d.queue_begin = begin;
     for (int k = 0; k < f.n_spheres; ++k) {
         for (int c = 0; c < 3; ++c) d.sph[k][c] = f.spheres[k].c[c];
         d.sph[k][3] = f.spheres[k].radius;

@@ -47,6 +47,7 @@ struct FrameDev {
     double r_hor; // horizon test radius r_h (1 + eps_rel) + eps_abs (negative: no test)
     double r_esc2; // escape radius beyond a wormhole's throat (l < 0; rout2)
     int64_t ray_begin;
+    int64_t queue_begin; // first queue position (frames start at multiples of 64)
     double sph[PRR_MAX_SPHERES][4]; // centre xyz, radius
     int32_t sph_tex[PRR_MAX_SPHERES];
 };
@@ -134,6 +135,7 @@ struct JobDev {
     int32_t nframes;
     int32_t refill;
     int64_t total;               // rays over all frames (replay: job indices)
+    int64_t qtotal;              // render queue length: total plus the padding between frames
     const double *explicit_x0u0; // prr_trace: 8 initial doubles per ray, frame 0 supplies the rest
     TracerDev tr;
     TexDev tex[PRR_MAX_TEXTURES];

@@ -60,6 +60,9 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
     int32_t end = 0; // replay: stop at this step count
     bool exhausted = false;
     unsigned acc_att = 0, acc_ok = 0, acc_64 = 0, acc_rays = 0;
+    // queue length: the render queue has padding between frames; adjoint
+    // passes and explicit initial states run over job.total entries
+    const int64_t qn = (Mt::ckpt || Mt::replay || job.explicit_x0u0) ? job.total : job.qtotal;
     while (true) {
         const unsigned long long idle = __ballot(mode == MODE_IDLE);
         const int nidle = __popcll(idle);
@@ -68,27 +71,31 @@ __global__ void __launch_bounds__(256) PRR_OCCUPANCY k_integrate(JobDev job) {
             unsigned long long base = 0;
             if (lane == leader) base = atomicAdd(&job.ctr->queue, (unsigned long long)nidle);
             base = shfl_u64(base, leader);
-            if (base + (unsigned long long)nidle >= (unsigned long long)job.total) exhausted = true;
+            if (base + (unsigned long long)nidle >= (unsigned long long)qn) exhausted = true;
             if (mode == MODE_IDLE) {
                 const unsigned rank =
                     __builtin_amdgcn_mbcnt_hi(unsigned(idle >> 32), __builtin_amdgcn_mbcnt_lo(unsigned(idle), 0u));
                 const int64_t idx = int64_t(base) + rank;
-                if (idx < job.total) {
+                if (idx < qn) {
                     if (replay) { // task idx: segment adj.seg[idx] of ray adj.rays[idx]
                         const int64_t rid = job.adj.rays[idx];
                         const int32_t seg = job.adj.seg[idx];
                         lane_load_checkpoint<Mt>(job, L, rid, seg);
                         L.rec_i = idx;
                         end = min((seg + 1) * job.adj.K, job.adj.nsteps[rid]);
+                        mode = MODE_ACTIVE;
                     } else {
                         // a checkpoint pass over some of the rays takes them from adj.rays
                         const bool some = Mt::ckpt && job.adj.mode == ADJ_CHECKPOINT && job.adj.rays;
-                        lane_load<Mt>(job, L, some ? job.adj.rays[idx] : queue_ray(job, idx));
-                        if (Mt::ckpt && job.adj.mode == ADJ_CHECKPOINT)
-                            store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, 0);
+                        const int64_t rid = some ? job.adj.rays[idx] : queue_ray(job, idx);
+                        if (rid >= 0) { // -1: padding between frames, the lane stays idle
+                            lane_load<Mt>(job, L, rid);
+                            if (Mt::ckpt && job.adj.mode == ADJ_CHECKPOINT)
+                                store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, 0);
+                            mode = MODE_ACTIVE;
+                        }
                     }
-                    mode = MODE_ACTIVE;
-                    if (!finite_state<Mt>(L.y)) {
+                    if (mode == MODE_ACTIVE && !finite_state<Mt>(L.y)) {
                         if (!replay) lane_store<Mt>(job, L, PRR_NAN);
                         acc_rays++;
                         mode = MODE_IDLE;

@@ -153,7 +153,19 @@ PRR_HD int find_frame(const JobDev &job, int64_t id) {
     return lo;
 }
 
-// Queue position -> ray id.  The queue hands a wave 64 consecutive positions;
+// Frame of a queue position (frames' queue ranges start at multiples of 64).
+PRR_HD int find_frame_q(const JobDev &job, int64_t q) {
+    int lo = 0, hi = job.nframes - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (job.frames[mid].queue_begin <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+// Queue position -> ray id, or -1 for the padding after a frame whose ray
+// count is not a multiple of 64.  The queue hands a wave 64 consecutive positions;
 // within each frame they map to a tile of pixels (all samples of each pixel)
 // rather than to a run of one image row, so that a wave's rays are close
 // together on the sky and follow similar paths; tiles alternate between the
@@ -165,16 +177,17 @@ PRR_HD int find_frame(const JobDev &job, int64_t id) {
 // the identity.
 PRR_HD int64_t queue_ray(const JobDev &job, int64_t q) {
     if (job.explicit_x0u0) return q;
-    const FrameDev &fr = job.frames[find_frame(job, q)];
+    const FrameDev &fr = job.frames[find_frame_q(job, q)];
     const int64_t s = fr.s;
-    if (s <= 0 || 64 % s != 0) return q;
+    const int64_t local = q - fr.queue_begin;
+    if (local >= int64_t(fr.W) * fr.H * s) return -1;
+    if (s <= 0 || 64 % s != 0) return fr.ray_begin + local;
     const int64_t P = 64 / s;
     int64_t th = 1;
     while ((th * 2) * (th * 2) <= P) th *= 2;
     const int64_t tw = P / th;
     const int64_t W = fr.W, H = fr.H;
     const int64_t wc = (W / tw) * tw, hc = (H / th) * th;
-    const int64_t local = q - fr.ray_begin;
     const int64_t ncore = wc * hc * s;
     int64_t i, j, k;
     if (local < ncore) {
@@ -280,6 +293,16 @@ PRR_HD double step_factor(double errn) {
     const double y2 = y * y;
     const double y5 = y2 * y2 * y;
     return 0.9 * (y * (1.2 - 0.2 * (x * y5)));
+}
+
+// True when c holds for any lane of the wave that executes this call (on the
+// host, which runs one lane at a time: c itself).
+PRR_HD bool wave_any(bool c) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    return __ballot(c) != 0;
+#else
+    return c;
+#endif
 }
 
 // Six RK stages in precision T from the FP64 state y (plain or FP32 form of
@@ -1010,8 +1033,16 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
             const double mu_reach = fabs(L.y[2]) + hs * fabs(L.y[3]);
             use64 = 1.0 - mu_reach * mu_reach < kAxisFP64;
         }
-        L.att64 += use64 ? 1 : 0;
     }
+    // FP32-bulk render with whole-wave refills (each wave takes 64 consecutive
+    // queue entries at once): a trip takes the FP64 stages for all lanes of
+    // the wave when any of them needs them, so that no trip executes both
+    // stage forms (on a SIMD, both cost the same).  The decision depends only
+    // on those 64 rays, so results do not depend on scheduling.  The adjoint's passes keep the per-lane choice: its
+    // replay must repeat each ray's steps bit for bit outside its wave.
+    if constexpr (MIXED && !Mt::ckpt && !Mt::replay)
+        if (job.refill >= 64) use64 = wave_any(use64);
+    if (mode != MODE_LOCATE) L.att64 += use64 ? 1 : 0;
     double d4[N], e[N];
     if (use64) rkf_stages<Mt, double, Tab>(L.y, hs, L.cs, d4, e);
     else if (Mt::mino) kerr_stages_fp32<Tab>(L.y, hs, L.cs, d4, e);
