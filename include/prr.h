@@ -172,7 +172,8 @@ typedef struct prr_ray_out {
 typedef struct prr_launch {
     int32_t block;         /* workgroup size, multiple of 64, <= 1024 */
     int32_t blocks_per_cu; /* resident workgroups per CU in the persistent grid */
-    int32_t refill;        /* a wave refills once at least this many lanes are idle (1..64) */
+    int32_t refill;        /* a wave refills once at least this many lanes are idle (1..64);
+                              PRR_FP32_BULK always uses 64 */
     int32_t checkpoint;    /* adjoint: accepted steps between per-ray checkpoints */
 } prr_launch;
 

@@ -75,7 +75,9 @@ def tune_launch(ctx, frames, metric, tracer, device_name="gpu0", budget_s=60.0, 
                 break
     results.sort()
     _, block, bpc, _ = results[0]
-    for refill in REFILLS:
+    # FP32-bulk always refills whole waves (its stage precision is chosen per
+    # wave), so the refill threshold is not a dimension there
+    for refill in (REFILLS if tracer.precision != A.PRR_FP32_BULK else ()):
         ctx.set_launch(block, bpc, refill, 64)
         results.append((_time_integrate(ctx, frames), block, bpc, refill))
         if time.perf_counter() - start > budget_s:
