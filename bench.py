@@ -34,7 +34,7 @@ P32, P64 = 157.3e12, 78.6e12
 P32_NONPACKED = 78.6e12
 BATCH = 4
 # k_integrate<Kerr, RKF45, FP32-bulk> on one C1 frame: 1849.9 MB read + 1473.7 MB written
-TRAFFIC_FP32_BULK_C1 = (1849.9 + 1473.7) * 1e6
+TRAFFIC_FP32_BULK_C1 = (1881.1 + 1493.2) * 1e6
 METRIC = "Kerr RKF45 1080p rays/s + adjoint rays/s, FP32-bulk vs FP64 at ≤1e-6 rad"
 
 
