@@ -97,11 +97,11 @@ def run_adjoint(api, A, CF, reps):
                 loss=float(((I - T) ** 2).mean()))
 
 
-def run_c1_batched(api, A, CF, ctx, steps, warmup, torch):
-    """BATCH C1 frames in one persistent launch from a hipGraph (FP32-bulk)."""
+def run_c1_batched(api, A, CF, ctx, steps, warmup, torch, precision=None):
+    """BATCH C1 frames in one persistent launch from a hipGraph (FP32-bulk unless precision says otherwise)."""
     from parrelray_amd import autotune
     frames = [CF.frame_c1() for _ in range(BATCH)]
-    tracer = CF.tracer_kerr(precision=A.PRR_FP32_BULK)
+    tracer = CF.tracer_kerr(precision=A.PRR_FP32_BULK if precision is None else precision)
     ctx.set_tracer(tracer)
     shape = autotune.tune_launch(ctx, frames[:1], A.PRR_KERR_BL, tracer, use_cache=False)
     rgb8 = [torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
@@ -211,8 +211,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the C3 adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the C2 64-frame batch leg")
-    ap.add_argument("--legs", default="fp64,fp32,adjoint,c2,batched,cpu",
-                    help="comma-separated legs to run: fp64, fp32, adjoint, c2, batched, cpu, copy "
+    ap.add_argument("--legs", default="fp64,fp32,adjoint,c2,batched,batched64,cpu",
+                    help="comma-separated legs to run: fp64, fp32, adjoint, c2, batched, batched64 (the same "
+                         "batch in plain FP64), cpu, copy "
                          "(copy: a 1 GiB device-to-device copy, the calibration kernel for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -243,7 +244,7 @@ def main():
     ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
     out8 = torch.empty((frame.height, frame.width, 3), dtype=torch.uint8, device="cuda")
     outf = torch.empty((frame.height, frame.width, 3), dtype=torch.float32, device="cuda")
-    fp64 = fp32 = adj = batch = bat = None
+    fp64 = fp32 = adj = batch = bat = bat64 = None
     if "fp64" in legs:
         fp64 = run_form(ctx, frame, A.PRR_FP64, args.steps, args.warmup, out8, outf, torch)
         progress(fp64_ms_per_step=fp64["ms_per_step"])
@@ -259,6 +260,9 @@ def main():
     if "batched" in legs:
         bat = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch)
         progress(c1_batched_ms_per_frame=bat["ms_per_frame"])
+    if "batched64" in legs:
+        bat64 = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch, precision=A.PRR_FP64)
+        progress(c1_batched64_ms_per_frame=bat64["ms_per_frame"])
     cpu = cpu_baseline(frame) if "cpu" in legs else None
     roof = None
     if fp32:
@@ -312,6 +316,10 @@ def main():
         line["fp32_bulk_batched"] = bat
         if fp64:
             line["speedup_batched_fp32_bulk_vs_plain_fp64"] = bat["rays_per_s"] / fp64["rays_per_s"]
+    if bat64:
+        line["fp64_batched"] = bat64
+        if bat:
+            line["speedup_batched_fp32_bulk_vs_batched_fp64"] = bat["rays_per_s"] / bat64["rays_per_s"]
     if adj:
         line["adjoint_rays_per_s"] = adj["rays_per_s"]
         line["adjoint"] = adj
