@@ -675,7 +675,11 @@ template <class Mt, bool MIXED> PRR_HD int euler_step(const JobDev &job, Lane<Mt
     constexpr int N = Mt::N;
     const TracerDev &tr = job.tr;
     if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max) return PRR_STEP_CAP;
-    const bool fp64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
+    bool fp64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
+    // FP32-bulk render with whole-wave refills: one precision per wave and
+    // step, as for the Runge-Kutta attempts (lane_iterate)
+    if constexpr (MIXED && !Mt::ckpt && !Mt::replay)
+        if (job.refill >= 64) fp64 = wave_any(fp64);
     double f[N];
     if (fp64) euler_rhs<Mt, double>(L.y, L.cs, f);
     else euler_rhs<Mt, float>(L.y, L.cs, f);

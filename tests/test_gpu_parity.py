@@ -546,3 +546,37 @@ def test_fp32_bulk_close_to_fp64():
     m = same & ((c64["cls"] == A.PRR_ESCAPE) | (c64["cls"] == A.PRR_SPHERE))
     ang = angle(c64["endpoint"][m], c32["endpoint"][m])
     assert np.median(ang) < 1e-6
+
+
+@pytest.mark.parametrize("metric", [A.PRR_SCHW_KS_CART, A.PRR_KERR_BL])
+def test_fp32_bulk_euler_tracks_fp64_and_does_not_depend_on_launch_shape(metric):
+    """Euler steps in FP32-bulk: FP32 right-hand side outside r_fp64 (the
+    whole wave in FP64 when any of its lanes is inside), FP64 state.  Classes
+    agree with the FP64 Euler render; images and step counts do not depend on
+    the launch shape, and a frame renders the same alone or after a frame
+    whose ray count is not a multiple of 64."""
+    def ctx_for(precision):
+        T = A.make_tracer(solver=A.PRR_EULER, h=0.02, r_escape=40.0, horizon_eps_rel=1e-3, horizon_eps_abs=0.0,
+                          lambda_max=1e9, max_steps=6000, max_attempts=6000)
+        T.precision = precision
+        ctx = api.Context(0)
+        ctx.set_spacetime(metric, 1.0, 0.95 if metric == A.PRR_KERR_BL else 0.0)
+        ctx.set_tracer(T)
+        ctx.set_texture(0, CF.sky_texture())
+        ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
+        return ctx
+    f = CF.frame_c1(96, 54)
+    odd = CF.frame_c1(37, 23)  # 851 pixels x 4 samples: 3404 rays, not a multiple of 64
+    c64 = ctx_for(A.PRR_FP64).render(f, rays=True)[0]
+    ctx = ctx_for(A.PRR_FP32_BULK)
+    ref = ctx.render(f, rays=True)[0]
+    assert (ref["cls"] == c64["cls"]).mean() >= 0.99
+    assert (ref["cls"] == A.PRR_HORIZON).sum() >= 100 and (ref["cls"] == A.PRR_ESCAPE).sum() >= 100
+    st = ctx.stats()
+    assert 0 < st["attempts_fp64"] < st["attempts"]
+    ctx.set_launch(block=64, blocks_per_cu=1, refill=1)
+    out = ctx.render(f, rays=True)[0]
+    assert np.array_equal(ref["rgbf"], out["rgbf"]) and np.array_equal(ref["steps"], out["steps"])
+    ctx.set_launch()
+    both = ctx.render([odd, f], rays=True)
+    assert np.array_equal(both[1]["rgbf"], ref["rgbf"]) and np.array_equal(both[1]["steps"], ref["steps"])
