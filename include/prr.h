@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define PRR_ABI_VERSION 2
+#define PRR_ABI_VERSION 3
 #define PRR_MAX_SPHERES 8
 #define PRR_MAX_TEXTURES 10 /* slot 0 = celestial sphere, slots 1..8 = sphere textures, 9 = second sky */
 #define PRR_SKY2_SLOT 9     /* celestial sphere seen through a wormhole's throat (bimage2) */
@@ -175,6 +175,8 @@ typedef struct prr_launch {
     int32_t refill;        /* a wave refills once at least this many lanes are idle (1..64);
                               PRR_FP32_BULK always uses 64 */
     int32_t checkpoint;    /* adjoint: accepted steps between per-ray checkpoints */
+    int64_t adjoint_record_bytes; /* adjoint: device memory for one chunk of replayed steps (48 B per
+                                     step; 0: 8 GiB); a smaller budget replays in more chunks */
 } prr_launch;
 
 /* Counters and HIP-event timings of the last prr_render / graph launch. */
@@ -260,7 +262,12 @@ prr_status prr_set_spheres(prr_ctx *ctx, const prr_sphere *spheres, int n);
 
 /* Capture the whole multi-frame render (counter reset, persistent
  * integrator, SSAA, FXAA+RGB8) in a hipGraph; images must be device
- * buffers.  Replay with prr_graph_launch. */
+ * buffers and stay allocated while the graph lives.  Replay with
+ * prr_graph_launch.  The graph owns its frame table and device buffers and
+ * keeps the textures it captured: later calls on the context (renders, traces,
+ * adjoints, prr_set_launch, prr_set_texture) do not change what it replays.
+ * A context destroyed while graphs live is released with its last graph; its
+ * graphs then fail to launch. */
 prr_status prr_graph_create(prr_ctx *ctx, const prr_frame *frames, int n, const prr_image *images, prr_graph **out);
 prr_status prr_graph_launch(prr_graph *g, void *stream);
 void prr_graph_destroy(prr_graph *g);

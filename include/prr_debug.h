@@ -79,11 +79,6 @@ prr_status prr_debug_trace_rays(int32_t metric, const prr_tracer *tracer, const 
 prr_status prr_debug_trace_log(int32_t metric, const prr_tracer *tracer, const prr_frame *f, int64_t id, int max_n,
                                double *log, int *n_out);
 
-/* Device memory for one chunk of the adjoint's replayed steps (48 B per
- * step; 0: the default, 8 GiB).  A smaller budget splits the replay into more
- * chunks of segment levels; process-wide, for tests of the chunked path. */
-prr_status prr_debug_adjoint_record_budget(uint64_t bytes);
-
 #ifdef __cplusplus
 }
 #endif

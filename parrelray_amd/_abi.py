@@ -5,7 +5,7 @@ and offsets against the compiled library.
 """
 import ctypes as C
 
-PRR_ABI_VERSION = 2
+PRR_ABI_VERSION = 3
 PRR_MAX_SPHERES = 8
 PRR_MAX_TEXTURES = 10
 PRR_SKY2_SLOT = 9
@@ -74,7 +74,7 @@ class RayOut(C.Structure):
 
 class Launch(C.Structure):
     _fields_ = [("block", C.c_int32), ("blocks_per_cu", C.c_int32), ("refill", C.c_int32),
-                ("checkpoint", C.c_int32)]
+                ("checkpoint", C.c_int32), ("adjoint_record_bytes", C.c_int64)]
 
 
 class Stats(C.Structure):

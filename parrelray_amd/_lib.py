@@ -102,7 +102,6 @@ def lib():
                                          _vp, _vp, _vp])
     _sig(L, "prr_debug_trace_log", st, [_i32, C.POINTER(A.Tracer), C.POINTER(A.Frame), _i64, C.c_int, _vp,
                                         C.POINTER(C.c_int)])
-    _sig(L, "prr_debug_adjoint_record_budget", st, [C.c_uint64])
     _sig(L, "prr_frame_from_orbit", st, [C.POINTER(A.Frame), C.POINTER(A.Orbit), C.POINTER(A.Frame)])
     _sig(L, "prr_adjoint", st, [_vp, C.POINTER(A.Frame), C.POINTER(A.Orbit), _vp, _i32, _dp, _vp])
     if L.prr_abi_version() != A.PRR_ABI_VERSION:

@@ -148,14 +148,15 @@ class Context:
             arr[i].tex = tex
         self._check(lib().prr_set_spheres(self.h, arr, len(spheres)))
 
-    def set_launch(self, block=256, blocks_per_cu=0, refill=64, checkpoint=64):
-        l = A.Launch(block, blocks_per_cu, refill, checkpoint)
+    def set_launch(self, block=256, blocks_per_cu=0, refill=64, checkpoint=64, adjoint_record_bytes=0):
+        l = A.Launch(block, blocks_per_cu, refill, checkpoint, adjoint_record_bytes)
         self._check(lib().prr_set_launch(self.h, C.byref(l)))
 
     def get_launch(self):
         l = A.Launch()
         self._check(lib().prr_get_launch(self.h, C.byref(l)))
-        return dict(block=l.block, blocks_per_cu=l.blocks_per_cu, refill=l.refill, checkpoint=l.checkpoint)
+        return dict(block=l.block, blocks_per_cu=l.blocks_per_cu, refill=l.refill, checkpoint=l.checkpoint,
+                    adjoint_record_bytes=l.adjoint_record_bytes)
 
     def stats(self):
         s = A.Stats()
@@ -243,11 +244,15 @@ class Context:
 
 
 class Graph:
-    """A captured multi-frame render (prr_graph_create / prr_graph_launch)."""
+    """A captured multi-frame render (prr_graph_create / prr_graph_launch).
+
+    The graph writes into the given device tensors on every replay; it keeps
+    references to them (and to its context) for as long as it lives."""
 
     def __init__(self, ctx, frames, rgb8=None, rgbf=None):
         n = len(frames)
         self.ctx = ctx
+        self._images = (list(rgb8) if rgb8 is not None else [], list(rgbf) if rgbf is not None else [])
         self._frames = (A.Frame * n)(*frames)
         imgs = (A.Image * n)()
         for i in range(n):

@@ -13,6 +13,8 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -20,6 +22,7 @@
 #include "../../include/prr_debug.h"
 #include "frame_host.hpp"
 #include "kernels.hpp"
+#include "context.hpp"
 #include "prr_host.hpp"
 
 using namespace prr;
@@ -28,54 +31,38 @@ namespace {
 
 thread_local std::string g_error;
 
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-};
-
 } // namespace
 
 namespace prr {
 void set_global_error(const std::string &msg) { g_error = msg; }
+
+// Lays the workspace out from `base` (null: offsets only) and returns its size in bytes.
+size_t carve_workspace(void *base, const WorkspaceShape &s, Workspace *ws) {
+    size_t off = 0;
+    auto take = [&](size_t bytes) -> void * {
+        void *p = base ? static_cast<char *>(base) + off : nullptr;
+        off += (bytes + 255) / 256 * 256;
+        return p;
+    };
+    const size_t R = size_t(std::max<int64_t>(s.rays, 1));
+    Workspace w;
+    w.frames = static_cast<FrameDev *>(take(sizeof(FrameDev) * size_t(std::max(s.nframes, 1))));
+    w.post = static_cast<PostFrameDev *>(take(sizeof(PostFrameDev) * size_t(std::max(s.nframes, 1))));
+    w.counters = static_cast<CountersDev *>(take(sizeof(CountersDev)));
+    w.color = static_cast<float *>(take(12 * R));
+    w.img_scratch = s.fxaa_pixels > 0 ? static_cast<float *>(take(12 * size_t(s.fxaa_pixels))) : nullptr;
+    w.rs_y = static_cast<double *>(take(8 * size_t(s.state_width) * R));
+    w.rs_cls = static_cast<int8_t *>(take(R));
+    w.rs_hit = static_cast<int8_t *>(take(R));
+    w.rs_steps = static_cast<int32_t *>(take(4 * R));
+    w.rs_attempts = static_cast<int32_t *>(take(4 * R));
+    w.loc = static_cast<LocateCtx *>(take(sizeof(LocateCtx) * size_t(std::max<int64_t>(s.lanes, 1))));
+    if (ws) *ws = w;
+    return off;
+}
 } // namespace prr
 
-struct prr_ctx {
-    int device = 0;
-    int cus = 0;
-    hipStream_t stream = nullptr;
-    hipStream_t copy_stream = nullptr; // the adjoint's host-to-device copy of dL/dI (created on first use)
-    hipEvent_t copy_done = nullptr;
-    std::string err;
-    int32_t metric = PRR_SCHW_KS_CART;
-    double M = 1.0, a = 0.0, q = 0.0, p[16] = {};
-    prr_tracer tracer{};
-    prr_launch launch{256, 0, 64, 64}; // refill 64: a wave takes 64 consecutive rays once all its lanes are idle
-    TexDev tex[PRR_MAX_TEXTURES] = {};
-    std::vector<prr_sphere> spheres;
-    // device buffers, grown on demand
-    DevBuf frames, counters, color, post, img_scratch, stage_rgbf, stage_rgb8, cls, endpoint, steps, attempts,
-        state8, x0u0;
-    // ray-state arrays and per-lane event location contexts
-    DevBuf rs_y, rs_cls, rs_hit, rs_steps, rs_attempts, loc;
-    // adjoint
-    DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_ckpt2, adj_event, adj_rays, adj_rec, adj_lam, adj_grad_ray,
-        adj_task_ray, adj_task_seg, adj_levelstart, adj_nseg, adj_over, adj_plan,
-        adj_grad;
-    hipEvent_t ev[6] = {};
-    prr_stats stats{};
-};
-
-struct prr_graph {
-    prr_ctx *ctx = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
-    int64_t total_rays = 0, pixels = 0;
-    double ssaa_bytes = 0, fxaa_bytes = 0;
-};
-
-namespace {
-
-prr_status fail(prr_ctx *ctx, prr_status st, const char *fmt, ...) {
+prr_status prr::fail(prr_ctx *ctx, prr_status st, const char *fmt, ...) {
     char buf[512];
     va_list ap;
     va_start(ap, fmt);
@@ -85,8 +72,6 @@ prr_status fail(prr_ctx *ctx, prr_status st, const char *fmt, ...) {
     g_error = buf;
     return st;
 }
-
-} // namespace
 
 // An exception that reached the C ABI (none may cross it): its status and
 // message, on the context when there is one.
@@ -102,15 +87,7 @@ prr_status prr::abi_exception(prr_ctx *ctx) noexcept {
     }
 }
 
-namespace {
-
-#define HIPTRY(ctx, call)                                                                                       \
-    do {                                                                                                        \
-        hipError_t e_ = (call);                                                                                 \
-        if (e_ != hipSuccess) return fail((ctx), PRR_EHIP, "%s failed: %s", #call, hipGetErrorString(e_));     \
-    } while (0)
-
-prr_status ensure(prr_ctx *ctx, DevBuf &b, size_t bytes) {
+prr_status prr::ensure(prr_ctx *ctx, DevBuf &b, size_t bytes) {
     if (b.bytes >= bytes && b.p) return PRR_OK;
     if (b.p) {
         hipError_t e = hipFree(b.p);
@@ -124,6 +101,8 @@ prr_status ensure(prr_ctx *ctx, DevBuf &b, size_t bytes) {
     b.bytes = alloc;
     return PRR_OK;
 }
+
+namespace {
 
 void release(DevBuf &b) {
     if (b.p) (void)hipFree(b.p);
@@ -206,7 +185,6 @@ double metric_param(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx-
 // Device memory for the replayed steps of one chunk of adjoint tasks (48 B
 // per step, K steps per task); larger replays run in several chunks.
 constexpr size_t kRecBudget = size_t(8) << 30;
-std::atomic<uint64_t> g_rec_budget{0}; // prr_debug_adjoint_record_budget (0: kRecBudget)
 
 double far_escape(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx->p[1] : 0.0; }
 
@@ -306,21 +284,16 @@ prr_status prr_create(prr_ctx **out, int hip_device) try {
     return prr::abi_exception(nullptr);
 }
 
-void prr_destroy(prr_ctx *ctx) {
-    if (!ctx) return;
+namespace {
+void free_context(prr_ctx *ctx) {
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-    for (auto &t : ctx->tex)
-        if (t.texels) (void)hipFree(const_cast<float4 *>(t.texels));
-    DevBuf *bufs[] = {&ctx->frames, &ctx->counters, &ctx->color, &ctx->post, &ctx->img_scratch,
-                      &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
+    for (auto &t : ctx->tex) t.reset();
+    DevBuf *bufs[] = {&ctx->ws, &ctx->stage_rgbf, &ctx->stage_rgb8, &ctx->cls, &ctx->endpoint, &ctx->steps,
                       &ctx->attempts, &ctx->state8, &ctx->x0u0, &ctx->adj_img, &ctx->adj_dldi,
                       &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_ckpt2, &ctx->adj_event, &ctx->adj_rays, &ctx->adj_rec,
                       &ctx->adj_task_ray, &ctx->adj_task_seg, &ctx->adj_levelstart, &ctx->adj_nseg,
-                      &ctx->adj_over, &ctx->adj_plan,
-                      &ctx->adj_lam, &ctx->rs_y, &ctx->rs_cls, &ctx->rs_hit, &ctx->rs_steps, &ctx->rs_attempts,
-                      &ctx->loc, &ctx->adj_grad_ray,
-                      &ctx->adj_grad};
+                      &ctx->adj_over, &ctx->adj_plan, &ctx->adj_lam, &ctx->adj_grad_ray, &ctx->adj_grad};
     for (DevBuf *b : bufs) release(*b);
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
@@ -328,6 +301,18 @@ void prr_destroy(prr_ctx *ctx) {
     if (ctx->copy_done) (void)hipEventDestroy(ctx->copy_done);
     if (ctx->copy_stream) (void)hipStreamDestroy(ctx->copy_stream);
     delete ctx;
+}
+} // namespace
+
+// A context with live graphs is released when its last graph is destroyed
+// (the graphs replay on its stream and report into its statistics).
+void prr_destroy(prr_ctx *ctx) {
+    if (!ctx) return;
+    if (ctx->live_graphs > 0) {
+        ctx->destroyed = true;
+        return;
+    }
+    free_context(ctx);
 }
 
 prr_status prr_set_spacetime(prr_ctx *ctx, int32_t metric, double M, double a, double q, const double p[16]) {
@@ -365,6 +350,7 @@ prr_status prr_set_launch(prr_ctx *ctx, const prr_launch *l) {
     if (l->blocks_per_cu < 0 || l->blocks_per_cu > 32) return fail(ctx, PRR_EINVAL, "blocks_per_cu in [0, 32]");
     if (l->refill < 1 || l->refill > 64) return fail(ctx, PRR_EINVAL, "refill in [1, 64]");
     if (l->checkpoint < 1) return fail(ctx, PRR_EINVAL, "checkpoint must be >= 1");
+    if (l->adjoint_record_bytes < 0) return fail(ctx, PRR_EINVAL, "adjoint_record_bytes must be >= 0");
     ctx->launch = *l;
     return PRR_OK;
 }
@@ -380,31 +366,31 @@ prr_status prr_set_texture(prr_ctx *ctx, int slot, const uint8_t *rgb8, int w, i
     if (slot < 0 || slot >= PRR_MAX_TEXTURES) return fail(ctx, PRR_EINVAL, "texture slot %d out of range", slot);
     if (filter != PRR_FILTER_SQAVG && filter != PRR_FILTER_BILINEAR)
         return fail(ctx, PRR_EINVAL, "unknown filter %d", filter);
+    if (rgb8 && w > 0 && h > 0 && (w < 2 || h < 2)) return fail(ctx, PRR_EINVAL, "texture must be at least 2x2");
     HIPTRY(ctx, hipSetDevice(ctx->device));
-    TexDev &t = ctx->tex[slot];
-    if (t.texels) {
+    if (ctx->tex[slot]) {
+        // work queued on the context's stream may still read the texels; a
+        // graph that captured them keeps its own reference
         HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
-        HIPTRY(ctx, hipFree(const_cast<float4 *>(t.texels)));
-        t = TexDev{};
+        ctx->tex[slot].reset();
     }
     if (!rgb8 || w <= 0 || h <= 0) return PRR_OK; // slot cleared
-    if (w < 2 || h < 2) return fail(ctx, PRR_EINVAL, "texture must be at least 2x2");
     std::vector<float4> host(size_t(w) * h);
     for (size_t i = 0; i < host.size(); ++i)
         host[i] = make_float4(float(rgb8[3 * i]) / 255.0f, float(rgb8[3 * i + 1]) / 255.0f,
                               float(rgb8[3 * i + 2]) / 255.0f, 1.0f);
+    auto t = std::make_shared<TexAlloc>();
+    t->device = ctx->device;
     float4 *d = nullptr;
     hipError_t e = hipMalloc(&d, host.size() * sizeof(float4));
     if (e != hipSuccess) return fail(ctx, PRR_ENOMEM, "texture hipMalloc: %s", hipGetErrorString(e));
+    t->t.texels = d;
     e = hipMemcpy(d, host.data(), host.size() * sizeof(float4), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-        (void)hipFree(d);
-        return fail(ctx, PRR_EHIP, "texture upload: %s", hipGetErrorString(e));
-    }
-    t.texels = d;
-    t.w = w;
-    t.h = h;
-    t.filter = filter;
+    if (e != hipSuccess) return fail(ctx, PRR_EHIP, "texture upload: %s", hipGetErrorString(e));
+    t->t.w = w;
+    t->t.h = h;
+    t->t.filter = filter;
+    ctx->tex[slot] = std::move(t);
     return PRR_OK;
 } catch (...) {
     return prr::abi_exception(ctx);
@@ -431,33 +417,26 @@ prr_status prr_get_stats(const prr_ctx *ctx, prr_stats *out) {
 
 namespace {
 
-struct RenderPlan {
-    std::vector<FrameDev> fd;
-    std::vector<PostFrameDev> pf;
-    int64_t total = 0, qtotal = 0, pixels = 0, max_pix = 0;
-    int max_w = 0, max_h = 0;
-    bool any_fxaa = false;
-    double ssaa_bytes = 0, fxaa_bytes = 0;
-};
+int rs_width(int metric) { return metric == PRR_KERR_BL ? 9 : 8; }
 
-prr_status ensure_pipeline(prr_ctx *ctx, int64_t total);
+int integrate_grid(const prr_ctx *ctx) { return ctx->cus * blocks_per_cu(ctx); }
 
-// Builds the device frame table and post-processing descriptors and makes
-// sure every buffer exists.  Images given on the device are written in
-// place; host images go through staging buffers.
-prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_image *images, RenderPlan &P,
-                       bool need_host_staging) {
-    P.fd.resize(n);
+} // namespace
+
+// Validates the frames, fills the frame table fd[0..n) and the totals.
+prr_status prr::plan_frames(prr_ctx *ctx, const prr_frame *frames, int n, FrameDev *fd, RenderPlan &P) {
+    P.n = n;
+    P.fd = fd;
     for (int i = 0; i < n; ++i) {
         prr_status st = validate_frame(ctx, frames[i], i);
         if (st != PRR_OK) return st;
-        P.fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total, metric_param(ctx), far_escape(ctx));
+        fd[i] = frame_dev(ctx->metric, ctx->tracer, frames[i], P.total, metric_param(ctx), far_escape(ctx));
         const int64_t np = int64_t(frames[i].width) * frames[i].height;
         // each frame's queue starts at a multiple of 64, so that a whole-wave
         // refill never mixes two frames and a frame's waves are the same
         // whether it is rendered alone or in a batch
-        P.fd[i].queue_begin = (P.qtotal + 63) / 64 * 64;
-        P.qtotal = P.fd[i].queue_begin + np * frames[i].samples;
+        fd[i].queue_begin = (P.qtotal + 63) / 64 * 64;
+        P.qtotal = fd[i].queue_begin + np * frames[i].samples;
         P.total += np * frames[i].samples;
         P.pixels += np;
         P.max_pix = std::max(P.max_pix, np);
@@ -467,25 +446,42 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
         P.ssaa_bytes += double(np) * frames[i].samples * 12.0 + double(np) * (frames[i].fxaa ? 12.0 : 15.0);
         if (frames[i].fxaa) P.fxaa_bytes += double(np) * (12.0 + 15.0);
     }
-    prr_status st;
-    if ((st = ensure(ctx, ctx->frames, sizeof(FrameDev) * n)) != PRR_OK) return st;
-    if ((st = ensure(ctx, ctx->counters, sizeof(CountersDev))) != PRR_OK) return st;
-    if ((st = ensure(ctx, ctx->color, sizeof(float) * 3 * size_t(P.total))) != PRR_OK) return st;
-    if ((st = ensure(ctx, ctx->post, sizeof(PostFrameDev) * n)) != PRR_OK) return st;
-    if (P.any_fxaa && (st = ensure(ctx, ctx->img_scratch, sizeof(float) * 3 * size_t(P.pixels))) != PRR_OK)
-        return st;
-    if ((st = ensure_pipeline(ctx, P.total)) != PRR_OK) return st;
-    if (need_host_staging) {
-        if ((st = ensure(ctx, ctx->stage_rgbf, sizeof(float) * 3 * size_t(P.pixels))) != PRR_OK) return st;
-        if ((st = ensure(ctx, ctx->stage_rgb8, 3 * size_t(P.pixels))) != PRR_OK) return st;
-    }
-    P.pf.resize(n);
+    P.grid = integrate_grid(ctx);
+    P.block = ctx->launch.block;
+    return PRR_OK;
+}
+
+WorkspaceShape prr::workspace_shape(const prr_ctx *ctx, const RenderPlan &P) {
+    WorkspaceShape sh;
+    sh.nframes = P.n;
+    sh.rays = P.total;
+    sh.fxaa_pixels = P.any_fxaa ? P.pixels : 0;
+    sh.state_width = rs_width(ctx->metric);
+    sh.lanes = int64_t(P.grid) * P.block;
+    return sh;
+}
+
+// The context's own workspace (eager calls), grown to the plan's shape.
+prr_status prr::context_workspace(prr_ctx *ctx, const RenderPlan &P, Workspace &ws) {
+    const WorkspaceShape sh = workspace_shape(ctx, P);
+    prr_status st = ensure(ctx, ctx->ws, carve_workspace(nullptr, sh, nullptr));
+    if (st != PRR_OK) return st;
+    carve_workspace(ctx->ws.p, sh, &ws);
+    return PRR_OK;
+}
+
+// Post-processing descriptors pf[0..n): the frame's ray colours and FXAA
+// input in the workspace; outputs to device images in place, to host images
+// through the staging buffers (stage_rgbf / stage_rgb8, W*H of every frame).
+void prr::plan_post(const prr_frame *frames, const prr_image *images, const Workspace &ws, float *stage_rgbf,
+               uint8_t *stage_rgb8, RenderPlan &P, PostFrameDev *pf) {
+    P.pf = pf;
     int64_t pix_off = 0;
-    for (int i = 0; i < n; ++i) {
-        PostFrameDev &q = P.pf[i];
+    for (int i = 0; i < P.n; ++i) {
+        PostFrameDev &q = pf[i];
         const int64_t np = int64_t(frames[i].width) * frames[i].height;
-        q.ray_color = static_cast<float *>(ctx->color.p) + 3 * P.fd[i].ray_begin;
-        q.img = P.any_fxaa ? static_cast<float *>(ctx->img_scratch.p) + 3 * pix_off : nullptr;
+        q.ray_color = ws.color + 3 * P.fd[i].ray_begin;
+        q.img = P.any_fxaa ? ws.img_scratch + 3 * pix_off : nullptr;
         q.W = frames[i].width;
         q.H = frames[i].height;
         q.s = frames[i].samples;
@@ -495,38 +491,20 @@ prr_status plan_render(prr_ctx *ctx, const prr_frame *frames, int n, const prr_i
             q.final_rgbf = im->rgbf;
             q.rgb8 = im->rgb8;
         } else if (im) {
-            q.final_rgbf = im->rgbf ? static_cast<float *>(ctx->stage_rgbf.p) + 3 * pix_off : nullptr;
-            q.rgb8 = im->rgb8 ? static_cast<uint8_t *>(ctx->stage_rgb8.p) + 3 * pix_off : nullptr;
+            q.final_rgbf = im->rgbf ? stage_rgbf + 3 * pix_off : nullptr;
+            q.rgb8 = im->rgb8 ? stage_rgb8 + 3 * pix_off : nullptr;
         } else {
             q.final_rgbf = nullptr;
             q.rgb8 = nullptr;
         }
         pix_off += np;
     }
-    return PRR_OK;
 }
 
-int rs_width(int metric) { return metric == PRR_KERR_BL ? 9 : 8; }
-
-int integrate_grid(const prr_ctx *ctx) { return ctx->cus * blocks_per_cu(ctx); }
-
-// Ray-state arrays for `total` rays and one event-location context per lane.
-prr_status ensure_pipeline(prr_ctx *ctx, int64_t total) {
-    prr_status s;
-    const size_t T = size_t(total);
-    if ((s = ensure(ctx, ctx->rs_y, 8 * size_t(rs_width(ctx->metric)) * T)) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->rs_cls, T)) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->rs_hit, T)) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->rs_steps, 4 * T)) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->rs_attempts, 4 * T)) != PRR_OK) return s;
-    const size_t lanes = size_t(integrate_grid(ctx)) * size_t(ctx->launch.block);
-    return ensure(ctx, ctx->loc, sizeof(LocateCtx) * lanes);
-}
-
-JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
+JobDev prr::make_job(const prr_ctx *ctx, const RenderPlan &P, const Workspace &ws) {
     JobDev job{};
-    job.frames = static_cast<const FrameDev *>(ctx->frames.p);
-    job.nframes = int(P.fd.size());
+    job.frames = ws.frames;
+    job.nframes = P.n;
     // FP32-bulk always refills whole waves: its choice of stage precision is
     // made per wave (k_integrate), and whole-wave refills keep it independent
     // of the launch shape
@@ -535,44 +513,42 @@ JobDev make_job(prr_ctx *ctx, const RenderPlan &P) {
     job.qtotal = P.qtotal > 0 ? P.qtotal : P.total;
     job.explicit_x0u0 = nullptr;
     job.tr = tracer_dev(ctx->tracer);
-    for (int i = 0; i < PRR_MAX_TEXTURES; ++i) job.tex[i] = ctx->tex[i];
-    job.out.color = static_cast<float *>(ctx->color.p);
-    job.rs.y = static_cast<double *>(ctx->rs_y.p);
-    job.rs.cls = static_cast<int8_t *>(ctx->rs_cls.p);
-    job.rs.hit = static_cast<int8_t *>(ctx->rs_hit.p);
-    job.rs.steps = static_cast<int32_t *>(ctx->rs_steps.p);
-    job.rs.attempts = static_cast<int32_t *>(ctx->rs_attempts.p);
+    for (int i = 0; i < PRR_MAX_TEXTURES; ++i) job.tex[i] = ctx->tex[i] ? ctx->tex[i]->t : TexDev{};
+    job.out.color = ws.color;
+    job.rs.y = ws.rs_y;
+    job.rs.cls = ws.rs_cls;
+    job.rs.hit = ws.rs_hit;
+    job.rs.steps = ws.rs_steps;
+    job.rs.attempts = ws.rs_attempts;
     job.rs_total = P.total;
-    job.loc = static_cast<LocateCtx *>(ctx->loc.p);
-    job.ctr = static_cast<CountersDev *>(ctx->counters.p);
+    job.loc = ws.loc;
+    job.ctr = ws.counters;
     return job;
 }
 
 // Enqueue reset -> init -> integrate -> shade -> SSAA -> FXAA, with timing events.
 // job.adj.mode picks the integrator binary: the render one, or (the adjoint's
 // first pass) the checkpointing one, whose steps the replay must reproduce.
-prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, hipStream_t st, bool events) {
+prr_status prr::enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, const Workspace &ws,
+                          hipStream_t st, bool events) {
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     HIPTRY(ctx, launch_reset(job.ctr, st));
     HIPTRY(ctx, launch_init(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
-    HIPTRY(ctx, launch_integrate(ctx->metric, job, integrate_grid(ctx), ctx->launch.block, st));
+    HIPTRY(ctx, launch_integrate(ctx->metric, job, P.grid, P.block, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     HIPTRY(ctx, launch_shade(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[5], st));
-    if (!P.pf.empty())
-        HIPTRY(ctx, launch_ssaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_pix, st));
+    if (P.post) HIPTRY(ctx, launch_ssaa(ws.post, P.n, P.max_pix, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[3], st));
-    if (P.any_fxaa)
-        HIPTRY(ctx, launch_fxaa(static_cast<const PostFrameDev *>(ctx->post.p), int(P.pf.size()), P.max_w, P.max_h,
-                                st));
+    if (P.post && P.any_fxaa) HIPTRY(ctx, launch_fxaa(ws.post, P.n, P.max_w, P.max_h, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[4], st));
     return PRR_OK;
 }
 
-prr_status read_stats(prr_ctx *ctx, const RenderPlan &P, hipStream_t st, bool events) {
+prr_status prr::read_stats(prr_ctx *ctx, const RenderPlan &P, const CountersDev *counters, hipStream_t st, bool events) {
     CountersDev c{};
-    HIPTRY(ctx, hipMemcpyAsync(&c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipMemcpyAsync(&c, counters, sizeof c, hipMemcpyDeviceToHost, st));
     HIPTRY(ctx, hipStreamSynchronize(st));
     prr_stats &s = ctx->stats;
     s = prr_stats{};
@@ -604,8 +580,6 @@ prr_status read_stats(prr_ctx *ctx, const RenderPlan &P, hipStream_t st, bool ev
     return PRR_OK;
 }
 
-} // namespace
-
 extern "C" {
 
 prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *images, prr_ray_out *rays,
@@ -617,10 +591,20 @@ prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *i
     bool host_images = false;
     if (images)
         for (int i = 0; i < n; ++i) host_images = host_images || !images[i].on_device;
+    std::vector<FrameDev> fd(static_cast<size_t>(n));
+    std::vector<PostFrameDev> pf(static_cast<size_t>(n));
     RenderPlan P;
-    prr_status s = plan_render(ctx, frames, n, images, P, host_images);
+    prr_status s = plan_frames(ctx, frames, n, fd.data(), P);
     if (s != PRR_OK) return s;
-    JobDev job = make_job(ctx, P);
+    Workspace ws;
+    if ((s = context_workspace(ctx, P, ws)) != PRR_OK) return s;
+    if (host_images) {
+        if ((s = ensure(ctx, ctx->stage_rgbf, sizeof(float) * 3 * size_t(P.pixels))) != PRR_OK) return s;
+        if ((s = ensure(ctx, ctx->stage_rgb8, 3 * size_t(P.pixels))) != PRR_OK) return s;
+    }
+    plan_post(frames, images, ws, static_cast<float *>(ctx->stage_rgbf.p), static_cast<uint8_t *>(ctx->stage_rgb8.p),
+              P, pf.data());
+    JobDev job = make_job(ctx, P, ws);
     // optional per-ray diagnostics
     if (rays) {
         const size_t T = size_t(P.total);
@@ -643,10 +627,10 @@ prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *i
         if ((s = bind(rays->steps, ctx->steps, 4 * T, job.out.steps)) != PRR_OK) return s;
         if ((s = bind(rays->attempts, ctx->attempts, 4 * T, job.out.attempts)) != PRR_OK) return s;
     }
-    HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, st));
-    HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev) * n, hipMemcpyHostToDevice, st));
-    if ((s = enqueue_render(ctx, P, job, st, true)) != PRR_OK) return s;
-    if ((s = read_stats(ctx, P, st, true)) != PRR_OK) return s;
+    HIPTRY(ctx, hipMemcpyAsync(ws.frames, fd.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipMemcpyAsync(ws.post, pf.data(), sizeof(PostFrameDev) * n, hipMemcpyHostToDevice, st));
+    if ((s = enqueue_render(ctx, P, job, ws, st, true)) != PRR_OK) return s;
+    if ((s = read_stats(ctx, P, ws.counters, st, true)) != PRR_OK) return s;
     // copy results to host buffers
     if (images && host_images) {
         int64_t off = 0;
@@ -666,7 +650,7 @@ prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *i
     if (rays) {
         const size_t T = size_t(P.total);
         if (rays->color) {
-            HIPTRY(ctx, hipMemcpyAsync(rays->color, ctx->color.p, 12 * T,
+            HIPTRY(ctx, hipMemcpyAsync(rays->color, ws.color, 12 * T,
                                        rays->on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
         }
         if (!rays->on_device) {
@@ -697,34 +681,38 @@ prr_status prr_trace(prr_ctx *ctx, const double *x0u0, int n, double *xend, int3
     f.spin = ctx->a;
     f.n_spheres = int(ctx->spheres.size());
     for (int k = 0; k < f.n_spheres; ++k) f.spheres[k] = ctx->spheres[k];
+    FrameDev fd = frame_dev(ctx->metric, ctx->tracer, f, 0, metric_param(ctx), far_escape(ctx));
     RenderPlan P;
-    P.fd.push_back(frame_dev(ctx->metric, ctx->tracer, f, 0, metric_param(ctx), far_escape(ctx)));
-    P.total = n;
+    P.n = 1;
+    P.fd = &fd;
+    P.post = false;
+    P.total = P.qtotal = n;
+    P.grid = integrate_grid(ctx);
+    P.block = ctx->launch.block;
     prr_status s;
-    if ((s = ensure(ctx, ctx->frames, sizeof(FrameDev))) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->counters, sizeof(CountersDev))) != PRR_OK) return s;
+    Workspace ws;
+    if ((s = context_workspace(ctx, P, ws)) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->x0u0, 64 * size_t(n))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->state8, 64 * size_t(n))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->steps, 4 * size_t(n))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->attempts, 4 * size_t(n))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->cls, size_t(n))) != PRR_OK) return s;
-    if ((s = ensure_pipeline(ctx, n)) != PRR_OK) return s;
-    JobDev job = make_job(ctx, P);
+    JobDev job = make_job(ctx, P, ws);
     job.explicit_x0u0 = static_cast<const double *>(ctx->x0u0.p);
     job.out = OutDev{};
     job.out.state8 = static_cast<double *>(ctx->state8.p);
     job.out.steps = static_cast<int32_t *>(ctx->steps.p);
     job.out.attempts = static_cast<int32_t *>(ctx->attempts.p);
     job.out.cls = static_cast<int8_t *>(ctx->cls.p);
-    HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipMemcpyAsync(ws.frames, &fd, sizeof(FrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipMemcpyAsync(ctx->x0u0.p, x0u0, 64 * size_t(n), hipMemcpyHostToDevice, st));
-    if ((s = enqueue_render(ctx, P, job, st, true)) != PRR_OK) return s;
+    if ((s = enqueue_render(ctx, P, job, ws, st, true)) != PRR_OK) return s;
     std::vector<int8_t> c8(n);
     if (xend) HIPTRY(ctx, hipMemcpyAsync(xend, ctx->state8.p, 64 * size_t(n), hipMemcpyDeviceToHost, st));
     if (steps) HIPTRY(ctx, hipMemcpyAsync(steps, ctx->steps.p, 4 * size_t(n), hipMemcpyDeviceToHost, st));
     if (attempts) HIPTRY(ctx, hipMemcpyAsync(attempts, ctx->attempts.p, 4 * size_t(n), hipMemcpyDeviceToHost, st));
     HIPTRY(ctx, hipMemcpyAsync(c8.data(), ctx->cls.p, size_t(n), hipMemcpyDeviceToHost, st));
-    if ((s = read_stats(ctx, P, st, true)) != PRR_OK) return s;
+    if ((s = read_stats(ctx, P, ws.counters, st, true)) != PRR_OK) return s;
     if (cls)
         for (int i = 0; i < n; ++i) cls[i] = c8[i];
     return PRR_OK;
@@ -741,53 +729,60 @@ prr_status prr_graph_create(prr_ctx *ctx, const prr_frame *frames, int n, const 
         for (int i = 0; i < n; ++i)
             if (!images[i].on_device) return fail(ctx, PRR_EINVAL, "graph images must be device buffers");
     HIPTRY(ctx, hipSetDevice(ctx->device));
+    std::vector<FrameDev> fd(static_cast<size_t>(n));
+    std::vector<PostFrameDev> pf(static_cast<size_t>(n));
     RenderPlan P;
-    prr_status s = plan_render(ctx, frames, n, images, P, false);
+    prr_status s = plan_frames(ctx, frames, n, fd.data(), P);
     if (s != PRR_OK) return s;
-    JobDev job = make_job(ctx, P);
-    HIPTRY(ctx, hipMemcpy(ctx->frames.p, P.fd.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice));
-    HIPTRY(ctx, hipMemcpy(ctx->post.p, P.pf.data(), sizeof(PostFrameDev) * n, hipMemcpyHostToDevice));
-    prr_graph *g = new prr_graph();
-    g->ctx = ctx;
+    // the graph's own workspace: later eager calls on the context cannot
+    // overwrite its tables or move its buffers
+    const WorkspaceShape sh = workspace_shape(ctx, P);
+    const size_t bytes = carve_workspace(nullptr, sh, nullptr);
+    std::unique_ptr<prr_graph, void (*)(prr_graph *)> g(new prr_graph(), prr_graph_destroy);
+    hipError_t e = hipMalloc(&g->block, bytes);
+    if (e != hipSuccess) return fail(ctx, PRR_ENOMEM, "graph workspace hipMalloc(%zu): %s", bytes, hipGetErrorString(e));
+    Workspace ws;
+    carve_workspace(g->block, sh, &ws);
+    plan_post(frames, images, ws, nullptr, nullptr, P, pf.data());
+    JobDev job = make_job(ctx, P, ws);
+    HIPTRY(ctx, hipMemcpy(ws.frames, fd.data(), sizeof(FrameDev) * n, hipMemcpyHostToDevice));
+    HIPTRY(ctx, hipMemcpy(ws.post, pf.data(), sizeof(PostFrameDev) * n, hipMemcpyHostToDevice));
+    g->counters = ws.counters;
     g->total_rays = P.total;
     g->pixels = P.pixels;
     g->ssaa_bytes = P.ssaa_bytes;
     g->fxaa_bytes = P.fxaa_bytes;
-    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
-    if (e != hipSuccess) {
-        delete g;
-        return fail(ctx, PRR_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
-    }
-    s = enqueue_render(ctx, P, job, ctx->stream, false);
+    for (int i = 0; i < PRR_MAX_TEXTURES; ++i) g->tex[i] = ctx->tex[i]; // the captured texels stay alive
+    HIPTRY(ctx, hipStreamSynchronize(ctx->stream));
+    e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return fail(ctx, PRR_EHIP, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    s = enqueue_render(ctx, P, job, ws, ctx->stream, false);
     hipGraph_t graph = nullptr;
     e = hipStreamEndCapture(ctx->stream, &graph);
-    if (s != PRR_OK || e != hipSuccess) {
-        if (graph) (void)hipGraphDestroy(graph);
-        delete g;
-        return s != PRR_OK ? s : fail(ctx, PRR_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
-    }
     g->graph = graph;
+    if (s != PRR_OK) return s;
+    if (e != hipSuccess) return fail(ctx, PRR_EHIP, "hipStreamEndCapture: %s", hipGetErrorString(e));
     e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-    if (e != hipSuccess) {
-        prr_graph_destroy(g);
-        return fail(ctx, PRR_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
-    }
-    *out = g;
+    if (e != hipSuccess) return fail(ctx, PRR_EHIP, "hipGraphInstantiate: %s", hipGetErrorString(e));
+    g->ctx = ctx;
+    ctx->live_graphs++;
+    *out = g.release();
     return PRR_OK;
 } catch (...) {
     return prr::abi_exception(ctx);
 }
 
 prr_status prr_graph_launch(prr_graph *g, void *stream) {
-    if (!g) return fail(nullptr, PRR_EINVAL, "NULL graph");
+    if (!g || !g->ctx) return fail(nullptr, PRR_EINVAL, "NULL graph");
     prr_ctx *ctx = g->ctx;
+    if (ctx->destroyed) return fail(nullptr, PRR_EINVAL, "the graph's context was destroyed");
     HIPTRY(ctx, hipSetDevice(ctx->device));
     hipStream_t st = stream ? static_cast<hipStream_t>(stream) : ctx->stream;
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
     HIPTRY(ctx, hipGraphLaunch(g->exec, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[4], st));
     CountersDev c{};
-    HIPTRY(ctx, hipMemcpyAsync(&c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipMemcpyAsync(&c, g->counters, sizeof c, hipMemcpyDeviceToHost, st));
     HIPTRY(ctx, hipStreamSynchronize(st));
     prr_stats &s = ctx->stats;
     s = prr_stats{};
@@ -809,9 +804,17 @@ prr_status prr_graph_launch(prr_graph *g, void *stream) {
 
 void prr_graph_destroy(prr_graph *g) {
     if (!g) return;
+    prr_ctx *ctx = g->ctx;
+    if (ctx) {
+        (void)hipSetDevice(ctx->device);
+        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    }
     if (g->exec) (void)hipGraphExecDestroy(g->exec);
     if (g->graph) (void)hipGraphDestroy(g->graph);
+    if (g->block) (void)hipFree(g->block);
+    for (auto &t : g->tex) t.reset();
     delete g;
+    if (ctx && --ctx->live_graphs == 0 && ctx->destroyed) free_context(ctx);
 }
 
 // ---------------------------------------------------------------- adjoint
@@ -856,8 +859,13 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     const int64_t npix = int64_t(f.width) * f.height;
     if ((s = ensure(ctx, ctx->adj_img, 12 * size_t(npix))) != PRR_OK) return s;
     prr_image im{nullptr, static_cast<float *>(ctx->adj_img.p), 1, 0};
+    FrameDev fd;
+    PostFrameDev pf;
     RenderPlan P;
-    if ((s = plan_render(ctx, &f, 1, &im, P, false)) != PRR_OK) return s;
+    if ((s = plan_frames(ctx, &f, 1, &fd, P)) != PRR_OK) return s;
+    Workspace ws;
+    if ((s = context_workspace(ctx, P, ws)) != PRR_OK) return s;
+    plan_post(&f, &im, ws, nullptr, nullptr, P, &pf);
     const int64_t T = P.total;
     if ((s = ensure(ctx, ctx->steps, 4 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->cls, size_t(T))) != PRR_OK) return s;
@@ -872,18 +880,18 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     // segments are checkpointed again by a pass of their own
     const int cap = std::max(8, (512 + K - 1) / K);
     if ((s = ensure(ctx, ctx->adj_ckpt, 64 * size_t(cap) * size_t(T))) != PRR_OK) return s;
-    JobDev job = make_job(ctx, P);
+    JobDev job = make_job(ctx, P, ws);
     job.out.steps = static_cast<int32_t *>(ctx->steps.p);
     job.out.cls = static_cast<int8_t *>(ctx->cls.p);
     job.adj.mode = ADJ_CHECKPOINT;
     job.adj.K = K;
     job.adj.cap = cap;
     job.adj.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
-    CountersDev *ctr = static_cast<CountersDev *>(ctx->counters.p);
-    HIPTRY(ctx, hipMemcpyAsync(ctx->frames.p, P.fd.data(), sizeof(FrameDev), hipMemcpyHostToDevice, st));
-    HIPTRY(ctx, hipMemcpyAsync(ctx->post.p, P.pf.data(), sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
+    CountersDev *ctr = ws.counters;
+    HIPTRY(ctx, hipMemcpyAsync(ws.frames, &fd, sizeof(FrameDev), hipMemcpyHostToDevice, st));
+    HIPTRY(ctx, hipMemcpyAsync(ws.post, &pf, sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[0], st));
-    if ((s = enqueue_render(ctx, P, job, st, false)) != PRR_OK) return s;
+    if ((s = enqueue_render(ctx, P, job, ws, st, false)) != PRR_OK) return s;
     // dL/dI from the host: copied on a second stream while the first pass
     // runs; the reverse sweep waits for it
     const float *dldi = dL_dimg;
@@ -941,7 +949,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         steps_h.resize(size_t(T));
         HIPTRY(ctx, hipMemcpyAsync(steps_h.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
     }
-    if ((s = read_stats(ctx, P, st, false)) != PRR_OK) return s; // synchronises
+    if ((s = read_stats(ctx, P, ws.counters, st, false)) != PRR_OK) return s; // synchronises
     prr_stats fwd = ctx->stats;
     if (plan_dev) {
         nck2 = int64_t(pctr[0]);
@@ -1012,8 +1020,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         JobDev ji = j2;
         ji.total = T;
         HIPTRY(ctx, launch_init(ctx->metric, ji, st));
-        const int grid = int(std::min<int64_t>(integrate_grid(ctx), (j2.total + 63) / 64 + 1));
-        HIPTRY(ctx, launch_integrate(ctx->metric, j2, grid, ctx->launch.block, st));
+        const int grid = int(std::min<int64_t>(P.grid, (j2.total + 63) / 64 + 1));
+        HIPTRY(ctx, launch_integrate(ctx->metric, j2, grid, P.block, st));
         j2.total = T;
     }
     HIPTRY(ctx, launch_reset(ctr, st, true));
@@ -1033,7 +1041,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     plan.task_seg = static_cast<int32_t *>(ctx->adj_task_seg.p);
     HIPTRY(ctx, launch_adj_plan_tasks(plan, st));
     const size_t rec_task = 48 * size_t(K); // bytes of records per task
-    const uint64_t budget = g_rec_budget.load() ? g_rec_budget.load() : uint64_t(kRecBudget);
+    const uint64_t budget = ctx->launch.adjoint_record_bytes > 0 ? uint64_t(ctx->launch.adjoint_record_bytes)
+                                                                 : uint64_t(kRecBudget);
     const int64_t chunk_cap = std::max<int64_t>(1, int64_t(budget / rec_task));
     std::vector<std::pair<int32_t, int32_t>> chunks; // (jhi, jlo)
     for (int32_t j = nseg_max - 1; j >= 0;) {
@@ -1061,7 +1070,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     ad.lam = static_cast<double *>(ctx->adj_lam.p);
     ad.dLdI = dldi;
     ad.img = static_cast<const float *>(ctx->adj_img.p);
-    ad.ray_color = static_cast<const float *>(ctx->color.p);
+    ad.ray_color = ws.color;
     ad.grad_ray = static_cast<double *>(ctx->adj_grad_ray.p);
     ad.cam_phi = orbit->cam_phi;
     ad.sphere_r = orbit->sphere_r;
@@ -1074,8 +1083,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         j3.adj.rec_stride = t1 - t0;
         j3.total = t1 - t0;
         HIPTRY(ctx, launch_reset(ctr, st, false));
-        const int grid = int(std::min<int64_t>(integrate_grid(ctx), (j3.total + 63) / 64 + 1));
-        HIPTRY(ctx, launch_integrate(ctx->metric, j3, grid, ctx->launch.block, st));
+        const int grid = int(std::min<int64_t>(P.grid, (j3.total + 63) / 64 + 1));
+        HIPTRY(ctx, launch_integrate(ctx->metric, j3, grid, P.block, st));
         ad.jhi = c.first;
         ad.jlo = c.second;
         ad.t0 = t0;
@@ -1092,7 +1101,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
                                    static_cast<double *>(ctx->adj_grad.p), st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     CountersDev c{};
-    HIPTRY(ctx, hipMemcpyAsync(&c, ctx->counters.p, sizeof c, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipMemcpyAsync(&c, ws.counters, sizeof c, hipMemcpyDeviceToHost, st));
     HIPTRY(ctx, hipMemcpyAsync(grad, ctx->adj_grad.p, 32, hipMemcpyDeviceToHost, st));
     if (img_out)
         HIPTRY(ctx, hipMemcpyAsync(img_out, ctx->adj_img.p, 12 * size_t(npix),
@@ -1113,9 +1122,5 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     return prr::abi_exception(ctx);
 }
 
-prr_status prr_debug_adjoint_record_budget(uint64_t bytes) {
-    g_rec_budget.store(bytes);
-    return PRR_OK;
-}
 
 } // extern "C"

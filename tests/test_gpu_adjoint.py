@@ -209,7 +209,6 @@ def test_gradient_does_not_depend_on_replay_chunks_or_where_rays_are_binned(setu
     record budget (lambda passes between chunks through memory), and rays are
     binned by segment count on the host when the tracer's step cap allows
     more than 4096 segments per ray: neither changes the gradient."""
-    from parrelray_amd._lib import check, lib
     tmpl, o0, T, ref = setup
     I, _ = render(ref, tmpl, o0)
     dLdI = (2.0 * (I - T) / I.size).astype(np.float32)
@@ -217,11 +216,9 @@ def test_gradient_does_not_depend_on_replay_chunks_or_where_rays_are_binned(setu
     ctx = context(A.PRR_FP64, 1e-10)
     ctx.set_launch(checkpoint=K)
     g0 = ctx.adjoint(tmpl, o0, dLdI)
-    try:
-        check(lib().prr_debug_adjoint_record_budget(48 * K * 300))  # about 300 tasks per chunk
-        g1 = ctx.adjoint(tmpl, o0, dLdI)
-    finally:
-        check(lib().prr_debug_adjoint_record_budget(0))
+    ctx.set_launch(checkpoint=K, adjoint_record_bytes=48 * K * 300)  # about 300 tasks per chunk
+    g1 = ctx.adjoint(tmpl, o0, dLdI)
+    ctx.set_launch(checkpoint=K)
     tr = CF.tracer_kerr(precision=A.PRR_FP64, tol=1e-10)
     tr.max_steps = 4096 * K + 64  # no ray comes near it: same steps, host-side binning
     tr.max_attempts = 10 * tr.max_steps

@@ -3,6 +3,7 @@ hipGraph, and the launch-shape autotuner."""
 import numpy as np
 import pytest
 
+import oracle as O
 from parrelray_amd import _abi as A
 from parrelray_amd import api
 from parrelray_amd import configs as CF
@@ -37,6 +38,57 @@ def test_graph_replays_match_eager_render():
         for e, b8, bf in zip(eager, rgb8, rgbf):
             assert np.array_equal(b8.cpu().numpy(), e["rgb8"]), rep
             assert np.array_equal(bf.cpu().numpy(), e["rgbf"]), rep
+    g.close()
+
+
+def test_graph_replays_survive_later_calls_on_its_context():
+    """A graph owns its workspace and keeps the textures it captured: between
+    replays, a larger eager batch (which grows the context's buffers), a
+    trace, a larger launch shape and a new texture in a captured slot leave
+    its replays equal to the render eagerly made before the capture."""
+    torch = pytest.importorskip("torch")
+    frames = CF.frames_c2(nframes=6, scale=0.05)
+    ctx = context()
+    eager = ctx.render(frames)
+    rgb8 = [torch.zeros((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
+    g = api.Graph(ctx, frames, rgb8=rgb8)
+    del rgb8  # the graph keeps its output tensors alive
+    bigger = CF.frames_c2(nframes=8, scale=0.12)
+    x, u = O.init_ray(A.PRR_KERR_BL, 1.0, 0.0, [20.0, 3.0, 1.0], [-0.9, 0.1, 0.0])
+    x0 = np.tile(np.concatenate([x, u]), (100, 1))
+    other_sky = api.checker_texture(512, 256, 8, 4, 99)
+    steps = [lambda: ctx.render(bigger),
+             lambda: ctx.trace(x0),
+             lambda: ctx.set_launch(block=256, blocks_per_cu=4, refill=64),
+             lambda: ctx.set_texture(0, other_sky)]
+    for rep, step in enumerate([None] + steps):
+        if step is not None:
+            step()
+        g.launch()
+        st = ctx.stats()
+        assert st["rays"] == sum(f.nrays for f in frames), rep
+        for e, b8 in zip(eager, g._images[0]):
+            assert np.array_equal(b8.cpu().numpy(), e["rgb8"]), rep
+    # the context itself now renders with the new sky
+    now = ctx.render(frames[:1])[0]["rgb8"]
+    assert not np.array_equal(now, eager[0]["rgb8"])
+    g.close()
+
+
+def test_graph_outlives_a_destroyed_context():
+    """prr_destroy with a live graph defers the release to the graph's
+    destruction; the graph then refuses to launch."""
+    torch = pytest.importorskip("torch")
+    from parrelray_amd._lib import PrrError, lib
+    frames = CF.frames_c2(nframes=2, scale=0.04)
+    ctx = context()
+    rgb8 = [torch.zeros((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
+    g = api.Graph(ctx, frames, rgb8=rgb8)
+    g.launch()
+    lib().prr_destroy(ctx.h)
+    ctx.h = None
+    with pytest.raises(PrrError, match="destroyed"):
+        g.launch()
     g.close()
 
 

@@ -500,6 +500,25 @@ def test_launch_shape_does_not_change_results(refill, block):
     assert np.array_equal(ref["steps"], out["steps"])
 
 
+@pytest.mark.parametrize("refill", [1, 32])
+def test_fp64_partial_refills_batch_equals_single(refill):
+    """Plain FP64 with partial refills (a wave takes new rays once `refill`
+    lanes are idle: ballot + mbcnt ranks, padding positions between frames):
+    a batch holding a frame whose ray count is not a multiple of 64 equals
+    the frames rendered alone, bit for bit, and every ray is integrated once."""
+    odd = CF.frame_c1(37, 23)  # 851 pixels x 4 samples: 3404 rays
+    frames = [odd, CF.frame_c1(64, 36), CF.frame_c1(29, 17, ssaa=1, fxaa=0)]
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP64))
+    ctx.set_launch(block=128, blocks_per_cu=1, refill=refill)
+    batch = ctx.render(frames, rays=True)
+    assert ctx.stats()["rays"] == sum(f.nrays for f in frames)
+    ctx.set_launch()
+    for f, b in zip(frames, batch):
+        single = ctx.render(f, rays=True)[0]
+        for key in ("cls", "steps", "attempts", "rgb8", "rgbf"):
+            assert np.array_equal(single[key], b[key]), (refill, key)
+
+
 @pytest.mark.parametrize("metric", [A.PRR_SCHW_KS_CART, A.PRR_SCHW_SPH, A.PRR_ELLIS])
 def test_fp32_bulk_second_order_metrics_track_fp64(metric):
     """The 8-component metrics in FP32-bulk: FP32 stages of the full
