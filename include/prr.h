@@ -250,6 +250,25 @@ prr_status prr_render(prr_ctx *ctx, const prr_frame *frames, int n, prr_image *i
                       void *stream);
 prr_status prr_get_stats(const prr_ctx *ctx, prr_stats *stats);
 
+/* Allocation-free workspace of prr_render_noalloc: its size on every device
+ * the library runs on, and the most frames one call renders. */
+#define PRR_NOALLOC_SCRATCH_BYTES ((size_t)64 << 20)
+#define PRR_NOALLOC_MAX_FRAMES 16
+
+/* prr_render without any host or device allocation, for callers that cannot
+ * allocate (code under stream capture, tight serving loops).  No counterpart
+ * in the reference.  The same kernels and results as prr_render, with the
+ * workspace in a static device array of PRR_NOALLOC_SCRATCH_BYTES per device
+ * (held until the process ends) instead of buffers the context allocates.
+ * Images must be device buffers (on_device = 1); no per-ray outputs.  Up to
+ * PRR_NOALLOC_MAX_FRAMES frames whose rays fit the array (about 94 bytes per
+ * ray, 12 per FXAA pixel, beside the integrator's per-lane contexts, whose
+ * grid shrinks to fit); larger inputs give PRR_EBUDGET.  Calls that overlap
+ * on one device share the array and race; the call returns when the images
+ * are written. */
+prr_status prr_render_noalloc(prr_ctx *ctx, const prr_frame *frames, int n, const prr_image *images,
+                              void *stream);
+
 /* Integrator alone from explicit initial states: x0u0 is 8 doubles per ray
  * (x^1, x^2, x^3, t, u^1, u^2, u^3, u^t) in the metric's coordinates, the
  * state the reference's Ray carries (main/base.h:172-176).  xend receives the

@@ -60,6 +60,7 @@ def lib():
     _sig(L, "prr_set_tracer", st, [_vp, C.POINTER(A.Tracer)])
     _sig(L, "prr_set_texture", st, [_vp, C.c_int, _vp, C.c_int, C.c_int, _i32])
     _sig(L, "prr_set_launch", st, [_vp, C.POINTER(A.Launch)])
+    _sig(L, "prr_render_noalloc", st, [_vp, C.POINTER(A.Frame), C.c_int, C.POINTER(A.Image), _vp])
     _sig(L, "prr_get_launch", st, [_vp, C.POINTER(A.Launch)])
     _sig(L, "prr_render", st, [_vp, C.POINTER(A.Frame), C.c_int, _vp, C.POINTER(A.RayOut), _vp])
     _sig(L, "prr_get_stats", st, [_vp, C.POINTER(A.Stats)])

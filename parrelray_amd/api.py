@@ -213,6 +213,19 @@ class Context:
             imgs[i].on_device = 1
         self._check(lib().prr_render(self.h, farr, n, imgs, None, C.c_void_p(stream) if stream else None))
 
+    def render_noalloc(self, frames, rgb8=None, rgbf=None, stream=None):
+        """prr_render_noalloc: render into device buffers (lists of torch tensors) without allocating."""
+        if isinstance(frames, A.Frame):
+            frames = [frames]
+        n = len(frames)
+        farr = (A.Frame * n)(*frames)
+        imgs = (A.Image * n)()
+        for i in range(n):
+            imgs[i].rgb8 = rgb8[i].data_ptr() if rgb8 is not None else None
+            imgs[i].rgbf = rgbf[i].data_ptr() if rgbf is not None else None
+            imgs[i].on_device = 1
+        self._check(lib().prr_render_noalloc(self.h, farr, n, imgs, C.c_void_p(stream) if stream else None))
+
     def adjoint(self, tmpl, orbit, dL_dimg, img_out=None):
         """Gradient of L w.r.t. (spin, r_cam, incl, sphere_phi) given dL/dI of the frame's image.
 

@@ -258,6 +258,8 @@ prr_status prr_create(prr_ctx **out, int hip_device) try {
     if (hip_device < 0 || hip_device >= n) return fail(nullptr, PRR_EINVAL, "device %d out of range", hip_device);
     hipError_t e = hipSetDevice(hip_device);
     if (e != hipSuccess) return fail(nullptr, PRR_EHIP, "hipSetDevice: %s", hipGetErrorString(e));
+    const prr_status ns = noalloc_setup(hip_device);
+    if (ns != PRR_OK) return ns;
     hipDeviceProp_t prop;
     e = hipGetDeviceProperties(&prop, hip_device);
     if (e != hipSuccess) return fail(nullptr, PRR_EHIP, "hipGetDeviceProperties: %s", hipGetErrorString(e));

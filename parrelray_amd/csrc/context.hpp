@@ -143,6 +143,9 @@ prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, 
                           bool events);
 // Counters of the finished call (synchronises the stream) into ctx->stats.
 prr_status read_stats(prr_ctx *ctx, const RenderPlan &P, const CountersDev *counters, hipStream_t st, bool events);
+// With `device` current: the device address of its copy of the allocation-free
+// workspace, looked up once (noalloc.cpp; called by prr_create).
+prr_status noalloc_setup(int device);
 
 } // namespace prr
 
