@@ -39,7 +39,11 @@
 //   SSAA / FXAA / RGB8 hand-computed RMS values and sample order, FXAA
 //                      fixed points (constant images, axis-aligned edges)
 //                      and its equivariance under grid flips / transpose,
-//                      RGB8 ties of the FP32 product (test_oracle_images).
+//                      RGB8 ties of the FP32 product (test_oracle_images);
+//                      FXAA outputs of small images in closed form, which
+//                      fix the luma weights and /255 (D14), REDUCE_MIN,
+//                      REDUCE_MUL, SPAN_MAX, the A / B choice and the
+//                      clamp-to-edge taps (test_oracle_fxaa_golden).
 //   escape / sphere    closed-form line intersections in flat space and an
 //   location (D21/22)  independent scipy event search on the Schwarzschild
 //                      right-hand side (test_oracle_events).
