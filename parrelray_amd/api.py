@@ -200,8 +200,13 @@ class Context:
                 off += k
         return outs
 
-    def render_device(self, frames, rgb8=None, rgbf=None, stream=None):
-        """Render into device buffers (lists of torch tensors, one per frame, or None)."""
+    def render_device(self, frames, rgb8=None, rgbf=None, rays=None, stream=None):
+        """Render into device buffers (lists of torch tensors, one per frame, or None).
+
+        rays: optional dict of device tensors over all rays of the call (frame
+        order) that receive per-ray outputs: 'cls' (int8), 'endpoint'
+        (float64, n x 3), 'steps', 'attempts' (int32), 'color' (float32, n x 3).
+        """
         if isinstance(frames, A.Frame):
             frames = [frames]
         n = len(frames)
@@ -211,7 +216,12 @@ class Context:
             imgs[i].rgb8 = rgb8[i].data_ptr() if rgb8 is not None else None
             imgs[i].rgbf = rgbf[i].data_ptr() if rgbf is not None else None
             imgs[i].on_device = 1
-        self._check(lib().prr_render(self.h, farr, n, imgs, None, C.c_void_p(stream) if stream else None))
+        ro = None
+        if rays is not None:
+            ptr = lambda k: rays[k].data_ptr() if rays.get(k) is not None else None  # noqa: E731
+            ro = A.RayOut(ptr("cls"), ptr("endpoint"), ptr("steps"), ptr("attempts"), ptr("color"), 1, 0)
+        self._check(lib().prr_render(self.h, farr, n, imgs, C.byref(ro) if ro is not None else None,
+                                     C.c_void_p(stream) if stream else None))
 
     def render_noalloc(self, frames, rgb8=None, rgbf=None, stream=None):
         """prr_render_noalloc: render into device buffers (lists of torch tensors) without allocating."""
