@@ -341,6 +341,16 @@ prr_status prr_params_frame(const prr_params *p, int i, prr_frame *out);
  * (PRR_EINVAL); a missing file gives PRR_EIO.  Messages via prr_global_error. */
 prr_status prr_image_read(const char *path, uint8_t *rgb8, int *w, int *h);
 
+/* ---- camera --------------------------------------------------------------- */
+
+/* Camera::computeVectors (main/Camera.cpp:19-28): forward, up and right unit
+ * vectors of a camera with azimuth alpha and tilt beta (radians). */
+prr_status prr_camera_basis(double alpha, double beta, double forward[3], double up[3], double right[3]);
+/* Camera::rayDir_PinHole / rayDir_Spherical (main/Camera.h:30-42), by the
+ * frame's projection: the unit direction through screen position (u, v) in
+ * [0, 1]^2 (u to the right, v down), as the kernels compute it. */
+prr_status prr_camera_dir(const prr_frame *f, double u, double v, double d[3]);
+
 /* ---- procedural inputs -------------------------------------------------- */
 
 /* Seeded RGB8 textures used in place of the reference's image files:

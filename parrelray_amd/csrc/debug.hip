@@ -254,6 +254,23 @@ prr_status prr_debug_queue_rays(const prr_frame *f, const int64_t *q, int64_t n,
     return PRR_OK;
 }
 
+prr_status prr_camera_basis(double alpha, double beta, double forward[3], double up[3], double right[3]) {
+    if (!forward || !up || !right) return PRR_EINVAL;
+    camera_basis(alpha, beta, forward, up, right);
+    return PRR_OK;
+}
+
+prr_status prr_camera_dir(const prr_frame *f, double u, double v, double d[3]) {
+    if (!f || !d || f->width <= 0 || f->height <= 0) return PRR_EINVAL;
+    prr_tracer tr{};
+    const FrameDev fr = frame_dev(PRR_FLAT, tr, *f, 0);
+    const D3 r = camera_dir(fr, u, v);
+    d[0] = r.x;
+    d[1] = r.y;
+    d[2] = r.z;
+    return PRR_OK;
+}
+
 prr_status prr_debug_camera_ray(const prr_frame *f, int64_t local, double d[3]) {
     if (!f || !d) return PRR_EINVAL;
     prr_tracer tr{};

@@ -618,9 +618,9 @@ PRR_HD void ray_uv(const FrameDev &fr, int64_t local, double &u, double &v) {
     }
 }
 
-PRR_HD D3 camera_ray(const FrameDev &fr, int64_t local) {
-    double u, v;
-    ray_uv(fr, local, u, v);
+// Direction of the camera ray through screen position (u, v) in [0, 1]^2
+// (Camera::rayDir_PinHole / rayDir_Spherical, main/Camera.h:30-42).
+PRR_HD D3 camera_dir(const FrameDev &fr, double u, double v) {
     if (fr.projection == PRR_PROJ_SPHERICAL) {
         const double nu = (0.5 - v) * fr.vfov, mu = (0.5 - u) * fr.aspect * fr.vfov;
         double sb, cb, sa, ca;
@@ -632,6 +632,12 @@ PRR_HD D3 camera_ray(const FrameDev &fr, int64_t local) {
     const double cu = (1.0 - 2.0 * v) * fr.tan_half;
     return normalize3(d3(fr.F[0], fr.F[1], fr.F[2]) + cr * d3(fr.R[0], fr.R[1], fr.R[2]) +
                       cu * d3(fr.U[0], fr.U[1], fr.U[2]));
+}
+
+PRR_HD D3 camera_ray(const FrameDev &fr, int64_t local) {
+    double u, v;
+    ray_uv(fr, local, u, v);
+    return camera_dir(fr, u, v);
 }
 
 // Larger root of g_tt (u^t)^2 + B u^t + C = 0 with u0 < 0 -> 1
