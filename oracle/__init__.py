@@ -13,7 +13,8 @@ import numpy as np
 from parrelray_amd import _abi as A
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# PRR_ORACLE_LIB: another build of the same source (the sanitizer build of tests/run_sanitized.sh)
+_LIB = os.environ.get("PRR_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 _dp = C.POINTER(C.c_double)
