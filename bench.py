@@ -20,6 +20,11 @@ FP32 as well, the rate of the non-packed FP32 instructions the kernels are compi
 
 cpu_baseline: the CPU FP64 oracle (oracle/, a restatement of the reference
 solver) on a stratified row subset of the same frame, all host cores, OpenMP.
+The same leg times the oracle on the whole C0 frame (the reference's
+CPU-runnable case, beside the GPU's C0 leg) and, as the checker only, runs
+the oracle at tol 1e-12 on seeded subsets of the rays the GPU legs rendered:
+the `accuracy` block (per config and form: class agreement with the rays
+within 1e-6 rad of a class boundary excused, and the endpoint-angle error).
 """
 import argparse
 import json
@@ -36,6 +41,25 @@ BATCH = 4
 # k_integrate<Kerr, RKF45, FP32-bulk> on one C1 frame: 1849.9 MB read + 1473.7 MB written
 TRAFFIC_FP32_BULK_C1 = (1881.1 + 1493.2) * 1e6
 METRIC = "Kerr RKF45 1080p rays/s + adjoint rays/s, FP32-bulk vs FP64 at ≤1e-6 rad"
+
+
+def subset(n, k, seed):
+    """A seeded uniform subset of k of n ray ids, sorted."""
+    import numpy as np
+    return np.sort(np.random.default_rng(seed).choice(n, size=min(k, n), replace=False))
+
+
+def gpu_rays(ctx, frames, ids, torch, base=0):
+    """Per-ray class and endpoint of rays base + ids of one more (untimed) render of frames."""
+    total = sum(f.nrays for f in frames)
+    rays = dict(cls=torch.empty(total, dtype=torch.int8, device="cuda"),
+                endpoint=torch.empty((total, 3), dtype=torch.float64, device="cuda"))
+    ctx.render_device(frames, rays=rays)
+    idx = torch.from_numpy(base + ids).cuda()
+    out = {k: v.index_select(0, idx).cpu().numpy() for k, v in rays.items()}
+    del rays
+    torch.cuda.synchronize()
+    return out
 
 
 def run_form(ctx, frame, precision, steps, warmup, out8, outf, torch):
@@ -69,7 +93,7 @@ def run_form(ctx, frame, precision, steps, warmup, out8, outf, torch):
                 fxaa_gbs=st["fxaa_bytes"] / max(1e-9, sum(s["fxaa_ms"] for s in per) / steps / 1e3) / 1e9)
 
 
-def run_adjoint(api, A, CF, reps):
+def run_adjoint(api, A, CF, reps, torch=None, ids=None):
     """C3: 512x512 Kerr a=0.7, FP32-bulk RKF45 tol 1e-8; gradient of the L2 loss against the target
     render w.r.t. (a, r_cam, incl, sphere phi).  Rays/s counts primary rays through the whole
     prr_adjoint call (forward render, checkpointed forward, replay and reverse sweep)."""
@@ -90,11 +114,14 @@ def run_adjoint(api, A, CF, reps):
         g = ctx.adjoint(tmpl, o0, dLdI)
     sec = (time.perf_counter() - t0) / reps
     st = ctx.stats()
-    return dict(rays=tmpl.nrays, seconds=sec, rays_per_s=tmpl.nrays / sec, adjoint_ms=st["adjoint_ms"],
-                total_ms=st["total_ms"], steps_swept=st["adjoint_steps"],
-                steps_per_s=st["adjoint_steps"] / (st["adjoint_ms"] / 1e3),
-                grad={"a": g[0], "r_cam": g[1], "incl": g[2], "sphere_phi": g[3]},
-                loss=float(((I - T) ** 2).mean()))
+    out = dict(rays=tmpl.nrays, seconds=sec, rays_per_s=tmpl.nrays / sec, adjoint_ms=st["adjoint_ms"],
+               total_ms=st["total_ms"], steps_swept=st["adjoint_steps"],
+               steps_per_s=st["adjoint_steps"] / (st["adjoint_ms"] / 1e3),
+               grad={"a": g[0], "r_cam": g[1], "incl": g[2], "sphere_phi": g[3]},
+               loss=float(((I - T) ** 2).mean()))
+    if ids is not None:
+        out["_rays"] = gpu_rays(ctx, [api.frame_from_orbit(tmpl, o0)], ids, torch)
+    return out
 
 
 def run_c1_batched(api, A, CF, ctx, steps, warmup, torch, precision=None):
@@ -120,6 +147,64 @@ def run_c1_batched(api, A, CF, ctx, steps, warmup, torch, precision=None):
     return dict(frames=BATCH, rays=st["rays"], ms_per_step=sec * 1e3, rays_per_s=st["rays"] / sec, launch=shape,
                 ms_per_frame=sec * 1e3 / BATCH, attempts=st["attempts"], attempts_fp64=st["attempts_fp64"],
                 graph_ms=st["total_ms"])
+
+
+def run_c4(api, A, CF, torch, steps, ids):
+    """C4: Kerr a = 0.998, camera r = 6M edge-on, 2048x2048 with 4x4 SSAA (67,108,864 rays) + FXAA,
+    tol 1e-10, one frame per step, in both precision forms; plus the per-ray results of a subset."""
+    f = CF.frame_c4()
+    out = {"rays_per_frame": f.nrays}
+    img = torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda")
+    for name, prec in (("fp64", A.PRR_FP64), ("fp32_bulk", A.PRR_FP32_BULK)):
+        ctx = api.Context(0)
+        ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.998)
+        ctx.set_tracer(CF.tracer_c4(precision=prec))
+        ctx.set_texture(0, CF.sky_texture())
+        ctx.render_device([f], rgb8=[img])  # warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        per = []
+        for _ in range(steps):
+            ctx.render_device([f], rgb8=[img])
+            per.append(ctx.stats())
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        st = per[-1]
+        integ = sum(p["integrate_ms"] for p in per) / steps
+        out[name] = dict(ms_per_frame=ms, rays_per_s=f.nrays / (ms / 1e3), integrate_ms=integ,
+                         attempts=st["attempts"], attempts_fp64=st["attempts_fp64"],
+                         attempts_per_s=st["attempts"] / (integ / 1e3),
+                         roofline_frac=(st["flops_fp32"] / P32 + st["flops_fp64"] / P64) / (integ / 1e3))
+        out[name]["_rays"] = gpu_rays(ctx, [f], ids, torch)
+        ctx.close()
+    out["speedup_fp32_bulk_vs_fp64"] = out["fp64"]["ms_per_frame"] / out["fp32_bulk"]["ms_per_frame"]
+    return out
+
+
+def run_c0(api, A, CF, torch, steps):
+    """C0, the reference's CPU-runnable case: Schwarzschild (Schwarzschild coordinates), fixed-step
+    Euler h = 0.01 M in FP64, 256x256, camera r = 30M at 85 deg, 10,000-step cap; per-ray results of
+    all its rays for the checker."""
+    import numpy as np
+    f = CF.frame_c0()
+    ctx = api.Context(0)
+    ctx.set_spacetime(A.PRR_SCHW_SPH, 1.0, 0.0)
+    ctx.set_tracer(CF.tracer_c0())
+    ctx.set_texture(0, CF.sky_texture())
+    img = torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda")
+    ctx.render_device([f], rgb8=[img])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ctx.render_device([f], rgb8=[img])
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / steps
+    st = ctx.stats()
+    out = dict(rays=f.nrays, ms_per_frame=ms, rays_per_s=f.nrays / (ms / 1e3), integrate_ms=st["integrate_ms"],
+               euler_steps=st["accepted"], steps_per_s=st["accepted"] / (st["integrate_ms"] / 1e3))
+    out["_rays"] = gpu_rays(ctx, [f], np.arange(f.nrays), torch)
+    ctx.close()
+    return out
 
 
 def run_copy(torch, nbytes=1 << 30):
@@ -183,7 +268,71 @@ def cpu_model():
     return "unknown"
 
 
-def cpu_baseline(frame, row_stride=16):
+def angle(a, b):
+    import numpy as np
+    a = a / np.linalg.norm(a, axis=-1, keepdims=True)
+    b = b / np.linalg.norm(b, axis=-1, keepdims=True)
+    return np.arctan2(np.linalg.norm(np.cross(a, b), axis=-1), np.clip(np.sum(a * b, axis=-1), -1.0, 1.0))
+
+
+def check_rays(S, frame, ids, orc, gpu, cores):
+    """The GPU's per-ray class and endpoint against the oracle's (orc) on rays ids of frame: class
+    mismatches, those not within 1e-6 rad of a class boundary (none of the oracle's four +-1e-6 rad
+    perturbations of the ray changes its class), and the endpoint angle of agreeing escape / sphere rays."""
+    import numpy as np
+
+    from parrelray_amd import _abi as A
+    same = gpu["cls"] == orc["cls"]
+    bad = np.nonzero(~same)[0]
+    excused = np.zeros(bad.size, bool)
+    if bad.size:
+        for axis in (1, 2, 3, 4):
+            r = S.render_rays(frame, ids[bad], delta=1e-6, axis=axis, nthreads=cores)
+            excused |= r["cls"] != orc["cls"][bad]
+    m = same & ((orc["cls"] == A.PRR_ESCAPE) | (orc["cls"] == A.PRR_SPHERE))
+    ang = angle(gpu["endpoint"][m], orc["endpoint"][m])
+    st = {}
+    if ang.size:
+        st = dict(max=float(ang.max()), mean=float(ang.mean()), median=float(np.median(ang)),
+                  p99=float(np.quantile(ang, 0.99)), frac_le_1e6=float((ang <= 1e-6).mean()), compared=int(ang.size))
+    return dict(rays=int(ids.size), class_mismatch=int(bad.size), class_mismatch_unexcused=int((~excused).sum()),
+                endpoint_angle_rad=st)
+
+
+def cpu_leg(frame, c0=None, checks=()):
+    """The CPU leg: the FP64 oracle timed on every 16th row of the C1 frame (cpu_baseline) and on the
+    whole C0 frame, then, as the checker only, the oracle at tol 1e-12 on the rays the GPU legs kept.
+    checks: (name, frame, oracle tracer, ray ids, {form: gpu rays}) per config."""
+    import numpy as np
+
+    import oracle as O
+    from parrelray_amd import _abi as A
+    from parrelray_amd import configs as CF
+    cores = len(os.sched_getaffinity(0))
+    base = cpu_baseline(frame, cores=cores)
+    accuracy = {}
+    if c0 is not None:
+        f0 = CF.frame_c0()
+        S0 = O.Scene(A.PRR_SCHW_SPH, 1.0, 0.0, CF.tracer_c0())
+        S0.set_texture(0, CF.sky_texture())
+        ids = np.arange(f0.nrays)
+        r = S0.render_rays(f0, ids, nthreads=cores)
+        c0["cpu"] = dict(rays_per_s=f0.nrays / r["seconds"], seconds=r["seconds"], cores=cores, kind="port",
+                         sample="the whole C0 frame (65,536 rays), FP64 Euler h = 0.01")
+        c0["speedup_gpu_vs_cpu"] = c0["rays_per_s"] / c0["cpu"]["rays_per_s"]
+        accuracy["C0"] = {"fp64": check_rays(S0, f0, ids, r, c0.pop("_rays"), cores)}
+    for name, f, tracer, ids, forms in checks:
+        S = O.Scene(A.PRR_KERR_BL, 1.0, 0.0, tracer)
+        t0 = time.perf_counter()
+        orc = S.render_rays(f, ids, nthreads=cores)
+        accuracy[name] = {"oracle": f"FP64 RKF45 tol {tracer.atol:g}, {ids.size} seeded rays of {f.nrays}",
+                          "oracle_seconds": time.perf_counter() - t0}
+        for form, g in forms.items():
+            accuracy[name][form] = check_rays(S, f, ids, orc, g, cores)
+    return base, accuracy
+
+
+def cpu_baseline(frame, row_stride=16, cores=None):
     """Oracle (FP64, Christoffel form) on every row_stride-th image row, all cores."""
     import numpy as np
 
@@ -196,7 +345,7 @@ def cpu_baseline(frame, row_stride=16):
     per_row = frame.width * frame.samples
     rows = np.arange(0, frame.height, row_stride)
     ids = (rows[:, None] * per_row + np.arange(per_row)[None, :]).ravel()
-    cores = len(os.sched_getaffinity(0))
+    cores = cores or len(os.sched_getaffinity(0))
     r = S.render_rays(frame, ids, nthreads=cores)
     return dict(value=ids.size / r["seconds"], unit="rays/s", cores=cores, kind="port", model=cpu_model(),
                 sample=f"C1 rows j % {row_stride} == 0 ({ids.size} rays of {frame.nrays}), FP64 RKF45 tol 1e-8, "
@@ -208,12 +357,12 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline, C0 on the CPU, accuracy)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the C3 adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the C2 64-frame batch leg")
-    ap.add_argument("--legs", default="fp64,fp32,adjoint,c2,batched,batched64,cpu",
+    ap.add_argument("--legs", default="fp64,fp32,adjoint,c2,batched,batched64,c4,c0,cpu",
                     help="comma-separated legs to run: fp64, fp32, adjoint, c2, batched, batched64 (the same "
-                         "batch in plain FP64), cpu, copy "
+                         "batch in plain FP64), c4, c0, cpu, copy "
                          "(copy: a 1 GiB device-to-device copy, the calibration kernel for HBM byte counters)")
     args = ap.parse_args()
     legs = set(args.legs.split(","))
@@ -238,22 +387,34 @@ def main():
         copy = run_copy(torch)
         progress(copy=copy)
     frame = CF.frame_c1()
+    ids_c1 = subset(frame.nrays, 4000, 101)
+    ids_c3 = subset(CF.frame_c3().nrays, 2000, 103)
+    ids_c4 = subset(CF.frame_c4().nrays, 4000, 104)
     ctx = api.Context(0)
     ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.95)
     ctx.set_texture(0, CF.sky_texture())
     ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
     out8 = torch.empty((frame.height, frame.width, 3), dtype=torch.uint8, device="cuda")
     outf = torch.empty((frame.height, frame.width, 3), dtype=torch.float32, device="cuda")
-    fp64 = fp32 = adj = batch = bat = bat64 = None
+    fp64 = fp32 = adj = batch = bat = bat64 = c4 = c0 = None
+    c1_rays = {}
     if "fp64" in legs:
         fp64 = run_form(ctx, frame, A.PRR_FP64, args.steps, args.warmup, out8, outf, torch)
+        c1_rays["fp64"] = gpu_rays(ctx, [frame], ids_c1, torch)
         progress(fp64_ms_per_step=fp64["ms_per_step"])
     if "fp32" in legs:
         fp32 = run_form(ctx, frame, A.PRR_FP32_BULK, args.steps, args.warmup, out8, outf, torch)
+        c1_rays["fp32_bulk"] = gpu_rays(ctx, [frame], ids_c1, torch)
         progress(fp32_bulk_ms_per_step=fp32["ms_per_step"])
     if "adjoint" in legs:
-        adj = run_adjoint(api, A, CF, max(1, args.steps // 2))
-        progress(adjoint=adj)
+        adj = run_adjoint(api, A, CF, max(1, args.steps // 2), torch, ids_c3)
+        progress(adjoint={k: v for k, v in adj.items() if not k.startswith("_")})
+    if "c4" in legs:
+        c4 = run_c4(api, A, CF, torch, 2, ids_c4)
+        progress(c4={k: v for k, v in c4.items() if k.startswith("speedup")})
+    if "c0" in legs:
+        c0 = run_c0(api, A, CF, torch, args.steps)
+        progress(c0={k: v for k, v in c0.items() if not k.startswith("_")})
     if "c2" in legs:
         batch = run_batch(api, A, CF, 2, torch)
         progress(batch=batch)
@@ -263,7 +424,25 @@ def main():
     if "batched64" in legs:
         bat64 = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch, precision=A.PRR_FP64)
         progress(c1_batched64_ms_per_frame=bat64["ms_per_frame"])
-    cpu = cpu_baseline(frame) if "cpu" in legs else None
+    cpu = accuracy = None
+    if "cpu" in legs:
+        checks = []
+        if c1_rays:
+            checks.append(("C1", frame, CF.tracer_kerr(tol=1e-12), ids_c1, c1_rays))
+        if adj and "_rays" in adj:
+            checks.append(("C3", api.frame_from_orbit(CF.frame_c3(), CF.c3_orbit()), CF.tracer_kerr(tol=1e-12),
+                           ids_c3, {"fp32_bulk": adj.pop("_rays")}))
+        if c4:
+            checks.append(("C4", CF.frame_c4(), CF.tracer_kerr(tol=1e-12, max_steps=400000), ids_c4,
+                           {k: c4[k].pop("_rays") for k in ("fp64", "fp32_bulk")}))
+        cpu, accuracy = cpu_leg(frame, c0, checks)
+        progress(accuracy=accuracy)
+    for leg in (adj, c0):
+        if leg:
+            leg.pop("_rays", None)
+    if c4:
+        for k in ("fp64", "fp32_bulk"):
+            c4[k].pop("_rays", None)
     roof = None
     if fp32:
         t_int = fp32["integrate_ms"] / 1e3
@@ -316,6 +495,8 @@ def main():
         line["fp32_bulk_batched"] = bat
         if fp64:
             line["speedup_batched_fp32_bulk_vs_plain_fp64"] = bat["rays_per_s"] / fp64["rays_per_s"]
+            line["target_3x_fp32_bulk_vs_plain_fp64"] = "missed" if bat["rays_per_s"] < 3 * fp64["rays_per_s"] \
+                else "met"
     if bat64:
         line["fp64_batched"] = bat64
         if bat:
@@ -323,8 +504,14 @@ def main():
     if adj:
         line["adjoint_rays_per_s"] = adj["rays_per_s"]
         line["adjoint"] = adj
+    if c4:
+        line["c4"] = c4
+    if c0:
+        line["c0"] = c0
     if batch:
         line["batch_c2"] = batch
+    if accuracy:
+        line["accuracy"] = accuracy
     if copy:
         line["copy_calibration"] = copy
     print(json.dumps(line))
