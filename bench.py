@@ -124,13 +124,16 @@ def run_adjoint(api, A, CF, reps, torch=None, ids=None):
     return out
 
 
-def run_c1_batched(api, A, CF, ctx, steps, warmup, torch, precision=None):
-    """BATCH C1 frames in one persistent launch from a hipGraph (FP32-bulk unless precision says otherwise)."""
+def run_c1_batched(api, A, CF, ctx, steps, warmup, torch, precision=None, tune_precision=True):
+    """BATCH C1 frames in one persistent launch from a hipGraph (FP32-bulk unless precision says
+    otherwise), with the autotuned launch shape and, for FP32-bulk, the autotuned stage precision
+    (tune_precision: the faster of FP32 and FP64 stages; False keeps the FP32 stages)."""
     from parrelray_amd import autotune
     frames = [CF.frame_c1() for _ in range(BATCH)]
     tracer = CF.tracer_kerr(precision=A.PRR_FP32_BULK if precision is None else precision)
     ctx.set_tracer(tracer)
-    shape = autotune.tune_launch(ctx, frames[:1], A.PRR_KERR_BL, tracer, use_cache=False)
+    shape = autotune.tune_launch(ctx, frames[:1], A.PRR_KERR_BL, tracer, use_cache=False,
+                                 tune_precision=tune_precision)
     rgb8 = [torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda") for f in frames]
     g = api.Graph(ctx, frames, rgb8=rgb8)
     for _ in range(warmup):
@@ -146,7 +149,8 @@ def run_c1_batched(api, A, CF, ctx, steps, warmup, torch, precision=None):
     ctx.set_launch()  # back to the default shape for the other legs
     return dict(frames=BATCH, rays=st["rays"], ms_per_step=sec * 1e3, rays_per_s=st["rays"] / sec, launch=shape,
                 ms_per_frame=sec * 1e3 / BATCH, attempts=st["attempts"], attempts_fp64=st["attempts_fp64"],
-                graph_ms=st["total_ms"])
+                graph_ms=st["total_ms"],
+                stage_precision="fp64" if shape.get("precision", tracer.precision) == A.PRR_FP64 else "fp32")
 
 
 def run_c4(api, A, CF, torch, steps, ids):
@@ -360,8 +364,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU leg (baseline, C0 on the CPU, accuracy)")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the C3 adjoint leg")
     ap.add_argument("--no-batch", action="store_true", help="skip the C2 64-frame batch leg")
-    ap.add_argument("--legs", default="fp64,fp32,adjoint,c2,batched,batched64,c4,c0,cpu",
-                    help="comma-separated legs to run: fp64, fp32, adjoint, c2, batched, batched64 (the same "
+    ap.add_argument("--legs", default="fp64,fp32,adjoint,c2,batched,batched32,batched64,c4,c0,cpu",
+                    help="comma-separated legs to run: fp64, fp32, adjoint, c2, batched (FP32-bulk form, autotuned "
+                         "stage precision), batched32 (the same batch with FP32 stages), batched64 (the same "
                          "batch in plain FP64), c4, c0, cpu, copy "
                          "(copy: a 1 GiB device-to-device copy, the calibration kernel for HBM byte counters)")
     args = ap.parse_args()
@@ -396,7 +401,7 @@ def main():
     ctx.set_texture(1, CF.sphere_texture(), A.PRR_FILTER_BILINEAR)
     out8 = torch.empty((frame.height, frame.width, 3), dtype=torch.uint8, device="cuda")
     outf = torch.empty((frame.height, frame.width, 3), dtype=torch.float32, device="cuda")
-    fp64 = fp32 = adj = batch = bat = bat64 = c4 = c0 = None
+    fp64 = fp32 = adj = batch = bat = bat32 = bat64 = c4 = c0 = None
     c1_rays = {}
     if "fp64" in legs:
         fp64 = run_form(ctx, frame, A.PRR_FP64, args.steps, args.warmup, out8, outf, torch)
@@ -421,6 +426,9 @@ def main():
     if "batched" in legs:
         bat = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch)
         progress(c1_batched_ms_per_frame=bat["ms_per_frame"])
+    if "batched32" in legs:
+        bat32 = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch, tune_precision=False)
+        progress(c1_batched32_ms_per_frame=bat32["ms_per_frame"])
     if "batched64" in legs:
         bat64 = run_c1_batched(api, A, CF, ctx, args.steps, args.warmup, torch, precision=A.PRR_FP64)
         progress(c1_batched64_ms_per_frame=bat64["ms_per_frame"])
@@ -475,7 +483,9 @@ def main():
         "data": "synthetic (seeded checker sky 512x256, value-noise sphere texture 1024x512)",
         "config": {"workload": (f"C1 x{BATCH} per step: " if bat else "C1, one frame per step: ")
                    + "Kerr a=0.95 BL, RKF45 rtol=atol=1e-8, 1920x1080, 2x2 SSAA + FXAA, camera r=20M i=80deg, "
-                   "sphere R=1M at r=8M; " + ("FP32-bulk, one persistent launch per batch from a hipGraph"
+                   "sphere R=1M at r=8M; " + (f"FP32-bulk form, one persistent launch per batch from a hipGraph, "
+                                              f"autotuned launch shape and stage precision "
+                                              f"({bat['stage_precision'].upper()} stages chosen)"
                                               if bat else ("FP32-bulk" if fp32 else "plain FP64")),
                    "rays_per_frame": frame.nrays, "frames_per_step": BATCH if bat else 1},
         "roofline": roof,
@@ -497,6 +507,10 @@ def main():
             line["speedup_batched_fp32_bulk_vs_plain_fp64"] = bat["rays_per_s"] / fp64["rays_per_s"]
             line["target_3x_fp32_bulk_vs_plain_fp64"] = "missed" if bat["rays_per_s"] < 3 * fp64["rays_per_s"] \
                 else "met"
+    if bat32:
+        line["fp32_stages_batched"] = bat32
+        if fp64:
+            line["speedup_batched_fp32_stages_vs_plain_fp64"] = bat32["rays_per_s"] / fp64["rays_per_s"]
     if bat64:
         line["fp64_batched"] = bat64
         if bat:

@@ -9,6 +9,12 @@ Results are cached per (device, metric, solver, precision) in a JSON file.
 Every launch shape yields bit-identical images (the per-ray work does not
 depend on which lanes share a wave), so tuning only changes speed; the
 tests check that invariance.
+
+For an FP32-bulk tracer it also times the plain FP64 stages at the tuned
+shape and keeps the faster of the two (tune_precision): both meet the
+FP32-bulk form's accuracy contract, and on devices whose non-packed FP32
+instructions issue at the FP64 rate (MI355X) the FP64 stages need fewer
+instructions than the FP32 difference stages with their conversions.
 """
 import json
 import os
@@ -52,13 +58,18 @@ def _time_integrate(ctx, frames, reps=2):
     return best
 
 
-def tune_launch(ctx, frames, metric, tracer, device_name="gpu0", budget_s=60.0, use_cache=True):
-    """Pick (block, blocks_per_cu, refill) for rendering `frames`; sets it on ctx and returns it."""
+def tune_launch(ctx, frames, metric, tracer, device_name="gpu0", budget_s=60.0, use_cache=True, tune_precision=True):
+    """Pick (block, blocks_per_cu, refill) for rendering `frames` and, for an FP32-bulk tracer, the
+    stage precision (tune_precision); sets them on ctx (launch shape, tracer) and returns them."""
     cache = _load()
-    key = _key(device_name, metric, tracer.solver, tracer.precision)
+    key = _key(device_name, metric, tracer.solver, tracer.precision) + ("" if tune_precision else "|stages")
     if use_cache and key in cache:
         best = cache[key]
         ctx.set_launch(best["block"], best["blocks_per_cu"], best["refill"], best.get("checkpoint", 64))
+        if best.get("precision", tracer.precision) != tracer.precision:
+            t = A.Tracer.from_buffer_copy(tracer)
+            t.precision = best["precision"]
+            ctx.set_tracer(t)
         return best
     start = time.perf_counter()
     results = []
@@ -85,8 +96,21 @@ def tune_launch(ctx, frames, metric, tracer, device_name="gpu0", budget_s=60.0, 
     results.sort()
     ms, block, bpc, refill = results[0]
     best = {"block": block, "blocks_per_cu": bpc, "refill": refill, "checkpoint": 64, "integrate_ms": ms,
-            "trials": len(results)}
-    ctx.set_launch(block, bpc, refill, 64)
+            "trials": len(results), "precision": int(tracer.precision)}
+    if tune_precision and tracer.precision == A.PRR_FP32_BULK:
+        # the FP64 stages at the same shape, refilling as the FP32-bulk form does (whole waves)
+        t64 = A.Tracer.from_buffer_copy(tracer)
+        t64.precision = A.PRR_FP64
+        ctx.set_tracer(t64)
+        ctx.set_launch(block, bpc, 64, 64)
+        ms64 = _time_integrate(ctx, frames)
+        best["integrate_ms_fp32_stages"], best["integrate_ms_fp64_stages"] = ms, ms64
+        best["trials"] += 1
+        if ms64 < ms:
+            best["precision"], best["integrate_ms"], best["refill"] = int(A.PRR_FP64), ms64, 64
+        else:
+            ctx.set_tracer(tracer)
+    ctx.set_launch(block, bpc, best["refill"], 64)
     cache[key] = best
     _save(cache)
     return best

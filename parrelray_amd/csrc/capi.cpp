@@ -188,9 +188,13 @@ constexpr size_t kRecBudget = size_t(8) << 30;
 
 double far_escape(const prr_ctx *ctx) { return ctx->metric == PRR_ELLIS ? ctx->p[1] : 0.0; }
 
+// Resident workgroups per CU of the persistent grid: the launch shape's, or
+// by default as many as the render instantiation's register budget lets
+// stay resident (its waves per SIMD x 4 SIMDs).
 int blocks_per_cu(const prr_ctx *ctx) {
     if (ctx->launch.blocks_per_cu > 0) return ctx->launch.blocks_per_cu;
-    return std::max(1, 2048 / std::max(64, ctx->launch.block) / 4); // 2 waves per SIMD
+    const int waves = integrate_waves(ctx->metric, ctx->tracer.solver);
+    return std::max(1, waves * 4 * 64 / std::max(64, ctx->launch.block));
 }
 
 // Algorithmic flops of one attempt.  Counting: FMA = 2; add, sub, mul, abs,

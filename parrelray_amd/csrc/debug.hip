@@ -76,7 +76,7 @@ prr_status host_ray(JobDev &job, const prr_tracer &tr, int64_t id, double *log, 
             row[2] = h_before;
             row[3] = L.att64 - a64;
             // the accepted state (also while its events are being located)
-            const double *yy = mode == MODE_LOCATE ? C.yn : L.y;
+            const double *yy = (mode == MODE_LOCATE || mode == MODE_SPHERE) ? C.yn : L.y;
             for (int i = 0; i < 8; ++i) row[4 + i] = i < Mt::N ? yy[i] : 0.0;
         }
         if (c >= 0) lane_store<Mt>(job, L, c);

@@ -24,6 +24,9 @@
 namespace prr {
 
 #define PRR_HD __host__ __device__ inline
+// Out of line: for code that is rare on the integrator's path, so that its
+// registers do not count against the loop's.
+#define PRR_NOINLINE __attribute__((noinline))
 
 // ----------------------------------------------------------------- helpers
 struct D3 {

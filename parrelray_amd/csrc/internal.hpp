@@ -81,21 +81,21 @@ struct RayStateDev {
     int32_t *attempts;
 };
 
-// Event location in progress for one lane (RKF45): the accepted step's end
-// state, the bracket / best states and the search variables.  One per lane
-// of the integrator grid, in device memory; only lanes that locate touch it.
+// Sphere-hit location in progress for one lane (RKF45): the accepted step's
+// end state, the bracket / best states and the search variables.  One per
+// lane of the integrator grid, in device memory; only lanes whose step chord
+// passes near a sphere touch it (escape location keeps its state in the lane).
 struct LocateCtx {
     double yn[8];    // end state of the accepted step
-    double yb[8];    // sphere: state at the inside end of the bracket
-    double ybest[8]; // sphere: state at the best hit so far
+    double yb[8];    // state at the inside end of the bracket
+    double ybest[8]; // state at the best hit so far
     double h;        // size of the step
     double th;       // fraction of the step of the pending sub-step
-    double ta, t1, tb, Ga, G1, Gb; // sphere: minimum search
-    double lo, hi, Glo, Ghi;       // sphere: entry bracket
+    double ta, t1, tb, Ga, G1, Gb; // minimum search
+    double lo, hi, Glo, Ghi;       // entry bracket
     double G0, best_th;
-    int32_t kind;  // 0 escape, 1 sphere
-    int32_t phase; // sphere: 0 next sphere, 1 first probe, 2 minimum search, 3 bracket
-    int32_t k, mask, it, side, best_k, pad;
+    int32_t phase; // 0 next sphere, 1 first probe, 2 minimum search, 3 bracket
+    int32_t k, mask, it, side, best_k;
 };
 
 // Adjoint modes of the integrator (AdjRunDev.mode).

@@ -21,13 +21,24 @@
 #include "kernels.hpp"
 #include "lane.hpp"
 
-// Register budget of the integrator: waves per SIMD the compiler must fit
-// (build-time knob, -DPRR_WAVES_PER_EU=n; unset = the compiler's choice).
-#ifdef PRR_WAVES_PER_EU
-#define PRR_OCCUPANCY __attribute__((amdgpu_waves_per_eu(PRR_WAVES_PER_EU, 8)))
-#else
-#define PRR_OCCUPANCY
+// Register budget of the integrator: waves per SIMD the compiler must fit.
+// PRR_WAVES_PER_EU (build-time knob, default 3: at most 168 VGPRs) for the
+// renders of the 6-component Kerr state and of the flat and Kerr-Schild
+// metrics with Euler, RK4 and RKF45, which fit it (escape location stays in
+// the loop, sphere location is out of line) with a few spills in branches
+// taken once per ray.  2 waves (at most 256 VGPRs) for the rest: the
+// 8-component spherical charts, Dormand-Prince's seven stages and the
+// adjoint's checkpointing and replay instantiations carry more live state
+// and would spill inside the loop.
+#ifndef PRR_WAVES_PER_EU
+#define PRR_WAVES_PER_EU 3
 #endif
+template <class Mt, int SOLVER> struct WavesOf {
+    static constexpr bool lean = !Mt::ckpt && !Mt::replay && SOLVER != PRR_DORMAND &&
+                                 (Mt::kind == PRR_KERR_BL || Mt::kind == PRR_FLAT || Mt::kind == PRR_SCHW_KS_CART);
+    static constexpr int value = lean ? PRR_WAVES_PER_EU : 2;
+};
+#define PRR_OCCUPANCY __attribute__((amdgpu_waves_per_eu(WavesOf<Mt, SOLVER>::value, 8)))
 
 namespace prr {
 
@@ -213,40 +224,69 @@ __device__ inline float3 load3(const float *im, int W, int x, int y) {
     return make_float3(p[0], p[1], p[2]);
 }
 
-// Clamp-to-edge bilinear tap at pixel-space (X, Y).
-__device__ inline float3 fxaa_tap(const float *im, int W, int H, float X, float Y) {
+// fxaa (kernels/ssaa.cuh:150-182) on every pixel with clamp-to-edge taps,
+// fused with the RGB8 quantisation.  A workgroup takes a 16 x 16 tile; its
+// pixels and a 1-pixel halo (clamped to the image) are staged in LDS with
+// their lumas, computed once per pixel.  A bilinear tap reads LDS when its
+// 2 x 2 footprint lies in the staged 18 x 18 block; with luma / 255 (D14) the
+// search span is at most about one pixel for colours in [0, 1], so every tap
+// of a render does.  Larger spans (colours above 1) read the image in global
+// memory.  The arithmetic is the same either way.
+constexpr int kFxTile = 16, kFxHalo = kFxTile + 2;
+
+__device__ inline float3 fxaa_tap_lds(const float *im, const float (*tile)[kFxHalo][3], int W, int H, int ox,
+                                      int oy, float X, float Y) {
 #pragma clang fp contract(off)
     const float fx0 = floorf(X), fy0 = floorf(Y);
     const float fx = X - fx0, fy = Y - fy0;
     const int x0 = int(fx0), y0 = int(fy0);
     const int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
     const int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
-    const float3 c00 = load3(im, W, xa, ya), c10 = load3(im, W, xb, ya), c01 = load3(im, W, xa, yb),
-                 c11 = load3(im, W, xb, yb);
+    // staged block: image rows oy - 1 .. oy + 16 and columns ox - 1 .. ox + 16, clamped
+    const int lxa = xa - ox + 1, lxb = xb - ox + 1, lya = ya - oy + 1, lyb = yb - oy + 1;
+    float3 c00, c10, c01, c11;
+    if (lxa >= 0 && lxb < kFxHalo && lya >= 0 && lyb < kFxHalo && lxb >= 0 && lya < kFxHalo) {
+        c00 = make_float3(tile[lya][lxa][0], tile[lya][lxa][1], tile[lya][lxa][2]);
+        c10 = make_float3(tile[lya][lxb][0], tile[lya][lxb][1], tile[lya][lxb][2]);
+        c01 = make_float3(tile[lyb][lxa][0], tile[lyb][lxa][1], tile[lyb][lxa][2]);
+        c11 = make_float3(tile[lyb][lxb][0], tile[lyb][lxb][1], tile[lyb][lxb][2]);
+    } else {
+        c00 = load3(im, W, xa, ya);
+        c10 = load3(im, W, xb, ya);
+        c01 = load3(im, W, xa, yb);
+        c11 = load3(im, W, xb, yb);
+    }
     const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy), w01 = (1.0f - fx) * fy, w11 = fx * fy;
     return make_float3(c00.x * w00 + c10.x * w10 + c01.x * w01 + c11.x * w11,
                        c00.y * w00 + c10.y * w10 + c01.y * w01 + c11.y * w11,
                        c00.z * w00 + c10.z * w10 + c01.z * w01 + c11.z * w11);
 }
 
-// fxaa (kernels/ssaa.cuh:150-182) on every pixel with clamp-to-edge taps,
-// fused with the RGB8 quantisation.  16x16 tiles; neighbour taps hit L1/L2.
 __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
 #pragma clang fp contract(off)
+    __shared__ float tile[kFxHalo][kFxHalo][3];
+    __shared__ float lum[kFxHalo][kFxHalo];
     const PostFrameDev pf = pfs[blockIdx.z];
-    const int x = int(blockIdx.x) * 16 + int(threadIdx.x & 15);
-    const int y = int(blockIdx.y) * 16 + int(threadIdx.x >> 4);
     const int W = pf.W, H = pf.H;
-    if (!pf.fxaa || x >= W || y >= H) return;
+    const int ox = int(blockIdx.x) * kFxTile, oy = int(blockIdx.y) * kFxTile;
+    if (!pf.fxaa || ox >= W || oy >= H) return; // uniform over the workgroup
     const float *im = pf.img;
-    auto L = [&](int xx, int yy) {
-        xx = min(max(xx, 0), W - 1);
-        yy = min(max(yy, 0), H - 1);
+    // stage the 18 x 18 block (clamp-to-edge) and its lumas
+    for (int t = int(threadIdx.x); t < kFxHalo * kFxHalo; t += int(blockDim.x)) {
+        const int ly = t / kFxHalo, lx = t - ly * kFxHalo;
+        const int xx = min(max(ox + lx - 1, 0), W - 1), yy = min(max(oy + ly - 1, 0), H - 1);
         const float3 c = load3(im, W, xx, yy);
-        return luma(c.x, c.y, c.z);
-    };
-    const float lNW = L(x - 1, y - 1), lNE = L(x + 1, y - 1), lSW = L(x - 1, y + 1), lSE = L(x + 1, y + 1);
-    const float lM = L(x, y);
+        tile[ly][lx][0] = c.x;
+        tile[ly][lx][1] = c.y;
+        tile[ly][lx][2] = c.z;
+        lum[ly][lx] = luma(c.x, c.y, c.z);
+    }
+    __syncthreads();
+    const int tx = int(threadIdx.x & 15), ty = int(threadIdx.x >> 4);
+    const int x = ox + tx, y = oy + ty;
+    if (x >= W || y >= H) return;
+    const float lNW = lum[ty][tx], lNE = lum[ty][tx + 2], lSW = lum[ty + 2][tx], lSE = lum[ty + 2][tx + 2];
+    const float lM = lum[ty + 1][tx + 1];
     const float lMin = fminf(lM, fminf(fminf(lNW, lNE), fminf(lSW, lSE)));
     const float lMax = fmaxf(lM, fmaxf(fmaxf(lNW, lNE), fmaxf(lSW, lSE)));
     float dx = -((lNW + lNE) - (lSW + lSE));
@@ -257,10 +297,10 @@ __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
     dy = fminf(fmaxf(dy * rcp, -8.0f), 8.0f);
     const float k1 = 1.0f / 3.0f - 0.5f, k2 = 2.0f / 3.0f - 0.5f;
     const float fxp = float(x), fyp = float(y);
-    const float3 t1 = fxaa_tap(im, W, H, fxp + dx * k1, fyp + dy * k1);
-    const float3 t2 = fxaa_tap(im, W, H, fxp + dx * k2, fyp + dy * k2);
-    const float3 t3 = fxaa_tap(im, W, H, fxp + dx * -0.5f, fyp + dy * -0.5f);
-    const float3 t4 = fxaa_tap(im, W, H, fxp + dx * 0.5f, fyp + dy * 0.5f);
+    const float3 t1 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k1, fyp + dy * k1);
+    const float3 t2 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k2, fyp + dy * k2);
+    const float3 t3 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * -0.5f, fyp + dy * -0.5f);
+    const float3 t4 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * 0.5f, fyp + dy * 0.5f);
     const float3 A = make_float3(t1.x * 0.5f + t2.x * 0.5f, t1.y * 0.5f + t2.y * 0.5f, t1.z * 0.5f + t2.z * 0.5f);
     const float3 B = make_float3(0.5f * A.x + (t3.x * 0.25f + t4.x * 0.25f), 0.5f * A.y + (t3.y * 0.25f + t4.y * 0.25f),
                                  0.5f * A.z + (t3.z * 0.25f + t4.z * 0.25f));
@@ -656,6 +696,25 @@ template <class Mt> void launch_integrate_t(const JobDev &job, dim3 g, dim3 b, h
     case PRR_RK4: launch_integrate_s<Mt, PRR_RK4>(job, g, b, st); break;
     case PRR_DORMAND: launch_integrate_s<Mt, PRR_DORMAND>(job, g, b, st); break;
     default: launch_integrate_s<Mt, PRR_RKF45>(job, g, b, st); break;
+    }
+}
+
+template <class Mt> static int waves_of(int solver) {
+    switch (solver) {
+    case PRR_EULER: return WavesOf<Mt, PRR_EULER>::value;
+    case PRR_RK4: return WavesOf<Mt, PRR_RK4>::value;
+    case PRR_DORMAND: return WavesOf<Mt, PRR_DORMAND>::value;
+    default: return WavesOf<Mt, PRR_RKF45>::value;
+    }
+}
+
+int integrate_waves(int metric, int solver) {
+    switch (metric) {
+    case PRR_FLAT: return waves_of<NoAdj<Flat>>(solver);
+    case PRR_SCHW_KS_CART: return waves_of<NoAdj<SchwKS>>(solver);
+    case PRR_SCHW_SPH: return waves_of<NoAdj<SchwSph>>(solver);
+    case PRR_KERR_BL: return waves_of<NoAdj<Kerr>>(solver);
+    default: return waves_of<NoAdj<Ellis>>(solver);
     }
 }
 

@@ -15,6 +15,8 @@ hipError_t launch_shade(int metric, const JobDev &job, hipStream_t st);
 // code; the Kerr adjoint's checkpointing passes and its replay each carry
 // their own path (other metrics have no adjoint: hipErrorInvalidValue).
 hipError_t launch_integrate(int metric, const JobDev &job, int grid, int block, hipStream_t st);
+// Waves per SIMD that the render instantiation of (metric, solver) is compiled for.
+int integrate_waves(int metric, int solver);
 // Zero the device counters and the ray-queue head (all: also the adjoint's mismatch count).
 hipError_t launch_reset(CountersDev *c, hipStream_t st, bool all = true);
 // SSAA combine for nframes frames (pfs is a device array).

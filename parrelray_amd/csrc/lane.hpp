@@ -129,6 +129,10 @@ constexpr double kAxisFP64 = 0.05;
 constexpr double kAxisStep = 0.25;
 
 // Lane state of one ray.
+// While the lane locates an escape (MODE_LOCATE), three fields that the ray
+// no longer needs hold the location's state, so that it stays in registers:
+// h is the size of the step being searched, lam the fraction theta of the
+// pending sub-step and hit the Newton iteration count.
 template <class Mt> struct Lane {
     double y[Mt::N];
     Consts<double> cs;
@@ -703,15 +707,29 @@ template <class Mt, bool MIXED> PRR_HD int euler_step(const JobDev &job, Lane<Mt
 }
 
 // ------------------------------------------------------------ RKF45 events
-enum { MODE_IDLE = 0, MODE_ACTIVE = 1, MODE_LOCATE = 2 };
-enum { LOC_ESCAPE = 0, LOC_SPHERE = 1 };
+// MODE_LOCATE: the escape crossing inside the last accepted step, located by
+// Newton sub-steps with its state in the lane (Lane); MODE_SPHERE: a sphere
+// hit, located through the lane's LocateCtx in device memory by out-of-line
+// functions (sphere_prepare / sphere_update) that take the lane's state by
+// value.  Sphere locations are rare (only steps whose chord passes near a
+// sphere), and keeping their code out of the loop leaves the loop the
+// registers of three waves per SIMD.
+enum { MODE_IDLE = 0, MODE_ACTIVE = 1, MODE_LOCATE = 2, MODE_SPHERE = 3 };
 enum { SPH_NEXT = 0, SPH_PROBE = 1, SPH_MIN = 2, SPH_BRACKET = 3 };
-constexpr int kNeedSubstep = -2; // locate_prepare: a sub-step of the returned size is due
+constexpr int kNeedSubstep = -2; // a sub-step of the returned size is due
 constexpr int kResume = -3;      // the step had no event after all: the lane goes on
+constexpr int kSphereHit = -4;   // sphere_prepare: the best hit (C.ybest, C.best_th, C.best_k) ends the ray
+constexpr int kSphereNone = -5;  // sphere_prepare: no sphere was hit in the step (its end state is C.yn)
 
 template <int N> PRR_HD void copy_n(double *d, const double *s) {
     for (int i = 0; i < N; ++i) d[i] = s[i];
 }
+
+// A state passed by value to the out-of-line sphere functions (the lane's
+// own array stays in registers).
+template <int N> struct StateN {
+    double v[N];
+};
 
 // Adjoint checkpoint passes: a checkpoint every K accepted steps while the lane integrates.
 template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<Mt> &L) {
@@ -734,15 +752,16 @@ template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &
     r[5 * T] = h;
 }
 
-// A located event ends the ray: the lane takes the event's state (adjoint
-// replay: the terminal step is recorded for the seed).
+// A located event of the step of size h from L.y ends the ray: the lane
+// takes the event's state ys (adjoint replay: the terminal step is recorded
+// for the seed).
 template <class Mt>
-PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, const LocateCtx &C, const double *ys, double theta, int cls,
+PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, double h, const double *ys, double theta, int cls,
                           int hit) {
     if (Mt::replay && job.adj.mode == ADJ_REPLAY && job.adj.event) {
         EventDev &ev = job.adj.event[L.rid];
         for (int i = 0; i < 8; ++i) ev.y_prev[i] = i < Mt::N ? L.y[i] : 0.0;
-        ev.h = C.h;
+        ev.h = h;
         ev.theta = theta;
         ev.cls = cls;
         ev.hit = hit;
@@ -754,9 +773,8 @@ PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, const LocateCtx &C, co
 }
 
 // Escape test of the accepted step L.y -> yn (size h); starts the location
-// when r crosses r_escape.  Then spherical wraps and the horizon.  Returns a
-// class, -1 (go on, L.y = yn) or MODE_LOCATE's kNeedSubstep marker through
-// `mode`.
+// (MODE_LOCATE) when r crosses r_escape.  Then spherical wraps and the
+// horizon.  Returns a class or -1 (go on: L.y = yn, or the location started).
 template <class Mt>
 PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h,
                                int &mode) {
@@ -766,13 +784,14 @@ PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, con
     if (r >= r_esc) {
         double go, gn, dg;
         escape_g<Mt>(L.y, r_esc, go, dg);
-        if (!(go < 0.0)) return finish_located(job, L, C, L.y, 0.0, PRR_ESCAPE, -1); // started outside
+        if (!(go < 0.0)) return finish_located(job, L, h, L.y, 0.0, PRR_ESCAPE, -1); // started outside
         escape_g<Mt>(yn, r_esc, gn, dg);
-        copy_n<Mt::N>(C.yn, yn);
-        C.h = h;
-        C.kind = LOC_ESCAPE;
-        C.th = fmin(fmax(go / (go - gn), 0.0), 1.0);
-        C.it = 0;
+#if !defined(__HIP_DEVICE_COMPILE__)
+        copy_n<Mt::N>(C.yn, yn); // the host-run loop logs the accepted state
+#endif
+        L.h = h;
+        L.lam = fmin(fmax(go / (go - gn), 0.0), 1.0);
+        L.hit = 0;
         mode = MODE_LOCATE;
         return -1;
     }
@@ -800,30 +819,33 @@ PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double
         if (mask) {
             copy_n<Mt::N>(C.yn, yn);
             C.h = h;
-            C.kind = LOC_SPHERE;
             C.phase = SPH_NEXT;
             C.mask = int32_t(mask);
             C.k = -1;
             C.best_k = -1;
             C.best_th = 2.0;
-            mode = MODE_LOCATE;
+            mode = MODE_SPHERE;
             return -1;
         }
     }
     return escape_wrap_horizon<Mt>(job, L, C, yn, h, mode);
 }
 
-// Event location before the sub-step of this trip: sphere bookkeeping that
-// needs no sub-step is done here.  Returns kNeedSubstep with *hs set, a class
-// (the ray ends), or -1 / kResume with the lane active again.
+// Result of sphere_prepare: kNeedSubstep (a sub-step of fraction C.th, size
+// hs), kSphereHit or kSphereNone.
+struct SphStep {
+    int32_t code;
+    int32_t pad;
+    double hs;
+};
+
+// Sphere location before the sub-step of this trip; y is the step's start
+// state.  Bookkeeping that needs no sub-step is done here.
 template <class Mt>
-PRR_HD int locate_prepare(const JobDev &job, Lane<Mt> &L, LocateCtx &C, double &hs, int &mode) {
+PRR_HD PRR_NOINLINE SphStep sphere_prepare(const FrameDev *frp, StateN<Mt::N> y, LocateCtx *Cp) {
     constexpr int N = Mt::N;
-    if (C.kind == LOC_ESCAPE) {
-        hs = C.th * C.h;
-        return kNeedSubstep;
-    }
-    const FrameDev &fr = job.frames[L.frame];
+    const FrameDev &fr = *frp;
+    LocateCtx &C = *Cp;
     for (;;) {
         if (C.phase == SPH_BRACKET) {
             const double R2 = fr.sph[C.k][3] * fr.sph[C.k][3];
@@ -833,8 +855,7 @@ PRR_HD int locate_prepare(const JobDev &job, Lane<Mt> &L, LocateCtx &C, double &
                 double t = (C.lo * C.Ghi - C.hi * C.Glo) / (C.Ghi - C.Glo);
                 if (!(t > C.lo && t < C.hi)) t = 0.5 * (C.lo + C.hi);
                 C.th = t;
-                hs = t * C.h;
-                return kNeedSubstep;
+                return SphStep{kNeedSubstep, 0, t * C.h};
             }
             if (C.hi < C.best_th) {
                 C.best_th = C.hi;
@@ -843,41 +864,22 @@ PRR_HD int locate_prepare(const JobDev &job, Lane<Mt> &L, LocateCtx &C, double &
             }
             C.phase = SPH_NEXT;
         }
-        if (C.phase == SPH_PROBE || C.phase == SPH_MIN) {
-            hs = C.th * C.h;
-            return kNeedSubstep;
-        }
+        if (C.phase == SPH_PROBE || C.phase == SPH_MIN) return SphStep{kNeedSubstep, 0, C.th * C.h};
         // SPH_NEXT: the next screened sphere
         int k = C.k + 1;
         while (k < fr.n_spheres && !((unsigned(C.mask) >> k) & 1u)) ++k;
-        if (k >= fr.n_spheres) {
-            if (C.best_k >= 0) {
-                mode = MODE_IDLE;
-                return finish_located(job, L, C, C.ybest, C.best_th, PRR_SPHERE, C.best_k);
-            }
-            // no hit: the step's other events
-            mode = MODE_ACTIVE;
-            double yn[N];
-            copy_n<N>(yn, C.yn);
-            const int r = escape_wrap_horizon<Mt>(job, L, C, yn, C.h, mode);
-            if (r >= 0) mode = MODE_IDLE;
-            if (mode == MODE_LOCATE) {
-                hs = C.th * C.h;
-                return kNeedSubstep;
-            }
-            return r >= 0 ? r : kResume;
-        }
+        if (k >= fr.n_spheres) return SphStep{C.best_k >= 0 ? kSphereHit : kSphereNone, 0, 0.0};
         C.k = k;
         const D3 c = d3(fr.sph[k][0], fr.sph[k][1], fr.sph[k][2]);
         const double R2 = fr.sph[k][3] * fr.sph[k][3];
-        const D3 P0 = cart_of<Mt>(L.y), P1 = cart_of<Mt>(C.yn);
+        const D3 P0 = cart_of<Mt>(y.v), P1 = cart_of<Mt>(C.yn);
         const D3 m = P0 - c;
         C.G0 = dot3(m, m) - R2;
         if (C.G0 <= 0.0) { // the step starts inside (reference: chord t = 0)
             if (0.0 < C.best_th) {
                 C.best_th = 0.0;
                 C.best_k = k;
-                copy_n<N>(C.ybest, L.y);
+                copy_n<N>(C.ybest, y.v);
             }
             continue;
         }
@@ -917,22 +919,11 @@ PRR_HD bool parabola_vertex(const LocateCtx &C, double &tv) {
     return tv > C.ta && tv < C.tb && fabs(tv - C.t1) >= 1e-9;
 }
 
-// The sub-step of fraction C.th gave yt: advance the location.  Returns a
-// class (located) or -1 (go on locating).
-template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yt) {
+// The sub-step of fraction C.th gave yt: advance the sphere location.
+template <class Mt> PRR_HD PRR_NOINLINE void sphere_update(const FrameDev *frp, StateN<Mt::N> yt, LocateCtx *Cp) {
     constexpr int N = Mt::N;
-    if (C.kind == LOC_ESCAPE) {
-        const double r_esc = escape_radius<Mt>(job, job.frames[L.frame], yt);
-        double g, dg;
-        escape_g<Mt>(yt, r_esc, g, dg);
-        const double tol = 1e-13 * (Mt::mino ? 1.0 / r_esc : r_esc);
-        if (fabs(g) <= tol || !(dg * C.h != 0.0) || C.it >= 6) return finish_located(job, L, C, yt, C.th, PRR_ESCAPE, -1);
-        C.th = fmin(fmax(C.th - g / (dg * C.h), 0.0), 1.0);
-        C.it++;
-        return -1;
-    }
-    const FrameDev &fr = job.frames[L.frame];
-    const double G = sphere_G<Mt>(fr, C.k, yt);
+    LocateCtx &C = *Cp;
+    const double G = sphere_G<Mt>(*frp, C.k, yt.v);
     if (C.phase == SPH_BRACKET) { // Illinois regula falsi
         if (G > 0.0) {
             C.lo = C.th;
@@ -942,23 +933,23 @@ template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, Loc
         } else {
             C.hi = C.th;
             C.Ghi = G;
-            copy_n<N>(C.yb, yt);
+            copy_n<N>(C.yb, yt.v);
             if (C.side == 1) C.Glo *= 0.5;
             C.side = 1;
         }
         C.it++;
-        return -1;
+        return;
     }
     if (G <= 0.0) { // the path is inside at this probe: bracket [0, th]
         C.lo = 0.0;
         C.Glo = C.G0;
         C.hi = C.th;
         C.Ghi = G;
-        copy_n<N>(C.yb, yt);
+        copy_n<N>(C.yb, yt.v);
         C.side = 0;
         C.it = 0;
         C.phase = C.hi < C.best_th ? SPH_BRACKET : SPH_NEXT;
-        return -1;
+        return;
     }
     if (C.phase == SPH_PROBE) {
         C.G1 = G;
@@ -991,6 +982,63 @@ template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, Loc
     } else {
         C.phase = SPH_NEXT; // the path stays outside: a miss
     }
+}
+
+// Event location before the sub-step of this trip (MODE_LOCATE or
+// MODE_SPHERE).  Returns kNeedSubstep with hs set, a class (the ray ends),
+// or kResume with the lane active again.
+template <class Mt>
+PRR_HD int locate_prepare(const JobDev &job, Lane<Mt> &L, LocateCtx &C, double &hs, int &mode) {
+    if (mode == MODE_LOCATE) {
+        hs = L.lam * L.h;
+        return kNeedSubstep;
+    }
+    StateN<Mt::N> y;
+    copy_n<Mt::N>(y.v, L.y);
+    const SphStep r = sphere_prepare<Mt>(&job.frames[L.frame], y, &C);
+    if (r.code == kNeedSubstep) {
+        hs = r.hs;
+        return kNeedSubstep;
+    }
+    if (r.code == kSphereHit) {
+        mode = MODE_IDLE;
+        double yb[Mt::N];
+        copy_n<Mt::N>(yb, C.ybest);
+        return finish_located(job, L, C.h, yb, C.best_th, PRR_SPHERE, C.best_k);
+    }
+    // no hit: the step's other events
+    mode = MODE_ACTIVE;
+    double yn[Mt::N];
+    copy_n<Mt::N>(yn, C.yn);
+    const int e = escape_wrap_horizon<Mt>(job, L, C, yn, C.h, mode);
+    if (e >= 0) {
+        mode = MODE_IDLE;
+        return e;
+    }
+    if (mode == MODE_LOCATE) {
+        hs = L.lam * L.h;
+        return kNeedSubstep;
+    }
+    return kResume;
+}
+
+// The sub-step gave yt: advance the location.  Returns a class (located) or
+// -1 (go on locating).
+template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yt, int mode) {
+    if (mode == MODE_LOCATE) {
+        const double r_esc = escape_radius<Mt>(job, job.frames[L.frame], yt);
+        double g, dg;
+        escape_g<Mt>(yt, r_esc, g, dg);
+        const double tol = 1e-13 * (Mt::mino ? 1.0 / r_esc : r_esc);
+        if (fabs(g) <= tol || !(dg * L.h != 0.0) || L.hit >= 6)
+            return finish_located(job, L, L.h, yt, L.lam, PRR_ESCAPE, -1);
+        L.lam = fmin(fmax(L.lam - g / (dg * L.h), 0.0), 1.0);
+        L.hit++;
+        return -1;
+    }
+    StateN<Mt::N> y;
+    copy_n<Mt::N>(y.v, yt);
+    sphere_update<Mt>(&job.frames[L.frame], y, &C);
     return -1;
 }
 
@@ -1010,7 +1058,8 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
     const TracerDev &tr = job.tr;
     double hs;
     bool use64 = true, at_min = false;
-    if (mode == MODE_LOCATE) {
+    const bool locating = mode == MODE_LOCATE || mode == MODE_SPHERE;
+    if (locating) {
         const int r = locate_prepare<Mt>(job, L, C, hs, mode);
         if (r == kResume) return -1;
         if (r != kNeedSubstep) {
@@ -1046,7 +1095,7 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
     // replay must repeat each ray's steps bit for bit outside its wave.
     if constexpr (MIXED && !Mt::ckpt && !Mt::replay)
         if (job.refill >= 64) use64 = wave_any(use64);
-    if (mode != MODE_LOCATE) L.att64 += use64 ? 1 : 0;
+    if (!locating) L.att64 += use64 ? 1 : 0;
     double d4[N], e[N];
     if (use64) rkf_stages<Mt, double, Tab>(L.y, hs, L.cs, d4, e);
     else if (Mt::mino) kerr_stages_fp32<Tab>(L.y, hs, L.cs, d4, e);
@@ -1054,8 +1103,8 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
     double yn[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) yn[i] = L.y[i] + d4[i];
-    if (mode == MODE_LOCATE) {
-        const int r = locate_update<Mt>(job, L, C, yn);
+    if (locating) {
+        const int r = locate_update<Mt>(job, L, C, yn, mode);
         if (r >= 0) mode = MODE_IDLE;
         return r;
     }

@@ -99,13 +99,39 @@ def test_autotuner_picks_a_shape_and_results_do_not_change(tmp_path, monkeypatch
     ctx = context()
     ref = ctx.render(frames)
     best = autotune.tune_launch(ctx, frames, A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK),
-                                budget_s=20.0, use_cache=False)
+                                budget_s=20.0, use_cache=False, tune_precision=False)
     assert best["block"] in autotune.BLOCKS and 1 <= best["refill"] <= 64
     out = ctx.render(frames)
     for a, b in zip(ref, out):
         assert np.array_equal(a["rgbf"], b["rgbf"])
-    again = autotune.tune_launch(ctx, frames, A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    again = autotune.tune_launch(ctx, frames, A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK),
+                                 tune_precision=False)
     assert again["block"] == best["block"] and again["refill"] == best["refill"]
+
+
+def test_autotuner_stage_precision_choice(tmp_path, monkeypatch):
+    """For an FP32-bulk tracer the autotuner times the FP64 stages too and
+    keeps the faster; the context then renders exactly what a render in the
+    chosen precision gives, and the cached choice is applied again."""
+    from parrelray_amd import autotune
+    monkeypatch.setattr(autotune, "CACHE", str(tmp_path / "tune.json"))
+    frames = CF.frames_c2(nframes=5, scale=0.05)
+    ctx = context()
+    best = autotune.tune_launch(ctx, frames, A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK),
+                                budget_s=20.0, use_cache=False)
+    assert best["precision"] in (A.PRR_FP32_BULK, A.PRR_FP64)
+    assert min(best["integrate_ms_fp32_stages"], best["integrate_ms_fp64_stages"]) == best["integrate_ms"]
+    out = ctx.render(frames)
+    ref_ctx = context()
+    ref_ctx.set_tracer(CF.tracer_kerr(precision=best["precision"]))
+    ref = ref_ctx.render(frames)
+    for a, b in zip(ref, out):
+        assert np.array_equal(a["rgbf"], b["rgbf"])
+    ctx2 = context()
+    again = autotune.tune_launch(ctx2, frames, A.PRR_KERR_BL, CF.tracer_kerr(precision=A.PRR_FP32_BULK))
+    assert again["precision"] == best["precision"]
+    for a, b in zip(ref, ctx2.render(frames)):
+        assert np.array_equal(a["rgbf"], b["rgbf"])
 
 
 def test_invalid_inputs_are_rejected_without_touching_the_gpu_state():
