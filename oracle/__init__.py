@@ -36,7 +36,7 @@ def lib():
         L.orc_scene_set_spheres.argtypes = [C.c_void_p, C.POINTER(A.Sphere), C.c_int]
         L.orc_render_rays.argtypes = [C.c_void_p, C.POINTER(A.Frame), C.c_void_p, C.c_int64, C.c_double, C.c_int,
                                       C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
-                                      C.c_int, _dp]
+                                      C.c_int, _dp, C.c_void_p]
         L.orc_trace.argtypes = [C.c_void_p, C.c_void_p, C.c_int64, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
                                 C.c_int]
         L.orc_render_frame.argtypes = [C.c_void_p, C.POINTER(A.Frame), C.c_void_p, C.c_void_p, C.c_void_p, C.c_int]
@@ -108,8 +108,9 @@ class Scene:
             arr[i].tex = tex
         lib().orc_scene_set_spheres(self.h, arr, len(spheres))
 
-    def render_rays(self, frame, ray_ids=None, delta=0.0, axis=0, nthreads=0, want_state=False):
-        """Per-ray results: dict of cls, endpoint, steps, attempts, color (+ state8), seconds."""
+    def render_rays(self, frame, ray_ids=None, delta=0.0, axis=0, nthreads=0, want_state=False, want_color64=False):
+        """Per-ray results: dict of cls, endpoint, steps, attempts, color (+ state8, + color64: the
+        colours through the FP64 colour path), seconds."""
         if ray_ids is None:
             n = frame.nrays
             ids = None
@@ -120,10 +121,13 @@ class Scene:
                    attempts=np.empty(n, np.int32), color=np.empty((n, 3), np.float32))
         if want_state:
             out["state8"] = np.empty((n, 8))
+        if want_color64:
+            out["color64"] = np.empty((n, 3))
         sec = C.c_double(0.0)
         lib().orc_render_rays(self.h, C.byref(frame), _ptr(ids), n, delta, axis, _ptr(out["cls"]),
                               _ptr(out["endpoint"]), _ptr(out["steps"]), _ptr(out["attempts"]),
-                              _ptr(out["color"]), _ptr(out.get("state8")), nthreads, C.byref(sec))
+                              _ptr(out["color"]), _ptr(out.get("state8")), nthreads, C.byref(sec),
+                              _ptr(out.get("color64")))
         out["seconds"] = sec.value
         return out
 

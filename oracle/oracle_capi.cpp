@@ -43,7 +43,7 @@ void orc_scene_set_spheres(void *s, const prr_sphere *sph, int n) {
 // Returns 0; *seconds receives the wall time of integration + shading.
 int orc_render_rays(void *scene, const prr_frame *f, const int64_t *ray_ids, int64_t n, double delta, int axis,
                     int8_t *cls, double *endpoint, int32_t *steps, int32_t *attempts, float *color, double *state8,
-                    int nthreads, double *seconds) {
+                    int nthreads, double *seconds, double *color64) {
     const Scene &S = *static_cast<Scene *>(scene);
     const Basis B = camera_basis(f->alpha, f->beta);
     const int64_t total = int64_t(f->width) * f->height * f->samples;
@@ -65,6 +65,11 @@ int orc_render_rays(void *scene, const prr_frame *f, const int64_t *ray_ids, int
             color[3 * i] = R.color.r;
             color[3 * i + 1] = R.color.g;
             color[3 * i + 2] = R.color.b;
+        }
+        if (color64) {
+            color64[3 * i] = R.color64.r;
+            color64[3 * i + 1] = R.color64.g;
+            color64[3 * i + 2] = R.color64.b;
         }
         if (state8)
             for (int c = 0; c < 4; ++c) {
@@ -108,7 +113,7 @@ int orc_render_frame(void *scene, const prr_frame *f, float *ray_color, float *i
     const int64_t total = int64_t(f->width) * f->height * f->samples;
     std::vector<float> col(size_t(total) * 3);
     orc_render_rays(scene, f, nullptr, total, 0.0, 0, nullptr, nullptr, nullptr, nullptr, col.data(), nullptr,
-                    nthreads, nullptr);
+                    nthreads, nullptr, nullptr);
     if (ray_color) std::copy(col.begin(), col.end(), ray_color);
     std::vector<float> ss(size_t(f->width) * f->height * 3);
     ssaa_combine(col.data(), f->width, f->height, f->samples, ss.data());

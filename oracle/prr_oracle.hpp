@@ -476,7 +476,8 @@ inline void init_ray(int kind, double M, double a, V3 pos, V3 d, double x[4], do
 // ------------------------------------------------------------------ textures
 struct Texture {
     int w = 0, h = 0, filter = PRR_FILTER_SQAVG;
-    std::vector<float> rgb; // linear, byte/255 (cudamaths/image.cu:15)
+    std::vector<float> rgb;   // linear, byte/255 (cudamaths/image.cu:15)
+    std::vector<double> rgbd; // the same texels in double (the FP64 colour path)
     RGB at(int i, int j) const {
         const float *p = &rgb[3 * (size_t(j) * w + i)];
         return RGB{p[0], p[1], p[2]};
@@ -488,7 +489,11 @@ inline void texture_from_rgb8(Texture &t, const uint8_t *rgb8, int w, int h, int
     t.h = h;
     t.filter = filter;
     t.rgb.resize(size_t(w) * h * 3);
-    for (size_t i = 0; i < t.rgb.size(); ++i) t.rgb[i] = float(rgb8[i]) / 255.0f;
+    t.rgbd.resize(t.rgb.size());
+    for (size_t i = 0; i < t.rgb.size(); ++i) {
+        t.rgb[i] = float(rgb8[i]) / 255.0f;
+        t.rgbd[i] = double(rgb8[i]) / 255.0;
+    }
 }
 
 // sqavg (cudamaths/f3.cuh:101-103)
@@ -551,6 +556,58 @@ inline RGB sphere_color(const Texture *t, V3 n) {
     return RGB{c.r * shade, c.g * shade, c.b * shade};
 }
 
+// The FP64 colour path: the same lookups (bilinear / block RMS, the sphere's
+// Lambert factor) with the texels as exact doubles byte / 255 and double
+// weights.  It serves the finite-difference references of the adjoint, whose
+// differences over steps of 1e-7 would drown in the float colours' rounding.
+struct RGBd {
+    double r, g, b;
+};
+inline RGBd texel_d(const Texture &t, int i, int j) {
+    const double *p = &t.rgbd[3 * (size_t(j) * t.w + i)];
+    return RGBd{p[0], p[1], p[2]};
+}
+inline RGBd bilinear_d(const Texture &t, double u, double v) {
+    u = u - std::floor(u);
+    v = std::min(std::max(v, 0.0), 1.0);
+    const double X = u * t.w, Y = v * t.h;
+    int i0 = int(std::floor(X)), j0 = int(std::floor(Y));
+    double fx = X - i0, fy = Y - j0;
+    if (i0 >= t.w) i0 -= t.w;
+    const int i1 = i0 + 1 >= t.w ? 0 : i0 + 1;
+    if (j0 >= t.h - 1) {
+        j0 = t.h - 1;
+        fy = 0.0;
+    }
+    const int j1 = std::min(j0 + 1, t.h - 1);
+    const RGBd c00 = texel_d(t, i0, j0), c10 = texel_d(t, i1, j0), c01 = texel_d(t, i0, j1), c11 = texel_d(t, i1, j1);
+    const double w00 = (1.0 - fx) * (1.0 - fy), w10 = fx * (1.0 - fy), w01 = (1.0 - fx) * fy, w11 = fx * fy;
+    return RGBd{c00.r * w00 + c10.r * w10 + c01.r * w01 + c11.r * w11, c00.g * w00 + c10.g * w10 + c01.g * w01 + c11.g * w11,
+                c00.b * w00 + c10.b * w10 + c01.b * w01 + c11.b * w11};
+}
+inline RGBd sky_lookup_d(const Texture &t, double theta, double phi) {
+    if (t.w == 0) return RGBd{0.0, 0.0, 0.0};
+    if (!(phi <= 2.0 * M_PI && phi >= 0.0)) return RGBd{0.5, 0.5, 0.0};
+    if (!(theta >= 0.0 && theta <= M_PI)) return RGBd{0.0, 0.5, 0.8};
+    if (t.filter == PRR_FILTER_BILINEAR) return bilinear_d(t, phi / (2.0 * M_PI), theta / M_PI);
+    const double uvx = phi / (2.0 * M_PI) * t.w, uvy = theta / M_PI * t.h;
+    int px = int(std::floor(uvx)), py = int(std::floor(uvy));
+    if (px == t.w) px -= 1;
+    if (py == t.h) py -= 1;
+    if (px < 0 || px >= t.w || py < 0 || py >= t.h) return RGBd{0.0, 0.5, 0.0};
+    if (px == t.w - 1 || py == t.h - 1) return texel_d(t, px, py);
+    const RGBd a = texel_d(t, px, py), b = texel_d(t, px, py + 1), c = texel_d(t, px + 1, py), d = texel_d(t, px + 1, py + 1);
+    auto q = [](double w, double x, double y, double z) { return std::sqrt(0.25 * (w * w + x * x + y * y + z * z)); };
+    return RGBd{q(a.r, b.r, c.r, d.r), q(a.g, b.g, c.g, d.g), q(a.b, b.b, c.b, d.b)};
+}
+inline RGBd sphere_color_d(const Texture *t, V3 n) {
+    const double th = std::acos(std::min(std::max(n.z, -1.0), 1.0));
+    const double ph = std::atan2(n.y, n.x) + M_PI;
+    const RGBd c = (t && t->w) ? bilinear_d(*t, ph / (2.0 * M_PI), th / M_PI) : RGBd{1.0, 1.0, 1.0};
+    const double shade = std::min(std::max(0.5 + 0.5 * n.z, 0.0), 1.0);
+    return RGBd{c.r * shade, c.g * shade, c.b * shade};
+}
+
 // ------------------------------------------------------------------ scene
 struct Scene {
     int metric = PRR_SCHW_KS_CART;
@@ -572,6 +629,7 @@ struct RayResult {
     double x[4], u[4];
     V3 endpoint{0, 0, 0};
     RGB color{0, 0, 0};
+    RGBd color64{0, 0, 0}; // the FP64 colour path
     int64_t steps = 0, attempts = 0;
     double lambda = 0.0;
 };
@@ -664,6 +722,7 @@ inline bool after_step(const Scene &S, double M, double a, const std::vector<prr
             R.endpoint = n;
             const int slot = sph[k].tex;
             R.color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &S.tex[slot] : nullptr, n);
+            R.color64 = sphere_color_d((slot > 0 && slot < PRR_MAX_TEXTURES) ? &S.tex[slot] : nullptr, n);
             return true;
         }
     }
@@ -964,6 +1023,7 @@ inline bool rkf_events(const Scene &S, double M, double a, const std::vector<prr
             R.endpoint = n;
             const int slot = sph[k].tex;
             R.color = sphere_color((slot > 0 && slot < PRR_MAX_TEXTURES) ? &S.tex[slot] : nullptr, n);
+            R.color64 = sphere_color_d((slot > 0 && slot < PRR_MAX_TEXTURES) ? &S.tex[slot] : nullptr, n);
             return true;
         }
     }
@@ -1103,12 +1163,14 @@ inline void shade(const Scene &S, RayResult &R) {
         // through a wormhole's throat (l < 0): the second sky (bimage2, main/base.h:168)
         const bool far_side = S.metric == PRR_ELLIS && R.x[0] < 0.0;
         R.color = sky_lookup(S.tex[far_side ? PRR_SKY2_SLOT : 0], th, ph);
+        R.color64 = sky_lookup_d(S.tex[far_side ? PRR_SKY2_SLOT : 0], th, ph);
         break;
     }
     case PRR_SPHERE: break; // colour set at the hit
     default:
         R.endpoint = cart_of(S.metric, R.x);
         R.color = RGB{0.f, 0.f, 0.f};
+        R.color64 = RGBd{0.0, 0.0, 0.0};
         break;
     }
 }
