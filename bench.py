@@ -19,7 +19,9 @@ the kernel's duration measured with HIP events on its own stream.  frac =
 FP32 as well, the rate of the non-packed FP32 instructions the kernels are compiled to.
 
 cpu_baseline: the CPU FP64 oracle (oracle/, a restatement of the reference
-solver) on a stratified row subset of the same frame, all host cores, OpenMP.
+solver) on a stratified row subset of the same frame, all host cores, OpenMP;
+beside it the same solver in FP32, the reference's own precision
+(fp32_reference_like, on a seeded sample of those rays).
 The same leg times the oracle on the whole C0 frame (the reference's
 CPU-runnable case, beside the GPU's C0 leg) and, as the checker only, runs
 the oracle at tol 1e-12 on seeded subsets of the rays the GPU legs rendered:
@@ -325,6 +327,13 @@ def cpu_leg(frame, c0=None, checks=()):
         c0["cpu"] = dict(rays_per_s=f0.nrays / r["seconds"], seconds=r["seconds"], cores=cores, kind="port",
                          sample="the whole C0 frame (65,536 rays), FP64 Euler h = 0.01")
         c0["speedup_gpu_vs_cpu"] = c0["rays_per_s"] / c0["cpu"]["rays_per_s"]
+        S0.set_fp32(True)
+        r32 = S0.render_rays(f0, ids, nthreads=cores)
+        S0.set_fp32(False)
+        c0["cpu_fp32_reference_like"] = dict(
+            rays_per_s=f0.nrays / r32["seconds"], seconds=r32["seconds"], cores=cores, kind="port",
+            class_agreement_with_fp64=float((r32["cls"] == r["cls"]).mean()),
+            sample="the whole C0 frame, Euler h = 0.01 with the state and right-hand side in FP32")
         accuracy["C0"] = {"fp64": check_rays(S0, f0, ids, r, c0.pop("_rays"), cores)}
     for name, f, tracer, ids, forms in checks:
         S = O.Scene(A.PRR_KERR_BL, 1.0, 0.0, tracer)
@@ -337,8 +346,9 @@ def cpu_leg(frame, c0=None, checks=()):
     return base, accuracy
 
 
-def cpu_baseline(frame, row_stride=16, cores=None):
-    """Oracle (FP64, Christoffel form) on every row_stride-th image row, all cores."""
+def cpu_baseline(frame, row_stride=16, cores=None, fp32_rays=96):
+    """Oracle (FP64, Christoffel form) on every row_stride-th image row, all cores; its FP32 variant
+    (the reference's precision) on fp32_rays of them."""
     import numpy as np
 
     import oracle as O
@@ -352,9 +362,25 @@ def cpu_baseline(frame, row_stride=16, cores=None):
     ids = (rows[:, None] * per_row + np.arange(per_row)[None, :]).ravel()
     cores = cores or len(os.sched_getaffinity(0))
     r = S.render_rays(frame, ids, nthreads=cores)
-    return dict(value=ids.size / r["seconds"], unit="rays/s", cores=cores, kind="port", model=cpu_model(),
+    base = dict(value=ids.size / r["seconds"], unit="rays/s", cores=cores, kind="port", model=cpu_model(),
                 sample=f"C1 rows j % {row_stride} == 0 ({ids.size} rays of {frame.nrays}), FP64 RKF45 tol 1e-8, "
                        f"{r['seconds']:.1f} s")
+    # The reference's own precision: the same solver with the state, the stages, the error norm and
+    # the controller in FP32, on a seeded sample of the same rows.  An FP32 state cannot meet
+    # rtol = atol = 1e-8 (FP32 rounding of the state is ~6e-8 of it), so most rays spin at the smallest
+    # steps until the attempt cap: the rate is reported with the share of rays that end there.
+    sub = np.sort(np.random.default_rng(32).choice(ids, size=min(fp32_rays, ids.size), replace=False))
+    r64 = {k: v[np.searchsorted(ids, sub)] for k, v in r.items() if k != "seconds"}
+    S.set_fp32(True)
+    r32 = S.render_rays(frame, sub, nthreads=cores)
+    cap = r32["attempts"] >= S.tracer.max_attempts
+    base["fp32_reference_like"] = dict(
+        value=sub.size / r32["seconds"], unit="rays/s", cores=cores, seconds=r32["seconds"],
+        sample=f"{sub.size} seeded rays of the same rows, RKF45 tol 1e-8 in FP32",
+        rays_at_attempt_cap=float(cap.mean()), mean_attempts=float(r32["attempts"].mean()),
+        mean_attempts_fp64=float(r64["attempts"].mean()),
+        class_agreement_with_fp64=float((r32["cls"] == r64["cls"]).mean()))
+    return base
 
 
 def main():

@@ -31,6 +31,7 @@ def lib():
         L.orc_scene_new.argtypes = [C.c_int, C.c_double, C.c_double, C.POINTER(A.Tracer)]
         L.orc_scene_free.argtypes = [C.c_void_p]
         L.orc_scene_set_far_escape.argtypes = [C.c_void_p, C.c_double]
+        L.orc_scene_set_fp32.argtypes = [C.c_void_p, C.c_int]
         L.orc_scene_set_tracer.argtypes = [C.c_void_p, C.POINTER(A.Tracer)]
         L.orc_scene_set_texture.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_int, C.c_int, C.c_int]
         L.orc_scene_set_spheres.argtypes = [C.c_void_p, C.POINTER(A.Sphere), C.c_int]
@@ -90,6 +91,12 @@ class Scene:
     def set_far_escape(self, r):
         """Escape radius beyond a wormhole's throat (rout2); 0: the tracer's r_escape."""
         lib().orc_scene_set_far_escape(self.h, r)
+
+    def set_fp32(self, on=True):
+        """Integrate in FP32 (state, stages, error norm, controller), the
+        reference's own precision; the event tests take an FP64 copy of the
+        state.  For the CPU baseline's reference-like timing only."""
+        lib().orc_scene_set_fp32(self.h, int(bool(on)))
 
     def set_tracer(self, tracer):
         self.tracer = tracer

@@ -11,6 +11,11 @@ using namespace prro;
 
 namespace {
 int threads_or_default(int n) { return n > 0 ? n : omp_get_max_threads(); }
+// Rays per OpenMP chunk: 64 (the schedule(dynamic, 64) of the CPU baseline plan)
+// unless that leaves threads idle on a small ray set.
+int chunk_for(int64_t n, int threads) {
+    return int(std::max<int64_t>(1, std::min<int64_t>(64, n / (8 * int64_t(threads)))));
+}
 } // namespace
 
 extern "C" {
@@ -27,6 +32,9 @@ void *orc_scene_new(int metric, double M, double a, const prr_tracer *tr) {
 void orc_scene_free(void *s) { delete static_cast<Scene *>(s); }
 
 void orc_scene_set_far_escape(void *s, double r) { static_cast<Scene *>(s)->r_esc2 = r; }
+
+// 1: integrate in FP32 (the reference-like precision of the CPU baseline); 0: FP64.
+void orc_scene_set_fp32(void *s, int on) { static_cast<Scene *>(s)->fp32 = on != 0; }
 
 void orc_scene_set_tracer(void *s, const prr_tracer *tr) { static_cast<Scene *>(s)->tr = *tr; }
 
@@ -48,8 +56,9 @@ int orc_render_rays(void *scene, const prr_frame *f, const int64_t *ray_ids, int
     const Basis B = camera_basis(f->alpha, f->beta);
     const int64_t total = int64_t(f->width) * f->height * f->samples;
     if (!ray_ids) n = total;
+    const int threads = threads_or_default(nthreads);
     const auto t0 = std::chrono::steady_clock::now();
-#pragma omp parallel for schedule(dynamic, 64) num_threads(threads_or_default(nthreads))
+#pragma omp parallel for schedule(dynamic, chunk_for(n, threads)) num_threads(threads)
     for (int64_t i = 0; i < n; ++i) {
         const int64_t ray = ray_ids ? ray_ids[i] : i;
         const RayResult R = render_ray(S, *f, B, ray, delta, axis);

@@ -164,16 +164,18 @@ inline void metric_ellis(const double x[4], double b, double g[16]) {
     g[15] = -1.0;
 }
 
-// metrics/kerr.cuh:77-100 (Boyer-Lindquist, order r, theta, phi, t)
-inline void metric_kerr(const double x[4], double M, double a, double g[16]) {
-    const double rs = 2.0 * M, r = x[0], s = std::sin(x[1]), c = std::cos(x[1]);
-    const double s2 = s * s, Si = r * r + a * a * c * c, Dl = r * r - rs * r + a * a;
-    std::fill(g, g + 16, 0.0);
+// metrics/kerr.cuh:77-100 (Boyer-Lindquist, order r, theta, phi, t).  R is
+// double for the oracle; float gives the reference-like FP32 variant that
+// the CPU baseline times beside it (trace<float>).
+template <class R> inline void metric_kerr(const R x[4], R M, R a, R g[16]) {
+    const R rs = R(2.0) * M, r = x[0], s = std::sin(x[1]), c = std::cos(x[1]);
+    const R s2 = s * s, Si = r * r + a * a * c * c, Dl = r * r - rs * r + a * a;
+    std::fill(g, g + 16, R(0.0));
     g[0] = Si / Dl;
     g[5] = Si;
     g[10] = (r * r + a * a + rs * r * a * a / Si * s2) * s2;
     g[11] = g[14] = -rs * r * a * s2 / Si;
-    g[15] = -(1.0 - rs * r / Si);
+    g[15] = -(R(1.0) - rs * r / Si);
 }
 
 inline void metric(int kind, double M, double a, const double x[4], double g[16]) {
@@ -189,12 +191,12 @@ inline void metric(int kind, double M, double a, const double x[4], double g[16]
 // Analytic partial derivatives dg[k][16] = d g / d x^k of the Kerr BL metric
 // (only k = 0 (r) and k = 1 (theta) are non-zero).  Textbook GR: the
 // reference's getGeod for Kerr is a stub (metrics/kerr.cuh:72-74).
-inline void dmetric_kerr(const double x[4], double M, double a, double dg[4][16]) {
-    const double r = x[0], s = std::sin(x[1]), c = std::cos(x[1]);
-    const double a2 = a * a, s2 = s * s;
-    const double Si = r * r + a2 * c * c, Dl = r * r - 2.0 * M * r + a2;
-    const double Si_r = 2.0 * r, Si_t = -2.0 * a2 * c * s, Dl_r = 2.0 * r - 2.0 * M;
-    for (int k = 0; k < 4; ++k) std::fill(dg[k], dg[k] + 16, 0.0);
+template <class R> inline void dmetric_kerr(const R x[4], R M, R a, R dg[4][16]) {
+    const R r = x[0], s = std::sin(x[1]), c = std::cos(x[1]);
+    const R a2 = a * a, s2 = s * s;
+    const R Si = r * r + a2 * c * c, Dl = r * r - R(2.0) * M * r + a2;
+    const R Si_r = R(2.0) * r, Si_t = R(-2.0) * a2 * c * s, Dl_r = R(2.0) * r - R(2.0) * M;
+    for (int k = 0; k < 4; ++k) std::fill(dg[k], dg[k] + 16, R(0.0));
     // g_rr = Si / Dl
     dg[0][0] = (Si_r * Dl - Si * Dl_r) / (Dl * Dl);
     dg[1][0] = Si_t / Dl;
@@ -202,90 +204,90 @@ inline void dmetric_kerr(const double x[4], double M, double a, double dg[4][16]
     dg[0][5] = Si_r;
     dg[1][5] = Si_t;
     // A = 2 M r a^2 s^2 / Si ; g_phph = (r^2 + a^2 + A) s^2
-    const double A = 2.0 * M * r * a2 * s2 / Si;
-    const double A_r = 2.0 * M * a2 * s2 * (Si - r * Si_r) / (Si * Si);
-    const double A_t = 2.0 * M * r * a2 * (2.0 * s * c * Si - s2 * Si_t) / (Si * Si);
-    dg[0][10] = (2.0 * r + A_r) * s2;
-    dg[1][10] = A_t * s2 + (r * r + a2 + A) * 2.0 * s * c;
+    const R A = R(2.0) * M * r * a2 * s2 / Si;
+    const R A_r = R(2.0) * M * a2 * s2 * (Si - r * Si_r) / (Si * Si);
+    const R A_t = R(2.0) * M * r * a2 * (R(2.0) * s * c * Si - s2 * Si_t) / (Si * Si);
+    dg[0][10] = (R(2.0) * r + A_r) * s2;
+    dg[1][10] = A_t * s2 + (r * r + a2 + A) * R(2.0) * s * c;
     // g_tph = -2 M a r s^2 / Si
-    dg[0][11] = dg[0][14] = -2.0 * M * a * s2 * (Si - r * Si_r) / (Si * Si);
-    dg[1][11] = dg[1][14] = -2.0 * M * a * r * (2.0 * s * c * Si - s2 * Si_t) / (Si * Si);
+    dg[0][11] = dg[0][14] = R(-2.0) * M * a * s2 * (Si - r * Si_r) / (Si * Si);
+    dg[1][11] = dg[1][14] = R(-2.0) * M * a * r * (R(2.0) * s * c * Si - s2 * Si_t) / (Si * Si);
     // g_tt = -1 + 2 M r / Si
-    dg[0][15] = 2.0 * M * (Si - r * Si_r) / (Si * Si);
-    dg[1][15] = -2.0 * M * r * Si_t / (Si * Si);
+    dg[0][15] = R(2.0) * M * (Si - r * Si_r) / (Si * Si);
+    dg[1][15] = R(-2.0) * M * r * Si_t / (Si * Si);
 }
 
 // Inverse of a metric whose only off-diagonal block is (phi, t) = (2, 3).
-inline void inverse_block(const double g[16], double gi[16]) {
-    std::fill(gi, gi + 16, 0.0);
-    gi[0] = 1.0 / g[0];
-    gi[5] = 1.0 / g[5];
-    const double det = g[10] * g[15] - g[11] * g[14];
+template <class R> inline void inverse_block(const R g[16], R gi[16]) {
+    std::fill(gi, gi + 16, R(0.0));
+    gi[0] = R(1.0) / g[0];
+    gi[5] = R(1.0) / g[5];
+    const R det = g[10] * g[15] - g[11] * g[14];
     gi[10] = g[15] / det;
     gi[15] = g[10] / det;
     gi[11] = gi[14] = -g[11] / det;
 }
 
 // Christoffel symbols Gamma^al_{mu nu} = 1/2 g^{al be} (d_mu g_{be nu} + d_nu g_{be mu} - d_be g_{mu nu}).
-inline void christoffel_kerr(const double x[4], double M, double a, double G[4][4][4]) {
-    double g[16], gi[16], dg[4][16];
+template <class R> inline void christoffel_kerr(const R x[4], R M, R a, R G[4][4][4]) {
+    R g[16], gi[16], dg[4][16];
     metric_kerr(x, M, a, g);
     inverse_block(g, gi);
     dmetric_kerr(x, M, a, dg);
     for (int al = 0; al < 4; ++al)
         for (int mu = 0; mu < 4; ++mu)
             for (int nu = 0; nu < 4; ++nu) {
-                double acc = 0.0;
+                R acc = 0.0;
                 for (int be = 0; be < 4; ++be)
                     acc += gi[4 * al + be] * (dg[mu][4 * be + nu] + dg[nu][4 * be + mu] - dg[be][4 * mu + nu]);
-                G[al][mu][nu] = 0.5 * acc;
+                G[al][mu][nu] = R(0.5) * acc;
             }
 }
 
 // metrics/schwarzschild.cuh:90-119, the reference's expanded form.
-inline void geod_ks(const double p[4], const double u[4], double M, double du[4]) {
-    const double rs = 2.0 * M;
-    const double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
-    const double x2 = p[0] * p[0], y2 = p[1] * p[1], z2 = p[2] * p[2];
-    const double cross_terms = p[0] * p[1] * u[0] * u[1] + p[0] * p[2] * u[0] * u[2] + p[1] * p[2] * u[1] * u[2];
-    const double pu = p[0] * u[0] + p[1] * u[1] + p[2] * u[2];
-    double Phi = (r * (x2 - 2.0 * y2 - 2.0 * z2) + rs * x2) * u[0] * u[0] +
-                 (r * (-2.0 * x2 + y2 - 2.0 * z2) + rs * y2) * u[1] * u[1] +
-                 (r * (-2.0 * x2 - 2.0 * y2 + z2) + rs * z2) * u[2] * u[2] + 2.0 * (3.0 * r + rs) * cross_terms +
-                 r * r * (rs - r) * u[3] * u[3] + 2.0 * r * rs * u[3] * pu;
-    Phi *= rs / (2.0 * std::pow(r, 6));
+template <class R> inline void geod_ks(const R p[4], const R u[4], R M, R du[4]) {
+    const R rs = R(2.0) * M;
+    const R r = std::sqrt(p[0] * p[0] + p[1] * p[1] + p[2] * p[2]);
+    const R x2 = p[0] * p[0], y2 = p[1] * p[1], z2 = p[2] * p[2];
+    const R cross_terms = p[0] * p[1] * u[0] * u[1] + p[0] * p[2] * u[0] * u[2] + p[1] * p[2] * u[1] * u[2];
+    const R pu = p[0] * u[0] + p[1] * u[1] + p[2] * u[2];
+    R Phi = (r * (x2 - R(2.0) * y2 - R(2.0) * z2) + rs * x2) * u[0] * u[0] +
+            (r * (R(-2.0) * x2 + y2 - R(2.0) * z2) + rs * y2) * u[1] * u[1] +
+            (r * (R(-2.0) * x2 - R(2.0) * y2 + z2) + rs * z2) * u[2] * u[2] + R(2.0) * (R(3.0) * r + rs) * cross_terms +
+            r * r * (rs - r) * u[3] * u[3] + R(2.0) * r * rs * u[3] * pu;
+    Phi *= rs / (R(2.0) * R(std::pow(r, R(6))));
     du[0] = p[0] * Phi;
     du[1] = p[1] * Phi;
     du[2] = p[2] * Phi;
-    double w = (2.0 * r * (-x2 + y2 + z2) - rs * x2) * u[0] * u[0] + (2.0 * r * (x2 - y2 + z2) - rs * y2) * u[1] * u[1] +
-               (2.0 * r * (x2 + y2 - z2) - rs * z2) * u[2] * u[2] - 2.0 * (4.0 * r + rs) * cross_terms -
-               r * r * rs * u[3] * u[3] - 2.0 * r * (r + rs) * u[3] * pu;
-    du[3] = w * rs / (2.0 * std::pow(r, 5));
+    R w = (R(2.0) * r * (-x2 + y2 + z2) - rs * x2) * u[0] * u[0] + (R(2.0) * r * (x2 - y2 + z2) - rs * y2) * u[1] * u[1] +
+          (R(2.0) * r * (x2 + y2 - z2) - rs * z2) * u[2] * u[2] - R(2.0) * (R(4.0) * r + rs) * cross_terms -
+          r * r * rs * u[3] * u[3] - R(2.0) * r * (r + rs) * u[3] * pu;
+    du[3] = w * rs / (R(2.0) * R(std::pow(r, R(5))));
 }
 
 // "metrics/schwarzschild - sph.cuh":94-115, including the tan/sin clamp at :98-101.
-inline void geod_sph(const double x[4], const double u[4], double M, double q, double du[4]) {
-    const double rs = 2.0 * M, r = x[0];
-    double tanth = std::tan(x[1]), sinth = std::sin(x[1]);
-    if (std::fabs(tanth) < 1e-4) {
-        tanth = 1e-4;
-        sinth = 1e-4;
+template <class R> inline void geod_sph(const R x[4], const R u[4], R M, R q, R du[4]) {
+    const R rs = R(2.0) * M, r = x[0];
+    R tanth = std::tan(x[1]), sinth = std::sin(x[1]);
+    if (std::fabs(tanth) < R(1e-4)) {
+        tanth = R(1e-4);
+        sinth = R(1e-4);
     }
-    if (q != 0.0) {
+    if (q != R(0.0)) {
         // static, spherically symmetric: Gamma^r_tt = f f'/2, Gamma^r_rr = -f'/(2f),
         // Gamma^r_thth = -r f, Gamma^t_tr = f'/(2f), f' = r_s/r^2 - 2 q^2/r^3
-        const double f = 1.0 - rs / r + q * q / (r * r), fp = rs / (r * r) - 2.0 * q * q / (r * r * r);
-        du[0] = -0.5 * f * fp * u[3] * u[3] + 0.5 * fp / f * u[0] * u[0] +
+        const R f = R(1.0) - rs / r + q * q / (r * r), fp = rs / (r * r) - R(2.0) * q * q / (r * r * r);
+        du[0] = R(-0.5) * f * fp * u[3] * u[3] + R(0.5) * fp / f * u[0] * u[0] +
                 r * f * (u[1] * u[1] + sinth * sinth * u[2] * u[2]);
-        du[1] = -2.0 / r * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
-        du[2] = -2.0 / r * u[0] * u[2] - 2.0 / tanth * u[1] * u[2];
+        du[1] = R(-2.0) / r * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
+        du[2] = R(-2.0) / r * u[0] * u[2] - R(2.0) / tanth * u[1] * u[2];
         du[3] = -fp / f * u[0] * u[3];
         return;
     }
-    du[0] = -0.5 * rs * (r - rs) / (r * r * r) * u[3] * u[3] + 0.5 * rs / (r * r - r * rs) * u[0] * u[0] +
+    du[0] = R(-0.5) * rs * (r - rs) / (r * r * r) * u[3] * u[3] + R(0.5) * rs / (r * r - r * rs) * u[0] * u[0] +
             (r - rs) * (u[1] * u[1] + sinth * sinth * u[2] * u[2]);
-    du[1] = -2.0 / r * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
-    du[2] = -2.0 / r * u[0] * u[2] - 2.0 / tanth * u[1] * u[2];
+    du[1] = R(-2.0) / r * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
+    du[2] = R(-2.0) / r * u[0] * u[2] - R(2.0) / tanth * u[1] * u[2];
     du[3] = -rs / (r * r - r * rs) * u[0] * u[3];
 }
 
@@ -294,25 +296,25 @@ inline void geod_sph(const double x[4], const double u[4], double M, double q, d
 // Gamma^th_lth = Gamma^ph_lph = l / R^2, Gamma^th_phph = -sin th cos th,
 // Gamma^ph_thph = cot th; tan and sin are clamped to magnitude 1e-4 near the
 // axis with their signs kept (rays cross the axis, axis_post_step).
-inline void geod_ellis(const double x[4], const double u[4], double b, double du[4]) {
-    const double l = x[0];
-    double tanth = std::tan(x[1]), sinth = std::sin(x[1]);
-    if (std::fabs(tanth) < 1e-4) {
-        tanth = std::copysign(1e-4, tanth);
-        sinth = std::copysign(1e-4, sinth);
+template <class R> inline void geod_ellis(const R x[4], const R u[4], R b, R du[4]) {
+    const R l = x[0];
+    R tanth = std::tan(x[1]), sinth = std::sin(x[1]);
+    if (std::fabs(tanth) < R(1e-4)) {
+        tanth = std::copysign(R(1e-4), tanth);
+        sinth = std::copysign(R(1e-4), sinth);
     }
-    const double k = l / (b * b + l * l);
+    const R k = l / (b * b + l * l);
     du[0] = l * (u[1] * u[1] + sinth * sinth * u[2] * u[2]);
-    du[1] = -2.0 * k * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
-    du[2] = -2.0 * k * u[0] * u[2] - 2.0 / tanth * u[1] * u[2];
+    du[1] = R(-2.0) * k * u[0] * u[1] + sinth * std::cos(x[1]) * u[2] * u[2];
+    du[2] = R(-2.0) * k * u[0] * u[2] - R(2.0) / tanth * u[1] * u[2];
     du[3] = 0.0;
 }
 
-inline void geod_kerr(const double x[4], const double u[4], double M, double a, double du[4]) {
-    double G[4][4][4];
+template <class R> inline void geod_kerr(const R x[4], const R u[4], R M, R a, R du[4]) {
+    R G[4][4][4];
     christoffel_kerr(x, M, a, G);
     for (int al = 0; al < 4; ++al) {
-        double acc = 0.0;
+        R acc = 0.0;
         for (int mu = 0; mu < 4; ++mu)
             for (int nu = 0; nu < 4; ++nu) acc += G[al][mu][nu] * u[mu] * u[nu];
         du[al] = -acc;
@@ -320,7 +322,7 @@ inline void geod_kerr(const double x[4], const double u[4], double M, double a, 
 }
 
 // getGeod of the selected metric (main/metric.h:12 selects it at compile time).
-inline void geod(int kind, double M, double a, const double x[4], const double u[4], double du[4]) {
+template <class R> inline void geod(int kind, R M, R a, const R x[4], const R u[4], R du[4]) {
     switch (kind) {
     case PRR_SCHW_KS_CART: geod_ks(x, u, M, du); break;
     case PRR_SCHW_SPH: geod_sph(x, u, M, a, du); break; // a: the charge q
@@ -616,6 +618,7 @@ struct Scene {
     prr_tracer tr{};
     std::vector<prr_sphere> spheres;
     Texture tex[PRR_MAX_TEXTURES];
+    bool fp32 = false; // the reference-like FP32 integration (trace_t<float>), CPU baseline only
 };
 
 // Escape radius on the state's side of a wormhole's throat: the reference's
@@ -795,38 +798,39 @@ inline const Tableau &tableau_of(int solver) {
 // One Runge-Kutta attempt of size h from y = (x, u) with the scene's solver:
 // the propagated state y4 and the scaled error norm (kernels/solver.cuh:142-184
 // with D2, D3, for RKF45).
-inline double rkf_attempt(const Scene &S, double M, double a, const double x[4], const double u[4], double h,
-                          double y4[8]) {
+// Real = float: the reference-like FP32 variant (stages, state, error norm).
+template <class Real>
+inline Real rkf_attempt(const Scene &S, Real M, Real a, const Real x[4], const Real u[4], Real h, Real y4[8]) {
     const Tableau &TB = tableau_of(S.tr.solver);
     const prr_tracer &T = S.tr;
-    double y[8], k[7][8];
+    Real y[8], k[7][8];
     for (int i = 0; i < 4; ++i) {
         y[i] = x[i];
         y[4 + i] = u[i];
     }
     for (int st = 0; st < TB.S; ++st) {
-        double ys[8];
+        Real ys[8];
         for (int i = 0; i < 8; ++i) {
-            double acc = y[i];
-            for (int j = 0; j < st; ++j) acc += TB.A[st][j] * k[j][i];
+            Real acc = y[i];
+            for (int j = 0; j < st; ++j) acc += Real(TB.A[st][j]) * k[j][i];
             ys[i] = acc;
         }
-        double du[4];
+        Real du[4];
         geod(S.metric, M, a, ys, ys + 4, du);
         for (int i = 0; i < 4; ++i) {
             k[st][i] = h * ys[4 + i]; // x' = u (D2)
             k[st][4 + i] = h * du[i];
         }
     }
-    double errn = 0.0;
+    Real errn = 0.0;
     for (int i = 0; i < 8; ++i) {
-        double s4 = y[i], s5 = y[i];
+        Real s4 = y[i], s5 = y[i];
         for (int st = 0; st < TB.S; ++st) {
-            s4 += TB.b[st] * k[st][i];
-            s5 += TB.bh[st] * k[st][i];
+            s4 += Real(TB.b[st]) * k[st][i];
+            s5 += Real(TB.bh[st]) * k[st][i];
         }
         y4[i] = s4;
-        const double sc = T.atol + T.rtol * std::max(std::fabs(y[i]), std::fabs(s4));
+        const Real sc = Real(T.atol) + Real(T.rtol) * std::max(std::fabs(y[i]), std::fabs(s4));
         if (s5 != s4) errn = std::max(errn, std::fabs(s5 - s4) / sc);
     }
     return errn;
@@ -1050,40 +1054,67 @@ inline bool rkf_events(const Scene &S, double M, double a, const std::vector<prr
 
 // trace_Euler (kernels/solver.cuh:68-116) and trace_RKF (:142-220) on the
 // 8-component state y = (x, u), with the resolutions listed at the top.
-inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr_sphere> &sph, const double x0[4],
-                       const double u0[4]) {
+// The integration loop.  Real = double is the oracle.  Real = float is the
+// reference-like FP32 variant that the CPU baseline times beside it: state,
+// stages, error norm and controller in FP32 as in kernels/solver.cuh:68-220;
+// the per-step event tests and locations (which run once per accepted step)
+// take an FP64 copy of the state.
+template <class Real>
+inline RayResult trace_t(const Scene &S, double M, double a, const std::vector<prr_sphere> &sph, const double x0[4],
+                         const double u0[4]) {
     RayResult R;
-    double x[4], u[4];
-    std::memcpy(x, x0, sizeof x);
-    std::memcpy(u, u0, sizeof u);
+    Real x[4], u[4];
+    double xd[4], ud[4]; // FP64 copy of the state for the event tests
+    for (int i = 0; i < 4; ++i) {
+        x[i] = Real(x0[i]);
+        u[i] = Real(u0[i]);
+    }
+    auto to_d = [&]() {
+        for (int i = 0; i < 4; ++i) {
+            xd[i] = double(x[i]);
+            ud[i] = double(u[i]);
+        }
+    };
+    auto from_d = [&]() {
+        for (int i = 0; i < 4; ++i) {
+            x[i] = Real(xd[i]);
+            u[i] = Real(ud[i]);
+        }
+    };
     const prr_tracer &T = S.tr;
+    const Real Mr = Real(M), ar = Real(a);
     auto finish = [&](int cls) {
         R.cls = cls;
-        std::memcpy(R.x, x, sizeof x);
-        std::memcpy(R.u, u, sizeof u);
+        to_d();
+        std::memcpy(R.x, xd, sizeof xd);
+        std::memcpy(R.u, ud, sizeof ud);
         return R;
     };
-    if (!finite8(x, u)) return finish(PRR_NAN);
+    to_d();
+    if (!finite8(xd, ud)) return finish(PRR_NAN);
     if (T.solver == PRR_EULER) {
-        const double h = T.h;
+        const Real h = Real(T.h);
         while (R.steps < T.max_steps && R.lambda < T.lambda_max) {
-            double du[4];
-            geod(S.metric, M, a, x, u, du);
+            Real du[4];
+            geod(S.metric, Mr, ar, x, u, du);
             if (!(std::isfinite(du[0]) && std::isfinite(du[1]) && std::isfinite(du[2]) && std::isfinite(du[3])))
                 return finish(PRR_NAN);
             double xo[4];
-            std::memcpy(xo, x, sizeof x);
+            for (int i = 0; i < 4; ++i) xo[i] = double(x[i]);
             for (int i = 0; i < 4; ++i) {
                 x[i] += u[i] * h;
                 u[i] += du[i] * h;
             }
             R.steps++;
             R.attempts++;
-            R.lambda += h;
-            if (!finite8(x, u)) return finish(PRR_NAN);
-            if (after_step(S, M, a, sph, xo, x, u, R)) {
-                std::memcpy(R.x, x, sizeof x);
-                std::memcpy(R.u, u, sizeof u);
+            R.lambda += T.h;
+            to_d();
+            if (!finite8(xd, ud)) return finish(PRR_NAN);
+            const bool stop = after_step(S, M, a, sph, xo, xd, ud, R);
+            from_d();
+            if (stop) {
+                std::memcpy(R.x, xd, sizeof xd);
+                std::memcpy(R.u, ud, sizeof ud);
                 return R;
             }
         }
@@ -1091,41 +1122,52 @@ inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr
     }
     // RKF45 (and Dormand-Prince 5(4) with the same controller; RK4: fixed h, no rejection)
     const bool adaptive = T.solver != PRR_RK4;
-    double h = T.h;
+    const Real hmin = Real(T.hmin), hmax = Real(T.hmax);
+    Real h = Real(T.h);
     while (R.steps < T.max_steps && R.lambda < T.lambda_max) {
         if (R.attempts >= T.max_attempts) return finish(PRR_STEP_CAP);
         R.attempts++;
-        double y4[8];
-        const double errn = rkf_attempt(S, M, a, x, u, h, y4);
+        Real y4[8];
+        const Real errn = rkf_attempt(S, Mr, ar, x, u, h, y4);
         // _h = clamp(0.9 h (eps/maxerr)^0.2, hmin, hmax) (kernels/solver.cuh:175-176)
-        double hn = h;
+        Real hn = h;
         if (adaptive) {
-            hn = 0.9 * h * std::pow(errn, -0.2);
-            hn = std::fmin(std::fmax(hn, T.hmin), T.hmax);
+            hn = Real(0.9) * h * std::pow(errn, Real(-0.2));
+            hn = std::fmin(std::fmax(hn, hmin), hmax);
         }
-        if (adaptive && errn > 1.0 && h != T.hmin) { // reject (:185-188)
+        if (adaptive && errn > Real(1.0) && h != hmin) { // reject (:185-188)
             h = hn;
             continue;
         }
         double xo[4], uo[4];
-        std::memcpy(xo, x, sizeof x);
-        std::memcpy(uo, u, sizeof u);
+        for (int i = 0; i < 4; ++i) {
+            xo[i] = double(x[i]);
+            uo[i] = double(u[i]);
+        }
         for (int i = 0; i < 4; ++i) {
             x[i] = y4[i];
             u[i] = y4[4 + i];
         }
-        if (!finite8(x, u)) return finish(PRR_NAN);
+        to_d();
+        if (!finite8(xd, ud)) return finish(PRR_NAN);
         R.steps++;
-        R.lambda += h;
-        const double hs = h;
+        R.lambda += double(h);
+        const Real hs = h;
         h = hn;
-        if (rkf_events(S, M, a, sph, xo, uo, hs, x, u, R)) {
-            std::memcpy(R.x, x, sizeof x);
-            std::memcpy(R.u, u, sizeof u);
+        const bool stop = rkf_events(S, M, a, sph, xo, uo, double(hs), xd, ud, R);
+        from_d();
+        if (stop) {
+            std::memcpy(R.x, xd, sizeof xd);
+            std::memcpy(R.u, ud, sizeof ud);
             return R;
         }
     }
     return finish(PRR_STEP_CAP);
+}
+
+inline RayResult trace(const Scene &S, double M, double a, const std::vector<prr_sphere> &sph, const double x0[4],
+                       const double u0[4]) {
+    return S.fp32 ? trace_t<float>(S, M, a, sph, x0, u0) : trace_t<double>(S, M, a, sph, x0, u0);
 }
 
 // Escape direction in Cartesian axes: the spatial velocity for Cartesian
