@@ -327,8 +327,11 @@ struct Kerr {
     static constexpr bool mino = true;
 
     // Coefficients in any scalar type (double, or the duals of the adjoint).
+    // Contraction is off so that every kernel that computes them (the lane at
+    // its ray's load, the adjoint's step transpose) gets the same bits.
     template <class T>
     PRR_HD static KerrCoef<T> coef_of(double M, const T &a, const T &E, const T &L, const T &Q) {
+#pragma clang fp contract(off)
         const T K = (L - a * E) * (L - a * E) + Q, A = a * (a * E - L);
         KerrCoef<T> k;
         k.c1 = T(2.0) * E * A - K;
@@ -343,9 +346,11 @@ struct Kerr {
         k.L = L;
         return k;
     }
-    // Per-attempt coefficients: computed in FP64, then rounded to T.
+    // Coefficients of a ray: computed in FP64, then rounded to T.
     template <class T> PRR_HD static KerrCoef<T> coef(const Consts<double> &c) {
-        const KerrCoef<double> d = coef_of<double>(c.M, c.a, c.E, c.L, c.Q);
+        return cast<T>(coef_of<double>(c.M, c.a, c.E, c.L, c.Q));
+    }
+    template <class T> PRR_HD static KerrCoef<T> cast(const KerrCoef<double> &d) {
         return KerrCoef<T>{T(d.c1), T(d.c2), T(d.c3), T(d.d1), T(d.d2), T(d.pa), T(d.pb), T(d.m2), T(d.a2), T(d.L)};
     }
 

@@ -128,6 +128,16 @@ constexpr double kAxisFP64 = 0.05;
 // Largest turn of a Kerr step, in units of the ray's angular distance from the polar axis.
 constexpr double kAxisStep = 0.25;
 
+// Kerr's right-hand-side coefficients of the lane's ray, computed once when
+// the lane takes the ray instead of once per attempt (other metrics: none).
+template <bool MINO> struct LaneCoef {
+    PRR_HD void set(const Consts<double> &) {}
+};
+template <> struct LaneCoef<true> {
+    KerrCoef<double> k;
+    PRR_HD void set(const Consts<double> &c) { k = Kerr::coef<double>(c); }
+};
+
 // Lane state of one ray.
 // While the lane locates an escape (MODE_LOCATE), three fields that the ray
 // no longer needs hold the location's state, so that it stays in registers:
@@ -136,6 +146,7 @@ constexpr double kAxisStep = 0.25;
 template <class Mt> struct Lane {
     double y[Mt::N];
     Consts<double> cs;
+    LaneCoef<Mt::mino> kc;
     double h;   // next step size in the integration variable (Mino time for Kerr)
     double lam; // affine parameter
     int32_t steps, attempts, att64;
@@ -312,10 +323,9 @@ PRR_HD bool wave_any(bool c) {
 // Six RK stages in precision T from the FP64 state y (plain or FP32 form of
 // the 8-component metrics, and Kerr's FP64 form); the 4th-order increment d4
 // and the error vector e in FP64.
-template <class Mt, class T, class Tab = Fehl>
-PRR_HD void rkf_stages(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
+template <class Mt, class T, class Tab = Fehl, class C>
+PRR_HD void rkf_stages_c(const double *y, double h, const C &c, double *d4, double *e) {
     constexpr int N = Mt::N, S = Tab::S;
-    const auto c = Mt::template coef<T>(cd);
     const T hT = T(h);
     T yT[N];
 #pragma unroll
@@ -351,6 +361,10 @@ PRR_HD void rkf_stages(const double *y, double h, const Consts<double> &cd, doub
         e[i] = ae;
     }
 }
+template <class Mt, class T, class Tab = Fehl>
+PRR_HD void rkf_stages(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
+    rkf_stages_c<Mt, T, Tab>(y, h, Mt::template coef<T>(cd), d4, e);
+}
 
 // Kerr's FP32-bulk attempt: stage 1 and the difference coefficients in FP64,
 // stages 2..S evaluate g_s = f(y + d_s) - f(y) in FP32 with
@@ -360,11 +374,11 @@ PRR_HD void rkf_stages(const double *y, double h, const Consts<double> &cd, doub
 // so FP32 rounding enters only through the small differences g_s and the
 // error estimate never sees the rounding of f itself.
 template <class Tab = Fehl>
-PRR_HD void kerr_stages_fp32(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
+PRR_HD void kerr_stages_fp32_c(const double *y, double h, const KerrCoef<double> &kc, double *d4, double *e) {
     constexpr int N = Kerr::N, S = Tab::S;
     double F[N];
     KerrDelta P;
-    Kerr::delta_prep(y, Kerr::coef<double>(cd), F, P);
+    Kerr::delta_prep(y, kc, F, P);
     const float hf = float(h);
     float Ff[N];
 #pragma unroll
@@ -396,6 +410,10 @@ PRR_HD void kerr_stages_fp32(const double *y, double h, const Consts<double> &cd
         d4[i] = h * F[i] + double(hf * a4);
         e[i] = double(hf * ae);
     }
+}
+template <class Tab = Fehl>
+PRR_HD void kerr_stages_fp32(const double *y, double h, const Consts<double> &cd, double *d4, double *e) {
+    kerr_stages_fp32_c<Tab>(y, h, Kerr::coef<double>(cd), d4, e);
 }
 
 template <class Mt, class T> PRR_HD void euler_rhs(const double *y, const Consts<double> &cd, double *f) {
@@ -514,6 +532,7 @@ template <class Mt> PRR_HD void lane_load(const JobDev &job, Lane<Mt> &L, int64_
     const int64_t T = job.rs_total, i = rs_index(job, id);
     for (int k = 0; k < Mt::N; ++k) L.y[k] = job.rs.y[k * T + i];
     load_consts<Mt>(job, id, fr, L.cs);
+    L.kc.set(L.cs);
     L.h = job.tr.h / lambda_rate<Mt>(L.y, L.cs);
     L.lam = 0.0;
     L.steps = L.attempts = L.att64 = 0;
@@ -542,6 +561,7 @@ template <class Mt> PRR_HD void lane_load_checkpoint(const JobDev &job, Lane<Mt>
     const double *c = ckpt_at(job, id, j);
     for (int k = 0; k < Mt::N; ++k) L.y[k] = c[k];
     load_consts<Mt>(job, id, fr, L.cs);
+    L.kc.set(L.cs);
     L.h = c[6];
     L.attempts = int32_t(c[7]);
     L.lam = Mt::mino ? L.y[5] : 0.0;
@@ -1097,9 +1117,13 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         if (job.refill >= 64) use64 = wave_any(use64);
     if (!locating) L.att64 += use64 ? 1 : 0;
     double d4[N], e[N];
-    if (use64) rkf_stages<Mt, double, Tab>(L.y, hs, L.cs, d4, e);
-    else if (Mt::mino) kerr_stages_fp32<Tab>(L.y, hs, L.cs, d4, e);
-    else rkf_stages<Mt, float, Tab>(L.y, hs, L.cs, d4, e);
+    if constexpr (Mt::mino) {
+        if (use64) rkf_stages_c<Mt, double, Tab>(L.y, hs, L.kc.k, d4, e);
+        else kerr_stages_fp32_c<Tab>(L.y, hs, L.kc.k, d4, e);
+    } else {
+        if (use64) rkf_stages<Mt, double, Tab>(L.y, hs, L.cs, d4, e);
+        else rkf_stages<Mt, float, Tab>(L.y, hs, L.cs, d4, e);
+    }
     double yn[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) yn[i] = L.y[i] + d4[i];
