@@ -482,11 +482,15 @@ PRR_HD bool sphere_screen(const FrameDev &fr, int k, const float P0[3], const fl
     const float mx = P0[0] - cx, my = P0[1] - cy, mz = P0[2] - cz;
     const float dx = P1[0] - P0[0], dy = P1[1] - P0[1], dz = P1[2] - P0[2];
     const float A = dx * dx + dy * dy + dz * dz;
+    const float scale = fabsf(cx) + fabsf(cy) + fabsf(cz) + R + fabsf(P0[0]) + fabsf(P0[1]) + fabsf(P0[2]);
+    // every point of the chord is at least |m| - |d| from the centre: steps that
+    // start farther than R + 1.5 |d| + the margin pass nowhere near (most steps)
+    const float far = R + 1.5f * sqrt_f32(A) + 2e-4f * scale;
+    if (mx * mx + my * my + mz * mz > far * far) return false;
     float t = A > 0.0f ? -(mx * dx + my * dy + mz * dz) * rcp_t(A) : 0.0f;
     t = fminf(fmaxf(t, 0.0f), 1.0f);
     const float qx = mx + t * dx, qy = my + t * dy, qz = mz + t * dz;
     const float dist = sqrt_f32(qx * qx + qy * qy + qz * qz);
-    const float scale = fabsf(cx) + fabsf(cy) + fabsf(cz) + R + fabsf(P0[0]) + fabsf(P0[1]) + fabsf(P0[2]);
     return dist <= R + 0.5f * sqrt_f32(A) + 1e-4f * scale;
 }
 
@@ -804,7 +808,7 @@ PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, double h, const double
 // horizon.  Returns a class or -1 (go on: L.y = yn, or the location started).
 template <class Mt>
 PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h,
-                               int &mode) {
+                               int &mode, const float *P1 = nullptr) {
     const FrameDev &fr = job.frames[L.frame];
     const double r = radius_of<Mt>(yn);
     const double r_esc = escape_radius<Mt>(job, fr, yn);
@@ -826,7 +830,17 @@ PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, con
     copy_n<Mt::N>(y, yn);
     chart_post_step<Mt>(y);
     copy_n<Mt::N>(L.y, y);
-    if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
+    if (fr.n_spheres > 0) {
+        // P1: the screen's FP32 position of yn, which is the new state's where the chart has no wraps
+        constexpr bool wraps = Mt::kind == PRR_SCHW_SPH || Mt::kind == PRR_ELLIS;
+        if (P1 && !wraps) {
+            L.P0[0] = P1[0];
+            L.P0[1] = P1[1];
+            L.P0[2] = P1[2];
+        } else {
+            cart_of_f32<Mt>(L.y, L.P0);
+        }
+    }
     if (r <= fr.r_hor) return PRR_HORIZON;
     maybe_checkpoint<Mt>(job, L);
     return -1;
@@ -837,8 +851,8 @@ PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, con
 template <class Mt>
 PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h, int &mode) {
     const FrameDev &fr = job.frames[L.frame];
+    float P1[3];
     if (fr.n_spheres > 0) {
-        float P1[3];
         cart_of_f32<Mt>(yn, P1);
         unsigned mask = 0u;
         for (int k = 0; k < fr.n_spheres; ++k)
@@ -855,7 +869,7 @@ PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double
             return -1;
         }
     }
-    return escape_wrap_horizon<Mt>(job, L, C, yn, h, mode);
+    return escape_wrap_horizon<Mt>(job, L, C, yn, h, mode, fr.n_spheres > 0 ? P1 : nullptr);
 }
 
 // Result of sphere_prepare: kNeedSubstep (a sub-step of fraction C.th, size
