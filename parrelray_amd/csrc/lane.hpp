@@ -1083,6 +1083,38 @@ template <class Mt> PRR_HD int locate_update(const JobDev &job, Lane<Mt> &L, Loc
     return -1;
 }
 
+// The tracer as the loop trip reads it.  On the device it is reloaded from
+// the kernel's argument segment each trip (scalar loads) instead of staying
+// in scalar registers across the persistent loop, where the RK tableau's FP64
+// constants leave no room for it and the compiler would spill it to vector
+// lanes (a VALU v_readlane per value and trip).  Valid in k_integrate, whose
+// only argument is the JobDev (at offset 0 of the argument segment).
+PRR_HD TracerDev trip_tracer(const JobDev &job) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    using KP = const __attribute__((address_space(4))) char *;
+    KP kp = (KP)__builtin_amdgcn_kernarg_segment_ptr();
+    __asm__ volatile("" : "+s"(kp));
+    const __attribute__((address_space(4))) TracerDev *t =
+        (const __attribute__((address_space(4))) TracerDev *)(kp + offsetof(JobDev, tr));
+    TracerDev r;
+    r.solver = t->solver;
+    r.precision = t->precision;
+    r.h = t->h;
+    r.hmin = t->hmin;
+    r.hmax = t->hmax;
+    r.atol = t->atol;
+    r.rtol = t->rtol;
+    r.r_escape = t->r_escape;
+    r.lambda_max = t->lambda_max;
+    r.r_fp64 = t->r_fp64;
+    r.max_steps = t->max_steps;
+    r.max_attempts = t->max_attempts;
+    return r;
+#else
+    return job.tr;
+#endif
+}
+
 // One loop trip of a lane: an Euler step, a Runge-Kutta attempt (RKF45,
 // Dormand-Prince 5(4) or fixed-step RK4), or one sub-step of an event
 // location.  Returns the class when the ray ends (mode becomes MODE_IDLE),
@@ -1096,7 +1128,9 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         if (r >= 0) mode = MODE_IDLE;
         return r;
     }
-    const TracerDev &tr = job.tr;
+    // (the FP32-bulk instantiation keeps the tracer in registers: reloading it
+    // there moves its stage registers into scratch memory)
+    const TracerDev tr = MIXED ? job.tr : trip_tracer(job);
     double hs;
     bool use64 = true, at_min = false;
     const bool locating = mode == MODE_LOCATE || mode == MODE_SPHERE;
