@@ -333,14 +333,14 @@ PRR_HD bool wave_any(bool c) {
 template <class Mt, class T, class Tab = Fehl, class C>
 PRR_HD void rkf_stages_c(const double *y, double h, const C &c, double *d4, double *e) {
     constexpr int N = Mt::N, S = Tab::S;
-    const T hT = T(h);
+    // The slopes f_s stay unscaled: h enters through the attempt's coefficients
+    // h a_sj, h b_s and h e_s, one multiply per coefficient instead of one per
+    // slope component.
     T yT[N];
 #pragma unroll
     for (int i = 0; i < N; ++i) yT[i] = T(y[i]);
-    T k[S][N];
-    Mt::template rhs<T>(yT, c, k[0]);
-#pragma unroll
-    for (int i = 0; i < N; ++i) k[0][i] *= hT;
+    T f[S][N];
+    Mt::template rhs<T>(yT, c, f[0]);
 #pragma unroll
     for (int s = 1; s < S; ++s) {
         T ys[N];
@@ -349,20 +349,18 @@ PRR_HD void rkf_stages_c(const double *y, double h, const C &c, double *d4, doub
             T acc = yT[i];
 #pragma unroll
             for (int j = 0; j < s; ++j)
-                if (Tab::a(s, j) != 0.0) acc += T(Tab::a(s, j)) * k[j][i];
+                if (Tab::a(s, j) != 0.0) acc += T(h * Tab::a(s, j)) * f[j][i];
             ys[i] = acc;
         }
-        Mt::template rhs<T>(ys, c, k[s]);
-#pragma unroll
-        for (int i = 0; i < N; ++i) k[s][i] *= hT;
+        Mt::template rhs<T>(ys, c, f[s]);
     }
 #pragma unroll
     for (int i = 0; i < N; ++i) {
         double a4 = 0.0, ae = 0.0;
 #pragma unroll
         for (int s = 0; s < S; ++s) {
-            if (Tab::b(s) != 0.0) a4 += Tab::b(s) * double(k[s][i]);
-            if (Tab::e(s) != 0.0) ae += Tab::e(s) * double(k[s][i]);
+            if (Tab::b(s) != 0.0) a4 += (h * Tab::b(s)) * double(f[s][i]);
+            if (Tab::e(s) != 0.0) ae += (h * Tab::e(s)) * double(f[s][i]);
         }
         d4[i] = a4;
         e[i] = ae;
@@ -398,11 +396,12 @@ PRR_HD void kerr_stages_fp32_c(const double *y, double h, const KerrCoef<double>
         float d[N];
 #pragma unroll
         for (int i = 0; i < N; ++i) {
-            float acc = float(Tab::c(s)) * Ff[i];
+            // h enters through the coefficients h c_s, h a_sj (one multiply each)
+            float acc = (hf * float(Tab::c(s))) * Ff[i];
 #pragma unroll
             for (int j = 1; j < s; ++j)
-                if (Tab::a(s, j) != 0.0) acc += float(Tab::a(s, j)) * g[j][i];
-            d[i] = hf * acc;
+                if (Tab::a(s, j) != 0.0) acc += (hf * float(Tab::a(s, j))) * g[j][i];
+            d[i] = acc;
         }
         Kerr::delta_rhs(d, P, g[s]);
     }
@@ -411,11 +410,11 @@ PRR_HD void kerr_stages_fp32_c(const double *y, double h, const KerrCoef<double>
         float a4 = 0.0f, ae = 0.0f;
 #pragma unroll
         for (int s = 1; s < S; ++s) {
-            if (Tab::b(s) != 0.0) a4 += float(Tab::b(s)) * g[s][i];
-            if (Tab::e(s) != 0.0) ae += float(Tab::e(s)) * g[s][i];
+            if (Tab::b(s) != 0.0) a4 += (hf * float(Tab::b(s))) * g[s][i];
+            if (Tab::e(s) != 0.0) ae += (hf * float(Tab::e(s))) * g[s][i];
         }
-        d4[i] = h * F[i] + double(hf * a4);
-        e[i] = double(hf * ae);
+        d4[i] = h * F[i] + double(a4);
+        e[i] = double(ae);
     }
 }
 template <class Tab = Fehl>
