@@ -240,17 +240,19 @@ __device__ inline float3 fxaa_tap_lds(const float *im, const float (*tile)[kFxHa
     const float fx0 = floorf(X), fy0 = floorf(Y);
     const float fx = X - fx0, fy = Y - fy0;
     const int x0 = int(fx0), y0 = int(fy0);
-    const int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
-    const int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
-    // staged block: image rows oy - 1 .. oy + 16 and columns ox - 1 .. ox + 16, clamped
-    const int lxa = xa - ox + 1, lxb = xb - ox + 1, lya = ya - oy + 1, lyb = yb - oy + 1;
+    // The staged block holds image columns ox - 1 .. ox + 16 and rows oy - 1 ..
+    // oy + 16 already clamped to the image, so a footprint inside it reads the
+    // clamp-to-edge texels without clamping its own coordinates.
+    const int lx = x0 - ox + 1, ly = y0 - oy + 1;
     float3 c00, c10, c01, c11;
-    if (lxa >= 0 && lxb < kFxHalo && lya >= 0 && lyb < kFxHalo && lxb >= 0 && lya < kFxHalo) {
-        c00 = make_float3(tile[lya][lxa][0], tile[lya][lxa][1], tile[lya][lxa][2]);
-        c10 = make_float3(tile[lya][lxb][0], tile[lya][lxb][1], tile[lya][lxb][2]);
-        c01 = make_float3(tile[lyb][lxa][0], tile[lyb][lxa][1], tile[lyb][lxa][2]);
-        c11 = make_float3(tile[lyb][lxb][0], tile[lyb][lxb][1], tile[lyb][lxb][2]);
+    if (unsigned(lx) < unsigned(kFxHalo - 1) && unsigned(ly) < unsigned(kFxHalo - 1)) {
+        c00 = make_float3(tile[ly][lx][0], tile[ly][lx][1], tile[ly][lx][2]);
+        c10 = make_float3(tile[ly][lx + 1][0], tile[ly][lx + 1][1], tile[ly][lx + 1][2]);
+        c01 = make_float3(tile[ly + 1][lx][0], tile[ly + 1][lx][1], tile[ly + 1][lx][2]);
+        c11 = make_float3(tile[ly + 1][lx + 1][0], tile[ly + 1][lx + 1][1], tile[ly + 1][lx + 1][2]);
     } else {
+        const int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
+        const int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
         c00 = load3(im, W, xa, ya);
         c10 = load3(im, W, xb, ya);
         c01 = load3(im, W, xa, yb);
