@@ -122,6 +122,22 @@ def run_adjoint(api, A, CF, reps, torch=None, ids=None):
                steps_per_s=st["adjoint_steps"] / (st["adjoint_ms"] / 1e3),
                grad={"a": g[0], "r_cam": g[1], "incl": g[2], "sphere_phi": g[3]},
                loss=float(((I - T) ** 2).mean()))
+    fd_path = os.path.join(ROOT, "tests", "golden", "c3_fd_gradient.npz")
+    if os.path.exists(fd_path):
+        # against the committed FP64 finite-difference reference (oracle at tol 1e-12 with FP64
+        # colours, central differences at h = 1e-6, 5e-7, 2.5e-7 of the same contraction w . I)
+        z = np.load(fd_path)
+        nkept = int(z["size"]) ** 2 - z["excluded_class"].size - z["excluded_fd"].size
+        w = z["s16"].astype(np.float32) * np.float32(2.0 / (3 * nkept))
+        gw = ctx.adjoint(tmpl, o0, w)
+        fd = z["fd"][-1]
+        names = ("a", "r_cam", "incl", "sphere_phi")
+        out["vs_fd"] = dict(
+            reference="FP64 oracle, tol 1e-12, FP64 colours; central differences, h = 1.25e-7; pixels whose "
+                      "class changes or whose own difference quotients do not converge left out of both sides",
+            adjoint={n: float(v) for n, v in zip(names, gw)}, fd={n: float(v) for n, v in zip(names, fd)},
+            rel_err={n: float(abs(a - b) / abs(b)) for n, a, b in zip(names, gw, fd)},
+            fd_spread_rel={n: float(np.abs(z["fd"][:, k] - fd[k]).max() / abs(fd[k])) for k, n in enumerate(names)})
     if ids is not None:
         out["_rays"] = gpu_rays(ctx, [api.frame_from_orbit(tmpl, o0)], ids, torch)
     return out
