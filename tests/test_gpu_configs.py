@@ -116,6 +116,46 @@ def test_c2_full_batch_matches_oracle():
     assert min(out["classes"][:2]) > 1000, out
 
 
+@pytest.mark.parametrize("precision", [A.PRR_FP64, A.PRR_FP32_BULK])
+def test_c1_full_frame_matches_oracle(precision):
+    """The whole C1 frame (1920 x 1080, 2 x 2 SSAA + FXAA, Kerr a = 0.95, tol
+    1e-8) in one launch, in both precision forms; of 24 seeded candidate
+    24 x 24-pixel blocks, the four whose GPU classes are most mixed and the
+    two with the most sphere rays are checked against the oracle (tol 1e-12):
+    classes, endpoint angle median < 1e-6 and 99th percentile < 1e-5 rad,
+    RGB8 within 1 LSB on >= 99.9 % of the compared pixels (the oracle runs
+    SSAA and FXAA on each block; its 2-pixel border is not compared)."""
+    torch = pytest.importorskip("torch")
+    f = CF.frame_c1()
+    assert f.nrays == 8294400 and f.fxaa == 1
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr(precision=precision))
+    rays = device_rays(torch, f.nrays)
+    rgb8 = torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda")
+    ctx.render_device([f], rgb8=[rgb8], rays=rays)
+    assert ctx.stats()["rays"] == f.nrays
+    rng = np.random.default_rng(0xC1)
+    n = 24
+    cands = []
+    for _ in range(24):
+        x0, y0 = int(rng.integers(0, f.width - n)), int(rng.integers(0, f.height - n))
+        ids = block_ids(f, x0, y0, n)
+        g = gather(torch, rays, 0, ids)
+        counts = np.bincount(g["cls"], minlength=5)
+        esc = int(counts[A.PRR_ESCAPE])
+        cands.append((min(esc, ids.size - esc), int(counts[A.PRR_SPHERE]), x0, y0, ids, g))
+    mixed = sorted(cands, key=lambda c: -c[0])[:4]
+    rest = [c for c in cands if all(c is not m for m in mixed)]
+    sphere = sorted(rest, key=lambda c: -c[1])[:2]
+    S = oracle_scene(A.PRR_KERR_BL, CF.tracer_kerr(tol=1e-12))
+    img = rgb8.cpu().numpy()
+    stats = [check_block(S, f, g, ids, img, x0, y0, n) for _, _, x0, y0, ids, g in mixed + sphere]
+    out = summarize("C1 %s" % ("FP64" if precision == A.PRR_FP64 else "FP32-bulk"), stats)
+    assert out["unexcused"] <= 2, out
+    assert out["angle_median"] < 1e-6 and out["angle_p99"] < 1e-5, out
+    assert out["rgb8_off"] <= 0.001 * out["pixels"], out
+    assert out["classes"][A.PRR_SPHERE] > 100 and min(out["classes"][:2]) > 1000, out
+
+
 def test_c3_forward_render_matches_oracle():
     """The C3 render that the adjoint differentiates (512 x 512, Kerr a = 0.7,
     camera r = 15M, i = 75 deg, sphere at phi = 0.3; FP32-bulk tol 1e-8,
