@@ -225,14 +225,16 @@ __device__ inline float3 load3(const float *im, int W, int x, int y) {
 }
 
 // fxaa (kernels/ssaa.cuh:150-182) on every pixel with clamp-to-edge taps,
-// fused with the RGB8 quantisation.  A workgroup takes a 16 x 16 tile; its
-// pixels and a 1-pixel halo (clamped to the image) are staged in LDS with
-// their lumas, computed once per pixel.  A bilinear tap reads LDS when its
-// 2 x 2 footprint lies in the staged 18 x 18 block; with luma / 255 (D14) the
-// search span is at most about one pixel for colours in [0, 1], so every tap
-// of a render does.  Larger spans (colours above 1) read the image in global
-// memory.  The arithmetic is the same either way.
-constexpr int kFxTile = 16, kFxHalo = kFxTile + 2;
+// fused with the RGB8 quantisation.  A workgroup of 256 threads takes a
+// 32 x 32 tile, four pixels per thread (rows ty, ty + 8, ty + 16, ty + 24):
+// its pixels and a 1-pixel halo (clamped to the image) are staged in LDS
+// with their lumas, computed once per pixel, the staging loads of a thread
+// all in flight together.  A bilinear tap reads LDS when its 2 x 2 footprint
+// lies in the staged 34 x 34 block; with luma / 255 (D14) the search span is
+// at most about one pixel for colours in [0, 1], so every tap of a render
+// does.  Larger spans (colours above 1) read the image in global memory.
+// The arithmetic is the same either way.
+constexpr int kFxTile = 32, kFxHalo = kFxTile + 2, kFxRows = 256 / kFxTile;
 
 __device__ inline float3 fxaa_tap_lds(const float *im, const float (*tile)[kFxHalo][3], int W, int H, int ox,
                                       int oy, float X, float Y) {
@@ -240,8 +242,8 @@ __device__ inline float3 fxaa_tap_lds(const float *im, const float (*tile)[kFxHa
     const float fx0 = floorf(X), fy0 = floorf(Y);
     const float fx = X - fx0, fy = Y - fy0;
     const int x0 = int(fx0), y0 = int(fy0);
-    // The staged block holds image columns ox - 1 .. ox + 16 and rows oy - 1 ..
-    // oy + 16 already clamped to the image, so a footprint inside it reads the
+    // The staged block holds image columns ox - 1 .. ox + 32 and rows oy - 1 ..
+    // oy + 32 already clamped to the image, so a footprint inside it reads the
     // clamp-to-edge texels without clamping its own coordinates.
     const int lx = x0 - ox + 1, ly = y0 - oy + 1;
     float3 c00, c10, c01, c11;
@@ -273,51 +275,71 @@ __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
     const int ox = int(blockIdx.x) * kFxTile, oy = int(blockIdx.y) * kFxTile;
     if (!pf.fxaa || ox >= W || oy >= H) return; // uniform over the workgroup
     const float *im = pf.img;
-    // stage the 18 x 18 block (clamp-to-edge) and its lumas
-    for (int t = int(threadIdx.x); t < kFxHalo * kFxHalo; t += int(blockDim.x)) {
-        const int ly = t / kFxHalo, lx = t - ly * kFxHalo;
-        const int xx = min(max(ox + lx - 1, 0), W - 1), yy = min(max(oy + ly - 1, 0), H - 1);
-        const float3 c = load3(im, W, xx, yy);
-        tile[ly][lx][0] = c.x;
-        tile[ly][lx][1] = c.y;
-        tile[ly][lx][2] = c.z;
-        lum[ly][lx] = luma(c.x, c.y, c.z);
+    // stage the 34 x 34 block (clamp-to-edge) and its lumas: 1156 pixels, at
+    // most five per thread, loaded before any is written
+    constexpr int kStage = kFxHalo * kFxHalo, kPer = (kStage + 255) / 256;
+    float3 c[kPer];
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int t = int(threadIdx.x) + 256 * r;
+        if (t < kStage) {
+            const int ly = t / kFxHalo, lx = t - ly * kFxHalo;
+            const int xx = min(max(ox + lx - 1, 0), W - 1), yy = min(max(oy + ly - 1, 0), H - 1);
+            c[r] = load3(im, W, xx, yy);
+        }
+    }
+#pragma unroll
+    for (int r = 0; r < kPer; ++r) {
+        const int t = int(threadIdx.x) + 256 * r;
+        if (t < kStage) {
+            const int ly = t / kFxHalo, lx = t - ly * kFxHalo;
+            tile[ly][lx][0] = c[r].x;
+            tile[ly][lx][1] = c[r].y;
+            tile[ly][lx][2] = c[r].z;
+            lum[ly][lx] = luma(c[r].x, c[r].y, c[r].z);
+        }
     }
     __syncthreads();
-    const int tx = int(threadIdx.x & 15), ty = int(threadIdx.x >> 4);
-    const int x = ox + tx, y = oy + ty;
-    if (x >= W || y >= H) return;
-    const float lNW = lum[ty][tx], lNE = lum[ty][tx + 2], lSW = lum[ty + 2][tx], lSE = lum[ty + 2][tx + 2];
-    const float lM = lum[ty + 1][tx + 1];
-    const float lMin = fminf(lM, fminf(fminf(lNW, lNE), fminf(lSW, lSE)));
-    const float lMax = fmaxf(lM, fmaxf(fmaxf(lNW, lNE), fmaxf(lSW, lSE)));
-    float dx = -((lNW + lNE) - (lSW + lSE));
-    float dy = ((lNW + lSW) - (lNE + lSE));
-    const float reduce = fmaxf((lNW + lNE + lSW + lSE) * (0.25f * (1.0f / 8.0f)), 1.0f / 128.0f);
-    const float rcp = 1.0f / (fminf(fabsf(dx), fabsf(dy)) + reduce);
-    dx = fminf(fmaxf(dx * rcp, -8.0f), 8.0f);
-    dy = fminf(fmaxf(dy * rcp, -8.0f), 8.0f);
-    const float k1 = 1.0f / 3.0f - 0.5f, k2 = 2.0f / 3.0f - 0.5f;
-    const float fxp = float(x), fyp = float(y);
-    const float3 t1 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k1, fyp + dy * k1);
-    const float3 t2 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k2, fyp + dy * k2);
-    const float3 t3 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * -0.5f, fyp + dy * -0.5f);
-    const float3 t4 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * 0.5f, fyp + dy * 0.5f);
-    const float3 A = make_float3(t1.x * 0.5f + t2.x * 0.5f, t1.y * 0.5f + t2.y * 0.5f, t1.z * 0.5f + t2.z * 0.5f);
-    const float3 B = make_float3(0.5f * A.x + (t3.x * 0.25f + t4.x * 0.25f), 0.5f * A.y + (t3.y * 0.25f + t4.y * 0.25f),
-                                 0.5f * A.z + (t3.z * 0.25f + t4.z * 0.25f));
-    const float lB = luma(B.x, B.y, B.z);
-    const float3 o = (lB < lMin || lB > lMax) ? A : B;
-    const int64_t p = int64_t(y) * W + x;
-    if (pf.final_rgbf) {
-        pf.final_rgbf[3 * p] = o.x;
-        pf.final_rgbf[3 * p + 1] = o.y;
-        pf.final_rgbf[3 * p + 2] = o.z;
-    }
-    if (pf.rgb8) {
-        pf.rgb8[3 * p] = to_rgb8(o.x);
-        pf.rgb8[3 * p + 1] = to_rgb8(o.y);
-        pf.rgb8[3 * p + 2] = to_rgb8(o.z);
+    const int tx = int(threadIdx.x) % kFxTile, ty0 = int(threadIdx.x) / kFxTile;
+    const int x = ox + tx;
+    if (x >= W) return;
+#pragma unroll
+    for (int k = 0; k < kFxTile / kFxRows; ++k) {
+        const int ty = ty0 + kFxRows * k, y = oy + ty;
+        if (y >= H) break;
+        const float lNW = lum[ty][tx], lNE = lum[ty][tx + 2], lSW = lum[ty + 2][tx], lSE = lum[ty + 2][tx + 2];
+        const float lM = lum[ty + 1][tx + 1];
+        const float lMin = fminf(lM, fminf(fminf(lNW, lNE), fminf(lSW, lSE)));
+        const float lMax = fmaxf(lM, fmaxf(fmaxf(lNW, lNE), fmaxf(lSW, lSE)));
+        float dx = -((lNW + lNE) - (lSW + lSE));
+        float dy = ((lNW + lSW) - (lNE + lSE));
+        const float reduce = fmaxf((lNW + lNE + lSW + lSE) * (0.25f * (1.0f / 8.0f)), 1.0f / 128.0f);
+        const float rcp = 1.0f / (fminf(fabsf(dx), fabsf(dy)) + reduce);
+        dx = fminf(fmaxf(dx * rcp, -8.0f), 8.0f);
+        dy = fminf(fmaxf(dy * rcp, -8.0f), 8.0f);
+        const float k1 = 1.0f / 3.0f - 0.5f, k2 = 2.0f / 3.0f - 0.5f;
+        const float fxp = float(x), fyp = float(y);
+        const float3 t1 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k1, fyp + dy * k1);
+        const float3 t2 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k2, fyp + dy * k2);
+        const float3 t3 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * -0.5f, fyp + dy * -0.5f);
+        const float3 t4 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * 0.5f, fyp + dy * 0.5f);
+        const float3 A = make_float3(t1.x * 0.5f + t2.x * 0.5f, t1.y * 0.5f + t2.y * 0.5f, t1.z * 0.5f + t2.z * 0.5f);
+        const float3 B = make_float3(0.5f * A.x + (t3.x * 0.25f + t4.x * 0.25f),
+                                     0.5f * A.y + (t3.y * 0.25f + t4.y * 0.25f),
+                                     0.5f * A.z + (t3.z * 0.25f + t4.z * 0.25f));
+        const float lB = luma(B.x, B.y, B.z);
+        const float3 o = (lB < lMin || lB > lMax) ? A : B;
+        const int64_t p = int64_t(y) * W + x;
+        if (pf.final_rgbf) {
+            pf.final_rgbf[3 * p] = o.x;
+            pf.final_rgbf[3 * p + 1] = o.y;
+            pf.final_rgbf[3 * p + 2] = o.z;
+        }
+        if (pf.rgb8) {
+            pf.rgb8[3 * p] = to_rgb8(o.x);
+            pf.rgb8[3 * p + 1] = to_rgb8(o.y);
+            pf.rgb8[3 * p + 2] = to_rgb8(o.z);
+        }
     }
 }
 
@@ -748,7 +770,9 @@ hipError_t launch_ssaa(const PostFrameDev *pfs, int nframes, int64_t max_pixels,
 }
 
 hipError_t launch_fxaa(const PostFrameDev *pfs, int nframes, int max_w, int max_h, hipStream_t st) {
-    const dim3 g(unsigned((max_w + 15) / 16), unsigned((max_h + 15) / 16), unsigned(nframes)), b(256);
+    const dim3 g(unsigned((max_w + kFxTile - 1) / kFxTile), unsigned((max_h + kFxTile - 1) / kFxTile),
+                 unsigned(nframes)),
+        b(256);
     hipLaunchKernelGGL(k_fxaa, g, b, 0, st, pfs);
     return hipGetLastError();
 }
