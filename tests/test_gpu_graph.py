@@ -148,6 +148,13 @@ def test_invalid_inputs_are_rejected_without_touching_the_gpu_state():
         ctx.render(bad)
     with pytest.raises(PrrError):
         ctx.set_tracer(A.make_tracer(hmin=0.0))
+    for w, h in ((0, 18), (32, 0), (70000, 2)):  # empty and oversized frames
+        bad = A.Frame.from_buffer_copy(f)
+        bad.width, bad.height = w, h
+        with pytest.raises(PrrError):
+            ctx.render(bad)
+    with pytest.raises(PrrError):  # no frames at all
+        ctx.render([])
     # the context still works
     out = ctx.render(f)[0]
     assert out["rgb8"].shape == (18, 32, 3)

@@ -13,7 +13,7 @@ from parrelray_amd import _abi as A
 from parrelray_amd import api
 from parrelray_amd import configs as CF
 
-from parity import angle, compare_rays, gpu_context, oracle_scene, rgb8_agreement
+from parity import angle, boundary_excused, compare_rays, gpu_context, oracle_scene, rgb8_agreement
 
 pytestmark = pytest.mark.gpu
 RNG = np.random.default_rng(11)
@@ -474,6 +474,29 @@ def test_aa_passes_bit_exact():
     ss = O.ssaa(out["color"], f.width, f.height, f.samples)
     assert np.array_equal(out["rgbf"], ss)
     assert np.array_equal(out["rgb8"], O.rgb8(ss))
+
+
+@pytest.mark.parametrize("width,height", [(1, 1), (1, 7), (7, 1), (2, 2), (33, 17), (65, 34)])
+def test_tiny_and_ragged_frames(width, height):
+    """Frames of one pixel, one row or column, and sizes that do not divide
+    the integrator's 4 x 4-pixel tiles or FXAA's 32 x 32 tiles, all in one
+    batch (2 x 2 SSAA + FXAA): every ray's class against the oracle (rays
+    within 1e-6 rad of a class boundary excused), and SSAA, FXAA and RGB8 of
+    the GPU's own ray colours equal the oracle's passes bit for bit (FXAA's
+    taps clamp to the edge of images as small as one pixel)."""
+    f = CF.frame_c1(width, height)
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr())
+    out = ctx.render([f, CF.frame_c1(16, 9)], rays=True)[0]
+    assert ctx.stats()["rays"] == f.nrays + 16 * 9 * 4
+    ss = O.ssaa(out["color"], f.width, f.height, f.samples)
+    fx = O.fxaa(ss)
+    assert np.array_equal(out["rgbf"], fx)
+    assert np.array_equal(out["rgb8"], O.rgb8(fx))
+    S = oracle_scene(A.PRR_KERR_BL, CF.tracer_kerr(tol=1e-12))
+    ids = np.arange(f.nrays, dtype=np.int64)
+    orc = S.render_rays(f, ids)
+    bad = np.nonzero(orc["cls"] != out["cls"])[0]
+    assert np.all(boundary_excused(S, f, ids[bad], orc["cls"][bad])), bad
 
 
 def test_every_ray_processed_once_and_batch_equals_single():
