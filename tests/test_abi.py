@@ -76,3 +76,30 @@ def test_no_context_calls_fail_cleanly():
     assert lib.prr_abi_version() == A.PRR_ABI_VERSION
     with pytest.raises(_lib.PrrError):
         _lib.check(lib.prr_set_tracer(None, None))
+
+
+def test_product_never_touches_the_oracle():
+    """The package and libprr.so neither import, link nor name the oracle
+    (test infrastructure), and the library links no CPU fallback: importing
+    the package and every module in it leaves oracle unimported, and the
+    shared library's dynamic dependencies do not include liboracle."""
+    import importlib
+    import pkgutil
+    import sys
+
+    import parrelray_amd
+    before = "oracle" in sys.modules
+    for m in pkgutil.iter_modules(parrelray_amd.__path__):
+        if m.name == "autograd":
+            pytest.importorskip("torch")
+        importlib.import_module("parrelray_amd." + m.name)
+    if not before:
+        assert "oracle" not in sys.modules
+    pkg = os.path.join(ROOT, "parrelray_amd")
+    for d, _, files in os.walk(pkg):
+        for f in files:
+            if f.endswith((".py", ".cpp", ".hpp", ".hip", ".h")):
+                text = open(os.path.join(d, f), errors="replace").read()
+                assert not re.search(r"\boracle\b|liboracle|orc_", text), os.path.join(d, f)
+    deps = subprocess.run(["ldd", _lib.LIB_PATH], capture_output=True, text=True).stdout
+    assert "oracle" not in deps
