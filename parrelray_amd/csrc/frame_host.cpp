@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <cmath>
 
+#include "geodesic.hpp"
 #include "prr_host.hpp"
 
 namespace prr {
@@ -73,6 +74,27 @@ FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t
         for (int c = 0; c < 3; ++c) d.sph[k][c] = f.spheres[k].c[c];
         d.sph[k][3] = f.spheres[k].radius;
         d.sph_tex[k] = f.spheres[k].tex;
+    }
+    // initRays' per-frame terms (the camera position through pos_cartTOspher,
+    // cudamaths/transform.cuh:15-22, and the Kerr metric there)
+    const double px = f.pos[0], py = f.pos[1], pz = f.pos[2];
+    const double x2y2 = px * px + py * py, r2 = x2y2 + pz * pz, r = std::sqrt(r2);
+    d.cam_r = r;
+    d.cam_th = std::acos(pz / r);
+    d.cam_ph = std::atan2(py, px);
+    d.cam_st = std::sin(d.cam_th);
+    d.cam_ct = std::cos(d.cam_th);
+    d.cam_ir = 1.0 / r;
+    d.cam_x2y2 = x2y2;
+    d.cam_ix2y2 = 1.0 / x2y2;
+    d.cam_iu1 = 1.0 / (r2 * std::sqrt(x2y2));
+    if (metric == PRR_KERR_BL) {
+        const KerrG g = kerr_g(f.mass, d.a, r, d.cam_th);
+        d.g_tt = g.tt;
+        d.g_tph = g.tph;
+        d.g_rr = g.rr;
+        d.g_sig = g.Sig;
+        d.g_phph = g.phph;
     }
     return d;
 }

@@ -677,6 +677,23 @@ PRR_HD KerrG kerr_g(double M, double a, double r, double th) {
     return g;
 }
 
+// Kerr's integrator state and constants at (r, theta, phi) from the metric
+// g there, sin / cos theta, 1 / r and u = dx/dlambda (cs.M and cs.a are set
+// by the caller).
+PRR_HD void kerr_state_at(const KerrG &g, double st, double ct, double ir, double ph, const double u[4], double a,
+                          double *y, Consts<double> &cs) {
+    cs.E = -(g.tt * u[3] + g.tph * u[2]);
+    cs.L = g.phph * u[2] + g.tph * u[3];
+    const double pth = g.Sig * u[1];
+    cs.Q = pth * pth + ct * ct * (cs.L * cs.L / (st * st) - a * a * cs.E * cs.E);
+    y[0] = ir;
+    y[1] = -g.Sig * u[0] * ir * ir;
+    y[2] = ct;
+    y[3] = -st * pth;
+    y[4] = ph;
+    y[5] = 0.0;
+}
+
 // (x, u) in the metric's coordinates -> integrator state y and per-ray
 // constants.  For the 8-component metrics y = (x, u).  For Kerr (x = (r, th,
 // phi, t), u = dx/dlambda): E = -p_t, L = p_phi, Q = p_th^2 + cos^2 th
@@ -695,19 +712,9 @@ template <class Mt> PRR_HD void state_from_xu(const double x[4], const double u[
         return;
     }
     const KerrG g = kerr_g(M, a, x[0], x[1]);
-    cs.E = -(g.tt * u[3] + g.tph * u[2]);
-    cs.L = g.phph * u[2] + g.tph * u[3];
     double st, ct;
     sincos(x[1], &st, &ct);
-    const double pth = g.Sig * u[1];
-    cs.Q = pth * pth + ct * ct * (cs.L * cs.L / (st * st) - a * a * cs.E * cs.E);
-    const double ir = 1.0 / x[0];
-    y[0] = ir;
-    y[1] = -g.Sig * u[0] * ir * ir;
-    y[2] = ct;
-    y[3] = -st * pth;
-    y[4] = x[2];
-    y[5] = 0.0;
+    kerr_state_at(g, st, ct, 1.0 / x[0], x[2], u, a, y, cs);
 }
 
 // Integrator state -> (x, u); for Kerr, t is returned as 0 and
@@ -796,6 +803,36 @@ template <class Mt> PRR_HD void init_state(D3 pos, D3 d, double M, double a, dou
     }
     u[3] = null_ut(A, B, C);
     state_from_xu<Mt>(x, u, M, a, y, cs);
+}
+
+// initRays for a camera ray of frame fr.  For Kerr the camera position's
+// coordinates, sin / cos theta and the metric there are per-frame constants
+// of the frame table (frame_dev), so a ray only converts its direction into
+// (dr, dtheta, dphi) and solves the null condition; the other metrics take
+// the general path.
+template <class Mt> PRR_HD void init_state_camera(const FrameDev &fr, D3 d, double *y, Consts<double> &cs) {
+    const D3 p = d3(fr.pos[0], fr.pos[1], fr.pos[2]);
+    if constexpr (Mt::kind == PRR_KERR_BL) {
+        double u[4];
+        u[0] = dot3(p, d) * fr.cam_ir;
+        u[1] = (p.z * (p.x * d.x + p.y * d.y) - fr.cam_x2y2 * d.z) * fr.cam_iu1;
+        u[2] = (p.x * d.y - p.y * d.x) * fr.cam_ix2y2;
+        const double A = fr.g_tt, B = 2.0 * fr.g_tph * u[2];
+        const double C = fr.g_rr * u[0] * u[0] + fr.g_sig * u[1] * u[1] + fr.g_phph * u[2] * u[2];
+        u[3] = null_ut(A, B, C);
+        KerrG g;
+        g.tt = fr.g_tt;
+        g.tph = fr.g_tph;
+        g.rr = fr.g_rr;
+        g.thth = g.Sig = fr.g_sig;
+        g.phph = fr.g_phph;
+        g.Del = 0.0;
+        cs.M = fr.M;
+        cs.a = fr.a;
+        kerr_state_at(g, fr.cam_st, fr.cam_ct, fr.cam_ir, fr.cam_ph, u, fr.a, y, cs);
+    } else {
+        init_state<Mt>(p, d, fr.M, fr.a, y, cs);
+    }
 }
 
 // User_ENDLOOP of the spherical metric ("metrics/schwarzschild - sph.cuh":69-78):

@@ -50,6 +50,13 @@ struct FrameDev {
     int64_t queue_begin; // first queue position (frames start at multiples of 64)
     double sph[PRR_MAX_SPHERES][4]; // centre xyz, radius
     int32_t sph_tex[PRR_MAX_SPHERES];
+    // initRays' per-frame terms for the spherical charts: the camera position
+    // in (r, theta, phi), sin and cos theta, x^2 + y^2 and the reciprocals
+    // that convert a direction into (dr, dtheta, dphi); for Kerr the metric
+    // components at the camera (g_tt, g_tphi, g_rr, g_thth = Sigma, g_phph)
+    double cam_r, cam_th, cam_ph, cam_st, cam_ct;
+    double cam_ir, cam_x2y2, cam_ix2y2, cam_iu1;
+    double g_tt, g_tph, g_rr, g_sig, g_phph;
 };
 
 // Terminal RKF45 step of a ray whose escape or sphere hit was located on

@@ -509,8 +509,7 @@ template <class Mt> PRR_HD void init_ray(const JobDev &job, int64_t id) {
         const double *s = job.explicit_x0u0 + 8 * id;
         state_from_xu<Mt>(s, s + 4, fr.M, fr.a, y, cs);
     } else {
-        const D3 d = camera_ray(fr, id - fr.ray_begin);
-        init_state<Mt>(d3(fr.pos[0], fr.pos[1], fr.pos[2]), d, fr.M, fr.a, y, cs);
+        init_state_camera<Mt>(fr, camera_ray(fr, id - fr.ray_begin), y, cs);
     }
     const int64_t T = job.rs_total, i = rs_index(job, id);
     for (int k = 0; k < Mt::N; ++k) job.rs.y[k * T + i] = y[k];
