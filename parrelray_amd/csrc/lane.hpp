@@ -179,12 +179,20 @@ PRR_HD int find_frame_q(const JobDev &job, int64_t q) {
     return lo;
 }
 
+// k-th index in [0, n) counted from the middle outwards, alternating sides:
+// n = 4 gives 2, 1, 3, 0 and n = 5 gives 2, 3, 1, 4, 0.
+PRR_HD int64_t centre_out(int64_t k, int64_t n) {
+    const int64_t h = n >> 1;
+    if (n & 1) return (k & 1) ? h + ((k + 1) >> 1) : h - (k >> 1);
+    return (k & 1) ? h - 1 - (k >> 1) : h + (k >> 1);
+}
+
 // Queue position -> ray id, or -1 for the padding after a frame whose ray
 // count is not a multiple of 64.  The queue hands a wave 64 consecutive positions;
 // within each frame they map to a tile of pixels (all samples of each pixel)
 // rather than to a run of one image row, so that a wave's rays are close
-// together on the sky and follow similar paths; tiles alternate between the
-// top and the bottom of the image.  Tiles hold 64 / s pixels
+// together on the sky and follow similar paths; tiles are ordered from the
+// middle of the image outwards (below).  Tiles hold 64 / s pixels
 // (s samples per pixel, a divisor of 64), Th x Tw with Th the largest power
 // of two whose square fits; the part of the image that whole tiles do not
 // cover (a right strip, then a bottom strip) keeps row order.  A bijection on
@@ -207,14 +215,16 @@ PRR_HD int64_t queue_ray(const JobDev &job, int64_t q) {
     int64_t i, j, k;
     if (local < ncore) {
         const int64_t per = P * s, q_t = local / per, w = local - q_t * per;
-        // tiles are handed out alternately from the first and the last tile
-        // row inwards, so that features mirrored about the middle of the image
-        // (the photon ring above and below the shadow, whose rays take the
-        // most steps) start early in launches whose rays fit the chip a few
-        // times over, instead of half of them starting last
-        const int64_t tiles_x = wc / tw, ntiles = tiles_x * (hc / th);
-        const int64_t t = (q_t & 1) ? ntiles - 1 - (q_t >> 1) : (q_t >> 1);
-        const int64_t ty = t / tiles_x, tx = t - ty * tiles_x;
+        // tiles are handed out a column at a time, columns from the middle
+        // of the image outwards and, within a column, rows from the middle
+        // outwards: the rays that take the most steps (those that pass close
+        // to the polar axis, whose image is the middle column when the spin
+        // axis projects upright, and those near the photon ring around the
+        // shadow) start first, and the cheap sky at the left and right edges
+        // fills the end of the launch
+        const int64_t tiles_x = wc / tw, tiles_y = hc / th;
+        const int64_t kc = q_t / tiles_y, kr = q_t - kc * tiles_y;
+        const int64_t tx = centre_out(kc, tiles_x), ty = centre_out(kr, tiles_y);
         const int64_t p = w / s;
         k = w - p * s;
         i = tx * tw + p % tw;
