@@ -92,8 +92,13 @@ def run_form(ctx, frame, precision, steps, warmup, out8, outf, torch):
                 accepted_per_s=st["accepted"] / t_int,
                 achieved_tflops=(f32 + f64) / t_int / 1e12,
                 roofline_frac=(f32 / P32 + f64 / P64) / t_int,
-                ssaa_gbs=st["ssaa_bytes"] / (sum(s["ssaa_ms"] for s in per) / steps / 1e3) / 1e9,
-                fxaa_gbs=st["fxaa_bytes"] / max(1e-9, sum(s["fxaa_ms"] for s in per) / steps / 1e3) / 1e9)
+                ssaa_gbs=gbs(st["ssaa_bytes"], sum(s["ssaa_ms"] for s in per) / steps),
+                fxaa_gbs=gbs(st["fxaa_bytes"], sum(s["fxaa_ms"] for s in per) / steps))
+
+
+def gbs(nbytes, ms):
+    """GB/s of a pass that moved nbytes in ms (None when the pass did not run)."""
+    return nbytes / (ms / 1e3) / 1e9 if nbytes > 0 and ms > 0 else None
 
 
 def run_adjoint(api, A, CF, reps, torch=None, ids=None):

@@ -187,13 +187,13 @@ typedef struct prr_stats {
     int64_t attempts_fp64;  /* attempts whose stages ran in FP64 */
     int64_t pixels;
     double integrate_ms;    /* persistent integrator kernel */
-    double ssaa_ms;         /* SSAA combine kernel */
-    double fxaa_ms;         /* FXAA + RGB8 kernel */
+    double ssaa_ms;         /* SSAA combine + RGB8 kernel of the frames without FXAA */
+    double fxaa_ms;         /* SSAA combine + FXAA + RGB8 kernel of the frames with FXAA */
     double total_ms;        /* whole call on the stream */
     double flops_fp64;      /* algorithmic FP64 flops of the integration */
     double flops_fp32;      /* algorithmic FP32 flops of the integration */
-    double ssaa_bytes;      /* algorithmic HBM bytes of the SSAA pass */
-    double fxaa_bytes;      /* algorithmic HBM bytes of the FXAA pass */
+    double ssaa_bytes;      /* algorithmic HBM bytes of the SSAA kernel (ray colours in, images out) */
+    double fxaa_bytes;      /* algorithmic HBM bytes of the FXAA kernel (ray colours in, images out) */
     double adjoint_ms;      /* prr_adjoint: checkpointed forward + backward passes */
     int64_t adjoint_steps;  /* prr_adjoint: accepted steps swept backwards */
     double init_ms;         /* ray set-up kernel */

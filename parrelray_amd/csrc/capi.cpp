@@ -50,7 +50,6 @@ size_t carve_workspace(void *base, const WorkspaceShape &s, Workspace *ws) {
     w.post = static_cast<PostFrameDev *>(take(sizeof(PostFrameDev) * size_t(std::max(s.nframes, 1))));
     w.counters = static_cast<CountersDev *>(take(sizeof(CountersDev)));
     w.color = static_cast<float *>(take(12 * R));
-    w.img_scratch = s.fxaa_pixels > 0 ? static_cast<float *>(take(12 * size_t(s.fxaa_pixels))) : nullptr;
     w.rs_y = static_cast<double *>(take(8 * size_t(s.state_width) * R));
     w.rs_cls = static_cast<int8_t *>(take(R));
     w.rs_hit = static_cast<int8_t *>(take(R));
@@ -449,8 +448,11 @@ prr_status prr::plan_frames(prr_ctx *ctx, const prr_frame *frames, int n, FrameD
         P.max_w = std::max(P.max_w, frames[i].width);
         P.max_h = std::max(P.max_h, frames[i].height);
         P.any_fxaa = P.any_fxaa || frames[i].fxaa;
-        P.ssaa_bytes += double(np) * frames[i].samples * 12.0 + double(np) * (frames[i].fxaa ? 12.0 : 15.0);
-        if (frames[i].fxaa) P.fxaa_bytes += double(np) * (12.0 + 15.0);
+        P.any_plain = P.any_plain || !frames[i].fxaa;
+        // the frame's ray colours in, the linear image and RGB8 out: k_ssaa
+        // for frames without FXAA, k_fxaa (which combines the samples itself)
+        // for the others
+        (frames[i].fxaa ? P.fxaa_bytes : P.ssaa_bytes) += double(np) * frames[i].samples * 12.0 + double(np) * 15.0;
     }
     P.grid = integrate_grid(ctx);
     P.block = ctx->launch.block;
@@ -461,7 +463,6 @@ WorkspaceShape prr::workspace_shape(const prr_ctx *ctx, const RenderPlan &P) {
     WorkspaceShape sh;
     sh.nframes = P.n;
     sh.rays = P.total;
-    sh.fxaa_pixels = P.any_fxaa ? P.pixels : 0;
     sh.state_width = rs_width(ctx->metric);
     sh.lanes = int64_t(P.grid) * P.block;
     return sh;
@@ -487,7 +488,6 @@ void prr::plan_post(const prr_frame *frames, const prr_image *images, const Work
         PostFrameDev &q = pf[i];
         const int64_t np = int64_t(frames[i].width) * frames[i].height;
         q.ray_color = ws.color + 3 * P.fd[i].ray_begin;
-        q.img = P.any_fxaa ? ws.img_scratch + 3 * pix_off : nullptr;
         q.W = frames[i].width;
         q.H = frames[i].height;
         q.s = frames[i].samples;
@@ -545,7 +545,7 @@ prr_status prr::enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
     HIPTRY(ctx, launch_shade(ctx->metric, job, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[5], st));
-    if (P.post) HIPTRY(ctx, launch_ssaa(ws.post, P.n, P.max_pix, st));
+    if (P.post && P.any_plain) HIPTRY(ctx, launch_ssaa(ws.post, P.n, P.max_pix, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[3], st));
     if (P.post && P.any_fxaa) HIPTRY(ctx, launch_fxaa(ws.post, P.n, P.max_w, P.max_h, st));
     if (events) HIPTRY(ctx, hipEventRecord(ctx->ev[4], st));

@@ -44,7 +44,6 @@ struct Workspace {
     PostFrameDev *post = nullptr;
     CountersDev *counters = nullptr;
     float *color = nullptr;       // 3 per ray
-    float *img_scratch = nullptr; // 3 per pixel of the FXAA frames (null without FXAA)
     double *rs_y = nullptr;       // rs_width(metric) per ray
     int8_t *rs_cls = nullptr, *rs_hit = nullptr;
     int32_t *rs_steps = nullptr, *rs_attempts = nullptr;
@@ -55,7 +54,6 @@ struct Workspace {
 struct WorkspaceShape {
     int nframes = 0;
     int64_t rays = 0;
-    int64_t fxaa_pixels = 0;
     int state_width = 0; // doubles of ray state per ray
     int64_t lanes = 0;
 };
@@ -72,7 +70,7 @@ struct RenderPlan {
     bool post = true; // SSAA / FXAA passes follow (not for prr_trace)
     int64_t total = 0, qtotal = 0, pixels = 0, max_pix = 0;
     int max_w = 0, max_h = 0;
-    bool any_fxaa = false;
+    bool any_fxaa = false, any_plain = false; // some frame with / without FXAA
     double ssaa_bytes = 0, fxaa_bytes = 0;
     int grid = 0, block = 0; // persistent integrator launch
 };

@@ -206,7 +206,6 @@ struct AdjPlanDev {
 // Post-processing of one frame (SSAA combine, FXAA, RGB8).
 struct PostFrameDev {
     const float *ray_color; // 3 * W*H*s, this frame's rays
-    float *img;             // SSAA result, W*H*3 (scratch when FXAA follows)
     float *final_rgbf;      // final linear image or null
     uint8_t *rgb8;          // final RGB8 or null
     int32_t W, H, s, fxaa;

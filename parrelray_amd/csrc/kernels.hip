@@ -173,42 +173,46 @@ __device__ inline uint8_t to_rgb8(float c) {
     return uint8_t(v);
 }
 
-// combineSamples (kernels/ssaa.cuh:109-131): one thread per pixel, frames
-// along grid.y.  Writes RGB8 directly when no FXAA follows.
-__global__ void __launch_bounds__(256) k_ssaa(const PostFrameDev *pfs) {
+// combineSamples (kernels/ssaa.cuh:109-131) of pixel p: the root mean square
+// of its s sample colours, each clamped to [0, 1] after squaring (one sample:
+// the clamped colour).  The result lies in [0, 1].
+__device__ inline float3 ssaa_pixel(const PostFrameDev &pf, int64_t p) {
 #pragma clang fp contract(off)
+    const int s = pf.s;
+    if (s == 1) {
+        const float *src = pf.ray_color + 3 * p;
+        return make_float3(clamp01(src[0]), clamp01(src[1]), clamp01(src[2]));
+    }
+    float acc[3] = {0.f, 0.f, 0.f};
+    const float *src = pf.ray_color + 3 * p * s;
+    for (int k = 0; k < s; ++k) {
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            const float c = src[3 * k + ch];
+            acc[ch] += clamp01(c * c);
+        }
+    }
+    return make_float3(sqrtf(acc[0] / float(s)), sqrtf(acc[1] / float(s)), sqrtf(acc[2] / float(s)));
+}
+
+// The SSAA pass of the frames without FXAA: one thread per pixel, frames
+// along grid.y, the linear image and RGB8 written directly.  (FXAA frames
+// combine their samples inside k_fxaa.)
+__global__ void __launch_bounds__(256) k_ssaa(const PostFrameDev *pfs) {
     const PostFrameDev pf = pfs[blockIdx.y];
     const int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const int64_t npix = int64_t(pf.W) * pf.H;
-    if (p >= npix) return;
-    const int s = pf.s;
-    float out[3];
-    if (s == 1) {
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) out[ch] = clamp01(pf.ray_color[3 * p + ch]);
-    } else {
-        float acc[3] = {0.f, 0.f, 0.f};
-        const float *src = pf.ray_color + 3 * p * s;
-        for (int k = 0; k < s; ++k) {
-#pragma unroll
-            for (int ch = 0; ch < 3; ++ch) {
-                const float c = src[3 * k + ch];
-                acc[ch] += clamp01(c * c);
-            }
-        }
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) out[ch] = sqrtf(acc[ch] / float(s));
+    if (pf.fxaa || p >= npix) return;
+    const float3 o = ssaa_pixel(pf, p);
+    if (pf.final_rgbf) {
+        pf.final_rgbf[3 * p] = o.x;
+        pf.final_rgbf[3 * p + 1] = o.y;
+        pf.final_rgbf[3 * p + 2] = o.z;
     }
-    float *dst = pf.fxaa ? pf.img : pf.final_rgbf;
-    if (dst) {
-        dst[3 * p] = out[0];
-        dst[3 * p + 1] = out[1];
-        dst[3 * p + 2] = out[2];
-    }
-    if (!pf.fxaa && pf.rgb8) {
-        pf.rgb8[3 * p] = to_rgb8(out[0]);
-        pf.rgb8[3 * p + 1] = to_rgb8(out[1]);
-        pf.rgb8[3 * p + 2] = to_rgb8(out[2]);
+    if (pf.rgb8) {
+        pf.rgb8[3 * p] = to_rgb8(o.x);
+        pf.rgb8[3 * p + 1] = to_rgb8(o.y);
+        pf.rgb8[3 * p + 2] = to_rgb8(o.z);
     }
 }
 
@@ -219,25 +223,37 @@ __device__ inline float luma(float r, float g, float b) {
     return float(double(r) * 0.299 * k + double(g) * 0.587 * k + double(b) * 0.114 * k);
 }
 
-__device__ inline float3 load3(const float *im, int W, int x, int y) {
-    const float *p = im + 3 * (int64_t(y) * W + x);
-    return make_float3(p[0], p[1], p[2]);
+// fxaa (kernels/ssaa.cuh:150-182) on every pixel with clamp-to-edge taps,
+// fused with the SSAA combine before it and the RGB8 quantisation after it,
+// so that the combined image never goes through HBM.  A workgroup of 512
+// threads takes a 32 x 32 tile, two pixels per thread (rows ty and
+// ty + 16): the combined colours of its pixels and of a 1-pixel
+// halo (clamped to the image) are computed from the ray colours into LDS with
+// their lumas, once per pixel, the loads of a thread all in flight together.
+// A bilinear tap reads LDS when its 2 x 2 footprint lies in the staged
+// 34 x 34 block; with luma / 255 (D14) and combined colours in [0, 1] the
+// search span is at most about one pixel, so every tap does.  A footprint
+// outside it would combine its pixels' samples from the ray colours again.
+// The arithmetic is the same either way.
+constexpr int kFxThreads = 512, kFxTile = 32, kFxHalo = kFxTile + 2, kFxRows = kFxThreads / kFxTile;
+
+// The 2 x 2 footprint at (x0, y0) of a tap outside the staged block, combined
+// from the ray colours (clamp-to-edge); out of line: no render reaches it.
+struct Footprint {
+    float3 c00, c10, c01, c11;
+};
+__device__ __noinline__ Footprint fxaa_footprint(const float *ray_color, int W, int H, int s, int x0, int y0) {
+    PostFrameDev pf{};
+    pf.ray_color = ray_color;
+    pf.s = s;
+    const int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
+    const int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
+    return Footprint{ssaa_pixel(pf, int64_t(ya) * W + xa), ssaa_pixel(pf, int64_t(ya) * W + xb),
+                     ssaa_pixel(pf, int64_t(yb) * W + xa), ssaa_pixel(pf, int64_t(yb) * W + xb)};
 }
 
-// fxaa (kernels/ssaa.cuh:150-182) on every pixel with clamp-to-edge taps,
-// fused with the RGB8 quantisation.  A workgroup of 256 threads takes a
-// 32 x 32 tile, four pixels per thread (rows ty, ty + 8, ty + 16, ty + 24):
-// its pixels and a 1-pixel halo (clamped to the image) are staged in LDS
-// with their lumas, computed once per pixel, the staging loads of a thread
-// all in flight together.  A bilinear tap reads LDS when its 2 x 2 footprint
-// lies in the staged 34 x 34 block; with luma / 255 (D14) the search span is
-// at most about one pixel for colours in [0, 1], so every tap of a render
-// does.  Larger spans (colours above 1) read the image in global memory.
-// The arithmetic is the same either way.
-constexpr int kFxTile = 32, kFxHalo = kFxTile + 2, kFxRows = 256 / kFxTile;
-
-__device__ inline float3 fxaa_tap_lds(const float *im, const float (*tile)[kFxHalo][3], int W, int H, int ox,
-                                      int oy, float X, float Y) {
+__device__ inline float3 fxaa_tap_lds(const PostFrameDev &pf, const float (*tile)[kFxHalo][3], int ox, int oy,
+                                      float X, float Y) {
 #pragma clang fp contract(off)
     const float fx0 = floorf(X), fy0 = floorf(Y);
     const float fx = X - fx0, fy = Y - fy0;
@@ -253,12 +269,11 @@ __device__ inline float3 fxaa_tap_lds(const float *im, const float (*tile)[kFxHa
         c01 = make_float3(tile[ly + 1][lx][0], tile[ly + 1][lx][1], tile[ly + 1][lx][2]);
         c11 = make_float3(tile[ly + 1][lx + 1][0], tile[ly + 1][lx + 1][1], tile[ly + 1][lx + 1][2]);
     } else {
-        const int xa = min(max(x0, 0), W - 1), xb = min(max(x0 + 1, 0), W - 1);
-        const int ya = min(max(y0, 0), H - 1), yb = min(max(y0 + 1, 0), H - 1);
-        c00 = load3(im, W, xa, ya);
-        c10 = load3(im, W, xb, ya);
-        c01 = load3(im, W, xa, yb);
-        c11 = load3(im, W, xb, yb);
+        const Footprint f = fxaa_footprint(pf.ray_color, pf.W, pf.H, pf.s, x0, y0);
+        c00 = f.c00;
+        c10 = f.c10;
+        c01 = f.c01;
+        c11 = f.c11;
     }
     const float w00 = (1.0f - fx) * (1.0f - fy), w10 = fx * (1.0f - fy), w01 = (1.0f - fx) * fy, w11 = fx * fy;
     return make_float3(c00.x * w00 + c10.x * w10 + c01.x * w01 + c11.x * w11,
@@ -266,40 +281,109 @@ __device__ inline float3 fxaa_tap_lds(const float *im, const float (*tile)[kFxHa
                        c00.z * w00 + c10.z * w10 + c01.z * w01 + c11.z * w11);
 }
 
-__global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
+// combineSamples of S samples held in v (3 S floats): the same arithmetic
+// as ssaa_pixel.
+template <int S> __device__ inline float3 ssaa_combine(const float *v) {
 #pragma clang fp contract(off)
-    __shared__ float tile[kFxHalo][kFxHalo][3];
-    __shared__ float lum[kFxHalo][kFxHalo];
-    const PostFrameDev pf = pfs[blockIdx.z];
-    const int W = pf.W, H = pf.H;
-    const int ox = int(blockIdx.x) * kFxTile, oy = int(blockIdx.y) * kFxTile;
-    if (!pf.fxaa || ox >= W || oy >= H) return; // uniform over the workgroup
-    const float *im = pf.img;
-    // stage the 34 x 34 block (clamp-to-edge) and its lumas: 1156 pixels, at
-    // most five per thread, loaded before any is written
-    constexpr int kStage = kFxHalo * kFxHalo, kPer = (kStage + 255) / 256;
-    float3 c[kPer];
+    if constexpr (S == 1) {
+        return make_float3(clamp01(v[0]), clamp01(v[1]), clamp01(v[2]));
+    } else {
+        float acc[3] = {0.f, 0.f, 0.f};
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        const int t = int(threadIdx.x) + 256 * r;
-        if (t < kStage) {
+        for (int k = 0; k < S; ++k) {
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) acc[ch] += clamp01(v[3 * k + ch] * v[3 * k + ch]);
+        }
+        return make_float3(sqrtf(acc[0] / float(S)), sqrtf(acc[1] / float(S)), sqrtf(acc[2] / float(S)));
+    }
+}
+
+// The 3 S floats of a pixel's samples at src into v, 16 bytes per load
+// (S a multiple of 4, src 16-byte aligned) or 4 (S = 1).
+template <int S> __device__ inline void load_samples(const float *src, float *v) {
+    if constexpr (S % 4 == 0) {
+#pragma unroll
+        for (int i = 0; i < 3 * S / 4; ++i) {
+            const float4 q = reinterpret_cast<const float4 *>(src)[i];
+            v[4 * i] = q.x;
+            v[4 * i + 1] = q.y;
+            v[4 * i + 2] = q.z;
+            v[4 * i + 3] = q.w;
+        }
+    } else {
+#pragma unroll
+        for (int i = 0; i < 3 * S; ++i) v[i] = src[i];
+    }
+}
+
+// This thread's pixels of the 34 x 34 block (1156 pixels, at most three per
+// thread) of the tile at (ox, oy), coordinates clamped to the image.  S: the
+// frame's sample count when it is 1, 4 or 16 (the last two with 16-byte-
+// aligned ray colours), else 0 (any count).  For S = 1 and 4 the raw samples
+// are kept, all of a thread's loads issued before fxaa_put combines them;
+// otherwise (16 samples would not fit the registers) the combined colours.
+constexpr int kFxStage = kFxHalo * kFxHalo, kFxPer = (kFxStage + kFxThreads - 1) / kFxThreads;
+template <int S> struct FxStaged {
+    static constexpr bool raw = S == 1 || S == 4;
+    float v[kFxPer][raw ? 3 * S : 3];
+};
+
+template <int S> __device__ inline void fxaa_get(const PostFrameDev &pf, int ox, int oy, FxStaged<S> &st) {
+    constexpr int kUnroll = S == 16 ? 1 : kFxPer;
+    const int W = pf.W, H = pf.H;
+#pragma unroll kUnroll
+    for (int r = 0; r < kFxPer; ++r) {
+        const int t = int(threadIdx.x) + kFxThreads * r;
+        if (t < kFxStage) {
             const int ly = t / kFxHalo, lx = t - ly * kFxHalo;
             const int xx = min(max(ox + lx - 1, 0), W - 1), yy = min(max(oy + ly - 1, 0), H - 1);
-            c[r] = load3(im, W, xx, yy);
+            const int64_t p = int64_t(yy) * W + xx;
+            if constexpr (FxStaged<S>::raw) {
+                load_samples<S>(pf.ray_color + 3 * S * p, st.v[r]);
+            } else {
+                float3 c;
+                if constexpr (S == 16) {
+                    float v[3 * S];
+                    load_samples<S>(pf.ray_color + 3 * S * p, v);
+                    c = ssaa_combine<S>(v);
+                } else {
+                    c = ssaa_pixel(pf, p);
+                }
+                st.v[r][0] = c.x;
+                st.v[r][1] = c.y;
+                st.v[r][2] = c.z;
+            }
         }
     }
+}
+
+// The staged pixels, combined, into LDS with their lumas.
+template <int S>
+__device__ inline void fxaa_put(const FxStaged<S> &st, float (*tile)[kFxHalo][3], float (*lum)[kFxHalo]) {
 #pragma unroll
-    for (int r = 0; r < kPer; ++r) {
-        const int t = int(threadIdx.x) + 256 * r;
-        if (t < kStage) {
+    for (int r = 0; r < kFxPer; ++r) {
+        const int t = int(threadIdx.x) + kFxThreads * r;
+        if (t < kFxStage) {
             const int ly = t / kFxHalo, lx = t - ly * kFxHalo;
-            tile[ly][lx][0] = c[r].x;
-            tile[ly][lx][1] = c[r].y;
-            tile[ly][lx][2] = c[r].z;
-            lum[ly][lx] = luma(c[r].x, c[r].y, c[r].z);
+            float3 c;
+            if constexpr (FxStaged<S>::raw)
+                c = ssaa_combine<S>(st.v[r]);
+            else
+                c = make_float3(st.v[r][0], st.v[r][1], st.v[r][2]);
+            tile[ly][lx][0] = c.x;
+            tile[ly][lx][1] = c.y;
+            tile[ly][lx][2] = c.z;
+            lum[ly][lx] = luma(c.x, c.y, c.z);
         }
     }
-    __syncthreads();
+}
+
+// FXAA of the staged tile at (ox, oy): two pixels per thread (rows ty and
+// ty + 16), written as the linear image and RGB8.
+__device__ inline void fxaa_filter(const PostFrameDev &pf, const float (*tile)[kFxHalo][3], const float (*lum)[kFxHalo],
+                                   int ox, int oy) {
+#pragma clang fp contract(off)
+    const int W = pf.W, H = pf.H;
     const int tx = int(threadIdx.x) % kFxTile, ty0 = int(threadIdx.x) / kFxTile;
     const int x = ox + tx;
     if (x >= W) return;
@@ -319,10 +403,10 @@ __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
         dy = fminf(fmaxf(dy * rcp, -8.0f), 8.0f);
         const float k1 = 1.0f / 3.0f - 0.5f, k2 = 2.0f / 3.0f - 0.5f;
         const float fxp = float(x), fyp = float(y);
-        const float3 t1 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k1, fyp + dy * k1);
-        const float3 t2 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * k2, fyp + dy * k2);
-        const float3 t3 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * -0.5f, fyp + dy * -0.5f);
-        const float3 t4 = fxaa_tap_lds(im, tile, W, H, ox, oy, fxp + dx * 0.5f, fyp + dy * 0.5f);
+        const float3 t1 = fxaa_tap_lds(pf, tile, ox, oy, fxp + dx * k1, fyp + dy * k1);
+        const float3 t2 = fxaa_tap_lds(pf, tile, ox, oy, fxp + dx * k2, fyp + dy * k2);
+        const float3 t3 = fxaa_tap_lds(pf, tile, ox, oy, fxp + dx * -0.5f, fyp + dy * -0.5f);
+        const float3 t4 = fxaa_tap_lds(pf, tile, ox, oy, fxp + dx * 0.5f, fyp + dy * 0.5f);
         const float3 A = make_float3(t1.x * 0.5f + t2.x * 0.5f, t1.y * 0.5f + t2.y * 0.5f, t1.z * 0.5f + t2.z * 0.5f);
         const float3 B = make_float3(0.5f * A.x + (t3.x * 0.25f + t4.x * 0.25f),
                                      0.5f * A.y + (t3.y * 0.25f + t4.y * 0.25f),
@@ -340,6 +424,34 @@ __global__ void __launch_bounds__(256) k_fxaa(const PostFrameDev *pfs) {
             pf.rgb8[3 * p + 1] = to_rgb8(o.y);
             pf.rgb8[3 * p + 2] = to_rgb8(o.z);
         }
+    }
+}
+
+// One tile: stage, then filter.
+template <int S>
+__device__ inline void fxaa_tile(const PostFrameDev &pf, float (*tile)[kFxHalo][3], float (*lum)[kFxHalo], int ox,
+                                 int oy) {
+    FxStaged<S> c;
+    fxaa_get<S>(pf, ox, oy, c);
+    fxaa_put<S>(c, tile, lum);
+    __syncthreads();
+    fxaa_filter(pf, tile, lum, ox, oy);
+}
+
+// fxaa with the SSAA combine (see above): grid (tiles in x, tiles in y,
+// frames), one 32 x 32 tile per workgroup.
+__global__ void __launch_bounds__(kFxThreads) k_fxaa(const PostFrameDev *pfs) {
+    __shared__ float tile[kFxHalo][kFxHalo][3];
+    __shared__ float lum[kFxHalo][kFxHalo];
+    const PostFrameDev pf = pfs[blockIdx.z];
+    const int ox = int(blockIdx.x) * kFxTile, oy = int(blockIdx.y) * kFxTile;
+    if (!pf.fxaa || ox >= pf.W || oy >= pf.H) return; // uniform over the workgroup
+    const bool al16 = (reinterpret_cast<uintptr_t>(pf.ray_color) & 15) == 0;
+    switch (pf.s) {
+    case 1: fxaa_tile<1>(pf, tile, lum, ox, oy); break;
+    case 4: al16 ? fxaa_tile<4>(pf, tile, lum, ox, oy) : fxaa_tile<0>(pf, tile, lum, ox, oy); break;
+    case 16: al16 ? fxaa_tile<16>(pf, tile, lum, ox, oy) : fxaa_tile<0>(pf, tile, lum, ox, oy); break;
+    default: fxaa_tile<0>(pf, tile, lum, ox, oy); break;
     }
 }
 
@@ -772,7 +884,7 @@ hipError_t launch_ssaa(const PostFrameDev *pfs, int nframes, int64_t max_pixels,
 hipError_t launch_fxaa(const PostFrameDev *pfs, int nframes, int max_w, int max_h, hipStream_t st) {
     const dim3 g(unsigned((max_w + kFxTile - 1) / kFxTile), unsigned((max_h + kFxTile - 1) / kFxTile),
                  unsigned(nframes)),
-        b(256);
+        b(kFxThreads);
     hipLaunchKernelGGL(k_fxaa, g, b, 0, st, pfs);
     return hipGetLastError();
 }

@@ -476,6 +476,24 @@ def test_aa_passes_bit_exact():
     assert np.array_equal(out["rgb8"], O.rgb8(ss))
 
 
+def test_aa_passes_bit_exact_every_sample_count():
+    """The FXAA pass combines each frame's samples itself: one batch of FXAA
+    frames with 1, 4, 9 and 16 samples per pixel, the 4-sample frame placed
+    after a frame of an odd ray count (its ray colours start off a 16-byte
+    boundary), and a frame without FXAA; SSAA, FXAA and RGB8 of the GPU's own
+    ray colours equal the oracle's passes bit for bit in every frame."""
+    frames = [CF.frame_c1(29, 17, ssaa=1, fxaa=1), CF.frame_c1(40, 24), CF.frame_c1(36, 20, ssaa=3, fxaa=1),
+              CF.frame_c1(24, 16, ssaa=4, fxaa=1), CF.frame_c1(21, 13, ssaa=2, fxaa=0)]
+    assert [f.samples for f in frames] == [1, 4, 9, 16, 4] and frames[0].nrays % 4 == 1
+    ctx = gpu_context(A.PRR_KERR_BL, CF.tracer_kerr())
+    outs = ctx.render(frames, rays=True)
+    for f, out in zip(frames, outs):
+        ss = O.ssaa(out["color"], f.width, f.height, f.samples)
+        img = O.fxaa(ss) if f.fxaa else ss
+        assert np.array_equal(out["rgbf"], img), f.samples
+        assert np.array_equal(out["rgb8"], O.rgb8(img)), f.samples
+
+
 @pytest.mark.parametrize("width,height", [(1, 1), (1, 7), (7, 1), (2, 2), (33, 17), (65, 34)])
 def test_tiny_and_ragged_frames(width, height):
     """Frames of one pixel, one row or column, and sizes that do not divide
