@@ -101,7 +101,7 @@ def gbs(nbytes, ms):
     return nbytes / (ms / 1e3) / 1e9 if nbytes > 0 and ms > 0 else None
 
 
-def run_adjoint(api, A, CF, reps, torch=None, ids=None):
+def run_adjoint(api, A, CF, reps, torch=None, ids=None, warmup=2):
     """C3: 512x512 Kerr a=0.7, FP32-bulk RKF45 tol 1e-8; gradient of the L2 loss against the target
     render w.r.t. (a, r_cam, incl, sphere phi).  Rays/s counts primary rays through the whole
     prr_adjoint call (forward render, checkpointed forward, replay and reverse sweep)."""
@@ -116,13 +116,18 @@ def run_adjoint(api, A, CF, reps, torch=None, ids=None):
     T = ctx.render(api.frame_from_orbit(tmpl, ot))[0]["rgbf"]
     I = ctx.render(api.frame_from_orbit(tmpl, o0))[0]["rgbf"]
     dLdI = (2.0 * (I - T) / I.size).astype(np.float32)
-    g = ctx.adjoint(tmpl, o0, dLdI)  # warm-up
-    t0 = time.perf_counter()
-    for _ in range(reps):
+    for _ in range(max(1, warmup)):
         g = ctx.adjoint(tmpl, o0, dLdI)
-    sec = (time.perf_counter() - t0) / reps
+    per = []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g = ctx.adjoint(tmpl, o0, dLdI)
+        per.append(time.perf_counter() - t0)
+    sec = sum(per) / reps
     st = ctx.stats()
-    out = dict(rays=tmpl.nrays, seconds=sec, rays_per_s=tmpl.nrays / sec, adjoint_ms=st["adjoint_ms"],
+    out = dict(rays=tmpl.nrays, seconds=sec, rays_per_s=tmpl.nrays / sec, call_ms=[round(x * 1e3, 3) for x in per],
+               median_call_ms=sorted(per)[len(per) // 2] * 1e3,
+               adjoint_ms=st["adjoint_ms"],
                total_ms=st["total_ms"], steps_swept=st["adjoint_steps"],
                steps_per_s=st["adjoint_steps"] / (st["adjoint_ms"] / 1e3),
                grad={"a": g[0], "r_cam": g[1], "incl": g[2], "sphere_phi": g[3]},
@@ -460,7 +465,7 @@ def main():
         c1_rays["fp32_bulk"] = gpu_rays(ctx, [frame], ids_c1, torch)
         progress(fp32_bulk_ms_per_step=fp32["ms_per_step"])
     if "adjoint" in legs:
-        adj = run_adjoint(api, A, CF, max(1, args.steps // 2), torch, ids_c3)
+        adj = run_adjoint(api, A, CF, max(1, args.steps // 2), torch, ids_c3, warmup=args.warmup)
         progress(adjoint={k: v for k, v in adj.items() if not k.startswith("_")})
     if "c4" in legs:
         c4 = run_c4(api, A, CF, torch, 2, ids_c4)

@@ -101,6 +101,23 @@ prr_status prr::ensure(prr_ctx *ctx, DevBuf &b, size_t bytes) {
     return PRR_OK;
 }
 
+prr_status prr::ensure_pinned(prr_ctx *ctx, size_t bytes) {
+    if (ctx->pinned_bytes >= bytes && ctx->pinned) return PRR_OK;
+    if (ctx->pinned) {
+        hipError_t e = hipHostFree(ctx->pinned);
+        if (e != hipSuccess) return fail(ctx, PRR_EHIP, "hipHostFree failed: %s", hipGetErrorString(e));
+        ctx->pinned = nullptr;
+        ctx->pinned_bytes = 0;
+    }
+    hipError_t e = hipHostMalloc(&ctx->pinned, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        ctx->pinned = nullptr;
+        return fail(ctx, PRR_ENOMEM, "hipHostMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+    }
+    ctx->pinned_bytes = bytes;
+    return PRR_OK;
+}
+
 namespace {
 
 void release(DevBuf &b) {
@@ -300,6 +317,7 @@ void free_context(prr_ctx *ctx) {
                       &ctx->adj_task_ray, &ctx->adj_task_seg, &ctx->adj_levelstart, &ctx->adj_nseg,
                       &ctx->adj_over, &ctx->adj_plan, &ctx->adj_lam, &ctx->adj_grad_ray, &ctx->adj_grad};
     for (DevBuf *b : bufs) release(*b);
+    if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (auto &ev : ctx->ev)
         if (ev) (void)hipEventDestroy(ev);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -556,6 +574,10 @@ prr_status prr::read_stats(prr_ctx *ctx, const RenderPlan &P, const CountersDev 
     CountersDev c{};
     HIPTRY(ctx, hipMemcpyAsync(&c, counters, sizeof c, hipMemcpyDeviceToHost, st));
     HIPTRY(ctx, hipStreamSynchronize(st));
+    return stats_from(ctx, P, c, events);
+}
+
+prr_status prr::stats_from(prr_ctx *ctx, const RenderPlan &P, const CountersDev &c, bool events) {
     prr_stats &s = ctx->stats;
     s = prr_stats{};
     s.rays = int64_t(c.rays);
@@ -921,13 +943,23 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     const int64_t seg_bound = (ctx->tracer.max_steps - 1) / K + 2; // exceeds every segment count
     const bool plan_dev = seg_bound <= kAdjPlanBins;
     std::vector<unsigned long long> hist; // rays per segment count 0..nseg_max
-    unsigned long long pctr[4] = {0, 0, 0, 0};
     int64_t nck2 = 0, swept = 0, n_over = 0;
     int32_t nseg_max = 0;
     if ((s = ensure(ctx, ctx->adj_nseg, 4 * size_t(T))) != PRR_OK) return s;
     if ((s = ensure(ctx, ctx->adj_over, 8 * size_t(T))) != PRR_OK) return s;
     // plan buffer: histogram, counters, then the bin starts and level starts (one copy from the host)
     if ((s = ensure(ctx, ctx->adj_plan, 8 * size_t(3 * kAdjPlanBins + 4))) != PRR_OK) return s;
+    // pinned staging: the histogram and plan counters, the first pass's
+    // counters, the bin and level starts, then the final counters and gradient
+    struct Pinned {
+        unsigned long long hist[kAdjPlanBins + 4];
+        CountersDev fwd;
+        unsigned long long cursor[2 * kAdjPlanBins];
+        CountersDev fin;
+        double grad[4];
+    };
+    if ((s = ensure_pinned(ctx, sizeof(Pinned))) != PRR_OK) return s;
+    Pinned *pin = static_cast<Pinned *>(ctx->pinned);
     AdjPlanDev plan{};
     plan.steps = static_cast<const int32_t *>(ctx->steps.p);
     plan.T = T;
@@ -945,24 +977,25 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     // host buffers of the copies below live until the final synchronisation
     std::vector<int64_t> order_h, off_h, over_h;
     std::vector<int32_t> steps_h;
-    std::vector<unsigned long long> cursor_h;
     if (plan_dev) {
         HIPTRY(ctx, launch_adj_plan_count(plan, st));
-        hist.assign(size_t(kAdjPlanBins), 0ull);
-        HIPTRY(ctx, hipMemcpyAsync(hist.data(), plan.hist, 8 * size_t(kAdjPlanBins), hipMemcpyDeviceToHost, st));
-        HIPTRY(ctx, hipMemcpyAsync(pctr, plan.ctr, sizeof pctr, hipMemcpyDeviceToHost, st));
+        // the histogram and the four plan counters after it, in one copy
+        HIPTRY(ctx, hipMemcpyAsync(pin->hist, plan.hist, sizeof pin->hist, hipMemcpyDeviceToHost, st));
     } else {
         steps_h.resize(size_t(T));
         HIPTRY(ctx, hipMemcpyAsync(steps_h.data(), ctx->steps.p, 4 * size_t(T), hipMemcpyDeviceToHost, st));
     }
-    if ((s = read_stats(ctx, P, ws.counters, st, false)) != PRR_OK) return s; // synchronises
+    HIPTRY(ctx, hipMemcpyAsync(&pin->fwd, ws.counters, sizeof pin->fwd, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipStreamSynchronize(st));
+    if ((s = stats_from(ctx, P, pin->fwd, false)) != PRR_OK) return s;
     prr_stats fwd = ctx->stats;
     if (plan_dev) {
+        const unsigned long long *pctr = pin->hist + kAdjPlanBins;
         nck2 = int64_t(pctr[0]);
         n_over = int64_t(pctr[1]);
         swept = int64_t(pctr[2]);
         nseg_max = int32_t(pctr[3]);
-        hist.resize(size_t(nseg_max) + 1);
+        hist.assign(pin->hist, pin->hist + nseg_max + 1);
     } else {
         std::vector<int32_t> nseg(static_cast<size_t>(T));
         off_h.resize(size_t(T));
@@ -1001,10 +1034,11 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         ntask += count[j];
     }
     if (nseg_max < kAdjPlanBins) { // bin starts and level starts in one copy
-        cursor_h.assign(size_t(2 * kAdjPlanBins), 0ull);
-        for (int32_t n = nseg_max - 1; n >= 0; --n) cursor_h[n] = cursor_h[n + 1] + hist[n + 1];
-        for (int32_t j = 0; j <= nseg_max; ++j) cursor_h[kAdjPlanBins + j] = uint64_t(levelstart[j]);
-        HIPTRY(ctx, hipMemcpyAsync(plan.cursor, cursor_h.data(), 8 * cursor_h.size(), hipMemcpyHostToDevice, st));
+        unsigned long long *cursor = pin->cursor;
+        std::fill(cursor, cursor + 2 * kAdjPlanBins, 0ull);
+        for (int32_t n = nseg_max - 1; n >= 0; --n) cursor[n] = cursor[n + 1] + hist[n + 1];
+        for (int32_t j = 0; j <= nseg_max; ++j) cursor[kAdjPlanBins + j] = uint64_t(levelstart[j]);
+        HIPTRY(ctx, hipMemcpyAsync(plan.cursor, cursor, sizeof pin->cursor, hipMemcpyHostToDevice, st));
     } else { // host plan with more levels than the plan buffer holds
         if ((s = ensure(ctx, ctx->adj_levelstart, 8 * levelstart.size())) != PRR_OK) return s;
         levelstart_d = static_cast<int64_t *>(ctx->adj_levelstart.p);
@@ -1106,13 +1140,14 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     HIPTRY(ctx, launch_grad_reduce(ad.grad_ray, T, static_cast<double *>(ctx->adj_grad.p) + 4,
                                    static_cast<double *>(ctx->adj_grad.p), st));
     HIPTRY(ctx, hipEventRecord(ctx->ev[2], st));
-    CountersDev c{};
-    HIPTRY(ctx, hipMemcpyAsync(&c, ws.counters, sizeof c, hipMemcpyDeviceToHost, st));
-    HIPTRY(ctx, hipMemcpyAsync(grad, ctx->adj_grad.p, 32, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipMemcpyAsync(&pin->fin, ws.counters, sizeof pin->fin, hipMemcpyDeviceToHost, st));
+    HIPTRY(ctx, hipMemcpyAsync(pin->grad, ctx->adj_grad.p, sizeof pin->grad, hipMemcpyDeviceToHost, st));
     if (img_out)
         HIPTRY(ctx, hipMemcpyAsync(img_out, ctx->adj_img.p, 12 * size_t(npix),
                                    on_device ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, st));
     HIPTRY(ctx, hipStreamSynchronize(st));
+    const CountersDev c = pin->fin;
+    std::copy(pin->grad, pin->grad + 4, grad);
     float ms_all = 0.f, ms_adj = 0.f;
     HIPTRY(ctx, hipEventElapsedTime(&ms_all, ctx->ev[0], ctx->ev[2]));
     HIPTRY(ctx, hipEventElapsedTime(&ms_adj, ctx->ev[1], ctx->ev[2]));

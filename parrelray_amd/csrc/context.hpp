@@ -98,6 +98,10 @@ struct prr_ctx {
     prr::DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_ckpt2, adj_event, adj_rays, adj_rec, adj_lam, adj_grad_ray,
         adj_task_ray, adj_task_seg, adj_levelstart, adj_nseg, adj_over, adj_plan,
         adj_grad;
+    // pinned host memory for the adjoint's small copies between passes
+    // (pageable copies cost tens of microseconds each)
+    void *pinned = nullptr;
+    size_t pinned_bytes = 0;
     hipEvent_t ev[6] = {};
     prr_stats stats{};
     int live_graphs = 0;    // graphs captured on this context and not yet destroyed
@@ -124,6 +128,8 @@ namespace prr {
 prr_status fail(prr_ctx *ctx, prr_status st, const char *fmt, ...);
 // Grows b to at least `bytes` (its contents are not kept).
 prr_status ensure(prr_ctx *ctx, DevBuf &b, size_t bytes);
+// Grows the context's pinned host block to at least `bytes` (contents not kept).
+prr_status ensure_pinned(prr_ctx *ctx, size_t bytes);
 
 // Validates the frames, fills the frame table fd[0..n) and the totals.
 prr_status plan_frames(prr_ctx *ctx, const prr_frame *frames, int n, FrameDev *fd, RenderPlan &P);
@@ -141,6 +147,8 @@ prr_status enqueue_render(prr_ctx *ctx, const RenderPlan &P, const JobDev &job, 
                           bool events);
 // Counters of the finished call (synchronises the stream) into ctx->stats.
 prr_status read_stats(prr_ctx *ctx, const RenderPlan &P, const CountersDev *counters, hipStream_t st, bool events);
+// ctx->stats from counters already copied to the host (the stream synchronised).
+prr_status stats_from(prr_ctx *ctx, const RenderPlan &P, const CountersDev &c, bool events);
 // With `device` current: the device address of its copy of the allocation-free
 // workspace, looked up once (noalloc.cpp; called by prr_create).
 prr_status noalloc_setup(int device);
