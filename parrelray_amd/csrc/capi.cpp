@@ -942,7 +942,10 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     // segments; else its checkpoints go to the overflow area at -1 - off[i].
     const int64_t seg_bound = (ctx->tracer.max_steps - 1) / K + 2; // exceeds every segment count
     const bool plan_dev = seg_bound <= kAdjPlanBins;
+    // length classes of the last segment per segment count in the device plan's bins
+    const int32_t L = int32_t(std::min<int64_t>(8, std::max<int64_t>(1, kAdjPlanBins / seg_bound)));
     std::vector<unsigned long long> hist; // rays per segment count 0..nseg_max
+    std::vector<unsigned long long> histb; // rays per bin of the order (device plan: (count, class))
     int64_t nck2 = 0, swept = 0, n_over = 0;
     int32_t nseg_max = 0;
     if ((s = ensure(ctx, ctx->adj_nseg, 4 * size_t(T))) != PRR_OK) return s;
@@ -966,6 +969,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     plan.K = K;
     plan.cap = cap;
     plan.nbins = kAdjPlanBins;
+    plan.L = L;
     plan.nseg = static_cast<int32_t *>(ctx->adj_nseg.p);
     plan.off = static_cast<int64_t *>(ctx->adj_off.p);
     plan.over = static_cast<int64_t *>(ctx->adj_over.p);
@@ -995,7 +999,9 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         n_over = int64_t(pctr[1]);
         swept = int64_t(pctr[2]);
         nseg_max = int32_t(pctr[3]);
-        hist.assign(pin->hist, pin->hist + nseg_max + 1);
+        histb.assign(pin->hist, pin->hist + size_t(nseg_max + 1) * L);
+        hist.assign(size_t(nseg_max) + 1, 0ull);
+        for (size_t b = 0; b < histb.size(); ++b) hist[b / L] += histb[b];
     } else {
         std::vector<int32_t> nseg(static_cast<size_t>(T));
         off_h.resize(size_t(T));
@@ -1015,6 +1021,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         n_over = int64_t(over_h.size());
         hist.assign(size_t(nseg_max) + 1, 0ull);
         for (int64_t i = 0; i < T; ++i) hist[nseg[i]]++;
+        histb = hist; // the host plan orders by segment count only
         std::vector<int64_t> start(size_t(nseg_max) + 2, 0);
         for (int32_t n = nseg_max; n >= 0; --n) start[n] = start[n + 1] + (n + 1 <= nseg_max ? int64_t(hist[n + 1]) : 0);
         for (int64_t i = 0; i < T; ++i) order_h[start[nseg[i]]++] = i;
@@ -1025,7 +1032,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     }
     // rays with more than j segments (count[j], a prefix of the order) and
     // the first task of each level j (levels from the last); device plan:
-    // the start of each segment count in the order (decreasing counts first)
+    // the start of each bin in the order (decreasing bins first)
     std::vector<int64_t> count(size_t(nseg_max) + 1, 0), levelstart(size_t(nseg_max) + 1, 0);
     for (int32_t j = nseg_max - 1; j >= 0; --j) count[j] = count[j + 1] + int64_t(hist[j + 1]);
     int64_t ntask = 0;
@@ -1036,7 +1043,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if (nseg_max < kAdjPlanBins) { // bin starts and level starts in one copy
         unsigned long long *cursor = pin->cursor;
         std::fill(cursor, cursor + 2 * kAdjPlanBins, 0ull);
-        for (int32_t n = nseg_max - 1; n >= 0; --n) cursor[n] = cursor[n + 1] + hist[n + 1];
+        for (int64_t b = int64_t(histb.size()) - 2; b >= 0; --b) cursor[b] = cursor[b + 1] + histb[size_t(b) + 1];
         for (int32_t j = 0; j <= nseg_max; ++j) cursor[kAdjPlanBins + j] = uint64_t(levelstart[j]);
         HIPTRY(ctx, hipMemcpyAsync(plan.cursor, cursor, sizeof pin->cursor, hipMemcpyHostToDevice, st));
     } else { // host plan with more levels than the plan buffer holds

@@ -188,15 +188,16 @@ struct AdjPlanDev {
     const int32_t *steps;        // accepted steps of every ray
     int64_t T;                   // rays
     int32_t K, cap;              // checkpoint interval; first-pass checkpoint slots per ray
-    int32_t nbins;               // histogram bins (> every segment count), at most kAdjPlanBins
+    int32_t nbins;               // histogram bins (> every bin below), at most kAdjPlanBins
+    int32_t L;                   // length classes of the last segment per segment count (bin n L + class)
     int32_t nlevels;             // tasks: segment levels (the largest segment count)
     int32_t *nseg;               // segments of every ray
     int64_t *off;                // checkpoint offsets (AdjRunDev.ckpt_off)
     int64_t *over;               // the rays with more than cap segments, any order
-    unsigned long long *hist;    // rays per segment count
+    unsigned long long *hist;    // rays per bin: segment count, then the last segment's length class
     unsigned long long *ctr;     // [0] overflow checkpoints, [1] overflow rays, [2] accepted steps, [3] max segments
-    unsigned long long *cursor;  // order: next position of each segment count (start positions from the host)
-    int64_t *order;              // rays by decreasing segment count
+    unsigned long long *cursor;  // order: next position of each bin (start positions from the host)
+    int64_t *order;              // rays by decreasing bin (segment count, then last segment's length)
     const int64_t *levelstart;   // tasks: first task of level j (levels descending: level nlevels-1 first)
     int64_t ntask;
     int64_t *task_ray;

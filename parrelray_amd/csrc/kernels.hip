@@ -639,6 +639,18 @@ __global__ void __launch_bounds__(256) k_adj_plan_clear(AdjPlanDev p) {
     if (i < 4) p.ctr[i] = 0ull;
 }
 
+// Histogram bin of a ray with st accepted steps: its segment count n, then
+// the length class of its last segment (1..K steps in L classes, longer in
+// higher bins).  Ordered by decreasing bin, the rays with more than j
+// segments stay a prefix of the order, and at each level the rays whose last
+// segment it is follow the full segments from the longest down, so that a
+// wave of replay or reverse-sweep tasks holds segments of similar length.
+__device__ inline int plan_bin(const AdjPlanDev &p, int32_t st) {
+    const int32_t n = st > 0 ? (st - 1) / p.K + 1 : 0;
+    const int32_t c = n > 0 ? (st - (n - 1) * p.K - 1) * p.L / p.K : 0;
+    return min(n * p.L + c, p.nbins - 1);
+}
+
 // Counts are gathered per workgroup in LDS and flushed with one global atomic
 // per non-empty bin, so that the few bins do not serialise every wave.
 __global__ void __launch_bounds__(256) k_adj_plan_count(AdjPlanDev p) {
@@ -660,7 +672,7 @@ __global__ void __launch_bounds__(256) k_adj_plan_count(AdjPlanDev p) {
             p.off[i] = -1 - int64_t(base);
             p.over[atomicAdd(&p.ctr[1], 1ull)] = i;
         }
-        atomicAdd(&lcount[min(n, p.nbins - 1)], 1u);
+        atomicAdd(&lcount[plan_bin(p, st)], 1u);
     }
     const unsigned long long sum = wave_sum((unsigned long long)st);
     unsigned mx = unsigned(n);
@@ -685,8 +697,8 @@ __global__ void __launch_bounds__(256) k_adj_plan_count(AdjPlanDev p) {
     }
 }
 
-// Each workgroup claims one run of positions per segment count it holds;
-// its rays take consecutive positions of that run.
+// Each workgroup claims one run of positions per bin it holds; its rays take
+// consecutive positions of that run.
 __global__ void __launch_bounds__(256) k_adj_plan_scatter(AdjPlanDev p) {
     __shared__ unsigned lcount[kAdjPlanBins];
     __shared__ unsigned long long lbase[kAdjPlanBins];
@@ -694,7 +706,7 @@ __global__ void __launch_bounds__(256) k_adj_plan_scatter(AdjPlanDev p) {
     __syncthreads();
     const int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     const bool on = i < p.T;
-    const int32_t n = on ? min(p.nseg[i], p.nbins - 1) : 0;
+    const int n = on ? plan_bin(p, p.steps[i]) : 0;
     const unsigned rank = on ? atomicAdd(&lcount[n], 1u) : 0u;
     __syncthreads();
     for (int b = threadIdx.x; b < p.nbins; b += blockDim.x)
