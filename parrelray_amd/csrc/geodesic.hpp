@@ -141,6 +141,21 @@ PRR_HD float rsqrt_f32(float x) {
 #endif
 }
 
+// max(|a|, |b|) for values that are never signalling NaNs.  On the device one
+// v_max_f64 with absolute-value source modifiers: fmax must quiet signalling
+// NaNs first, which costs a v_max_f64 per operand whose origin the compiler
+// cannot see (a loop-carried state).  NaN operands give the other operand,
+// as fmax does.
+PRR_HD double fmax_abs(double a, double b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    double r;
+    __asm__("v_max_f64 %0, |%1|, |%2|" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+#else
+    return fmax(fabs(a), fabs(b));
+#endif
+}
+
 // Per-ray constants: mass and spin, and for Kerr the conserved energy E,
 // axial angular momentum L and Carter constant Q.
 template <class T> struct Consts {

@@ -1205,7 +1205,7 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         // errn = max_i ei / sc_i; NaN components are ignored as by fmax
 #pragma unroll
         for (int i = 0; i < Mt::NERR; ++i) {
-            const double sc = tr.atol + tr.rtol * fmax(fabs(L.y[i]), fabs(yn[i]));
+            const double sc = tr.atol + tr.rtol * fmax_abs(L.y[i], yn[i]);
             const double ei = (Mt::mino && (i == 2 || i == 3)) ? fabs(e[i]) * ist : fabs(e[i]);
             errn = fmax(errn, ei * rcp_t(sc));
         }
