@@ -74,6 +74,8 @@ FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t
         for (int c = 0; c < 3; ++c) d.sph[k][c] = f.spheres[k].c[c];
         d.sph[k][3] = f.spheres[k].radius;
         d.sph_tex[k] = f.spheres[k].tex;
+        for (int c = 0; c < 4; ++c) d.sphf[k][c] = float(d.sph[k][c]);
+        d.sphf[k][4] = std::fabs(d.sphf[k][0]) + std::fabs(d.sphf[k][1]) + std::fabs(d.sphf[k][2]) + d.sphf[k][3];
     }
     // initRays' per-frame terms (the camera position through pos_cartTOspher,
     // cudamaths/transform.cuh:15-22, and the Kerr metric there)

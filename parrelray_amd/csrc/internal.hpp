@@ -50,6 +50,10 @@ struct FrameDev {
     int64_t queue_begin; // first queue position (frames start at multiples of 64)
     double sph[PRR_MAX_SPHERES][4]; // centre xyz, radius
     int32_t sph_tex[PRR_MAX_SPHERES];
+    // the FP32 screen's view of each sphere: centre and radius rounded to
+    // float, then |cx| + |cy| + |cz| + R summed in that order in float (one
+    // 32-byte record per sphere)
+    float sphf[PRR_MAX_SPHERES][8];
     // initRays' per-frame terms for the spherical charts: the camera position
     // in (r, theta, phi), sin and cos theta, x^2 + y^2 and the reciprocals
     // that convert a direction into (dr, dtheta, dphi); for Kerr the metric

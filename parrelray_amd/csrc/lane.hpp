@@ -486,12 +486,12 @@ template <class Mt> PRR_HD double sphere_G(const FrameDev &fr, int k, const doub
 // bound on the path's bulge for any step that turns by less than ~1 rad)
 // plus a relative margin for FP32 rounding.
 PRR_HD bool sphere_screen(const FrameDev &fr, int k, const float P0[3], const float P1[3]) {
-    const float cx = float(fr.sph[k][0]), cy = float(fr.sph[k][1]), cz = float(fr.sph[k][2]);
-    const float R = float(fr.sph[k][3]);
+    const float cx = fr.sphf[k][0], cy = fr.sphf[k][1], cz = fr.sphf[k][2], R = fr.sphf[k][3];
     const float mx = P0[0] - cx, my = P0[1] - cy, mz = P0[2] - cz;
     const float dx = P1[0] - P0[0], dy = P1[1] - P0[1], dz = P1[2] - P0[2];
     const float A = dx * dx + dy * dy + dz * dz;
-    const float scale = fabsf(cx) + fabsf(cy) + fabsf(cz) + R + fabsf(P0[0]) + fabsf(P0[1]) + fabsf(P0[2]);
+    // |cx| + |cy| + |cz| + R + |P0|_1, the sphere's part summed on the host
+    const float scale = fr.sphf[k][4] + fabsf(P0[0]) + fabsf(P0[1]) + fabsf(P0[2]);
     // every point of the chord is at least |m| - |d| from the centre: steps that
     // start farther than R + 1.5 |d| + the margin pass nowhere near (most steps)
     const float far = R + 1.5f * sqrt_f32(A) + 2e-4f * scale;
@@ -1200,8 +1200,7 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         // Kerr: mu = cos th and mu' are measured in units of th (divided by
         // sin th), so that the tolerance keeps its meaning near the poles, where a
         // fixed error in mu is a large relative error in 1 - mu^2 (a scale: FP32)
-        const double ist =
-            Mt::mino ? double(rsqrt_f32(fmaxf(float(fmax(1.0 - L.y[2] * L.y[2], 0.0)), 1e-24f))) : 1.0;
+        const double ist = Mt::mino ? double(rsqrt_f32(fmaxf(float(1.0 - L.y[2] * L.y[2]), 1e-24f))) : 1.0;
         // errn = max_i ei / sc_i; NaN components are ignored as by fmax
 #pragma unroll
         for (int i = 0; i < Mt::NERR; ++i) {
