@@ -29,6 +29,8 @@ TracerDev tracer_dev(const prr_tracer &t) {
     d.r_escape = t.r_escape;
     d.lambda_max = t.lambda_max;
     d.r_fp64 = t.r_fp64;
+    d.inv_r_escape = 1.0 / t.r_escape;
+    d.inv_r_fp64 = 1.0 / t.r_fp64;
     d.max_steps = t.max_steps;
     d.max_attempts = t.max_attempts;
     return d;
@@ -68,6 +70,7 @@ FrameDev frame_dev(int metric, const prr_tracer &tr, const prr_frame &f, int64_t
     d.r_hor = (metric == PRR_FLAT || metric == PRR_ELLIS) ? -1.0
                                       : rh * (1.0 + tr.horizon_eps_rel) + tr.horizon_eps_abs;
     d.r_esc2 = r_esc2 > 0.0 ? r_esc2 : tr.r_escape;
+    d.inv_r_hor = 1.0 / d.r_hor;
     d.ray_begin = begin;
     d.queue_begin = begin;
     for (int k = 0; k < f.n_spheres; ++k) {

@@ -18,6 +18,7 @@ struct TracerDev {
     double r_escape;
     double lambda_max;
     double r_fp64;
+    double inv_r_escape, inv_r_fp64; // 1 / r_escape, 1 / r_fp64 (Kerr's radius tests in u = 1 / r)
     int64_t max_steps;
     int64_t max_attempts;
 };
@@ -45,6 +46,7 @@ struct FrameDev {
     uint32_t seed; // RANDOM / JITTERED sample stream
     double M, a;
     double r_hor; // horizon test radius r_h (1 + eps_rel) + eps_abs (negative: no test)
+    double inv_r_hor; // 1 / r_hor (Kerr's horizon test in u = 1 / r)
     double r_esc2; // escape radius beyond a wormhole's throat (l < 0; rout2)
     int64_t ray_begin;
     int64_t queue_begin; // first queue position (frames start at multiples of 64)

@@ -456,6 +456,22 @@ template <class Mt> PRR_HD double escape_radius(const JobDev &job, const FrameDe
     return (Mt::kind == PRR_ELLIS && y[0] < 0.0) ? fr.r_esc2 : job.tr.r_escape;
 }
 
+// The radius tests of state y.  Kerr's state holds u = 1 / r: it compares u
+// with the reciprocals of the radii, computed once on the host, instead of
+// taking a reciprocal of u per step.
+template <class Mt> PRR_HD bool beyond_escape(const JobDev &job, const FrameDev &fr, const double *y) {
+    if constexpr (Mt::mino) return y[0] <= job.tr.inv_r_escape;
+    else return radius_of<Mt>(y) >= escape_radius<Mt>(job, fr, y);
+}
+template <class Mt> PRR_HD bool inside_horizon(const FrameDev &fr, const double *y) {
+    if constexpr (Mt::mino) return y[0] >= fr.inv_r_hor;
+    else return radius_of<Mt>(y) <= fr.r_hor;
+}
+template <class Mt> PRR_HD bool inside_r_fp64(const TracerDev &tr, const double *y) {
+    if constexpr (Mt::mino) return y[0] > tr.inv_r_fp64;
+    else return radius_of<Mt>(y) < tr.r_fp64;
+}
+
 // Escape crossing function g (g < 0 inside r_escape) and dg/dt along the
 // integration variable.
 template <class Mt> PRR_HD void escape_g(const double *y, double r_esc, double &g, double &dg) {
@@ -708,9 +724,8 @@ template <class Mt> PRR_HD int euler_events(const JobDev &job, Lane<Mt> &L, cons
     }
     chart_post_step<Mt>(yn);
     for (int i = 0; i < Mt::N; ++i) L.y[i] = yn[i];
-    const double r = radius_of<Mt>(yn);
-    if (r <= fr.r_hor) return PRR_HORIZON;
-    if (r >= escape_radius<Mt>(job, fr, yn)) return PRR_ESCAPE;
+    if (inside_horizon<Mt>(fr, yn)) return PRR_HORIZON;
+    if (beyond_escape<Mt>(job, fr, yn)) return PRR_ESCAPE;
     return -1;
 }
 
@@ -718,7 +733,7 @@ template <class Mt, bool MIXED> PRR_HD int euler_step(const JobDev &job, Lane<Mt
     constexpr int N = Mt::N;
     const TracerDev &tr = job.tr;
     if (L.steps >= tr.max_steps || L.lam >= tr.lambda_max) return PRR_STEP_CAP;
-    bool fp64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
+    bool fp64 = !MIXED || inside_r_fp64<Mt>(tr, L.y);
     // FP32-bulk render with whole-wave refills: one precision per wave and
     // step, as for the Runge-Kutta attempts (lane_iterate)
     if constexpr (MIXED && !Mt::ckpt && !Mt::replay)
@@ -818,9 +833,8 @@ template <class Mt>
 PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h,
                                int &mode, const float *P1 = nullptr) {
     const FrameDev &fr = job.frames[L.frame];
-    const double r = radius_of<Mt>(yn);
-    const double r_esc = escape_radius<Mt>(job, fr, yn);
-    if (r >= r_esc) {
+    if (beyond_escape<Mt>(job, fr, yn)) {
+        const double r_esc = escape_radius<Mt>(job, fr, yn);
         double go, gn, dg;
         escape_g<Mt>(L.y, r_esc, go, dg);
         if (!(go < 0.0)) return finish_located(job, L, h, L.y, 0.0, PRR_ESCAPE, -1); // started outside
@@ -849,7 +863,7 @@ PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, con
             cart_of_f32<Mt>(L.y, L.P0);
         }
     }
-    if (r <= fr.r_hor) return PRR_HORIZON;
+    if (inside_horizon<Mt>(fr, yn)) return PRR_HORIZON;
     maybe_checkpoint<Mt>(job, L);
     return -1;
 }
@@ -1115,6 +1129,8 @@ PRR_HD TracerDev trip_tracer(const JobDev &job) {
     r.r_escape = t->r_escape;
     r.lambda_max = t->lambda_max;
     r.r_fp64 = t->r_fp64;
+    r.inv_r_escape = t->inv_r_escape;
+    r.inv_r_fp64 = t->inv_r_fp64;
     r.max_steps = t->max_steps;
     r.max_attempts = t->max_attempts;
     return r;
@@ -1164,6 +1180,8 @@ PRR_HD int lane_iterate(const JobDev &job, Lane<Mt> &L, LocateCtx &C, int &mode)
         // FP64 stages: the plain form; in the FP32-bulk form inside r_fp64 and,
         // for Kerr, where the step can come within ~13 degrees of the polar
         // axis (1 - mu^2 and L / (1 - mu^2) lose their relative precision)
+        // (the radius, not inside_r_fp64: with the latter the FP32-bulk
+        // instantiation's register allocation spills twice as much)
         use64 = !MIXED || radius_of<Mt>(L.y) < tr.r_fp64;
         if (Mt::mino && !use64) {
             const double mu_reach = fabs(L.y[2]) + hs * fabs(L.y[3]);
