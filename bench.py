@@ -184,15 +184,24 @@ def run_c1_batched(api, A, CF, ctx, steps, warmup, torch, precision=None, tune_p
 
 def run_c4(api, A, CF, torch, steps, ids):
     """C4: Kerr a = 0.998, camera r = 6M edge-on, 2048x2048 with 4x4 SSAA (67,108,864 rays) + FXAA,
-    tol 1e-10, one frame per step, in both precision forms; plus the per-ray results of a subset."""
+    tol 1e-10, one frame per step: plain FP64, the FP32-bulk form with FP32 stages, and the FP32-bulk
+    form with the autotuned launch shape and stage precision; plus the per-ray results of a subset
+    in the first two."""
+    from parrelray_amd import autotune
     f = CF.frame_c4()
     out = {"rays_per_frame": f.nrays}
     img = torch.empty((f.height, f.width, 3), dtype=torch.uint8, device="cuda")
-    for name, prec in (("fp64", A.PRR_FP64), ("fp32_bulk", A.PRR_FP32_BULK)):
+    for name, prec in (("fp64", A.PRR_FP64), ("fp32_bulk", A.PRR_FP32_BULK), ("fp32_bulk_tuned", A.PRR_FP32_BULK)):
         ctx = api.Context(0)
         ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.998)
         ctx.set_tracer(CF.tracer_c4(precision=prec))
         ctx.set_texture(0, CF.sky_texture())
+        if name == "fp32_bulk_tuned":
+            # the FP32-bulk form with the autotuned launch shape and stage
+            # precision, tuned on a 512 x 512 frame of the same scene
+            out["tuned_launch"] = autotune.tune_launch(ctx, [CF.frame_c4(512, 512)], A.PRR_KERR_BL,
+                                                       CF.tracer_c4(precision=prec), use_cache=False,
+                                                       budget_s=20.0)
         ctx.render_device([f], rgb8=[img])  # warm-up
         torch.cuda.synchronize()
         t0 = time.perf_counter()
@@ -208,9 +217,11 @@ def run_c4(api, A, CF, torch, steps, ids):
                          attempts=st["attempts"], attempts_fp64=st["attempts_fp64"],
                          attempts_per_s=st["attempts"] / (integ / 1e3),
                          roofline_frac=(st["flops_fp32"] / P32 + st["flops_fp64"] / P64) / (integ / 1e3))
-        out[name]["_rays"] = gpu_rays(ctx, [f], ids, torch)
+        if name != "fp32_bulk_tuned":
+            out[name]["_rays"] = gpu_rays(ctx, [f], ids, torch)
         ctx.close()
     out["speedup_fp32_bulk_vs_fp64"] = out["fp64"]["ms_per_frame"] / out["fp32_bulk"]["ms_per_frame"]
+    out["speedup_fp32_bulk_tuned_vs_fp64"] = out["fp64"]["ms_per_frame"] / out["fp32_bulk_tuned"]["ms_per_frame"]
     return out
 
 
