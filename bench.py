@@ -104,8 +104,11 @@ def gbs(nbytes, ms):
 def run_adjoint(api, A, CF, reps, torch=None, ids=None, warmup=2):
     """C3: 512x512 Kerr a=0.7, FP32-bulk RKF45 tol 1e-8; gradient of the L2 loss against the target
     render w.r.t. (a, r_cam, incl, sphere phi).  Rays/s counts primary rays through the whole
-    prr_adjoint call (forward render, checkpointed forward, replay and reverse sweep)."""
+    prr_adjoint call (forward render, checkpointed forward, replay and reverse sweep).  The FP32-bulk
+    form's stage precision is autotuned for the adjoint call (the faster of FP32 and FP64 stages),
+    as the render legs' is."""
     import numpy as np
+    from parrelray_amd import autotune
     ctx = api.Context(0)
     ctx.set_spacetime(A.PRR_KERR_BL, 1.0, 0.7)
     ctx.set_tracer(CF.tracer_kerr(precision=A.PRR_FP32_BULK))
@@ -116,6 +119,7 @@ def run_adjoint(api, A, CF, reps, torch=None, ids=None, warmup=2):
     T = ctx.render(api.frame_from_orbit(tmpl, ot))[0]["rgbf"]
     I = ctx.render(api.frame_from_orbit(tmpl, o0))[0]["rgbf"]
     dLdI = (2.0 * (I - T) / I.size).astype(np.float32)
+    stages = autotune.tune_adjoint_precision(ctx, tmpl, o0, dLdI, CF.tracer_kerr(precision=A.PRR_FP32_BULK))
     for _ in range(max(1, warmup)):
         g = ctx.adjoint(tmpl, o0, dLdI)
     per = []
@@ -126,6 +130,7 @@ def run_adjoint(api, A, CF, reps, torch=None, ids=None, warmup=2):
     sec = sum(per) / reps
     st = ctx.stats()
     out = dict(rays=tmpl.nrays, seconds=sec, rays_per_s=tmpl.nrays / sec, call_ms=[round(x * 1e3, 3) for x in per],
+               stage_precision="fp64" if stages["precision"] == A.PRR_FP64 else "fp32", stage_tuning=stages,
                median_call_ms=sorted(per)[len(per) // 2] * 1e3,
                adjoint_ms=st["adjoint_ms"],
                total_ms=st["total_ms"], steps_swept=st["adjoint_steps"],

@@ -11,10 +11,11 @@ depend on which lanes share a wave), so tuning only changes speed; the
 tests check that invariance.
 
 For an FP32-bulk tracer it also times the plain FP64 stages at the tuned
-shape and keeps the faster of the two (tune_precision): both meet the
-FP32-bulk form's accuracy contract, and on devices whose non-packed FP32
-instructions issue at the FP64 rate (MI355X) the FP64 stages need fewer
-instructions than the FP32 difference stages with their conversions.
+shape and keeps the faster of the two (tune_precision; for the adjoint,
+tune_adjoint_precision): both meet the FP32-bulk form's accuracy contract,
+and on devices whose non-packed FP32 instructions issue at the FP64 rate
+(MI355X) the FP64 stages need fewer instructions than the FP32 difference
+stages with their conversions.
 """
 import json
 import os
@@ -135,3 +136,27 @@ def tune_checkpoint(ctx, tmpl, orbit, dLdI, device_name="gpu0", use_cache=True):
         _save(cache)
     ctx.set_launch(cur["block"], cur["blocks_per_cu"], cur["refill"], k)
     return k
+
+
+def tune_adjoint_precision(ctx, tmpl, orbit, dLdI, tracer, reps=3):
+    """For an FP32-bulk tracer: time prr_adjoint (the whole call, HIP events) with the FP32 and with
+    the FP64 stages and keep the faster on ctx; both meet the FP32-bulk form's accuracy contract
+    (the adjoint's passes all follow the tracer's stage precision, so the replay still repeats
+    the first pass bit for bit).  Returns {"precision", "ms_fp32_stages", "ms_fp64_stages"}."""
+    if tracer.precision != A.PRR_FP32_BULK:
+        return {"precision": int(tracer.precision)}
+    t64 = A.Tracer.from_buffer_copy(tracer)
+    t64.precision = A.PRR_FP64
+    ms = {}
+    for name, t in (("ms_fp32_stages", tracer), ("ms_fp64_stages", t64)):
+        ctx.set_tracer(t)
+        ctx.adjoint(tmpl, orbit, dLdI)  # warm-up (buffers, code objects)
+        best = float("inf")
+        for _ in range(reps):
+            ctx.adjoint(tmpl, orbit, dLdI)
+            best = min(best, ctx.stats()["total_ms"])
+        ms[name] = best
+    fp64 = ms["ms_fp64_stages"] < ms["ms_fp32_stages"]
+    ctx.set_tracer(t64 if fp64 else tracer)
+    return dict(precision=int(A.PRR_FP64 if fp64 else A.PRR_FP32_BULK), **ms)
+
