@@ -315,7 +315,8 @@ void free_context(prr_ctx *ctx) {
                       &ctx->attempts, &ctx->state8, &ctx->x0u0, &ctx->adj_img, &ctx->adj_dldi,
                       &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_ckpt2, &ctx->adj_event, &ctx->adj_rays, &ctx->adj_rec,
                       &ctx->adj_task_ray, &ctx->adj_task_seg, &ctx->adj_levelstart, &ctx->adj_nseg,
-                      &ctx->adj_over, &ctx->adj_plan, &ctx->adj_lam, &ctx->adj_grad_ray, &ctx->adj_grad};
+                      &ctx->adj_over, &ctx->adj_plan, &ctx->adj_lam, &ctx->adj_grad_ray, &ctx->adj_grad,
+                      &ctx->adj_jac, &ctx->adj_jacstart};
     for (DevBuf *b : bufs) release(*b);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (auto &ev : ctx->ev)
@@ -960,6 +961,7 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         unsigned long long cursor[2 * kAdjPlanBins];
         CountersDev fin;
         double grad[4];
+        int64_t jacstart[kAdjPlanBins + 1];
     };
     if ((s = ensure_pinned(ctx, sizeof(Pinned))) != PRR_OK) return s;
     Pinned *pin = static_cast<Pinned *>(ctx->pinned);
@@ -1107,6 +1109,24 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     j3.adj.rec = static_cast<double *>(ctx->adj_rec.p);
     j3.adj.event = static_cast<EventDev *>(ctx->adj_event.p);
     AdjDev ad{};
+    // long rays (all levels in one chunk): the reverse sweep takes their last
+    // segment, their other segments' transposed Jacobians are formed in
+    // parallel, and a chain applies them; level j holds the ranks below
+    // min(count[j + 1], nlong)
+    if (chunks.size() == 1 && nseg_max - 1 > kAdjLongLast && nseg_max < kAdjPlanBins) {
+        ad.nlong = count[kAdjLongLast + 1];
+        ad.jlevels = nseg_max - 1;
+        pin->jacstart[0] = 0;
+        for (int32_t j = 0; j < ad.jlevels; ++j)
+            pin->jacstart[j + 1] = pin->jacstart[j] + std::min<int64_t>(count[j + 1], ad.nlong);
+        ad.njac = pin->jacstart[ad.jlevels];
+        if ((s = ensure(ctx, ctx->adj_jac, 81 * 8 * size_t(std::max<int64_t>(ad.njac, 1)))) != PRR_OK) return s;
+        if ((s = ensure(ctx, ctx->adj_jacstart, 8 * size_t(ad.jlevels + 1))) != PRR_OK) return s;
+        HIPTRY(ctx, hipMemcpyAsync(ctx->adj_jacstart.p, pin->jacstart, 8 * size_t(ad.jlevels + 1),
+                                   hipMemcpyHostToDevice, st));
+        ad.jacstart = static_cast<const int64_t *>(ctx->adj_jacstart.p);
+        ad.jac = static_cast<double *>(ctx->adj_jac.p);
+    }
     ad.K = K;
     ad.rays = static_cast<const int64_t *>(ctx->adj_rays.p);
     ad.levelstart = levelstart_d;

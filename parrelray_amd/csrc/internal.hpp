@@ -183,7 +183,19 @@ struct AdjDev {
     double cam_phi;          // fixed orbit parameters
     double sphere_r;
     unsigned long long *mismatch; // rays whose replay did not end with their class
+    // Long rays (ranks < nlong: more than kAdjLongLast segments, all levels in
+    // this chunk): the sweep takes only their last segment, while the
+    // transposed Jacobians of their other segments are formed in parallel
+    // (jac, 81 doubles per (level j, rank) at task jacstart[j] + rank) and
+    // chained onto lambda afterwards.
+    int64_t nlong;
+    int64_t njac;                // Jacobian tasks (times 9 columns: threads)
+    int32_t jlevels;             // levels with Jacobian tasks: 0 .. jlevels - 1
+    const int64_t *jacstart;     // first task of level j, jlevels + 1 entries
+    double *jac;
 };
+// Rays whose last segment index exceeds this take the Jacobian path.
+constexpr int32_t kAdjLongLast = 1;
 
 // Segment-count histogram bins of the device-side adjoint plan; tracers whose
 // step cap allows more segments per ray plan on the host.

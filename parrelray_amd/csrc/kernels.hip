@@ -557,7 +557,45 @@ __global__ void __launch_bounds__(256) k_adj_seed(JobDev job, AdjDev ad) {
 // highest segment in the chunk down to jlo, reading segment j's steps from
 // task levelstart[j] + idx of the chunk's records.  lambda enters from ad.lam
 // (the seed, or the previous chunk) and leaves through it.
+// Column k of the transposed Jacobian of segment j of the ray at rank idx:
+// the unit cotangent e_k swept backwards through the segment's K steps.
+__device__ void adj_segment_jacobian(const JobDev &job, const AdjDev &ad, int64_t task, int k) {
+    int lo = 0, hi = ad.jlevels - 1; // the level: the largest j with jacstart[j] <= task
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (ad.jacstart[mid] <= task) lo = mid;
+        else hi = mid - 1;
+    }
+    const int32_t j = lo;
+    const int64_t idx = task - ad.jacstart[j];
+    const int64_t id = ad.rays[idx];
+    const FrameDev &fr = job.frames[find_frame(job, id)];
+    Consts<double> cs;
+    load_consts<Kerr>(job, id, fr, cs);
+    double lam[9];
+    for (int i = 0; i < 9; ++i) lam[i] = i == k ? 1.0 : 0.0;
+    const int64_t T = ad.rec_stride, col = ad.levelstart[j] - ad.t0 + idx;
+    const int K = ad.K;
+    for (int32_t n = (j + 1) * K - 1; n >= j * K; --n) { // a segment below the last: K steps
+        const double *r = ad.rec + (int64_t(n - j * K) * 6) * T + col;
+        double yn[6];
+        for (int c = 0; c < 5; ++c) yn[c] = r[c * T];
+        yn[5] = 0.0;
+        kerr_step_vjp(yn, r[5 * T], cs, lam);
+    }
+    double *J = ad.jac + 81 * task + 9 * k;
+    for (int i = 0; i < 9; ++i) J[i] = lam[i];
+}
+
+// Blocks below ceil(count / 256) sweep rays; the rest form the long rays'
+// segment Jacobians (9 threads per segment), concurrently with the sweeps.
 __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
+    const int64_t ray_blocks = (ad.count + 255) / 256;
+    if (int64_t(blockIdx.x) >= ray_blocks) {
+        const int64_t t = (int64_t(blockIdx.x) - ray_blocks) * blockDim.x + threadIdx.x;
+        if (t < 9 * ad.njac) adj_segment_jacobian(job, ad, t / 9, int(t % 9));
+        return;
+    }
     const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
     if (idx >= ad.count) return;
     const int64_t id = ad.rays[idx];
@@ -574,9 +612,11 @@ __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
     Consts<double> cs;
     load_consts<Kerr>(job, id, fr, cs);
     const int32_t jtop = min(last, ad.jhi);
+    // a long ray sweeps its last segment only; the chain applies the others
+    const int32_t jbot = idx < ad.nlong ? jtop : ad.jlo;
     {
         const int64_t T = ad.rec_stride;
-        for (int32_t j = jtop; j >= ad.jlo; --j) {
+        for (int32_t j = jtop; j >= jbot; --j) {
             const int64_t col = ad.levelstart[j] - ad.t0 + idx;
             const int32_t end = min((j + 1) * K, N - 1); // step N-1 is the located sub-step above
             for (int32_t n = end - 1; n >= j * K; --n) {
@@ -587,6 +627,32 @@ __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
                 kerr_step_vjp(yn, r[5 * T], cs, lam);
             }
         }
+    }
+    for (int i = 0; i < 9; ++i) lm[i] = lam[i];
+}
+
+// The long rays' lambda through their segments below the last, from the
+// highest: lambda <- J_j^T lambda.
+__global__ void __launch_bounds__(256) k_adj_chain(AdjDev ad) {
+    const int64_t idx = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+    if (idx >= ad.nlong) return;
+    const int64_t id = ad.rays[idx];
+    double *lm = ad.lam + 10 * id;
+    double lam[9];
+    bool zero = true;
+    for (int i = 0; i < 9; ++i) {
+        lam[i] = lm[i];
+        zero = zero && lam[i] == 0.0;
+    }
+    if (zero) return; // nothing flows back (as in the sweep)
+    const int32_t last = (ad.nsteps[id] - 1) / ad.K;
+    for (int32_t j = last - 1; j >= 0; --j) {
+        const double *J = ad.jac + 81 * (ad.jacstart[j] + idx);
+        double out[9];
+        for (int i = 0; i < 9; ++i) out[i] = 0.0;
+        for (int k = 0; k < 9; ++k)
+            for (int i = 0; i < 9; ++i) out[i] += lam[k] * J[9 * k + i];
+        for (int i = 0; i < 9; ++i) lam[i] = out[i];
     }
     for (int i = 0; i < 9; ++i) lm[i] = lam[i];
 }
@@ -790,7 +856,9 @@ hipError_t launch_adj_reverse(const JobDev &job, const AdjDev &ad, hipStream_t s
     if (ad.count > ad.seed_lo)
         hipLaunchKernelGGL(k_adj_seed, dim3(unsigned((ad.count - ad.seed_lo + 255) / 256)), dim3(256), 0, st, job,
                            ad);
-    hipLaunchKernelGGL(k_adj_reverse, dim3(unsigned((ad.count + 255) / 256)), dim3(256), 0, st, job, ad);
+    const int64_t blocks = (ad.count + 255) / 256 + (9 * ad.njac + 255) / 256;
+    hipLaunchKernelGGL(k_adj_reverse, dim3(unsigned(blocks)), dim3(256), 0, st, job, ad);
+    if (ad.nlong > 0) hipLaunchKernelGGL(k_adj_chain, dim3(unsigned((ad.nlong + 255) / 256)), dim3(256), 0, st, ad);
     return hipGetLastError();
 }
 
