@@ -175,8 +175,11 @@ typedef struct prr_launch {
     int32_t refill;        /* a wave refills once at least this many lanes are idle (1..64);
                               PRR_FP32_BULK always uses 64 */
     int32_t checkpoint;    /* adjoint: accepted steps between per-ray checkpoints */
-    int64_t adjoint_record_bytes; /* adjoint: device memory for one chunk of replayed steps (48 B per
-                                     step; 0: 8 GiB); a smaller budget replays in more chunks */
+    int64_t adjoint_record_bytes; /* adjoint: device memory for recorded steps (48 B per step; 0: 8 GiB).
+                                     When every ray's first max(8, 512 / checkpoint) * checkpoint steps
+                                     fit it, the first pass records them and nothing is replayed;
+                                     otherwise it bounds one chunk of replayed steps, and a smaller
+                                     budget replays in more chunks */
 } prr_launch;
 
 /* Counters and HIP-event timings of the last prr_render / graph launch. */

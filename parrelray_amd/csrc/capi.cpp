@@ -316,7 +316,7 @@ void free_context(prr_ctx *ctx) {
                       &ctx->adj_off, &ctx->adj_ckpt, &ctx->adj_ckpt2, &ctx->adj_event, &ctx->adj_rays, &ctx->adj_rec,
                       &ctx->adj_task_ray, &ctx->adj_task_seg, &ctx->adj_levelstart, &ctx->adj_nseg,
                       &ctx->adj_over, &ctx->adj_plan, &ctx->adj_lam, &ctx->adj_grad_ray, &ctx->adj_grad,
-                      &ctx->adj_jac, &ctx->adj_jacstart};
+                      &ctx->adj_jac, &ctx->adj_jacstart, &ctx->adj_recall};
     for (DevBuf *b : bufs) release(*b);
     if (ctx->pinned) (void)hipHostFree(ctx->pinned);
     for (auto &ev : ctx->ev)
@@ -916,6 +916,19 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     job.adj.K = K;
     job.adj.cap = cap;
     job.adj.ckpt = static_cast<double *>(ctx->adj_ckpt.p);
+    // when every ray's first cap * K steps fit the record budget, the first
+    // pass also records each accepted step and the terminal event, and the
+    // replay is skipped for calls whose rays all fit (no ray beyond cap
+    // segments); otherwise the replay records the steps chunk by chunk
+    const uint64_t rec_budget = ctx->launch.adjoint_record_bytes > 0 ? uint64_t(ctx->launch.adjoint_record_bytes)
+                                                                       : uint64_t(kRecBudget);
+    const int64_t rec_cap = int64_t(cap) * K;
+    if (48 * uint64_t(rec_cap) * uint64_t(T) <= rec_budget) {
+        if ((s = ensure(ctx, ctx->adj_recall, 48 * size_t(rec_cap) * size_t(T))) != PRR_OK) return s;
+        job.adj.rec_all = static_cast<double *>(ctx->adj_recall.p);
+        job.adj.rec_cap = rec_cap;
+        job.adj.event = static_cast<EventDev *>(ctx->adj_event.p);
+    }
     CountersDev *ctr = ws.counters;
     HIPTRY(ctx, hipMemcpyAsync(ws.frames, &fd, sizeof(FrameDev), hipMemcpyHostToDevice, st));
     HIPTRY(ctx, hipMemcpyAsync(ws.post, &pf, sizeof(PostFrameDev), hipMemcpyHostToDevice, st));
@@ -1057,6 +1070,10 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     HIPTRY(ctx, hipEventRecord(ctx->ev[1], st));
     JobDev j2 = job;
     j2.out = OutDev{};
+    j2.adj.rec_all = nullptr;
+    j2.adj.rec_cap = 0;
+    // the first pass's records serve the reverse sweep when every ray fits them
+    const bool direct = job.adj.rec_all != nullptr && n_over == 0;
     j2.adj.nsteps = static_cast<const int32_t *>(ctx->steps.p);
     j2.adj.ckpt_off = static_cast<const int64_t *>(ctx->adj_off.p);
     if (n_over > 0) {
@@ -1094,16 +1111,21 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
                                                                  : uint64_t(kRecBudget);
     const int64_t chunk_cap = std::max<int64_t>(1, int64_t(budget / rec_task));
     std::vector<std::pair<int32_t, int32_t>> chunks; // (jhi, jlo)
-    for (int32_t j = nseg_max - 1; j >= 0;) {
-        int32_t lo = j;
-        int64_t n = count[j];
-        while (lo > 0 && n + count[lo - 1] <= chunk_cap) n += count[--lo];
-        chunks.emplace_back(j, lo);
-        j = lo - 1;
+    if (direct) { // the first pass recorded every step: one sweep over all levels
+        if (nseg_max > 0) chunks.emplace_back(nseg_max - 1, 0);
+    } else {
+        for (int32_t j = nseg_max - 1; j >= 0;) {
+            int32_t lo = j;
+            int64_t n = count[j];
+            while (lo > 0 && n + count[lo - 1] <= chunk_cap) n += count[--lo];
+            chunks.emplace_back(j, lo);
+            j = lo - 1;
+        }
+        int64_t chunk_max = 0;
+        for (const auto &c : chunks)
+            chunk_max = std::max(chunk_max, levelstart[c.second] + count[c.second] - levelstart[c.first]);
+        if ((s = ensure(ctx, ctx->adj_rec, rec_task * size_t(std::max<int64_t>(chunk_max, 1)))) != PRR_OK) return s;
     }
-    int64_t chunk_max = 0;
-    for (const auto &c : chunks) chunk_max = std::max(chunk_max, levelstart[c.second] + count[c.second] - levelstart[c.first]);
-    if ((s = ensure(ctx, ctx->adj_rec, rec_task * size_t(std::max<int64_t>(chunk_max, 1)))) != PRR_OK) return s;
     JobDev j3 = j2;
     j3.adj.mode = ADJ_REPLAY;
     j3.adj.rec = static_cast<double *>(ctx->adj_rec.p);
@@ -1149,9 +1171,14 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
         j3.adj.seg = static_cast<const int32_t *>(ctx->adj_task_seg.p) + t0;
         j3.adj.rec_stride = t1 - t0;
         j3.total = t1 - t0;
-        HIPTRY(ctx, launch_reset(ctr, st, false));
-        const int grid = int(std::min<int64_t>(P.grid, (j3.total + 63) / 64 + 1));
-        HIPTRY(ctx, launch_integrate(ctx->metric, j3, grid, P.block, st));
+        if (direct) {
+            ad.rec = job.adj.rec_all;
+            ad.rec_cap = job.adj.rec_cap;
+        } else {
+            HIPTRY(ctx, launch_reset(ctr, st, false));
+            const int grid = int(std::min<int64_t>(P.grid, (j3.total + 63) / 64 + 1));
+            HIPTRY(ctx, launch_integrate(ctx->metric, j3, grid, P.block, st));
+        }
         ad.jhi = c.first;
         ad.jlo = c.second;
         ad.t0 = t0;

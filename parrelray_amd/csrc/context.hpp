@@ -96,7 +96,7 @@ struct prr_ctx {
     prr::DevBuf stage_rgbf, stage_rgb8, cls, endpoint, steps, attempts, state8, x0u0;
     // adjoint
     prr::DevBuf adj_img, adj_dldi, adj_off, adj_ckpt, adj_ckpt2, adj_event, adj_rays, adj_rec, adj_lam, adj_grad_ray,
-        adj_task_ray, adj_task_seg, adj_levelstart, adj_nseg, adj_over, adj_plan, adj_jac, adj_jacstart,
+        adj_task_ray, adj_task_seg, adj_levelstart, adj_nseg, adj_over, adj_plan, adj_jac, adj_jacstart, adj_recall,
         adj_grad;
     // pinned host memory for the adjoint's small copies between passes
     // (pageable copies cost tens of microseconds each)

@@ -128,8 +128,13 @@ struct AdjRunDev {
     const int32_t *seg;      // replay: segment index j of each job index (a task is one segment of one ray)
     double *rec;             // replay: K * 6 doubles per task, [slot * 6 + c] * rec_stride + task
     int64_t rec_stride;      // replay: tasks of this launch
-    EventDev *event;         // replay: terminal event of each ray
+    EventDev *event;         // replay (and a recording first pass): terminal event of each ray
     unsigned long long *mismatch; // replay: segments that did not reproduce their checkpoint
+    // first pass recording every accepted step (rec_cap > 0): 6 doubles
+    // (y[0..4], h) per step at rec_all + (ray * rec_cap + step) * 6, for the
+    // first rec_cap steps of each ray
+    double *rec_all;
+    int64_t rec_cap;
 };
 
 // Device-side counters; `queue` is the ray queue head.
@@ -173,7 +178,8 @@ struct AdjDev {
     int64_t rec_stride;      // tasks in the chunk
     const int32_t *nsteps;   // accepted steps of every ray
     const int8_t *cls;       // termination class of every ray
-    const double *rec;       // replayed steps of the chunk (AdjRunDev.rec layout)
+    const double *rec;       // replayed steps of the chunk (AdjRunDev.rec layout), or the first pass's records
+    int64_t rec_cap;         // > 0: rec holds the first pass's records (AdjRunDev.rec_all layout)
     const EventDev *event;   // terminal events
     double *lam;             // 10 doubles per ray: lambda (y[0..4], E, L, Q, a), dL/dphi_s
     const float *dLdI;       // dL/dI, W*H*3

@@ -557,6 +557,19 @@ __global__ void __launch_bounds__(256) k_adj_seed(JobDev job, AdjDev ad) {
 // highest segment in the chunk down to jlo, reading segment j's steps from
 // task levelstart[j] + idx of the chunk's records.  lambda enters from ad.lam
 // (the seed, or the previous chunk) and leaves through it.
+// Record of step n (in segment j) of ray id at rank idx: y[0..4] and h at
+// p[0], p[s], ..., p[5 s] (the replayed chunk's task-major records, or the
+// first pass's per-ray records).
+struct RecRef {
+    const double *p;
+    int64_t s;
+};
+__device__ inline RecRef adj_rec(const AdjDev &ad, int64_t id, int64_t idx, int32_t j, int32_t n) {
+    if (ad.rec_cap > 0) return RecRef{ad.rec + (id * ad.rec_cap + n) * 6, 1};
+    const int64_t T = ad.rec_stride;
+    return RecRef{ad.rec + (int64_t(n - j * ad.K) * 6) * T + (ad.levelstart[j] - ad.t0 + idx), T};
+}
+
 // Column k of the transposed Jacobian of segment j of the ray at rank idx:
 // the unit cotangent e_k swept backwards through the segment's K steps.
 __device__ void adj_segment_jacobian(const JobDev &job, const AdjDev &ad, int64_t task, int k) {
@@ -574,14 +587,13 @@ __device__ void adj_segment_jacobian(const JobDev &job, const AdjDev &ad, int64_
     load_consts<Kerr>(job, id, fr, cs);
     double lam[9];
     for (int i = 0; i < 9; ++i) lam[i] = i == k ? 1.0 : 0.0;
-    const int64_t T = ad.rec_stride, col = ad.levelstart[j] - ad.t0 + idx;
     const int K = ad.K;
     for (int32_t n = (j + 1) * K - 1; n >= j * K; --n) { // a segment below the last: K steps
-        const double *r = ad.rec + (int64_t(n - j * K) * 6) * T + col;
+        const RecRef r = adj_rec(ad, id, idx, j, n);
         double yn[6];
-        for (int c = 0; c < 5; ++c) yn[c] = r[c * T];
+        for (int c = 0; c < 5; ++c) yn[c] = r.p[c * r.s];
         yn[5] = 0.0;
-        kerr_step_vjp(yn, r[5 * T], cs, lam);
+        kerr_step_vjp(yn, r.p[5 * r.s], cs, lam);
     }
     double *J = ad.jac + 81 * task + 9 * k;
     for (int i = 0; i < 9; ++i) J[i] = lam[i];
@@ -614,18 +626,14 @@ __global__ void __launch_bounds__(256) k_adj_reverse(JobDev job, AdjDev ad) {
     const int32_t jtop = min(last, ad.jhi);
     // a long ray sweeps its last segment only; the chain applies the others
     const int32_t jbot = idx < ad.nlong ? jtop : ad.jlo;
-    {
-        const int64_t T = ad.rec_stride;
-        for (int32_t j = jtop; j >= jbot; --j) {
-            const int64_t col = ad.levelstart[j] - ad.t0 + idx;
-            const int32_t end = min((j + 1) * K, N - 1); // step N-1 is the located sub-step above
-            for (int32_t n = end - 1; n >= j * K; --n) {
-                const double *r = ad.rec + (int64_t(n - j * K) * 6) * T + col;
-                double yn[6];
-                for (int c = 0; c < 5; ++c) yn[c] = r[c * T];
-                yn[5] = 0.0;
-                kerr_step_vjp(yn, r[5 * T], cs, lam);
-            }
+    for (int32_t j = jtop; j >= jbot; --j) {
+        const int32_t end = min((j + 1) * K, N - 1); // step N-1 is the located sub-step above
+        for (int32_t n = end - 1; n >= j * K; --n) {
+            const RecRef r = adj_rec(ad, id, idx, j, n);
+            double yn[6];
+            for (int c = 0; c < 5; ++c) yn[c] = r.p[c * r.s];
+            yn[5] = 0.0;
+            kerr_step_vjp(yn, r.p[5 * r.s], cs, lam);
         }
     }
     for (int i = 0; i < 9; ++i) lm[i] = lam[i];

@@ -796,8 +796,18 @@ template <class Mt> PRR_HD void maybe_checkpoint(const JobDev &job, const Lane<M
     store_checkpoint(job, L.y, Mt::N, L.h, L.attempts, L.rid, L.steps / K);
 }
 
-// Adjoint replay: the step's start and size of every accepted step of the segment.
+// Adjoint replay: the step's start and size of every accepted step of the
+// segment.  A first pass with a record area records every step of the ray
+// instead (the reverse sweep then needs no replay).
 template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &L, double h) {
+    if constexpr (Mt::ckpt) {
+        if (job.adj.mode == ADJ_CHECKPOINT && job.adj.rec_all && L.steps <= job.adj.rec_cap) {
+            double *r = job.adj.rec_all + (L.rid * job.adj.rec_cap + (L.steps - 1)) * 6;
+            for (int c = 0; c < 5; ++c) r[c] = L.y[c];
+            r[5] = h;
+        }
+        return;
+    }
     if (!Mt::replay || job.adj.mode != ADJ_REPLAY) return;
     const int64_t slot = (L.steps - 1) - int64_t(L.seg) * job.adj.K; // L.steps already counts this step
     const int64_t T = job.adj.rec_stride;
@@ -812,7 +822,8 @@ template <class Mt> PRR_HD void maybe_record(const JobDev &job, const Lane<Mt> &
 template <class Mt>
 PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, double h, const double *ys, double theta, int cls,
                           int hit) {
-    if (Mt::replay && job.adj.mode == ADJ_REPLAY && job.adj.event) {
+    if ((Mt::replay && job.adj.mode == ADJ_REPLAY && job.adj.event) ||
+        (Mt::ckpt && job.adj.mode == ADJ_CHECKPOINT && job.adj.rec_all)) {
         EventDev &ev = job.adj.event[L.rid];
         for (int i = 0; i < 8; ++i) ev.y_prev[i] = i < Mt::N ? L.y[i] : 0.0;
         ev.h = h;

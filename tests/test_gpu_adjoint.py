@@ -179,10 +179,13 @@ def test_gradient_does_not_depend_on_the_checkpoint_interval(setup):
 
 
 def test_gradient_does_not_depend_on_replay_chunks_or_where_rays_are_binned(setup):
-    """The replay runs in chunks of segment levels when its records exceed the
-    record budget (lambda passes between chunks through memory), and rays are
+    """With the default record budget the first pass records every step (each
+    ray's first max(8, 512 / K) * K) and the reverse sweep reads those
+    records: no replay.  Below that budget the segments are replayed, in one
+    chunk of segment levels, or in several when the replay's records exceed
+    the budget (lambda passes between chunks through memory); and rays are
     binned by segment count on the host when the tracer's step cap allows
-    more than 4096 segments per ray: neither changes the gradient."""
+    more than 4096 segments per ray: none of these changes the gradient."""
     tmpl, o0, T, ref = setup
     I, _ = render(ref, tmpl, o0)
     dLdI = (2.0 * (I - T) / I.size).astype(np.float32)
@@ -190,6 +193,9 @@ def test_gradient_does_not_depend_on_replay_chunks_or_where_rays_are_binned(setu
     ctx = context(A.PRR_FP64, 1e-10)
     ctx.set_launch(checkpoint=K)
     g0 = ctx.adjoint(tmpl, o0, dLdI)
+    first_pass_records = 48 * max(8, 512 // K) * K * tmpl.nrays
+    ctx.set_launch(checkpoint=K, adjoint_record_bytes=first_pass_records - 48)  # replay, one chunk
+    g3 = ctx.adjoint(tmpl, o0, dLdI)
     ctx.set_launch(checkpoint=K, adjoint_record_bytes=48 * K * 300)  # about 300 tasks per chunk
     g1 = ctx.adjoint(tmpl, o0, dLdI)
     ctx.set_launch(checkpoint=K)
@@ -198,8 +204,8 @@ def test_gradient_does_not_depend_on_replay_chunks_or_where_rays_are_binned(setu
     tr.max_attempts = 10 * tr.max_steps
     ctx.set_tracer(tr)
     g2 = ctx.adjoint(tmpl, o0, dLdI)
-    print("grads", g0.tolist(), g1.tolist(), g2.tolist())
-    for g in (g1, g2):
+    print("grads", g0.tolist(), g1.tolist(), g2.tolist(), g3.tolist())
+    for g in (g1, g2, g3):
         assert np.abs(g - g0).max() <= 1e-12 * np.abs(g0).max(), (g.tolist(), g0.tolist())
 
 
