@@ -1098,14 +1098,16 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     // chunk in one launch, and each chunk's reverse sweep is one launch over
     // its rays: the replays of different segments run concurrently instead
     // of one launch per segment index.
-    if ((s = ensure(ctx, ctx->adj_task_ray, 8 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
-    if ((s = ensure(ctx, ctx->adj_task_seg, 4 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
-    plan.levelstart = levelstart_d;
-    plan.nlevels = nseg_max;
-    plan.ntask = ntask;
-    plan.task_ray = static_cast<int64_t *>(ctx->adj_task_ray.p);
-    plan.task_seg = static_cast<int32_t *>(ctx->adj_task_seg.p);
-    HIPTRY(ctx, launch_adj_plan_tasks(plan, st));
+    if (!direct) { // the replay's task list (the reverse sweep reads the first pass's records directly)
+        if ((s = ensure(ctx, ctx->adj_task_ray, 8 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
+        if ((s = ensure(ctx, ctx->adj_task_seg, 4 * size_t(std::max<int64_t>(ntask, 1)))) != PRR_OK) return s;
+        plan.levelstart = levelstart_d;
+        plan.nlevels = nseg_max;
+        plan.ntask = ntask;
+        plan.task_ray = static_cast<int64_t *>(ctx->adj_task_ray.p);
+        plan.task_seg = static_cast<int32_t *>(ctx->adj_task_seg.p);
+        HIPTRY(ctx, launch_adj_plan_tasks(plan, st));
+    }
     const size_t rec_task = 48 * size_t(K); // bytes of records per task
     const uint64_t budget = ctx->launch.adjoint_record_bytes > 0 ? uint64_t(ctx->launch.adjoint_record_bytes)
                                                                  : uint64_t(kRecBudget);
@@ -1167,14 +1169,14 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     if (!on_device) HIPTRY(ctx, hipStreamWaitEvent(st, ctx->copy_done, 0)); // dL/dI is on the device
     for (const auto &c : chunks) {
         const int64_t t0 = levelstart[c.first], t1 = levelstart[c.second] + count[c.second];
-        j3.adj.rays = static_cast<const int64_t *>(ctx->adj_task_ray.p) + t0;
-        j3.adj.seg = static_cast<const int32_t *>(ctx->adj_task_seg.p) + t0;
         j3.adj.rec_stride = t1 - t0;
         j3.total = t1 - t0;
         if (direct) {
             ad.rec = job.adj.rec_all;
             ad.rec_cap = job.adj.rec_cap;
         } else {
+            j3.adj.rays = static_cast<const int64_t *>(ctx->adj_task_ray.p) + t0;
+            j3.adj.seg = static_cast<const int32_t *>(ctx->adj_task_seg.p) + t0;
             HIPTRY(ctx, launch_reset(ctr, st, false));
             const int grid = int(std::min<int64_t>(P.grid, (j3.total + 63) / 64 + 1));
             HIPTRY(ctx, launch_integrate(ctx->metric, j3, grid, P.block, st));
