@@ -101,6 +101,22 @@ prr_status prr::ensure(prr_ctx *ctx, DevBuf &b, size_t bytes) {
     return PRR_OK;
 }
 
+bool prr::grow_quietly(DevBuf &b, size_t bytes) {
+    if (b.bytes >= bytes && b.p) return true;
+    if (b.p) {
+        if (hipFree(b.p) != hipSuccess) return false;
+        b.p = nullptr;
+        b.bytes = 0;
+    }
+    if (hipMalloc(&b.p, bytes) != hipSuccess) {
+        b.p = nullptr;
+        (void)hipGetLastError(); // the failed allocation leaves no error behind
+        return false;
+    }
+    b.bytes = bytes;
+    return true;
+}
+
 prr_status prr::ensure_pinned(prr_ctx *ctx, size_t bytes) {
     if (ctx->pinned_bytes >= bytes && ctx->pinned) return PRR_OK;
     if (ctx->pinned) {
@@ -923,8 +939,8 @@ prr_status prr_adjoint(prr_ctx *ctx, const prr_frame *tmpl, const prr_orbit *orb
     const uint64_t rec_budget = ctx->launch.adjoint_record_bytes > 0 ? uint64_t(ctx->launch.adjoint_record_bytes)
                                                                        : uint64_t(kRecBudget);
     const int64_t rec_cap = int64_t(cap) * K;
-    if (48 * uint64_t(rec_cap) * uint64_t(T) <= rec_budget) {
-        if ((s = ensure(ctx, ctx->adj_recall, 48 * size_t(rec_cap) * size_t(T))) != PRR_OK) return s;
+    if (48 * uint64_t(rec_cap) * uint64_t(T) <= rec_budget &&
+        grow_quietly(ctx->adj_recall, 48 * size_t(rec_cap) * size_t(T))) { // else: replay
         job.adj.rec_all = static_cast<double *>(ctx->adj_recall.p);
         job.adj.rec_cap = rec_cap;
         job.adj.event = static_cast<EventDev *>(ctx->adj_event.p);

@@ -128,6 +128,9 @@ namespace prr {
 prr_status fail(prr_ctx *ctx, prr_status st, const char *fmt, ...);
 // Grows b to at least `bytes` (its contents are not kept).
 prr_status ensure(prr_ctx *ctx, DevBuf &b, size_t bytes);
+// Like ensure, for optional buffers: false (and no error recorded) when the
+// memory is not there.
+bool grow_quietly(DevBuf &b, size_t bytes);
 // Grows the context's pinned host block to at least `bytes` (contents not kept).
 prr_status ensure_pinned(prr_ctx *ctx, size_t bytes);
 
