@@ -562,6 +562,15 @@ template <class Mt> PRR_HD D3 cart_of(const double *y) {
 }
 
 // FP32 Cartesian position for the sphere screen (accuracy ~1e-6 relative).
+// Kerr: the cylindrical radius and height (rho, z) of the screen's FP32
+// position, from the same r, sin th and cos th values as cart_of_f32.
+PRR_HD void kerr_rz_f32(const double *y, float P[2]) {
+    const float r = rcp_t(float(y[0])), ct = float(y[2]);
+    const float st = sqrt_f32(fmaxf(1.0f - ct * ct, 0.0f));
+    P[0] = r * st;
+    P[1] = r * ct;
+}
+
 template <class Mt> PRR_HD void cart_of_f32(const double *y, float P[3]) {
     if (!Mt::spherical) {
         P[0] = float(y[0]);

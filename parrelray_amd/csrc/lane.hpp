@@ -153,7 +153,11 @@ template <class Mt> struct Lane {
     int32_t frame;
     int32_t hit; // sphere index of a SPHERE termination
     int64_t rid;
-    float P0[3]; // FP32 Cartesian position at the start of the step (sphere screen)
+    // Sphere screen at the start of the step: the FP32 Cartesian position; for
+    // Kerr its (rho, z) in P0[0..1], and in gap a path length the ray can go
+    // before any sphere's FP32 screen can pass (not above 0: screen the step)
+    float P0[3];
+    float gap;
     int32_t seg;   // adjoint replay: the segment being replayed
     int64_t rec_i; // adjoint replay: the task's record column
 };
@@ -497,6 +501,16 @@ template <class Mt> PRR_HD double sphere_G(const FrameDev &fr, int k, const doub
     return dot3(m, m) - fr.sph[k][3] * fr.sph[k][3];
 }
 
+// Path length from Q, the end of a screened chord, before sphere k's FP32
+// screen can pass: |Q - c| - R less a margin for the screen's relative margin
+// and the FP32 rounding of the positions over the steps it then covers (the
+// screen's scale grows with the distance travelled, at most |Q - c|).
+PRR_HD float sphere_gap(const FrameDev &fr, int k, const float Q[3]) {
+    const float mx = Q[0] - fr.sphf[k][0], my = Q[1] - fr.sphf[k][1], mz = Q[2] - fr.sphf[k][2];
+    return sqrt_f32(mx * mx + my * my + mz * mz) - fr.sphf[k][3] -
+           5e-4f * (fr.sphf[k][4] + fabsf(Q[0]) + fabsf(Q[1]) + fabsf(Q[2]));
+}
+
 // FP32 screen: can the path of this step come near sphere k?  The chord's
 // closest distance to the centre against R plus half the chord length (a
 // bound on the path's bulge for any step that turns by less than ~1 rad)
@@ -517,6 +531,17 @@ PRR_HD bool sphere_screen(const FrameDev &fr, int k, const float P0[3], const fl
     const float qx = mx + t * dx, qy = my + t * dy, qz = mz + t * dz;
     const float dist = sqrt_f32(qx * qx + qy * qy + qz * qz);
     return dist <= R + 0.5f * sqrt_f32(A) + 1e-4f * scale;
+}
+
+// The screen's state where no budget carries over (a ray loaded, or resumed
+// after a location): the position of L.y, and no budget.
+template <class Mt> PRR_HD void screen_reset(const FrameDev &fr, Lane<Mt> &L) {
+    L.P0[0] = L.P0[1] = L.P0[2] = 0.0f;
+    L.gap = 0.0f;
+    if (fr.n_spheres > 0) {
+        if (Mt::mino) kerr_rz_f32(L.y, L.P0);
+        else cart_of_f32<Mt>(L.y, L.P0);
+    }
 }
 
 // ------------------------------------------------------------ ray state I/O
@@ -572,8 +597,7 @@ template <class Mt> PRR_HD void lane_load(const JobDev &job, Lane<Mt> &L, int64_
     L.lam = 0.0;
     L.steps = L.attempts = L.att64 = 0;
     L.hit = -1;
-    L.P0[0] = L.P0[1] = L.P0[2] = 0.0f;
-    if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
+    screen_reset<Mt>(fr, L);
     L.seg = 0;
     L.rec_i = 0;
 }
@@ -604,8 +628,7 @@ template <class Mt> PRR_HD void lane_load_checkpoint(const JobDev &job, Lane<Mt>
     L.seg = j;
     L.att64 = 0;
     L.hit = -1;
-    L.P0[0] = L.P0[1] = L.P0[2] = 0.0f;
-    if (fr.n_spheres > 0) cart_of_f32<Mt>(L.y, L.P0);
+    screen_reset<Mt>(fr, L);
 }
 
 PRR_HD void store_checkpoint(const JobDev &job, const double *y, int n, double h, int32_t attempts, int64_t id,
@@ -842,7 +865,7 @@ PRR_HD int finish_located(const JobDev &job, Lane<Mt> &L, double h, const double
 // horizon.  Returns a class or -1 (go on: L.y = yn, or the location started).
 template <class Mt>
 PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h,
-                               int &mode, const float *P1 = nullptr) {
+                               int &mode, const float *P1 = nullptr, float gap = 0.0f) {
     const FrameDev &fr = job.frames[L.frame];
     if (beyond_escape<Mt>(job, fr, yn)) {
         const double r_esc = escape_radius<Mt>(job, fr, yn);
@@ -864,14 +887,15 @@ PRR_HD int escape_wrap_horizon(const JobDev &job, Lane<Mt> &L, LocateCtx &C, con
     chart_post_step<Mt>(y);
     copy_n<Mt::N>(L.y, y);
     if (fr.n_spheres > 0) {
-        // P1: the screen's FP32 position of yn, which is the new state's where the chart has no wraps
+        // P1 (and gap): the screen's state at yn, which is the new state's where the chart has no wraps
         constexpr bool wraps = Mt::kind == PRR_SCHW_SPH || Mt::kind == PRR_ELLIS;
         if (P1 && !wraps) {
             L.P0[0] = P1[0];
             L.P0[1] = P1[1];
             L.P0[2] = P1[2];
+            L.gap = gap;
         } else {
-            cart_of_f32<Mt>(L.y, L.P0);
+            screen_reset<Mt>(fr, L);
         }
     }
     if (inside_horizon<Mt>(fr, yn)) return PRR_HORIZON;
@@ -885,11 +909,41 @@ template <class Mt>
 PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double *yn, double h, int &mode) {
     const FrameDev &fr = job.frames[L.frame];
     float P1[3];
+    float gap = 0.0f;
     if (fr.n_spheres > 0) {
-        cart_of_f32<Mt>(yn, P1);
         unsigned mask = 0u;
-        for (int k = 0; k < fr.n_spheres; ++k)
-            if (sphere_screen(fr, k, L.P0, P1)) mask |= 1u << k;
+        if constexpr (Mt::mino) {
+            // Kerr: the screen runs only when the path since the last screen
+            // may have used up the budget.  With (rho, z) of both ends,
+            // |P1 - P0|^2 = dz^2 + drho^2 + 2 rho0 rho1 (1 - cos dphi) <= D^2 =
+            // dz^2 + drho^2 + rho0 rho1 dphi^2, D inflated to cover the FP32
+            // rounding of the screen's positions (relative 1e-5).  The step is
+            // skipped only when gap - D > D / 2: the FP32 screen cannot pass
+            // (any point of this chord is at least gap + R - D from every
+            // centre, and the screen passes within R + |P1 - P0| / 2 plus its
+            // margin), so the steps taken are those of screening every step.
+            kerr_rz_f32(yn, P1);
+            P1[2] = 0.0f;
+            const float dphi = float(yn[4] - L.y[4]);
+            const float dr = P1[0] - L.P0[0], dz = P1[1] - L.P0[1];
+            const float D = sqrt_f32(dr * dr + dz * dz + (L.P0[0] * P1[0]) * (dphi * dphi)) * 1.00001f +
+                            1e-5f * (P1[0] + fabsf(P1[1]) + L.P0[0] + fabsf(L.P0[1]));
+            gap = L.gap - D;
+            if (!(gap > 0.5f * D)) { // (also for a NaN D)
+                float Q0[3], Q1[3];
+                cart_of_f32<Mt>(L.y, Q0);
+                cart_of_f32<Mt>(yn, Q1);
+                gap = 3.0e38f;
+                for (int k = 0; k < fr.n_spheres; ++k) {
+                    if (sphere_screen(fr, k, Q0, Q1)) mask |= 1u << k;
+                    gap = fminf(gap, sphere_gap(fr, k, Q1));
+                }
+            }
+        } else {
+            cart_of_f32<Mt>(yn, P1);
+            for (int k = 0; k < fr.n_spheres; ++k)
+                if (sphere_screen(fr, k, L.P0, P1)) mask |= 1u << k;
+        }
         if (mask) {
             copy_n<Mt::N>(C.yn, yn);
             C.h = h;
@@ -902,7 +956,7 @@ PRR_HD int rkf_events(const JobDev &job, Lane<Mt> &L, LocateCtx &C, const double
             return -1;
         }
     }
-    return escape_wrap_horizon<Mt>(job, L, C, yn, h, mode, fr.n_spheres > 0 ? P1 : nullptr);
+    return escape_wrap_horizon<Mt>(job, L, C, yn, h, mode, fr.n_spheres > 0 ? P1 : nullptr, gap);
 }
 
 // Result of sphere_prepare: kNeedSubstep (a sub-step of fraction C.th, size
