@@ -42,7 +42,7 @@ P32_NONPACKED = 78.6e12
 BATCH = 4
 # HBM bytes per launch on one C1 frame (profiles/k_integrate_kerr_c1_1080p.md: FETCH_SIZE x 2 + WRITE_SIZE,
 # calibrated on a 1 GiB copy): read + written
-TRAFFIC_C1 = {"fp64": (877.0 + 728.8) * 1e6, "fp32": (877.1 + 745.0) * 1e6}
+TRAFFIC_C1 = {"fp64": (884.5 + 741.9) * 1e6, "fp32": (875.7 + 744.8) * 1e6}
 METRIC = "Kerr RKF45 1080p rays/s + adjoint rays/s, FP32-bulk vs FP64 at ≤1e-6 rad"
 
 
